@@ -1,0 +1,219 @@
+/*
+ * x3dpt.h — C ABI of the MI355X (gfx950) path-tracing library, libx3dpt.so.
+ *
+ * Drop-in boundary for the per-sample path-tracing loop of DaviesX/x3dcore2 ("e8yescg"):
+ * scene + camera + spp in, linear-RGB film out. Plain pointers and sizes, opaque handles,
+ * int status codes (X3D_OK = 0) with a thread-local message from x3d_last_error(); no C++
+ * exception crosses the boundary. All host buffers are caller-owned. One call in flight per
+ * handle; distinct handles are independent.
+ *
+ * Each entry point names the reference interface it replaces (file:line under corelib/src or
+ * corelib/test of the reference).
+ */
+#ifndef X3DPT_H
+#define X3DPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define X3D_OK 0
+#define X3D_ERR_INVALID_ARGUMENT 1 /* null/empty/out-of-range input */
+#define X3D_ERR_STATE 2            /* e.g. scene not committed, edit after commit */
+#define X3D_ERR_DEVICE 3           /* HIP runtime failure or no GPU */
+#define X3D_ERR_OUT_OF_MEMORY 4
+#define X3D_ERR_UNSUPPORTED 5
+
+/* Last error message of the calling thread ("" when none). */
+const char *x3d_last_error(void);
+
+/* Library version string. */
+const char *x3d_version(void);
+
+/* ---- scene (replaces the if_obj_actuator back-ends bvh_path_space_layout,
+ *      default_material_container and basic_light_sources: pathspace.h:40-58, 76-164,
+ *      materialcontainer.h:14-53, lightsources.h:18-61) ---------------------------------- */
+typedef struct x3d_scene x3d_scene;
+
+/* Material kinds. OREN_NAYAR with sigma = 0 is the Lambertian of oren_nayar
+ * (material.cpp:108-169); COOK_TORR is cook_torr (GGX D, Smith G, complex-IOR Fresnel,
+ * alpha^2 = 2 r^2; material.cpp:171-252); MIXTURE is mat_mixture (material.cpp:78-106). */
+#define X3D_MAT_OREN_NAYAR 0
+#define X3D_MAT_COOK_TORR 1
+#define X3D_MAT_MIXTURE 2
+
+typedef struct x3d_material_desc {
+    int32_t kind;
+    float albedo[3];
+    float sigma;     /* OREN_NAYAR roughness (0 = Lambertian) */
+    float roughness; /* COOK_TORR r */
+    float ior_re, ior_im;
+    int32_t sub0, sub1; /* MIXTURE operands (material ids already added) */
+    float ratio;        /* MIXTURE probability of sub0 */
+} x3d_material_desc;
+
+int x3d_scene_create(x3d_scene **out);
+int x3d_scene_destroy(x3d_scene *scene);
+
+/* Adds a material; *out_id is its id (0, 1, 2, ... in call order). */
+int x3d_scene_add_material(x3d_scene *scene, const x3d_material_desc *desc, int32_t *out_id);
+
+/* Adds a world-space triangle mesh (replaces if_path_space::load of a trimesh,
+ * pathspace.cpp:15-19 / geometry.h:81-116). verts, normals: float[3*num_verts] (per-vertex
+ * shading normals); uvs: optional float[2*num_verts] (may be NULL); tris: uint32[3*num_tris].
+ * material_id < 0 selects the fail-safe Lambertian 0.8 (materialcontainer.cpp:7-17).
+ * Meshes are numbered 0, 1, 2, ... in call order; that number is the hit's `geo`. */
+int x3d_scene_add_mesh(x3d_scene *scene, const float *verts, const float *normals, const float *uvs,
+                       uint32_t num_verts, const uint32_t *tris, uint32_t num_tris,
+                       int32_t material_id, int32_t *out_mesh_id);
+
+/* Makes mesh `mesh_id` an area light of the given radiance (replaces area_light,
+ * light.cpp:17-85, loaded into basic_light_sources, lightsources.cpp:11-36). */
+int x3d_scene_add_area_light(x3d_scene *scene, int32_t mesh_id, const float radiance[3],
+                             int32_t *out_light_id);
+
+/* Builds the SAH BVH on the host with the reference's layout (bvh_path_space_layout::commit,
+ * pathspace.cpp:292-328), the light power CDF (lightsources.cpp:45-52), and uploads the
+ * scene to device memory. The scene is immutable afterwards. */
+int x3d_scene_commit(x3d_scene *scene);
+
+typedef struct x3d_scene_info {
+    uint32_t num_meshes, num_tris, num_nodes, num_interior, num_lights, num_materials;
+    uint32_t max_leaf_depth;   /* deepest leaf (root = depth 1) */
+    uint64_t device_bytes;     /* scene bytes resident in HBM */
+    double build_seconds;      /* host BVH build time */
+    double upload_seconds;     /* host-to-device copy time */
+} x3d_scene_info;
+int x3d_scene_get_info(const x3d_scene *scene, x3d_scene_info *info);
+
+/* Dumps the flattened BVH in the reference layout (flattened_node, pathspace.h:130-150):
+ * bounds float[6*num_nodes] (min xyz, max xyz); meta uint32[4*num_nodes] (num_prims,
+ * split_axis, next_child, prim_start); prims int32[2*num_tris] (mesh, triangle). */
+int x3d_scene_dump_bvh(const x3d_scene *scene, float *bounds, uint32_t *meta, int32_t *prims);
+
+/* ---- ray queries (replace bvh_path_space_layout::intersect / has_intersect,
+ *      pathspace.cpp:330-456) -------------------------------------------------------------- */
+typedef struct x3d_hit {
+    int32_t geo;    /* mesh id, -1 on miss */
+    int32_t tri;    /* triangle index within the mesh, -1 on miss */
+    int32_t prim;   /* index into the BVH primitive array, -1 on miss */
+    float t;        /* ray parameter of the hit (INFINITY on miss) */
+    float b1, b2;   /* barycentrics of vertex 1 and 2 (tensor.h:688-725) */
+    float vertex[3];
+    float normal[3]; /* interpolated, normalized shading normal (pathspace.cpp:385-409) */
+} x3d_hit;
+
+/* Traversal modes. REFERENCE visits every box overlapping [1e-4, 1000] in the reference's
+ * LIFO order (pathspace.cpp:343-383). CULLED (default) visits the nearer child first and skips
+ * boxes that start beyond the best hit; ties keep the reference's winner. Both return the same
+ * hits as the reference's intersector. */
+#define X3D_TRAVERSAL_CULLED 0
+#define X3D_TRAVERSAL_REFERENCE 1
+
+/* Closest hits over t in [1e-4, 1000]. rays: float[6*n] (origin, direction), host memory. */
+int x3d_intersect(const x3d_scene *scene, const float *rays, uint32_t n, int32_t traversal,
+                  x3d_hit *hits);
+
+/* Any hit with t in [tmin[k], tmax[k]] (has_intersect). occluded: uint8[n] (1 = blocked). */
+int x3d_occluded(const x3d_scene *scene, const float *rays, const float *tmin, const float *tmax,
+                 uint32_t n, uint8_t *occluded);
+
+/* Device-pointer variants for callers that keep rays in HBM (no host copies): d_rays, d_hits
+ * etc. are device pointers; the call is asynchronous on `stream` (a hipStream_t, or NULL for
+ * the default stream). */
+int x3d_intersect_device(const x3d_scene *scene, const float *d_rays, uint32_t n, int32_t traversal,
+                         x3d_hit *d_hits, void *stream);
+int x3d_occluded_device(const x3d_scene *scene, const float *d_rays, const float *d_tmin,
+                        const float *d_tmax, uint32_t n, uint8_t *d_occluded, void *stream);
+
+/* ---- BSDF batches (replace if_material::eval / sample, material.h:75-89; pdf is new:
+ *      the solid-angle density sample() returns, as a function of (o, i)) ----------------- */
+/* f: float[3*n]. normals, o, i: float[3*n]. */
+int x3d_bsdf_eval(const x3d_scene *scene, int32_t material_id, uint32_t n, const float *normals,
+                  const float *o, const float *i, float *f);
+int x3d_bsdf_pdf(const x3d_scene *scene, int32_t material_id, uint32_t n, const float *normals,
+                 const float *o, const float *i, float *pdf);
+/* Sample k draws from the PCG stream (seed, pixel = k, sample = 0). reference_mixture = 1 uses
+ * mat_mixture's density (chosen lobe's density times its probability, material.cpp:94-106);
+ * 0 the full mixture density. i: float[3*n], dens: float[n]. */
+int x3d_bsdf_sample(const x3d_scene *scene, int32_t material_id, uint64_t seed, uint32_t n,
+                    const float *normals, const float *o, int32_t reference_mixture, float *i,
+                    float *dens);
+
+/* ---- camera (replaces pinhole_camera, camera.cpp:23-71) --------------------------------- */
+typedef struct x3d_camera {
+    float sensor_size, focal_length, aspect;
+    float transform[16]; /* column-major 4x4; rotation = upper-left 3x3, position = column 3 */
+} x3d_camera;
+
+/* pinhole_camera::sample for every pixel (deterministic, no jitter). rays: float[6*w*h]. */
+int x3d_camera_rays(const x3d_camera *camera, uint32_t width, uint32_t height, float *rays);
+
+/* ---- renderer (replaces pt_image_renderer(factory, threads) + render(...),
+ *      renderer.h:30, 57-59, and pathtracer_factory, pathtracerfact.h:12-21) -------------- */
+typedef struct x3d_renderer x3d_renderer;
+
+/* Tracer types: pathtracer_factory::pt_type values (pathtracerfact.h:12) ... */
+#define X3D_PT_NORMAL 0
+#define X3D_PT_POSITION 1
+#define X3D_PT_DIRECT 2
+#define X3D_PT_UNIDIRECT 3     /* unidirect_path_tracer (pathtracer.cpp:528-585) */
+#define X3D_PT_UNIDIRECT_LT1 4 /* unidirect_lt1_path_tracer (pathtracer.cpp:587-653) */
+#define X3D_PT_BIDIRECT_LT2 5
+#define X3D_PT_BIDIRECT_MIS 6  /* bidirect_mis_path_tracer (pathtracer.cpp:754-803) */
+/* ... plus the fixed-depth unidirectional tracer with light + BSDF sampling combined by the
+ * power heuristic (no reference counterpart). */
+#define X3D_PT_UNIDIRECT_MIS 16
+
+/* Execution forms of the same estimator; every form gives the bit-identical film. */
+#define X3D_FORM_PLAIN 0     /* one thread per pixel path, one kernel (the drop-in) */
+#define X3D_FORM_WAVEFRONT 1 /* ray queues sorted by material/depth, bounce loop in a hipGraph */
+
+typedef struct x3d_render_options {
+    int32_t pt_type;       /* X3D_PT_* */
+    int32_t form;          /* X3D_FORM_* */
+    uint32_t max_depth;    /* UNIDIRECT_MIS: bounces after the first hit */
+    uint32_t max_path_len; /* BIDIRECT_MIS: vertices per subpath (reference: 8) */
+    uint64_t seed;         /* PCG seed; sample s of pixel p uses stream (seed, p, s) */
+    uint32_t workgroup_size; /* 0 = library default / autotuned */
+    uint32_t reserved[7];
+} x3d_render_options;
+
+/* Fills *opts with the defaults for pt_type. */
+int x3d_render_options_default(int32_t pt_type, x3d_render_options *opts);
+
+int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out);
+int x3d_renderer_destroy(x3d_renderer *r);
+
+typedef struct x3d_render_stats {
+    uint64_t num_samples;       /* w*h*spp */
+    uint64_t closest_queries;   /* closest-hit rays traced (including the first-hit pass) */
+    uint64_t anyhit_queries;    /* shadow rays traced */
+    double kernel_ms;           /* device time of the render kernels (HIP events) */
+    double total_ms;            /* wall time of the call including copies */
+} x3d_render_stats;
+
+/* Renders the film: rgba float[4*w*h], pixel (x, y) at index x + y*w, row 0 = top; rgb is the
+ * mean radiance over spp samples, alpha = 1/spp (renderer.cpp:270-295 with one task).
+ * firefly_filter must be 0 (the filter is not implemented). stats may be NULL. */
+int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+               uint32_t height, uint32_t spp, int32_t firefly_filter, float *rgba,
+               x3d_render_stats *stats);
+
+/* Same, leaving the film in device memory (d_rgba: float[4*w*h] device pointer). */
+int x3d_render_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera,
+                      uint32_t width, uint32_t height, uint32_t spp, float *d_rgba,
+                      x3d_render_stats *stats);
+
+/* PCG draws of stream (seed, pixel, sample) computed on the device (for RNG parity tests). */
+int x3d_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* X3DPT_H */

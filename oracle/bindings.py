@@ -1,0 +1,268 @@
+"""ctypes bindings for the CPU checkers (test infrastructure only).
+
+`Oracle` wraps oracle/liboracle.so (the C++ restatement); `Reference` wraps
+oracle/_ref/libx3dref.so (the reference's own sources built by `make -C oracle ref`). Only
+tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may use this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libx3dref.so")
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+TRAV_REF, TRAV_CULLED, TRAV_BRUTE = 0, 1, 2
+# Tracer ids: pathtracer_factory::pt_type order (corelib/src/pathtracerfact.h:12) plus the
+# config-mode MIS tracer.
+TR_NORMAL, TR_POSITION, TR_DIRECT, TR_UNIDIRECT, TR_UNIDIRECT_LT1, TR_BIDIRECT_LT2, TR_BIDIRECT_MIS = range(7)
+TR_UNIDIRECT_MIS = 16
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def oracle_available() -> bool:
+    return os.path.exists(ORACLE_SO)
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+class _Backend:
+    prefix = ""
+
+    def _load_desc(self, desc):
+        p = self.prefix
+        lib = self.lib
+        mat_ids = []
+        for m in desc.materials:
+            if m.kind == 0:
+                mat_ids.append(getattr(lib, p + "add_oren_nayar")(self.h, _f32(m.albedo), C.c_float(m.sigma)))
+            elif m.kind == 1:
+                mat_ids.append(getattr(lib, p + "add_cook_torr")(self.h, _f32(m.albedo), C.c_float(m.roughness),
+                                                                 C.c_float(m.ior[0]), C.c_float(m.ior[1])))
+            else:
+                mat_ids.append(getattr(lib, p + "add_mixture")(self.h, mat_ids[m.sub0], mat_ids[m.sub1],
+                                                               C.c_float(m.ratio)))
+        for mesh in desc.meshes:
+            getattr(lib, p + "add_mesh")(self.h, _f32(mesh.verts), _f32(mesh.normals), mesh.verts.shape[0],
+                                         np.ascontiguousarray(mesh.tris, np.uint32), mesh.tris.shape[0],
+                                         mat_ids[mesh.material] if mesh.material >= 0 else -1)
+        for mi, rad in desc.lights:
+            getattr(lib, p + "add_area_light")(self.h, mi, _f32(rad))
+        getattr(lib, p + "commit")(self.h)
+
+    def bvh(self):
+        nn, np_ = C.c_uint32(), C.c_uint32()
+        getattr(self.lib, self.prefix + "bvh_size")(self.h, C.byref(nn), C.byref(np_))
+        bounds = np.zeros((nn.value, 6), np.float32)
+        meta = np.zeros((nn.value, 4), np.uint32)
+        prims = np.zeros((np_.value, 2), np.int32)
+        getattr(self.lib, self.prefix + "bvh_dump")(self.h, bounds, meta, prims)
+        return bounds, meta, prims
+
+
+class Oracle(_Backend):
+    prefix = "orc_"
+
+    def __init__(self, desc=None):
+        lib = C.CDLL(ORACLE_SO)
+        self.lib = lib
+        lib.orc_scene_new.restype = C.c_void_p
+        lib.orc_scene_free.argtypes = [C.c_void_p]
+        lib.orc_add_oren_nayar.argtypes = [C.c_void_p, _f32p, C.c_float]
+        lib.orc_add_cook_torr.argtypes = [C.c_void_p, _f32p, C.c_float, C.c_float, C.c_float]
+        lib.orc_add_mixture.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float]
+        lib.orc_add_mesh.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, _u32p, C.c_uint32, C.c_int]
+        lib.orc_add_area_light.argtypes = [C.c_void_p, C.c_int, _f32p]
+        lib.orc_commit.argtypes = [C.c_void_p]
+        lib.orc_bvh_size.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        lib.orc_bvh_dump.argtypes = [C.c_void_p, _f32p, _u32p, _i32p]
+        lib.orc_intersect.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_int, _i32p, _i32p, _i32p, _f32p, _f32p,
+                                      _f32p, _f32p, C.c_void_p]
+        lib.orc_occluded.argtypes = [C.c_void_p, _f32p, _f32p, _f32p, C.c_uint32, C.c_int, _u8p, C.c_void_p]
+        lib.orc_bsdf_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _f32p, _f32p, _f32p, C.c_int, _f64p]
+        lib.orc_bsdf_pdf.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _f32p, _f32p, _f32p, C.c_int, _f64p]
+        lib.orc_bsdf_sample.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint32, _f32p, _f32p, C.c_int,
+                                        C.c_int, _f64p, _f64p]
+        lib.orc_pcg_draws.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _u32p]
+        lib.orc_sincos_2pi.argtypes = [C.c_uint32, _f32p, _f32p, _f32p]
+        lib.orc_camera_rays.argtypes = [C.c_float, C.c_float, C.c_float, _f32p, C.c_uint32, C.c_uint32, _f32p]
+        lib.orc_render.argtypes = [C.c_void_p, _f32p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int,
+                                   C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int,
+                                   C.c_uint32, C.c_void_p, C.c_void_p]
+        self.h = C.c_void_p(lib.orc_scene_new())
+        self.desc = desc
+        if desc is not None:
+            self._load_desc(desc)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_scene_free(self.h)
+            self.h = None
+
+    def intersect(self, rays, mode=TRAV_CULLED, stats=None):
+        rays = _f32(rays)
+        n = rays.shape[0]
+        geo, tri, prim = (np.zeros(n, np.int32) for _ in range(3))
+        t = np.zeros(n, np.float32)
+        b = np.zeros((n, 2), np.float32)
+        vtx = np.zeros((n, 3), np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        st = stats if stats is not None else None
+        self.lib.orc_intersect(self.h, rays, n, mode, geo, tri, prim, t, b, vtx, nrm,
+                               st.ctypes.data if st is not None else None)
+        return dict(geo=geo, tri=tri, prim=prim, t=t, b=b, vertex=vtx, normal=nrm)
+
+    def occluded(self, rays, tmin, tmax, mode=TRAV_REF, stats=None):
+        rays = _f32(rays)
+        n = rays.shape[0]
+        out = np.zeros(n, np.uint8)
+        self.lib.orc_occluded(self.h, rays, _f32(tmin), _f32(tmax), n, mode, out,
+                              stats.ctypes.data if stats is not None else None)
+        return out
+
+    def bsdf_eval(self, mat, nrm, o, i, prec=0):
+        n = nrm.shape[0]
+        f = np.zeros((n, 3), np.float64)
+        self.lib.orc_bsdf_eval(self.h, mat, n, _f32(nrm), _f32(o), _f32(i), prec, f)
+        return f
+
+    def bsdf_pdf(self, mat, nrm, o, i, prec=0):
+        n = nrm.shape[0]
+        p = np.zeros(n, np.float64)
+        self.lib.orc_bsdf_pdf(self.h, mat, n, _f32(nrm), _f32(o), _f32(i), prec, p)
+        return p
+
+    def bsdf_sample(self, mat, seed, nrm, o, prec=0, ref_mixture=False):
+        n = nrm.shape[0]
+        i = np.zeros((n, 3), np.float64)
+        d = np.zeros(n, np.float64)
+        self.lib.orc_bsdf_sample(self.h, mat, seed, n, _f32(nrm), _f32(o), prec, int(ref_mixture), i, d)
+        return i, d
+
+    def pcg_draws(self, seed, pixel, sample, n):
+        out = np.zeros(n, np.uint32)
+        self.lib.orc_pcg_draws(seed, pixel, sample, n, out)
+        return out
+
+    def sincos_2pi(self, u):
+        u = _f32(u)
+        s = np.zeros_like(u)
+        c = np.zeros_like(u)
+        self.lib.orc_sincos_2pi(u.shape[0], u, s, c)
+        return s, c
+
+    def camera_rays(self, cam, w, h):
+        rays = np.zeros((w * h, 6), np.float32)
+        self.lib.orc_camera_rays(cam.sensor, cam.focal, cam.aspect, _f32(cam.transform), w, h, rays)
+        return rays
+
+    def render(self, tracer, width, height, spp, seed=1, max_depth=5, max_path_len=8, traversal=TRAV_CULLED,
+               pixels: Optional[np.ndarray] = None, prec=0, threads=None, stats=None, camera=None):
+        cam = (camera or self.desc.camera).packed()
+        npix = width * height if pixels is None else int(pixels.shape[0])
+        out = np.zeros((npix, 4), np.float32 if prec == 0 else np.float64)
+        px = None if pixels is None else np.ascontiguousarray(pixels, np.uint32)
+        threads = threads or os.cpu_count() or 1
+        self.lib.orc_render(self.h, cam, tracer, max_depth, max_path_len, seed, traversal, width, height, spp,
+                            px.ctypes.data if px is not None else None, npix, prec, threads, out.ctypes.data,
+                            stats.ctypes.data if stats is not None else None)
+        return out
+
+
+class Reference(_Backend):
+    prefix = "ref_"
+
+    def __init__(self, desc=None):
+        lib = C.CDLL(REF_SO)
+        self.lib = lib
+        lib.ref_scene_new.restype = C.c_void_p
+        lib.ref_scene_free.argtypes = [C.c_void_p]
+        lib.ref_add_oren_nayar.argtypes = [C.c_void_p, _f32p, C.c_float]
+        lib.ref_add_cook_torr.argtypes = [C.c_void_p, _f32p, C.c_float, C.c_float, C.c_float]
+        lib.ref_add_mixture.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float]
+        lib.ref_add_mesh.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, _u32p, C.c_uint32, C.c_int]
+        lib.ref_add_area_light.argtypes = [C.c_void_p, C.c_int, _f32p]
+        lib.ref_commit.argtypes = [C.c_void_p]
+        lib.ref_bvh_size.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        lib.ref_bvh_dump.argtypes = [C.c_void_p, _f32p, _u32p, _i32p]
+        lib.ref_intersect.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_int, _i32p, _i32p, _f32p, _f32p, _f32p]
+        lib.ref_occluded.argtypes = [C.c_void_p, _f32p, _f32p, _f32p, C.c_uint32, C.c_int, _u8p]
+        lib.ref_bsdf_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _f32p, _f32p, _f32p, _f32p]
+        lib.ref_bsdf_sample.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, _f32p, _f32p, _f32p, _f32p]
+        lib.ref_rng_draws.argtypes = [C.c_uint32, C.c_uint32, _f32p]
+        lib.ref_camera_rays.argtypes = [C.c_float, C.c_float, C.c_float, _f32p, C.c_uint32, C.c_uint32, _f32p]
+        lib.ref_render.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, _f32p, C.c_uint32,
+                                   C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
+        lib.ref_render.restype = C.c_uint32
+        self.h = C.c_void_p(lib.ref_scene_new())
+        self.desc = desc
+        if desc is not None:
+            self._load_desc(desc)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ref_scene_free(self.h)
+            self.h = None
+
+    def intersect(self, rays, linear=False):
+        rays = _f32(rays)
+        n = rays.shape[0]
+        geo, tri = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        vtx = np.zeros((n, 3), np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        self.lib.ref_intersect(self.h, rays, n, int(linear), geo, tri, t, vtx, nrm)
+        return dict(geo=geo, tri=tri, t=t, vertex=vtx, normal=nrm)
+
+    def occluded(self, rays, tmin, tmax, linear=False):
+        rays = _f32(rays)
+        out = np.zeros(rays.shape[0], np.uint8)
+        self.lib.ref_occluded(self.h, rays, _f32(tmin), _f32(tmax), rays.shape[0], int(linear), out)
+        return out
+
+    def bsdf_eval(self, mat, nrm, o, i):
+        n = nrm.shape[0]
+        f = np.zeros((n, 3), np.float32)
+        self.lib.ref_bsdf_eval(self.h, mat, n, _f32(nrm), _f32(o), _f32(i), f)
+        return f
+
+    def bsdf_sample(self, mat, seed, nrm, o):
+        n = nrm.shape[0]
+        i = np.zeros((n, 3), np.float32)
+        d = np.zeros(n, np.float32)
+        self.lib.ref_bsdf_sample(self.h, mat, seed, n, _f32(nrm), _f32(o), i, d)
+        return i, d
+
+    def rng_draws(self, seed, n):
+        out = np.zeros(n, np.float32)
+        self.lib.ref_rng_draws(seed, n, out)
+        return out
+
+    def camera_rays(self, cam, w, h):
+        rays = np.zeros((w * h, 6), np.float32)
+        self.lib.ref_camera_rays(cam.sensor, cam.focal, cam.aspect, _f32(cam.transform), w, h, rays)
+        return rays
+
+    def render(self, tracer, width, height, spp, threads=1, camera=None):
+        cam = camera or self.desc.camera
+        out = np.zeros((width * height, 4), np.float32)
+        taken = self.lib.ref_render(self.h, tracer, cam.sensor, cam.focal, cam.aspect, _f32(cam.transform), width,
+                                    height, spp, threads, out)
+        return out, taken

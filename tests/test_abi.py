@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports every entry point include/x3dpt.h declares (no GPU calls)."""
+
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "x3dpt.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(x3d_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "x3d_render" in names and "x3d_intersect" in names and "x3d_scene_commit" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol(x3d):
+    lib = x3d.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(declared_functions()) == set(x3d.EXPORTS)
+
+
+def test_version_and_error_strings(x3d):
+    lib = x3d.lib()
+    assert lib.x3d_version().decode().startswith("x3dpt")
+    assert isinstance(lib.x3d_last_error().decode(), str)
+
+
+def test_scene_edit_errors_without_gpu(x3d):
+    """Host-side validation runs before any device call."""
+    import numpy as np
+    import pytest
+
+    s = x3d.Scene()
+    with pytest.raises(x3d.X3DError) as e:
+        s.add_mesh(np.zeros((3, 3)), np.zeros((3, 3)), np.array([[0, 1, 5]]), -1)
+    assert "out of range" in str(e.value)
+    with pytest.raises(x3d.X3DError):
+        s.add_mesh(np.full((3, 3), np.nan), np.zeros((3, 3)), np.array([[0, 1, 2]]), -1)
+    with pytest.raises(x3d.X3DError):
+        s.add_area_light(3, (1, 1, 1))
+    from x3dcore2_amd import scenes
+
+    with pytest.raises(x3d.X3DError):
+        s.add_material(scenes.Material(scenes.COOK_TORR, roughness=0.0))
+    with pytest.raises(x3d.X3DError):
+        s.add_material(scenes.Material(scenes.MIXTURE, sub0=0, sub1=9))

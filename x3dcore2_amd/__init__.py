@@ -1,0 +1,293 @@
+"""MI355X-native path-tracing hot path of DaviesX/x3dcore2, behind the C ABI of libx3dpt.so.
+
+Python is a thin ctypes layer over include/x3dpt.h; every computation runs in the HIP library.
+Importing the package fails loudly when libx3dpt.so has not been built (there is no CPU
+fallback). Build with ``python -m x3dcore2_amd.build`` or ``__graft_entry__.build()``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import scenes  # noqa: F401  (re-exported)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libx3dpt.so")
+
+# Tracer types (include/x3dpt.h): pathtracer_factory::pt_type order plus the MIS tracer.
+PT_NORMAL, PT_POSITION, PT_DIRECT, PT_UNIDIRECT, PT_UNIDIRECT_LT1, PT_BIDIRECT_LT2, PT_BIDIRECT_MIS = range(7)
+PT_UNIDIRECT_MIS = 16
+FORM_PLAIN, FORM_WAVEFRONT = 0, 1
+TRAVERSAL_CULLED, TRAVERSAL_REFERENCE = 0, 1
+
+
+class X3DError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"x3dpt error {code}: {msg}")
+        self.code = code
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("albedo", C.c_float * 3), ("sigma", C.c_float), ("roughness", C.c_float),
+                ("ior_re", C.c_float), ("ior_im", C.c_float), ("sub0", C.c_int32), ("sub1", C.c_int32),
+                ("ratio", C.c_float)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("num_meshes", C.c_uint32), ("num_tris", C.c_uint32), ("num_nodes", C.c_uint32),
+                ("num_interior", C.c_uint32), ("num_lights", C.c_uint32), ("num_materials", C.c_uint32),
+                ("max_leaf_depth", C.c_uint32), ("device_bytes", C.c_uint64), ("build_seconds", C.c_double),
+                ("upload_seconds", C.c_double)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("geo", C.c_int32), ("tri", C.c_int32), ("prim", C.c_int32), ("t", C.c_float), ("b1", C.c_float),
+                ("b2", C.c_float), ("vertex", C.c_float * 3), ("normal", C.c_float * 3)]
+
+
+HIT_DTYPE = np.dtype([("geo", np.int32), ("tri", np.int32), ("prim", np.int32), ("t", np.float32),
+                      ("b1", np.float32), ("b2", np.float32), ("vertex", np.float32, 3),
+                      ("normal", np.float32, 3)])
+assert HIT_DTYPE.itemsize == C.sizeof(Hit)
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("sensor_size", C.c_float), ("focal_length", C.c_float), ("aspect", C.c_float),
+                ("transform", C.c_float * 16)]
+
+
+class RenderOptions(C.Structure):
+    _fields_ = [("pt_type", C.c_int32), ("form", C.c_int32), ("max_depth", C.c_uint32),
+                ("max_path_len", C.c_uint32), ("seed", C.c_uint64), ("workgroup_size", C.c_uint32),
+                ("reserved", C.c_uint32 * 7)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("num_samples", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
+                ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+_lib: Optional[C.CDLL] = None
+
+# Every exported symbol declared in include/x3dpt.h.
+EXPORTS = [
+    "x3d_last_error", "x3d_version", "x3d_scene_create", "x3d_scene_destroy", "x3d_scene_add_material",
+    "x3d_scene_add_mesh", "x3d_scene_add_area_light", "x3d_scene_commit", "x3d_scene_get_info",
+    "x3d_scene_dump_bvh", "x3d_intersect", "x3d_occluded", "x3d_intersect_device", "x3d_occluded_device",
+    "x3d_bsdf_eval", "x3d_bsdf_pdf", "x3d_bsdf_sample", "x3d_camera_rays", "x3d_render_options_default",
+    "x3d_renderer_create", "x3d_renderer_destroy", "x3d_render", "x3d_render_device", "x3d_pcg_draws",
+]
+
+
+def lib() -> C.CDLL:
+    """Loads libx3dpt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m x3dcore2_amd.build` "
+                          "(the renderer has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, i32, u64 = C.c_void_p, C.c_uint32, C.c_int32, C.c_uint64
+    fp = C.POINTER(C.c_float)
+    L.x3d_last_error.restype = C.c_char_p
+    L.x3d_version.restype = C.c_char_p
+    for name in EXPORTS:
+        f = getattr(L, name)
+        if name not in ("x3d_last_error", "x3d_version"):
+            f.restype = C.c_int
+    L.x3d_scene_create.argtypes = [C.POINTER(vp)]
+    L.x3d_scene_destroy.argtypes = [vp]
+    L.x3d_scene_add_material.argtypes = [vp, C.POINTER(MaterialDesc), C.POINTER(i32)]
+    L.x3d_scene_add_mesh.argtypes = [vp, vp, vp, vp, u32, vp, u32, i32, C.POINTER(i32)]
+    L.x3d_scene_add_area_light.argtypes = [vp, i32, fp, C.POINTER(i32)]
+    L.x3d_scene_commit.argtypes = [vp]
+    L.x3d_scene_get_info.argtypes = [vp, C.POINTER(SceneInfo)]
+    L.x3d_scene_dump_bvh.argtypes = [vp, vp, vp, vp]
+    L.x3d_intersect.argtypes = [vp, vp, u32, i32, vp]
+    L.x3d_occluded.argtypes = [vp, vp, vp, vp, u32, vp]
+    L.x3d_intersect_device.argtypes = [vp, vp, u32, i32, vp, vp]
+    L.x3d_occluded_device.argtypes = [vp, vp, vp, vp, u32, vp, vp]
+    L.x3d_bsdf_eval.argtypes = [vp, i32, u32, vp, vp, vp, vp]
+    L.x3d_bsdf_pdf.argtypes = [vp, i32, u32, vp, vp, vp, vp]
+    L.x3d_bsdf_sample.argtypes = [vp, i32, u64, u32, vp, vp, i32, vp, vp]
+    L.x3d_camera_rays.argtypes = [C.POINTER(CameraDesc), u32, u32, vp]
+    L.x3d_render_options_default.argtypes = [i32, C.POINTER(RenderOptions)]
+    L.x3d_renderer_create.argtypes = [C.POINTER(RenderOptions), C.POINTER(vp)]
+    L.x3d_renderer_destroy.argtypes = [vp]
+    L.x3d_render.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, i32, vp, C.POINTER(RenderStats)]
+    L.x3d_render_device.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp, C.POINTER(RenderStats)]
+    L.x3d_pcg_draws.argtypes = [u64, u64, u32, u32, vp]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise X3DError(rc, lib().x3d_last_error().decode())
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def camera_desc(cam) -> CameraDesc:
+    c = CameraDesc()
+    c.sensor_size, c.focal_length, c.aspect = cam.sensor, cam.focal, cam.aspect
+    for k in range(16):
+        c.transform[k] = float(cam.transform[k])
+    return c
+
+
+class Scene:
+    """A committed scene resident in device memory (x3d_scene)."""
+
+    def __init__(self, desc=None, commit: bool = True):
+        L = lib()
+        h = C.c_void_p()
+        _check(L.x3d_scene_create(C.byref(h)))
+        self.h = h
+        self.desc = desc
+        self._keep = []
+        if desc is not None:
+            self.load(desc)
+            if commit:
+                self.commit()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.x3d_scene_destroy(self.h)
+            self.h = None
+
+    def add_material(self, m) -> int:
+        d = MaterialDesc()
+        d.kind = m.kind
+        for k in range(3):
+            d.albedo[k] = m.albedo[k]
+        d.sigma, d.roughness = m.sigma, m.roughness
+        d.ior_re, d.ior_im = m.ior
+        d.sub0, d.sub1, d.ratio = m.sub0, m.sub1, m.ratio
+        out = C.c_int32()
+        _check(lib().x3d_scene_add_material(self.h, C.byref(d), C.byref(out)))
+        return out.value
+
+    def add_mesh(self, verts, normals, tris, material=-1) -> int:
+        v, n = _f32(verts), _f32(normals)
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        out = C.c_int32()
+        _check(lib().x3d_scene_add_mesh(self.h, _ptr(v), _ptr(n), None, v.shape[0], _ptr(t), t.shape[0],
+                                        int(material), C.byref(out)))
+        return out.value
+
+    def add_area_light(self, mesh_id, radiance) -> int:
+        r = (C.c_float * 3)(*radiance)
+        out = C.c_int32()
+        _check(lib().x3d_scene_add_area_light(self.h, int(mesh_id), r, C.byref(out)))
+        return out.value
+
+    def load(self, desc):
+        ids = [self.add_material(m) for m in desc.materials]
+        for mesh in desc.meshes:
+            self.add_mesh(mesh.verts, mesh.normals, mesh.tris, ids[mesh.material] if mesh.material >= 0 else -1)
+        for mi, rad in desc.lights:
+            self.add_area_light(mi, rad)
+
+    def commit(self):
+        _check(lib().x3d_scene_commit(self.h))
+
+    def info(self) -> SceneInfo:
+        i = SceneInfo()
+        _check(lib().x3d_scene_get_info(self.h, C.byref(i)))
+        return i
+
+    def bvh(self):
+        i = self.info()
+        bounds = np.zeros((i.num_nodes, 6), np.float32)
+        meta = np.zeros((i.num_nodes, 4), np.uint32)
+        prims = np.zeros((i.num_tris, 2), np.int32)
+        _check(lib().x3d_scene_dump_bvh(self.h, _ptr(bounds), _ptr(meta), _ptr(prims)))
+        return bounds, meta, prims
+
+    def intersect(self, rays, traversal=TRAVERSAL_CULLED) -> np.ndarray:
+        r = _f32(rays)
+        hits = np.zeros(r.shape[0], HIT_DTYPE)
+        _check(lib().x3d_intersect(self.h, _ptr(r), r.shape[0], traversal, _ptr(hits)))
+        return hits
+
+    def occluded(self, rays, tmin, tmax) -> np.ndarray:
+        r = _f32(rays)
+        out = np.zeros(r.shape[0], np.uint8)
+        _check(lib().x3d_occluded(self.h, _ptr(r), _ptr(_f32(tmin)), _ptr(_f32(tmax)), r.shape[0], _ptr(out)))
+        return out
+
+    def bsdf_eval(self, mat, nrm, o, i) -> np.ndarray:
+        n = nrm.shape[0]
+        f = np.zeros((n, 3), np.float32)
+        _check(lib().x3d_bsdf_eval(self.h, mat, n, _ptr(_f32(nrm)), _ptr(_f32(o)), _ptr(_f32(i)), _ptr(f)))
+        return f
+
+    def bsdf_pdf(self, mat, nrm, o, i) -> np.ndarray:
+        n = nrm.shape[0]
+        p = np.zeros(n, np.float32)
+        _check(lib().x3d_bsdf_pdf(self.h, mat, n, _ptr(_f32(nrm)), _ptr(_f32(o)), _ptr(_f32(i)), _ptr(p)))
+        return p
+
+    def bsdf_sample(self, mat, seed, nrm, o, reference_mixture=False):
+        n = nrm.shape[0]
+        i = np.zeros((n, 3), np.float32)
+        d = np.zeros(n, np.float32)
+        _check(lib().x3d_bsdf_sample(self.h, mat, seed, n, _ptr(_f32(nrm)), _ptr(_f32(o)), int(reference_mixture),
+                                     _ptr(i), _ptr(d)))
+        return i, d
+
+
+class Renderer:
+    """pt_image_renderer equivalent (x3d_renderer)."""
+
+    def __init__(self, pt_type=PT_UNIDIRECT_MIS, form=FORM_PLAIN, max_depth=5, max_path_len=8, seed=1):
+        L = lib()
+        o = RenderOptions()
+        _check(L.x3d_render_options_default(pt_type, C.byref(o)))
+        o.form, o.max_depth, o.max_path_len, o.seed = form, max_depth, max_path_len, seed
+        h = C.c_void_p()
+        _check(L.x3d_renderer_create(C.byref(o), C.byref(h)))
+        self.h = h
+        self.opts = o
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.x3d_renderer_destroy(self.h)
+            self.h = None
+
+    def render(self, scene: Scene, camera, width, height, spp):
+        rgba = np.zeros((height * width, 4), np.float32)
+        st = RenderStats()
+        _check(lib().x3d_render(self.h, scene.h, C.byref(camera_desc(camera)), width, height, spp, 0, _ptr(rgba),
+                                C.byref(st)))
+        return rgba, st
+
+    def render_device(self, scene: Scene, camera, width, height, spp, d_rgba: int):
+        st = RenderStats()
+        _check(lib().x3d_render_device(self.h, scene.h, C.byref(camera_desc(camera)), width, height, spp,
+                                       C.c_void_p(d_rgba), C.byref(st)))
+        return st
+
+
+def camera_rays(camera, width, height) -> np.ndarray:
+    rays = np.zeros((width * height, 6), np.float32)
+    _check(lib().x3d_camera_rays(C.byref(camera_desc(camera)), width, height, _ptr(rays)))
+    return rays
+
+
+def pcg_draws(seed, pixel, sample, n) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    _check(lib().x3d_pcg_draws(seed, pixel, sample, n, _ptr(out)))
+    return out

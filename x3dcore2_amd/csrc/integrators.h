@@ -1,0 +1,250 @@
+// Per-sample estimators executed by the render kernels. Each function consumes the PCG stream
+// of one (seed, pixel, sample) and returns that sample's radiance; the float operation order is
+// the one of the CPU oracle, so the film is reproducible bit for bit.
+#pragma once
+
+#include "traverse.h"
+
+namespace x3d {
+
+constexpr uint32_t kRefModeMaxDepth = 64; // RR p = 0.5 from depth 2: P(depth >= 64) = 2^-62
+
+struct LightSample {
+    int32_t light;
+    f3 p, n;
+    float area_dens; // area density times light selection probability
+};
+
+// basic_light_sources::sample_light (lightsources.cpp:59-75), then a point on the chosen
+// emitter: triangle by area, sqrt-barycentric point and interpolated normal
+// (geometry.cpp:44-66). Draws consumed: 4.
+X3D_DI LightSample sample_light(DevScene const &S, Pcg &g) {
+    LightSample ls;
+    float e = g.draw() * S.total_power;
+    uint32_t lo = 0, hi = S.num_lights;
+    while (lo < hi) {
+        uint32_t mi = (lo + hi) >> 1;
+        if (S.light_cum_power[mi] < e)
+            lo = mi + 1;
+        else
+            hi = mi;
+    }
+    if (lo >= S.num_lights)
+        lo = S.num_lights - 1;
+    DevLight const L = S.lights[lo];
+    float ea = g.draw() * L.area;
+    uint32_t a = 0, b = L.tri_count;
+    const float *cum = S.light_tri_cum + L.tri_begin;
+    while (a < b) {
+        uint32_t mi = (a + b) >> 1;
+        if (cum[mi] < ea)
+            a = mi + 1;
+        else
+            b = mi;
+    }
+    if (a >= L.tri_count)
+        a = L.tri_count - 1;
+    float u = g.draw();
+    float v = g.draw();
+    float r = __builtin_sqrtf(u);
+    float b0 = 1.0f - r;
+    float b1 = r * v;
+    float b2 = 1.0f - b0 - b1;
+    const float4 *lt = static_cast<const float4 *>(S.light_tris) + 6u * (L.tri_begin + a);
+    ls.light = static_cast<int32_t>(lo);
+    ls.p = b0 * xyz(lt[0]) + b1 * xyz(lt[1]) + b2 * xyz(lt[2]);
+    ls.n = normalize(b0 * xyz(lt[3]) + b1 * xyz(lt[4]) + b2 * xyz(lt[5]));
+    ls.area_dens = (1.0f / L.area) * L.prob;
+    return ls;
+}
+
+X3D_DI f3 light_rad(DevScene const &S, int32_t l) {
+    DevLight const &L = S.lights[l];
+    return mk(L.rad[0], L.rad[1], L.rad[2]);
+}
+
+// area_light::radiance (light.cpp:62-71): one-sided.
+X3D_DI f3 light_radiance(DevScene const &S, int32_t l, f3 w, f3 n) {
+    return dot(n, w) > 0.0f ? light_rad(S, l) : mk(0.0f, 0.0f, 0.0f);
+}
+
+template <int WG, int LDSD> struct Tracer {
+    DevScene const &S;
+    TravStack<WG, LDSD> &st;
+    uint32_t pops = 0, tests = 0, spops = 0, stests = 0, nclosest = 0, nany = 0;
+
+    X3D_DI bool trace(f3 o, f3 v, HitRec &out) {
+        nclosest++;
+        ClosestHit h = closest_hit<WG, LDSD, false>(S, o, v, st, pops, tests);
+        if (h.prim < 0)
+            return false;
+        out = finish_hit(S, h);
+        return true;
+    }
+    X3D_DI bool occluded(f3 o, f3 v, float tmin, float tmax) {
+        nany++;
+        return any_hit<WG, LDSD>(S, o, v, tmin, tmax, st, spops, stests);
+    }
+
+    // transport_illum_source (pathtracer.cpp:152-176) divided by the light sample's area
+    // density (pathtracer.cpp:236-239). On success returns true with the unweighted estimate,
+    // the direction to the light, the squared distance and the cosine at the light.
+    X3D_DI bool direct(LightSample const &ls, HitRec const &x, f3 wo, f3 &contrib, f3 &wi, float &r2,
+                       float &cos_l) {
+        f3 l = x.p - ls.p;
+        r2 = dot(l, l);
+        f3 in = l / __builtin_sqrtf(r2);
+        float co = dot(ls.n, in);
+        float ci = dot(x.n, -in);
+        if (!(co > 0 && ci > 0))
+            return false; // area_light::eval is zero (light.cpp:41-52)
+        f3 illum = ((light_rad(S, ls.light) * ci) * co) / r2;
+        if (near0(illum))
+            return false;
+        float dist = norm(l);
+        wi = -l / dist;
+        if (occluded(x.p, wi, 1e-4f, dist - 1e-3f))
+            return false;
+        f3 f = bsdf_eval(S.mats, x.mat, x.n, wo, wi);
+        contrib = mul(illum, f) / ls.area_dens;
+        cos_l = dot(ls.n, -wi);
+        return true;
+    }
+
+    // Fixed-depth unidirectional path tracing: next-event estimation and BSDF sampling combined
+    // with the power heuristic (Veach 1997); up to max_depth bounces after the first hit.
+    X3D_DI f3 li_mis(HitRec const &first, bool valid, f3 ray_v, Pcg &g, uint32_t max_depth) {
+        f3 L = mk(0.0f, 0.0f, 0.0f), beta = mk(1.0f, 1.0f, 1.0f);
+        if (!valid)
+            return L;
+        HitRec x = first;
+        f3 wo = -ray_v;
+        if (x.light >= 0)
+            L = L + light_radiance(S, x.light, wo, x.n);
+        for (uint32_t k = 0; k < max_depth; k++) {
+            if (S.num_lights > 0) {
+                LightSample ls = sample_light(S, g);
+                f3 d, wi;
+                float r2, cl;
+                if (direct(ls, x, wo, d, wi, r2, cl)) {
+                    float pl = ls.area_dens * r2 / cl;
+                    float pb = bsdf_pdf(S.mats, x.mat, x.n, wo, wi);
+                    float w = pl * pl / (pl * pl + pb * pb);
+                    L = L + mul(beta, d * w);
+                }
+            }
+            float dens;
+            f3 wi = bsdf_sample(S.mats, x.mat, x.n, wo, g, &dens, false);
+            if (!(dens > 0.0f))
+                break;
+            HitRec nx;
+            if (!trace(x.p, wi, nx) || dot(nx.n, -wi) <= 0.0f)
+                break;
+            float cw = dot(x.n, wi);
+            f3 f = bsdf_eval(S.mats, x.mat, x.n, wo, wi);
+            beta = mul(beta, f) * cw / dens;
+            if (nx.light >= 0) {
+                float cl = dot(nx.n, -wi);
+                DevLight const &Ln = S.lights[nx.light];
+                float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
+                float w = dens * dens / (dens * dens + pl * pl);
+                L = L + mul(beta, light_radiance(S, nx.light, -wi, nx.n) * w);
+            }
+            x = nx;
+            wo = -wi;
+        }
+        return L;
+    }
+
+    // unidirect_lt1_path_tracer (pathtracer.cpp:587-653): NEE without MIS, Russian roulette
+    // p = 0.5 from depth 2, emission only at the first hit; recursion evaluated front to back.
+    X3D_DI f3 li_lt1(HitRec const &first, bool valid, f3 ray_v, Pcg &g) {
+        f3 L = mk(0.0f, 0.0f, 0.0f), beta = mk(1.0f, 1.0f, 1.0f);
+        if (!valid)
+            return L;
+        HitRec x = first;
+        f3 wo = -ray_v;
+        for (uint32_t depth = 0; depth < kRefModeMaxDepth; depth++) {
+            if (depth >= 2) {
+                if (g.draw() >= 0.5f)
+                    break;
+                beta = beta / 0.5f;
+            }
+            if (S.num_lights > 0) {
+                LightSample ls = sample_light(S, g);
+                f3 d = mk(0.0f, 0.0f, 0.0f), wi;
+                float r2, cl;
+                if (!direct(ls, x, wo, d, wi, r2, cl))
+                    d = mk(0.0f, 0.0f, 0.0f);
+                L = L + mul(beta, d);
+            }
+            float dens;
+            f3 wi = bsdf_sample(S.mats, x.mat, x.n, wo, g, &dens, true);
+            if (dens == 0.0f)
+                break;
+            HitRec nx;
+            if (!trace(x.p, wi, nx) || dot(nx.n, -wi) <= 0.0f)
+                break;
+            float cw = dot(x.n, wi);
+            f3 f = bsdf_eval(S.mats, x.mat, x.n, wo, wi);
+            beta = mul(beta, f) * cw / dens;
+            x = nx;
+            wo = -wi;
+        }
+        if (first.light >= 0)
+            L = L + light_radiance(S, first.light, -ray_v, first.n);
+        return L;
+    }
+
+    // unidirect_path_tracer (pathtracer.cpp:528-585): BSDF sampling only, emission at every
+    // vertex, Russian roulette p = 0.5 from depth 2.
+    X3D_DI f3 li_unidirect(HitRec const &first, bool valid, f3 ray_v, Pcg &g) {
+        f3 L = mk(0.0f, 0.0f, 0.0f), beta = mk(1.0f, 1.0f, 1.0f);
+        if (!valid)
+            return L;
+        HitRec x = first;
+        f3 wo = -ray_v;
+        for (uint32_t depth = 0; depth < kRefModeMaxDepth; depth++) {
+            if (depth >= 2) {
+                if (g.draw() >= 0.5f)
+                    break;
+                beta = beta / 0.5f;
+            }
+            if (x.light >= 0)
+                L = L + mul(beta, light_radiance(S, x.light, wo, x.n));
+            float dens;
+            f3 wi = bsdf_sample(S.mats, x.mat, x.n, wo, g, &dens, true);
+            if (dens == 0.0f)
+                break;
+            HitRec nx;
+            if (!trace(x.p, wi, nx) || dot(nx.n, -wi) <= 0.0f)
+                break;
+            float cw = dot(x.n, wi);
+            f3 f = bsdf_eval(S.mats, x.mat, x.n, wo, wi);
+            beta = mul(beta, f) * cw / dens;
+            x = nx;
+            wo = -wi;
+        }
+        return L;
+    }
+
+    // direct_path_tracer (pathtracer.cpp:508-526).
+    X3D_DI f3 li_direct(HitRec const &first, bool valid, f3 ray_v, Pcg &g) {
+        f3 L = mk(0.0f, 0.0f, 0.0f);
+        if (!valid)
+            return L;
+        f3 wo = -ray_v;
+        if (S.num_lights > 0) {
+            LightSample ls = sample_light(S, g);
+            f3 d, wi;
+            float r2, cl;
+            if (direct(ls, first, wo, d, wi, r2, cl))
+                L = d;
+        }
+        if (first.light >= 0 && dot(first.n, wo) > 0.0f)
+            L = L + light_rad(S, first.light) * dot(first.n, wo);
+        return L;
+    }
+};
+
+} // namespace x3d

@@ -1,0 +1,993 @@
+// HIP kernels of the path-tracing hot path and the C ABI of libx3dpt.so (include/x3dpt.h).
+//
+// Plain form (the drop-in): a first-hit pass caches each pixel's deterministic camera hit
+// (pathtracer.cpp:456-472, reused by every sample as in renderer.cpp:121-122), then one
+// persistent, grid-strided kernel runs one lane per pixel and loops over that pixel's samples,
+// accumulating them in order in registers; a pixel's film value is therefore independent of the
+// launch shape and identical to the CPU oracle's.
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/x3dpt.h"
+#include "integrators.h"
+#include "scene.h"
+
+namespace x3d {
+
+constexpr int kWG = 256;       // lanes per workgroup (4 waves)
+constexpr int kLdsDepth = 16;  // LDS stack entries per lane (16 x 8 B x 256 = 32 KiB per WG)
+
+struct Counters { // device-side totals (uint64)
+    unsigned long long closest, anyhit, pops, tests, spops, stests;
+};
+
+struct CamParams {
+    float sensor, focal, aspect;
+    float m[16];
+};
+
+// pinhole_camera::sample (camera.cpp:38-48) after pinhole_camera::transform (camera.cpp:63-71).
+X3D_DI void camera_ray(CamParams const &c, uint32_t x, uint32_t y, uint32_t w, uint32_t h, f3 &o, f3 &v) {
+    float s = c.sensor;
+    float dx = (static_cast<float>(x) / static_cast<float>(w - 1) * s - s / 2.0f) * c.aspect;
+    float dy = (static_cast<float>(h - 1) - static_cast<float>(y)) / static_cast<float>(h - 1) * s - s / 2.0f;
+    float dz = -c.focal;
+    float r[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float a = 0.0f;
+        a = a + c.m[0 * 4 + i] * dx;
+        a = a + c.m[1 * 4 + i] * dy;
+        a = a + c.m[2 * 4 + i] * dz;
+        a = a + 0.0f * 0.0f;
+        r[i] = a;
+    }
+    o = mk(c.m[12], c.m[13], c.m[14]);
+    v = normalize(mk(r[0], r[1], r[2]));
+}
+
+// Wave-level sum then one atomic per wave.
+X3D_DI void flush_counter(unsigned long long *dst, uint32_t v) {
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1)
+        x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0 && x != 0)
+        atomicAdd(dst, x);
+}
+
+template <int WG> X3D_DI TravStack<WG, kLdsDepth> make_stack(uint2 *lds, uint2 *spill, uint32_t spill_per_thread) {
+    TravStack<WG, kLdsDepth> st;
+    st.lds = lds + threadIdx.x;
+    uint32_t gtid = blockIdx.x * WG + threadIdx.x;
+    st.spill = spill + static_cast<size_t>(gtid) * spill_per_thread;
+    st.sp = 0;
+    return st;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ray queries.
+// ---------------------------------------------------------------------------------------------
+struct QueryParams {
+    DevScene S;
+    const float *rays;
+    const float *tmin, *tmax;
+    uint32_t n;
+    int32_t ref_order;
+    x3d_hit *hits;
+    uint8_t *occ;
+    uint2 *spill;
+    uint32_t spill_per_thread;
+    Counters *cnt;
+};
+
+__global__ __launch_bounds__(kWG) void k_intersect(QueryParams P) {
+    __shared__ uint2 lds[kLdsDepth * kWG];
+    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
+    uint32_t pops = 0, tests = 0;
+    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < P.n; k += gridDim.x * kWG) {
+        const float *r = P.rays + 6ull * k;
+        f3 o = mk(r[0], r[1], r[2]), v = mk(r[3], r[4], r[5]);
+        ClosestHit h = P.ref_order ? closest_hit<kWG, kLdsDepth, true>(P.S, o, v, st, pops, tests)
+                                   : closest_hit<kWG, kLdsDepth, false>(P.S, o, v, st, pops, tests);
+        x3d_hit out;
+        if (h.prim >= 0) {
+            HitRec rec = finish_hit(P.S, h);
+            const float4 *tris = static_cast<const float4 *>(P.S.tris);
+            out.geo = rec.geo;
+            out.tri = static_cast<int32_t>(__float_as_uint(tris[3 * h.prim + 1].w));
+            out.prim = h.prim;
+            out.t = h.t;
+            out.b1 = h.b1;
+            out.b2 = h.b2;
+            out.vertex[0] = rec.p.x;
+            out.vertex[1] = rec.p.y;
+            out.vertex[2] = rec.p.z;
+            out.normal[0] = rec.n.x;
+            out.normal[1] = rec.n.y;
+            out.normal[2] = rec.n.z;
+        } else {
+            out.geo = out.tri = out.prim = -1;
+            out.t = __builtin_inff();
+            out.b1 = out.b2 = 0.0f;
+            out.vertex[0] = out.vertex[1] = out.vertex[2] = 0.0f;
+            out.normal[0] = out.normal[1] = out.normal[2] = 0.0f;
+        }
+        P.hits[k] = out;
+    }
+    if (P.cnt) {
+        flush_counter(&P.cnt->pops, pops);
+        flush_counter(&P.cnt->tests, tests);
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_occluded(QueryParams P) {
+    __shared__ uint2 lds[kLdsDepth * kWG];
+    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
+    uint32_t pops = 0, tests = 0;
+    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < P.n; k += gridDim.x * kWG) {
+        const float *r = P.rays + 6ull * k;
+        f3 o = mk(r[0], r[1], r[2]), v = mk(r[3], r[4], r[5]);
+        P.occ[k] = any_hit<kWG, kLdsDepth>(P.S, o, v, P.tmin[k], P.tmax[k], st, pops, tests) ? 1 : 0;
+    }
+    if (P.cnt) {
+        flush_counter(&P.cnt->spops, pops);
+        flush_counter(&P.cnt->stests, tests);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// BSDF batches, camera rays, PCG draws.
+// ---------------------------------------------------------------------------------------------
+struct BsdfParams {
+    const DevMaterial *mats;
+    int32_t mat;
+    uint32_t n;
+    const float *nrm, *o, *i;
+    float *out_f, *out_i, *out_pdf;
+    uint64_t seed;
+    int32_t ref_mixture;
+    int32_t op; // 0 eval, 1 pdf, 2 sample
+};
+
+__global__ void k_bsdf(BsdfParams P) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n)
+        return;
+    f3 n = mk(P.nrm[3 * k], P.nrm[3 * k + 1], P.nrm[3 * k + 2]);
+    f3 o = mk(P.o[3 * k], P.o[3 * k + 1], P.o[3 * k + 2]);
+    if (P.op == 0) {
+        f3 i = mk(P.i[3 * k], P.i[3 * k + 1], P.i[3 * k + 2]);
+        f3 f = bsdf_eval(P.mats, P.mat, n, o, i);
+        P.out_f[3 * k] = f.x;
+        P.out_f[3 * k + 1] = f.y;
+        P.out_f[3 * k + 2] = f.z;
+    } else if (P.op == 1) {
+        f3 i = mk(P.i[3 * k], P.i[3 * k + 1], P.i[3 * k + 2]);
+        P.out_pdf[k] = bsdf_pdf(P.mats, P.mat, n, o, i);
+    } else {
+        Pcg g = pcg_for(P.seed, k, 0);
+        float d;
+        f3 i = bsdf_sample(P.mats, P.mat, n, o, g, &d, P.ref_mixture != 0);
+        P.out_i[3 * k] = i.x;
+        P.out_i[3 * k + 1] = i.y;
+        P.out_i[3 * k + 2] = i.z;
+        P.out_pdf[k] = d;
+    }
+}
+
+__global__ void k_camera_rays(CamParams c, uint32_t w, uint32_t h, float *rays) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= w * h)
+        return;
+    f3 o, v;
+    camera_ray(c, k % w, k / w, w, h, o, v);
+    float *r = rays + 6ull * k;
+    r[0] = o.x;
+    r[1] = o.y;
+    r[2] = o.z;
+    r[3] = v.x;
+    r[4] = v.y;
+    r[5] = v.z;
+}
+
+__global__ void k_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    Pcg g = pcg_for(seed, pixel, sample);
+    for (uint32_t k = 0; k < n; k++)
+        out[k] = g.next();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Rendering.
+// ---------------------------------------------------------------------------------------------
+struct RenderParams {
+    DevScene S;
+    CamParams cam;
+    uint32_t width, height;
+    uint32_t s_begin, s_end, spp_total;
+    uint64_t seed;
+    uint32_t max_depth, max_path_len;
+    float4 *first; // 3 x float4 per pixel: (p, t), (n, geo), (mat, light, valid, -)
+    float4 *acc;   // running rgb sum per pixel
+    float4 *out;   // final rgba per pixel (written when finalize != 0)
+    uint2 *spill;
+    uint32_t spill_per_thread;
+    int32_t finalize;
+    Counters *cnt;
+};
+
+__global__ __launch_bounds__(kWG) void k_first_hit(RenderParams P) {
+    __shared__ uint2 lds[kLdsDepth * kWG];
+    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
+    uint32_t pops = 0, tests = 0, nq = 0;
+    uint32_t npix = P.width * P.height;
+    for (uint32_t pix = blockIdx.x * kWG + threadIdx.x; pix < npix; pix += gridDim.x * kWG) {
+        f3 o, v;
+        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        nq++;
+        ClosestHit h = closest_hit<kWG, kLdsDepth, false>(P.S, o, v, st, pops, tests);
+        float4 a = make_float4(0, 0, 0, 0), b = make_float4(0, 0, 0, 0), c = make_float4(0, 0, 0, 0);
+        if (h.prim >= 0) {
+            HitRec r = finish_hit(P.S, h);
+            // compute_first_hit drops hits seen from behind (pathtracer.cpp:463-464).
+            if (!(dot(r.n, -v) <= 0.0f)) {
+                a = make_float4(r.p.x, r.p.y, r.p.z, r.t);
+                b = make_float4(r.n.x, r.n.y, r.n.z, __uint_as_float(static_cast<uint32_t>(r.geo)));
+                c = make_float4(__uint_as_float(static_cast<uint32_t>(r.mat)),
+                                __uint_as_float(static_cast<uint32_t>(r.light)), 1.0f, 0.0f);
+            }
+        }
+        P.first[3 * pix] = a;
+        P.first[3 * pix + 1] = b;
+        P.first[3 * pix + 2] = c;
+    }
+    if (P.cnt) {
+        flush_counter(&P.cnt->closest, nq);
+        flush_counter(&P.cnt->pops, pops);
+        flush_counter(&P.cnt->tests, tests);
+    }
+}
+
+template <int TRACER> __global__ __launch_bounds__(kWG) void k_render_plain(RenderParams P) {
+    __shared__ uint2 lds[kLdsDepth * kWG];
+    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
+    Tracer<kWG, kLdsDepth> tr{P.S, st};
+    uint32_t npix = P.width * P.height;
+    float scale = 1.0f / static_cast<float>(P.spp_total);
+    for (uint32_t pix = blockIdx.x * kWG + threadIdx.x; pix < npix; pix += gridDim.x * kWG) {
+        f3 o, v;
+        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        float4 a = P.first[3 * pix], b = P.first[3 * pix + 1], c = P.first[3 * pix + 2];
+        HitRec first;
+        first.p = mk(a.x, a.y, a.z);
+        first.t = a.w;
+        first.n = mk(b.x, b.y, b.z);
+        first.geo = static_cast<int32_t>(__float_as_uint(b.w));
+        first.mat = static_cast<int32_t>(__float_as_uint(c.x));
+        first.light = static_cast<int32_t>(__float_as_uint(c.y));
+        bool valid = c.z != 0.0f;
+        f3 acc = mk(0.0f, 0.0f, 0.0f);
+        if (P.s_begin > 0) {
+            float4 q = P.acc[pix];
+            acc = mk(q.x, q.y, q.z);
+        }
+        for (uint32_t s = P.s_begin; s < P.s_end; s++) {
+            Pcg g = pcg_for(P.seed, pix, s);
+            f3 L;
+            if (TRACER == X3D_PT_UNIDIRECT_MIS)
+                L = tr.li_mis(first, valid, v, g, P.max_depth);
+            else if (TRACER == X3D_PT_UNIDIRECT_LT1)
+                L = tr.li_lt1(first, valid, v, g);
+            else if (TRACER == X3D_PT_UNIDIRECT)
+                L = tr.li_unidirect(first, valid, v, g);
+            else if (TRACER == X3D_PT_DIRECT)
+                L = tr.li_direct(first, valid, v, g);
+            else // X3D_PT_NORMAL (normal_tracer, pathtracer.cpp:492-506)
+                L = valid ? (first.n + mk(1.0f, 1.0f, 1.0f)) / 2.0f : mk(0.0f, 0.0f, 0.0f);
+            acc = acc + L;
+        }
+        if (P.finalize)
+            P.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
+        else
+            P.acc[pix] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+    }
+    if (P.cnt) {
+        flush_counter(&P.cnt->closest, tr.nclosest);
+        flush_counter(&P.cnt->anyhit, tr.nany);
+        flush_counter(&P.cnt->pops, tr.pops);
+        flush_counter(&P.cnt->tests, tr.tests);
+        flush_counter(&P.cnt->spops, tr.spops);
+        flush_counter(&P.cnt->stests, tr.stests);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------------------------
+thread_local std::string g_last_error;
+
+int fail(int code, std::string const &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define X3D_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(X3D_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+struct DeviceInfo {
+    int ok = -1;
+    int cus = 0;
+    std::string err;
+};
+
+DeviceInfo &device_info() {
+    static DeviceInfo info;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n == 0) {
+            info.ok = 0;
+            info.err = "no HIP device available (libx3dpt requires an AMD GPU)";
+            return;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+            info.ok = 0;
+            info.err = "hipGetDeviceProperties failed";
+            return;
+        }
+        info.cus = prop.multiProcessorCount;
+        info.ok = 1;
+    });
+    return info;
+}
+
+int require_device() {
+    DeviceInfo &d = device_info();
+    if (d.ok != 1)
+        return fail(X3D_ERR_DEVICE, d.err);
+    return X3D_OK;
+}
+
+// Persistent grid: enough workgroups to fill every CU at the kernel's occupancy.
+template <class K> uint32_t persistent_grid(K kernel, uint64_t items) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWG, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    uint64_t want = (items + kWG - 1) / kWG;
+    uint64_t cap = static_cast<uint64_t>(device_info().cus) * static_cast<uint64_t>(per_cu);
+    uint64_t g = want < cap ? want : cap;
+    return static_cast<uint32_t>(g < 1 ? 1 : g);
+}
+
+// Growable device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p)
+            (void)hipFree(p);
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= bytes)
+            return hipSuccess;
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess)
+            bytes = n;
+        return e;
+    }
+};
+
+uint32_t spill_entries(DevScene const &S) {
+    return S.max_stack > static_cast<uint32_t>(kLdsDepth) ? S.max_stack - kLdsDepth + 1 : 1;
+}
+
+} // namespace x3d
+
+struct x3d_scene {
+    x3d::Scene s;
+};
+
+struct x3d_renderer {
+    x3d_render_options opt;
+    x3d::DevBuf first, acc, out, spill, counters;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~x3d_renderer() {
+        if (ev0)
+            (void)hipEventDestroy(ev0);
+        if (ev1)
+            (void)hipEventDestroy(ev1);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+using namespace x3d;
+
+namespace {
+
+bool finite3(const float *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
+
+CamParams cam_params(x3d_camera const &c) {
+    CamParams p;
+    p.sensor = c.sensor_size;
+    p.focal = c.focal_length;
+    p.aspect = c.aspect;
+    std::memcpy(p.m, c.transform, sizeof p.m);
+    return p;
+}
+
+int check_scene(const x3d_scene *scene) {
+    if (!scene)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene is null");
+    if (!scene->s.committed)
+        return fail(X3D_ERR_STATE, "scene is not committed");
+    return X3D_OK;
+}
+
+int run_queries(const x3d_scene *scene, const float *d_rays, const float *d_tmin, const float *d_tmax,
+                uint32_t n, int32_t traversal, x3d_hit *d_hits, uint8_t *d_occ, hipStream_t stream,
+                x3d::DevBuf &spill) {
+    QueryParams P{};
+    P.S = scene->s.dev;
+    P.rays = d_rays;
+    P.tmin = d_tmin;
+    P.tmax = d_tmax;
+    P.n = n;
+    P.ref_order = traversal == X3D_TRAVERSAL_REFERENCE;
+    P.hits = d_hits;
+    P.occ = d_occ;
+    P.spill_per_thread = spill_entries(P.S);
+    uint32_t grid = d_hits ? persistent_grid(k_intersect, n) : persistent_grid(k_occluded, n);
+    X3D_HIP(spill.ensure(static_cast<size_t>(grid) * kWG * P.spill_per_thread * sizeof(uint2)));
+    P.spill = static_cast<uint2 *>(spill.p);
+    if (d_hits)
+        hipLaunchKernelGGL(k_intersect, dim3(grid), dim3(kWG), 0, stream, P);
+    else
+        hipLaunchKernelGGL(k_occluded, dim3(grid), dim3(kWG), 0, stream, P);
+    X3D_HIP(hipGetLastError());
+    return X3D_OK;
+}
+
+} // namespace
+
+namespace {
+
+template <int T> void launch_plain(uint32_t grid, hipStream_t s, RenderParams const &P) {
+    hipLaunchKernelGGL(k_render_plain<T>, dim3(grid), dim3(kWG), 0, s, P);
+}
+
+int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                uint32_t height, uint32_t spp, float *d_out, x3d_render_stats *stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    RenderParams P{};
+    P.S = scene->s.dev;
+    P.cam = cam_params(*camera);
+    P.width = width;
+    P.height = height;
+    P.spp_total = spp;
+    P.seed = r->opt.seed;
+    P.max_depth = r->opt.max_depth;
+    P.max_path_len = r->opt.max_path_len;
+    P.spill_per_thread = spill_entries(P.S);
+    uint64_t npix = static_cast<uint64_t>(width) * height;
+    X3D_HIP(r->first.ensure(npix * 3 * sizeof(float4)));
+    X3D_HIP(r->acc.ensure(npix * sizeof(float4)));
+    X3D_HIP(r->counters.ensure(sizeof(Counters)));
+    P.first = static_cast<float4 *>(r->first.p);
+    P.acc = static_cast<float4 *>(r->acc.p);
+    P.out = reinterpret_cast<float4 *>(d_out);
+    P.cnt = static_cast<Counters *>(r->counters.p);
+    int tracer = r->opt.pt_type;
+    uint32_t grid_first = persistent_grid(k_first_hit, npix);
+    uint32_t grid_render = 1;
+    switch (tracer) {
+    case X3D_PT_UNIDIRECT_MIS: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT_MIS>, npix); break;
+    case X3D_PT_UNIDIRECT_LT1: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT_LT1>, npix); break;
+    case X3D_PT_UNIDIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT>, npix); break;
+    case X3D_PT_DIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_DIRECT>, npix); break;
+    default: grid_render = persistent_grid(k_render_plain<X3D_PT_NORMAL>, npix); break;
+    }
+    uint32_t grid = grid_first > grid_render ? grid_first : grid_render;
+    X3D_HIP(r->spill.ensure(static_cast<size_t>(grid) * kWG * P.spill_per_thread * sizeof(uint2)));
+    P.spill = static_cast<uint2 *>(r->spill.p);
+    X3D_HIP(hipMemsetAsync(P.cnt, 0, sizeof(Counters), r->stream));
+    X3D_HIP(hipEventRecord(r->ev0, r->stream));
+    hipLaunchKernelGGL(k_first_hit, dim3(grid_first), dim3(kWG), 0, r->stream, P);
+    X3D_HIP(hipGetLastError());
+    P.s_begin = 0;
+    P.s_end = spp;
+    P.finalize = 1;
+    switch (tracer) {
+    case X3D_PT_UNIDIRECT_MIS: launch_plain<X3D_PT_UNIDIRECT_MIS>(grid_render, r->stream, P); break;
+    case X3D_PT_UNIDIRECT_LT1: launch_plain<X3D_PT_UNIDIRECT_LT1>(grid_render, r->stream, P); break;
+    case X3D_PT_UNIDIRECT: launch_plain<X3D_PT_UNIDIRECT>(grid_render, r->stream, P); break;
+    case X3D_PT_DIRECT: launch_plain<X3D_PT_DIRECT>(grid_render, r->stream, P); break;
+    default: launch_plain<X3D_PT_NORMAL>(grid_render, r->stream, P); break;
+    }
+    X3D_HIP(hipGetLastError());
+    X3D_HIP(hipEventRecord(r->ev1, r->stream));
+    X3D_HIP(hipEventSynchronize(r->ev1));
+    if (stats) {
+        Counters c;
+        X3D_HIP(hipMemcpy(&c, P.cnt, sizeof c, hipMemcpyDeviceToHost));
+        float ms = 0;
+        X3D_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+        stats->num_samples = npix * spp;
+        stats->closest_queries = c.closest;
+        stats->anyhit_queries = c.anyhit;
+        stats->kernel_ms = ms;
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return X3D_OK;
+}
+
+int check_render_args(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                      uint32_t height, uint32_t spp) {
+    if (!r || !camera)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_scene(scene);
+    if (rc)
+        return rc;
+    if (width < 2 || height < 2)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "film must be at least 2 x 2 (camera.cpp:38-48 divides by w-1, h-1)");
+    if (static_cast<uint64_t>(width) * height > (1ull << 31))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "film too large");
+    if (spp == 0)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "spp must be > 0");
+    return X3D_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *x3d_last_error(void) { return g_last_error.c_str(); }
+
+const char *x3d_version(void) { return "x3dpt 0.1 (gfx950)"; }
+
+int x3d_scene_create(x3d_scene **out) {
+    if (!out)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "out is null");
+    *out = new (std::nothrow) x3d_scene();
+    return *out ? X3D_OK : fail(X3D_ERR_OUT_OF_MEMORY, "allocation failed");
+}
+
+int x3d_scene_destroy(x3d_scene *scene) {
+    delete scene;
+    return X3D_OK;
+}
+
+int x3d_scene_add_material(x3d_scene *scene, const x3d_material_desc *d, int32_t *out_id) {
+    if (!scene || !d)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene or desc is null");
+    if (scene->s.committed)
+        return fail(X3D_ERR_STATE, "scene already committed");
+    DevMaterial m{};
+    m.kind = d->kind;
+    for (int c = 0; c < 3; c++)
+        m.albedo[c] = d->albedo[c];
+    if (!finite3(d->albedo))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "albedo must be finite");
+    if (d->kind == X3D_MAT_OREN_NAYAR) {
+        // oren_nayar constructor (material.cpp:108-119).
+        float sigma2 = d->sigma * d->sigma;
+        m.on_a = 1 - 0.5f * sigma2 / (sigma2 + 0.33f);
+        m.on_b = 0.45f * sigma2 / (sigma2 + 0.09f);
+        m.sigma = d->sigma;
+    } else if (d->kind == X3D_MAT_COOK_TORR) {
+        if (!(d->roughness > 0.0f))
+            return fail(X3D_ERR_INVALID_ARGUMENT, "cook_torr roughness must be > 0");
+        m.alpha2 = 2 * d->roughness * d->roughness; // material.cpp:379
+        m.roughness = d->roughness;
+        m.ior_re = d->ior_re;
+        m.ior_im = d->ior_im;
+    } else if (d->kind == X3D_MAT_MIXTURE) {
+        int32_t n = static_cast<int32_t>(scene->s.materials.size());
+        if (d->sub0 < 0 || d->sub0 >= n || d->sub1 < 0 || d->sub1 >= n)
+            return fail(X3D_ERR_INVALID_ARGUMENT, "mixture operand out of range");
+        if (scene->s.materials[d->sub0].kind == kMatMixture || scene->s.materials[d->sub1].kind == kMatMixture)
+            return fail(X3D_ERR_UNSUPPORTED, "nested mixtures are not supported");
+        if (!(d->ratio >= 0.0f && d->ratio <= 1.0f))
+            return fail(X3D_ERR_INVALID_ARGUMENT, "mixture ratio must be in [0, 1]");
+        m.sub0 = d->sub0;
+        m.sub1 = d->sub1;
+        m.ratio = d->ratio;
+    } else {
+        return fail(X3D_ERR_INVALID_ARGUMENT, "unknown material kind");
+    }
+    scene->s.materials.push_back(m);
+    if (out_id)
+        *out_id = static_cast<int32_t>(scene->s.materials.size()) - 1;
+    return X3D_OK;
+}
+
+int x3d_scene_add_mesh(x3d_scene *scene, const float *verts, const float *normals, const float *uvs,
+                       uint32_t num_verts, const uint32_t *tris, uint32_t num_tris, int32_t material_id,
+                       int32_t *out_mesh_id) {
+    (void)uvs;
+    if (!scene || (!verts && num_verts) || (!normals && num_verts) || (!tris && num_tris))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null mesh array");
+    if (scene->s.committed)
+        return fail(X3D_ERR_STATE, "scene already committed");
+    if (material_id >= static_cast<int32_t>(scene->s.materials.size()))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "material id out of range");
+    HostMesh m;
+    m.verts.resize(num_verts);
+    m.normals.resize(num_verts);
+    for (uint32_t i = 0; i < num_verts; i++) {
+        if (!finite3(verts + 3 * i) || !finite3(normals + 3 * i))
+            return fail(X3D_ERR_INVALID_ARGUMENT, "vertex " + std::to_string(i) + " is not finite");
+        // The reference loads meshes through an identity transform (pathspace.cpp:15-19,
+        // geometry.cpp:76-87), which maps -0 to +0; adding +0 does the same.
+        m.verts[i] = {verts[3 * i] + 0.0f, verts[3 * i + 1] + 0.0f, verts[3 * i + 2] + 0.0f};
+        m.normals[i] = {normals[3 * i] + 0.0f, normals[3 * i + 1] + 0.0f, normals[3 * i + 2] + 0.0f};
+    }
+    m.tris.assign(tris, tris + 3ull * num_tris);
+    for (uint32_t v : m.tris)
+        if (v >= num_verts)
+            return fail(X3D_ERR_INVALID_ARGUMENT, "triangle index out of range");
+    m.material = material_id < 0 ? -1 : material_id;
+    scene->s.meshes.push_back(std::move(m));
+    if (out_mesh_id)
+        *out_mesh_id = static_cast<int32_t>(scene->s.meshes.size()) - 1;
+    return X3D_OK;
+}
+
+int x3d_scene_add_area_light(x3d_scene *scene, int32_t mesh_id, const float radiance[3], int32_t *out_light_id) {
+    if (!scene || !radiance)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    if (scene->s.committed)
+        return fail(X3D_ERR_STATE, "scene already committed");
+    if (mesh_id < 0 || mesh_id >= static_cast<int32_t>(scene->s.meshes.size()))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "mesh id out of range");
+    if (!finite3(radiance))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "radiance must be finite");
+    HostLight l;
+    l.mesh = mesh_id;
+    l.rad[0] = radiance[0];
+    l.rad[1] = radiance[1];
+    l.rad[2] = radiance[2];
+    scene->s.lights.push_back(l);
+    if (out_light_id)
+        *out_light_id = static_cast<int32_t>(scene->s.lights.size()) - 1;
+    return X3D_OK;
+}
+
+int x3d_scene_commit(x3d_scene *scene) {
+    if (!scene)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene is null");
+    int rc = require_device();
+    if (rc)
+        return rc;
+    std::string e = scene->s.commit();
+    if (!e.empty())
+        return fail(e.find("hip") != std::string::npos ? X3D_ERR_DEVICE : X3D_ERR_INVALID_ARGUMENT, e);
+    return X3D_OK;
+}
+
+int x3d_scene_get_info(const x3d_scene *scene, x3d_scene_info *info) {
+    if (!scene || !info)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    Scene const &s = scene->s;
+    info->num_meshes = static_cast<uint32_t>(s.meshes.size());
+    info->num_tris = static_cast<uint32_t>(s.prims.size());
+    info->num_nodes = static_cast<uint32_t>(s.flat.size());
+    info->num_interior = s.num_interior;
+    info->num_lights = static_cast<uint32_t>(s.lights.size());
+    info->num_materials = static_cast<uint32_t>(s.materials.size());
+    info->max_leaf_depth = s.max_leaf_depth;
+    info->device_bytes = s.buf.bytes;
+    info->build_seconds = s.build_seconds;
+    info->upload_seconds = s.upload_seconds;
+    return X3D_OK;
+}
+
+int x3d_scene_dump_bvh(const x3d_scene *scene, float *bounds, uint32_t *meta, int32_t *prims) {
+    int rc = check_scene(scene);
+    if (rc)
+        return rc;
+    Scene const &s = scene->s;
+    for (size_t n = 0; n < s.flat.size(); n++) {
+        FlatNode const &f = s.flat[n];
+        if (bounds)
+            for (int i = 0; i < 3; i++) {
+                bounds[6 * n + i] = f.mn[i];
+                bounds[6 * n + 3 + i] = f.mx[i];
+            }
+        if (meta) {
+            meta[4 * n] = f.num_prims;
+            meta[4 * n + 1] = f.split_axis;
+            meta[4 * n + 2] = f.next_child;
+            meta[4 * n + 3] = f.prim_start;
+        }
+    }
+    if (prims)
+        for (size_t i = 0; i < s.prims.size(); i++) {
+            prims[2 * i] = static_cast<int32_t>(s.prims[i].mesh);
+            prims[2 * i + 1] = static_cast<int32_t>(s.prims[i].tri);
+        }
+    return X3D_OK;
+}
+
+int x3d_intersect_device(const x3d_scene *scene, const float *d_rays, uint32_t n, int32_t traversal,
+                         x3d_hit *d_hits, void *stream) {
+    int rc = check_scene(scene);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return X3D_OK;
+    if (!d_rays || !d_hits)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null device pointer");
+    static thread_local x3d::DevBuf spill;
+    return run_queries(scene, d_rays, nullptr, nullptr, n, traversal, d_hits, nullptr,
+                       static_cast<hipStream_t>(stream), spill);
+}
+
+int x3d_occluded_device(const x3d_scene *scene, const float *d_rays, const float *d_tmin, const float *d_tmax,
+                        uint32_t n, uint8_t *d_occ, void *stream) {
+    int rc = check_scene(scene);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return X3D_OK;
+    if (!d_rays || !d_tmin || !d_tmax || !d_occ)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null device pointer");
+    static thread_local x3d::DevBuf spill;
+    return run_queries(scene, d_rays, d_tmin, d_tmax, n, X3D_TRAVERSAL_CULLED, nullptr, d_occ,
+                       static_cast<hipStream_t>(stream), spill);
+}
+
+int x3d_intersect(const x3d_scene *scene, const float *rays, uint32_t n, int32_t traversal, x3d_hit *hits) {
+    int rc = check_scene(scene);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return X3D_OK;
+    if (!rays || !hits)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null host pointer");
+    if (traversal != X3D_TRAVERSAL_CULLED && traversal != X3D_TRAVERSAL_REFERENCE)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "unknown traversal mode");
+    x3d::DevBuf dr, dh, spill;
+    X3D_HIP(dr.ensure(sizeof(float) * 6ull * n));
+    X3D_HIP(dh.ensure(sizeof(x3d_hit) * static_cast<size_t>(n)));
+    X3D_HIP(hipMemcpy(dr.p, rays, sizeof(float) * 6ull * n, hipMemcpyHostToDevice));
+    rc = run_queries(scene, static_cast<float *>(dr.p), nullptr, nullptr, n, traversal,
+                     static_cast<x3d_hit *>(dh.p), nullptr, nullptr, spill);
+    if (rc)
+        return rc;
+    X3D_HIP(hipMemcpy(hits, dh.p, sizeof(x3d_hit) * static_cast<size_t>(n), hipMemcpyDeviceToHost));
+    return X3D_OK;
+}
+
+int x3d_occluded(const x3d_scene *scene, const float *rays, const float *tmin, const float *tmax, uint32_t n,
+                 uint8_t *occ) {
+    int rc = check_scene(scene);
+    if (rc)
+        return rc;
+    if (n == 0)
+        return X3D_OK;
+    if (!rays || !tmin || !tmax || !occ)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null host pointer");
+    x3d::DevBuf dr, d0, d1, dout, spill;
+    X3D_HIP(dr.ensure(sizeof(float) * 6ull * n));
+    X3D_HIP(d0.ensure(sizeof(float) * n));
+    X3D_HIP(d1.ensure(sizeof(float) * n));
+    X3D_HIP(dout.ensure(n));
+    X3D_HIP(hipMemcpy(dr.p, rays, sizeof(float) * 6ull * n, hipMemcpyHostToDevice));
+    X3D_HIP(hipMemcpy(d0.p, tmin, sizeof(float) * n, hipMemcpyHostToDevice));
+    X3D_HIP(hipMemcpy(d1.p, tmax, sizeof(float) * n, hipMemcpyHostToDevice));
+    rc = run_queries(scene, static_cast<float *>(dr.p), static_cast<float *>(d0.p), static_cast<float *>(d1.p), n,
+                     X3D_TRAVERSAL_CULLED, nullptr, static_cast<uint8_t *>(dout.p), nullptr, spill);
+    if (rc)
+        return rc;
+    X3D_HIP(hipMemcpy(occ, dout.p, n, hipMemcpyDeviceToHost));
+    return X3D_OK;
+}
+
+static int bsdf_batch(const x3d_scene *scene, int32_t mat, uint32_t n, const float *nrm, const float *o,
+                      const float *i, int op, uint64_t seed, int32_t ref_mixture, float *out_f, float *out_i,
+                      float *out_pdf) {
+    int rc = check_scene(scene);
+    if (rc)
+        return rc;
+    if (mat < -1 || mat >= static_cast<int32_t>(scene->s.materials.size()) - 1)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "material id out of range");
+    if (n == 0)
+        return X3D_OK;
+    if (mat < 0)
+        mat = static_cast<int32_t>(scene->s.dev.fail_safe_mat);
+    x3d::DevBuf dn, dd, di, df, dI, dp;
+    size_t v3 = sizeof(float) * 3ull * n;
+    X3D_HIP(dn.ensure(v3));
+    X3D_HIP(dd.ensure(v3));
+    X3D_HIP(hipMemcpy(dn.p, nrm, v3, hipMemcpyHostToDevice));
+    X3D_HIP(hipMemcpy(dd.p, o, v3, hipMemcpyHostToDevice));
+    if (op != 2) {
+        X3D_HIP(di.ensure(v3));
+        X3D_HIP(hipMemcpy(di.p, i, v3, hipMemcpyHostToDevice));
+    }
+    X3D_HIP(df.ensure(v3));
+    X3D_HIP(dI.ensure(v3));
+    X3D_HIP(dp.ensure(sizeof(float) * n));
+    BsdfParams P{};
+    P.mats = scene->s.dev.mats;
+    P.mat = mat;
+    P.n = n;
+    P.nrm = static_cast<float *>(dn.p);
+    P.o = static_cast<float *>(dd.p);
+    P.i = static_cast<float *>(di.p);
+    P.out_f = static_cast<float *>(df.p);
+    P.out_i = static_cast<float *>(dI.p);
+    P.out_pdf = static_cast<float *>(dp.p);
+    P.seed = seed;
+    P.ref_mixture = ref_mixture;
+    P.op = op;
+    hipLaunchKernelGGL(k_bsdf, dim3((n + 255) / 256), dim3(256), 0, nullptr, P);
+    X3D_HIP(hipGetLastError());
+    if (op == 0)
+        X3D_HIP(hipMemcpy(out_f, df.p, v3, hipMemcpyDeviceToHost));
+    if (op == 1)
+        X3D_HIP(hipMemcpy(out_pdf, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (op == 2) {
+        X3D_HIP(hipMemcpy(out_i, dI.p, v3, hipMemcpyDeviceToHost));
+        X3D_HIP(hipMemcpy(out_pdf, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    }
+    return X3D_OK;
+}
+
+int x3d_bsdf_eval(const x3d_scene *scene, int32_t material_id, uint32_t n, const float *normals, const float *o,
+                  const float *i, float *f) {
+    if (n && (!normals || !o || !i || !f))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null host pointer");
+    return bsdf_batch(scene, material_id, n, normals, o, i, 0, 0, 0, f, nullptr, nullptr);
+}
+
+int x3d_bsdf_pdf(const x3d_scene *scene, int32_t material_id, uint32_t n, const float *normals, const float *o,
+                 const float *i, float *pdf) {
+    if (n && (!normals || !o || !i || !pdf))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null host pointer");
+    return bsdf_batch(scene, material_id, n, normals, o, i, 1, 0, 0, nullptr, nullptr, pdf);
+}
+
+int x3d_bsdf_sample(const x3d_scene *scene, int32_t material_id, uint64_t seed, uint32_t n, const float *normals,
+                    const float *o, int32_t reference_mixture, float *i, float *dens) {
+    if (n && (!normals || !o || !i || !dens))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null host pointer");
+    return bsdf_batch(scene, material_id, n, normals, o, nullptr, 2, seed, reference_mixture, nullptr, i, dens);
+}
+
+int x3d_camera_rays(const x3d_camera *camera, uint32_t width, uint32_t height, float *rays) {
+    if (!camera || !rays)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    if (width < 2 || height < 2)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "film must be at least 2 x 2 (camera.cpp:38-48 divides by w-1, h-1)");
+    int rc = require_device();
+    if (rc)
+        return rc;
+    size_t bytes = sizeof(float) * 6ull * width * height;
+    x3d::DevBuf d;
+    X3D_HIP(d.ensure(bytes));
+    uint32_t n = width * height;
+    hipLaunchKernelGGL(k_camera_rays, dim3((n + 255) / 256), dim3(256), 0, nullptr, cam_params(*camera), width,
+                       height, static_cast<float *>(d.p));
+    X3D_HIP(hipGetLastError());
+    X3D_HIP(hipMemcpy(rays, d.p, bytes, hipMemcpyDeviceToHost));
+    return X3D_OK;
+}
+
+int x3d_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out) {
+    if (!out && n)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "out is null");
+    int rc = require_device();
+    if (rc)
+        return rc;
+    if (n == 0)
+        return X3D_OK;
+    x3d::DevBuf d;
+    X3D_HIP(d.ensure(sizeof(uint32_t) * n));
+    hipLaunchKernelGGL(k_pcg_draws, dim3(1), dim3(64), 0, nullptr, seed, pixel, sample, n,
+                       static_cast<uint32_t *>(d.p));
+    X3D_HIP(hipGetLastError());
+    X3D_HIP(hipMemcpy(out, d.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    return X3D_OK;
+}
+
+int x3d_render_options_default(int32_t pt_type, x3d_render_options *opts) {
+    if (!opts)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "opts is null");
+    std::memset(opts, 0, sizeof(*opts));
+    opts->pt_type = pt_type;
+    opts->form = X3D_FORM_PLAIN;
+    opts->max_depth = 5;
+    opts->max_path_len = 8; // bidirect_mis_path_tracer::m_max_path_len (pathtracer.h:201)
+    opts->seed = 1;
+    return X3D_OK;
+}
+
+int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
+    if (!opts || !out)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    switch (opts->pt_type) {
+    case X3D_PT_NORMAL:
+    case X3D_PT_DIRECT:
+    case X3D_PT_UNIDIRECT:
+    case X3D_PT_UNIDIRECT_LT1:
+    case X3D_PT_UNIDIRECT_MIS:
+        break;
+    default:
+        return fail(X3D_ERR_UNSUPPORTED, "tracer type " + std::to_string(opts->pt_type) + " is not implemented");
+    }
+    if (opts->form != X3D_FORM_PLAIN)
+        return fail(X3D_ERR_UNSUPPORTED, "render form " + std::to_string(opts->form) + " is not implemented");
+    int rc = require_device();
+    if (rc)
+        return rc;
+    auto r = std::unique_ptr<x3d_renderer>(new (std::nothrow) x3d_renderer());
+    if (!r)
+        return fail(X3D_ERR_OUT_OF_MEMORY, "allocation failed");
+    r->opt = *opts;
+    X3D_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    X3D_HIP(hipEventCreate(&r->ev0));
+    X3D_HIP(hipEventCreate(&r->ev1));
+    *out = r.release();
+    return X3D_OK;
+}
+
+int x3d_renderer_destroy(x3d_renderer *r) {
+    delete r;
+    return X3D_OK;
+}
+
+
+int x3d_render_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                      uint32_t height, uint32_t spp, float *d_rgba, x3d_render_stats *stats) {
+    int rc = check_render_args(r, scene, camera, width, height, spp);
+    if (rc)
+        return rc;
+    if (!d_rgba)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "d_rgba is null");
+    return render_impl(r, scene, camera, width, height, spp, d_rgba, stats);
+}
+
+int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width, uint32_t height,
+               uint32_t spp, int32_t firefly_filter, float *rgba, x3d_render_stats *stats) {
+    int rc = check_render_args(r, scene, camera, width, height, spp);
+    if (rc)
+        return rc;
+    if (firefly_filter)
+        return fail(X3D_ERR_UNSUPPORTED, "firefly filter is not implemented");
+    if (!rgba)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "rgba is null");
+    uint64_t npix = static_cast<uint64_t>(width) * height;
+    X3D_HIP(r->out.ensure(npix * sizeof(float4)));
+    auto t0 = std::chrono::steady_clock::now();
+    rc = render_impl(r, scene, camera, width, height, spp, static_cast<float *>(r->out.p), stats);
+    if (rc)
+        return rc;
+    X3D_HIP(hipMemcpy(rgba, r->out.p, npix * sizeof(float4), hipMemcpyDeviceToHost));
+    if (stats)
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return X3D_OK;
+}
+
+} // extern "C"

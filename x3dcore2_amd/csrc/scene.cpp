@@ -1,0 +1,427 @@
+// Host-side scene commit: SAH BVH build reproducing the reference layout, light tables and the
+// HBM packing the kernels read (scene_layout.h).
+//
+// BVH build, following bvh_path_space_layout::commit (corelib/src/pathspace.cpp:121-328):
+// 20 SAH buckets on the widest axis, cost 1 + 8 * sum(SA_part / SA * count), split when cheaper
+// than 8 * n or when n > 4, std::partition at the chosen bucket boundary, or std::sort on the
+// centroid and a median cut when depth > log2(N). The flattened pre-order array (left child at
+// n + 1, right child at next_child) and the primitive permutation come out identical to the
+// reference's, so primitive indices and hit ids can be compared directly. The construction is
+// written directly in pre-order, and the bucket costs are computed with one prefix and one
+// suffix sweep instead of the reference's quadratic loop; both produce the same floats.
+
+#include "scene.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace x3d {
+
+namespace {
+
+// aabb (corelib/src/tensor.h:733-813) with its default-constructed empty box (min = max = 0).
+struct Box {
+    float mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    bool empty = true;
+};
+inline float fmin_std(float a, float b) { return (b < a) ? b : a; } // std::min
+inline float fmax_std(float a, float b) { return (a < b) ? b : a; } // std::max
+inline Box make_box(const float *a, const float *b) {
+    Box r;
+    for (int i = 0; i < 3; i++) {
+        r.mn[i] = a[i];
+        r.mx[i] = b[i];
+    }
+    r.empty = a[0] > b[0] || a[1] > b[1] || a[2] > b[2];
+    return r;
+}
+// aabb::operator+(aabb): a non-empty box takes min/max with the other's stored corners even
+// when the other box is empty.
+inline Box join(Box const &a, Box const &b) {
+    if (a.empty)
+        return b;
+    float lo[3], hi[3];
+    for (int i = 0; i < 3; i++) {
+        lo[i] = fmin_std(a.mn[i], b.mn[i]);
+        hi[i] = fmax_std(a.mx[i], b.mx[i]);
+    }
+    return make_box(lo, hi);
+}
+inline Box join_point(Box const &a, HostVec3 p) {
+    float q[3] = {p.x, p.y, p.z};
+    if (a.empty)
+        return make_box(q, q);
+    float lo[3], hi[3];
+    for (int i = 0; i < 3; i++) {
+        lo[i] = fmin_std(a.mn[i], q[i]);
+        hi[i] = fmax_std(a.mx[i], q[i]);
+    }
+    return make_box(lo, hi);
+}
+inline float area_of(Box const &b) { // aabb::surf_area
+    if (b.empty)
+        return 0.0f;
+    float d0 = b.mx[0] - b.mn[0], d1 = b.mx[1] - b.mn[1], d2 = b.mx[2] - b.mn[2];
+    return 2.0f * (d0 * d1 + d0 * d2 + d1 * d2);
+}
+
+struct BuildPrim {
+    PrimRef ref;
+    Box bound;
+    float c[3]; // centroid (max + min) / 2
+};
+
+constexpr unsigned kBuckets = 20, kLeafMax = 4, kTriCost = 8, kBoxCost = 1;
+
+struct Builder {
+    std::vector<BuildPrim> &p;
+    std::vector<FlatNode> &out;
+    double log2n;
+    uint32_t max_depth = 0;
+
+    void emit_leaf(Box const &b, unsigned start, unsigned count, unsigned depth) {
+        FlatNode f;
+        std::memcpy(f.mn, b.mn, sizeof f.mn);
+        std::memcpy(f.mx, b.mx, sizeof f.mx);
+        f.num_prims = count;
+        f.split_axis = 0xFF;
+        f.next_child = 0xFFFFFFFFu;
+        f.prim_start = start;
+        out.push_back(f);
+        max_depth = std::max(max_depth, depth + 1);
+    }
+
+    // Costs of the 19 bucket boundaries (pathspace.cpp:147-164) with prefix / suffix sweeps.
+    // For the suffix, the reference's left fold over buckets i+1..19 equals the union of the
+    // non-empty buckets, plus the origin when an empty bucket follows the first non-empty one.
+    void bucket_costs(Box const *bk_box, unsigned const *bk_n, Box const &b, float *cost) {
+        float sab = area_of(b);
+        Box first;
+        unsigned cf = 0;
+        float first_area[kBuckets];
+        unsigned first_n[kBuckets];
+        for (unsigned i = 0; i < kBuckets - 1; i++) {
+            first = join(first, bk_box[i]);
+            cf += bk_n[i];
+            first_area[i] = area_of(first);
+            first_n[i] = cf;
+        }
+        Box uni;
+        bool any_nonempty = false, any_empty = false, origin = false;
+        unsigned cs = 0;
+        for (int i = kBuckets - 2; i >= 0; i--) {
+            unsigned j = static_cast<unsigned>(i) + 1; // bucket prepended to the suffix
+            cs += bk_n[j];
+            if (!bk_box[j].empty) {
+                origin = any_empty;
+                uni = any_nonempty ? join(bk_box[j], uni) : bk_box[j];
+                any_nonempty = true;
+            }
+            any_empty = any_empty || bk_box[j].empty;
+            Box second;
+            if (any_nonempty) {
+                second = uni;
+                if (origin) {
+                    HostVec3 o{0.0f, 0.0f, 0.0f};
+                    second = join_point(second, o);
+                }
+            }
+            cost[i] = kBoxCost + kTriCost * (+first_area[i] / sab * first_n[i] +
+                                             area_of(second) / sab * cs);
+        }
+        cost[kBuckets - 1] = INFINITY;
+    }
+
+    void build(unsigned start, unsigned end, unsigned depth) {
+        unsigned n = end - start;
+        if (n == 1) {
+            emit_leaf(p[start].bound, start, 1, depth);
+            return;
+        }
+        Box b;
+        for (unsigned i = start; i < end; i++)
+            b = join(b, p[i].bound);
+        float range[3] = {b.mx[0] - b.mn[0], b.mx[1] - b.mn[1], b.mx[2] - b.mn[2]};
+        unsigned axis = (range[0] > range[1] && range[0] > range[2]) ? 0 : (range[1] > range[2] ? 1 : 2);
+        float cost_leaf = n * kTriCost;
+
+        Box bk_box[kBuckets];
+        unsigned bk_n[kBuckets] = {0};
+        float width = range[axis] / kBuckets;
+        for (unsigned i = start; i < end; i++) {
+            float q = (p[i].c[axis] - b.mn[axis]) / width;
+            unsigned ib = (q != q) ? 0u : static_cast<unsigned>(q); // NaN only if width == 0
+            if (ib == kBuckets)
+                ib = kBuckets - 1;
+            bk_box[ib] = join(bk_box[ib], p[i].bound);
+            bk_n[ib]++;
+        }
+        float cost[kBuckets];
+        bucket_costs(bk_box, bk_n, b, cost);
+        // The reference starts its minimum search at bucket 1 with bucket 0's cost.
+        unsigned best = 1;
+        float best_cost = cost[0];
+        for (unsigned i = 1; i < kBuckets - 1; i++)
+            if (cost[i] < best_cost) {
+                best = i;
+                best_cost = cost[i];
+            }
+        if (!(best_cost < cost_leaf || n > kLeafMax)) {
+            emit_leaf(b, start, n, depth);
+            return;
+        }
+        unsigned mid;
+        if (depth > log2n) {
+            std::sort(p.begin() + start, p.begin() + end,
+                      [axis](BuildPrim const &x, BuildPrim const &y) { return x.c[axis] < y.c[axis]; });
+            mid = (start + end) >> 1;
+        } else {
+            float split = b.mn[axis] + (best + 1) * range[axis] / kBuckets;
+            auto it = std::partition(p.begin() + start, p.begin() + end,
+                                     [axis, split](BuildPrim const &x) { return x.c[axis] < split; });
+            mid = static_cast<unsigned>(it - p.begin());
+            if (mid == start)
+                mid++;
+            else if (mid == end)
+                mid--;
+        }
+        size_t self = out.size();
+        out.push_back(FlatNode{});
+        build(start, mid, depth + 1);
+        FlatNode &f = out[self];
+        std::memcpy(f.mn, b.mn, sizeof f.mn);
+        std::memcpy(f.mx, b.mx, sizeof f.mx);
+        f.num_prims = 0;
+        f.split_axis = axis;
+        f.next_child = static_cast<uint32_t>(out.size());
+        f.prim_start = 0xFFFFFFFFu;
+        build(mid, end, depth + 1);
+    }
+};
+
+template <class T> std::string device_copy(void **dst, std::vector<T> const &v, uint64_t *bytes) {
+    size_t n = std::max<size_t>(v.size() * sizeof(T), 16);
+    if (hipMalloc(dst, n) != hipSuccess)
+        return "hipMalloc failed (" + std::to_string(n) + " bytes)";
+    if (!v.empty() && hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return "hipMemcpy to device failed";
+    *bytes += n;
+    return "";
+}
+
+struct F4 {
+    float x, y, z, w;
+};
+inline float bits_f(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+} // namespace
+
+Scene::~Scene() {
+    void *ptrs[] = {buf.nodes, buf.tris, buf.tri_attr, buf.mats, buf.lights, buf.light_cum_power,
+                    buf.light_tris, buf.light_tri_cum};
+    for (void *q : ptrs)
+        if (q)
+            (void)hipFree(q);
+}
+
+void Scene::build_bvh() {
+    std::vector<BuildPrim> p;
+    size_t total = 0;
+    for (auto const &m : meshes)
+        total += m.tris.size() / 3;
+    p.reserve(total);
+    for (uint32_t g = 0; g < meshes.size(); g++) {
+        HostMesh const &m = meshes[g];
+        for (uint32_t t = 0; t < m.tris.size() / 3; t++) {
+            BuildPrim bp; // primitive_details (pathspace.h:97-113)
+            bp.ref = {g, t};
+            bp.bound = join_point(bp.bound, m.verts[m.tris[3 * t]]);
+            bp.bound = join_point(bp.bound, m.verts[m.tris[3 * t + 1]]);
+            bp.bound = join_point(bp.bound, m.verts[m.tris[3 * t + 2]]);
+            for (int i = 0; i < 3; i++)
+                bp.c[i] = (bp.bound.mx[i] + bp.bound.mn[i]) / 2.0f;
+            p.push_back(bp);
+        }
+    }
+    flat.clear();
+    flat.reserve(2 * p.size());
+    Builder bld{p, flat, std::log2(static_cast<double>(p.size()))};
+    if (!p.empty())
+        bld.build(0, static_cast<unsigned>(p.size()), 0);
+    max_leaf_depth = bld.max_depth;
+    prims.resize(p.size());
+    for (size_t i = 0; i < p.size(); i++)
+        prims[i] = p[i].ref;
+}
+
+std::string Scene::commit() {
+    if (committed)
+        return "scene already committed";
+    auto t0 = std::chrono::steady_clock::now();
+    size_t total_tris = 0;
+    for (auto const &m : meshes)
+        total_tris += m.tris.size() / 3;
+    if (total_tris >= kMaxPrims)
+        return "too many triangles (limit 2^28)";
+    for (auto &m : meshes)
+        m.light = -1;
+    for (size_t l = 0; l < lights.size(); l++) {
+        HostMesh &m = meshes[lights[l].mesh];
+        if (m.light >= 0)
+            return "mesh " + std::to_string(lights[l].mesh) + " carries two lights";
+        if (m.tris.empty())
+            return "light mesh " + std::to_string(lights[l].mesh) + " has no triangles";
+        m.light = static_cast<int32_t>(l);
+    }
+    // Fail-safe material (mat_fail_safe, material.cpp:54-76): Lambertian 0.8 for meshes
+    // without a material.
+    DevMaterial fs{};
+    fs.kind = kMatFailSafe;
+    fs.albedo[0] = fs.albedo[1] = fs.albedo[2] = 0.8f;
+    fs.on_a = 1.0f;
+    fs.on_b = 0.0f;
+    uint32_t fail_safe = static_cast<uint32_t>(materials.size());
+    materials.push_back(fs);
+
+    build_bvh();
+    auto t1 = std::chrono::steady_clock::now();
+    build_seconds = std::chrono::duration<double>(t1 - t0).count();
+
+    std::string err = upload();
+    upload_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+    if (!err.empty())
+        return err;
+    dev.fail_safe_mat = fail_safe;
+    committed = true;
+    return "";
+}
+
+std::string Scene::upload() {
+    // Interior nodes are numbered by their order in the flattened array.
+    std::vector<uint32_t> interior_index(flat.size(), 0xFFFFFFFFu);
+    num_interior = 0;
+    for (size_t n = 0; n < flat.size(); n++)
+        if (flat[n].num_prims == 0)
+            interior_index[n] = num_interior++;
+    auto ref_of = [&](uint32_t n) -> uint32_t {
+        FlatNode const &f = flat[n];
+        return f.num_prims > 0 ? leaf_ref(f.prim_start, f.num_prims) : interior_index[n];
+    };
+    std::vector<F4> nodes(4 * static_cast<size_t>(num_interior));
+    for (size_t n = 0; n < flat.size(); n++) {
+        if (flat[n].num_prims > 0)
+            continue;
+        FlatNode const &l = flat[n + 1];
+        FlatNode const &r = flat[flat[n].next_child];
+        F4 *q = &nodes[4 * static_cast<size_t>(interior_index[n])];
+        q[0] = {l.mn[0], l.mx[0], l.mn[1], l.mx[1]};
+        q[1] = {r.mn[0], r.mx[0], r.mn[1], r.mx[1]};
+        q[2] = {l.mn[2], l.mx[2], r.mn[2], r.mx[2]};
+        q[3] = {bits_f(ref_of(static_cast<uint32_t>(n + 1))), bits_f(ref_of(flat[n].next_child)), 0.0f, 0.0f};
+    }
+    std::vector<F4> tris(3 * prims.size()), attr(3 * prims.size());
+    for (size_t i = 0; i < prims.size(); i++) {
+        HostMesh const &m = meshes[prims[i].mesh];
+        uint32_t t = prims[i].tri;
+        HostVec3 a = m.verts[m.tris[3 * t]], b = m.verts[m.tris[3 * t + 1]], c = m.verts[m.tris[3 * t + 2]];
+        HostVec3 na = m.normals[m.tris[3 * t]], nb = m.normals[m.tris[3 * t + 1]],
+                 nc = m.normals[m.tris[3 * t + 2]];
+        uint32_t mat = m.material >= 0 ? static_cast<uint32_t>(m.material) : static_cast<uint32_t>(materials.size() - 1);
+        tris[3 * i] = {a.x, a.y, a.z, bits_f(prims[i].mesh)};
+        tris[3 * i + 1] = {b.x, b.y, b.z, bits_f(t)};
+        tris[3 * i + 2] = {c.x, c.y, c.z, 0.0f};
+        attr[3 * i] = {na.x, na.y, na.z, bits_f(mat)};
+        attr[3 * i + 1] = {nb.x, nb.y, nb.z, bits_f(static_cast<uint32_t>(m.light))};
+        attr[3 * i + 2] = {nc.x, nc.y, nc.z, 0.0f};
+    }
+    // Lights: per-mesh area CDF (geometry.cpp:105-118) and power CDF (lightsources.cpp:45-52),
+    // power = pi * area * radiance (light.cpp:20-23).
+    std::vector<DevLight> dl;
+    std::vector<float> cum_power, tri_cum;
+    std::vector<F4> ltris;
+    float total_power = 0;
+    for (auto const &L : lights) {
+        HostMesh const &m = meshes[L.mesh];
+        DevLight d{};
+        d.tri_begin = static_cast<uint32_t>(tri_cum.size());
+        d.tri_count = static_cast<uint32_t>(m.tris.size() / 3);
+        float cum = 0;
+        for (uint32_t t = 0; t < d.tri_count; t++) {
+            HostVec3 a = m.verts[m.tris[3 * t]], b = m.verts[m.tris[3 * t + 1]], c = m.verts[m.tris[3 * t + 2]];
+            HostVec3 na = m.normals[m.tris[3 * t]], nb = m.normals[m.tris[3 * t + 1]],
+                     nc = m.normals[m.tris[3 * t + 2]];
+            float ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z;
+            float vx = c.x - a.x, vy = c.y - a.y, vz = c.z - a.z;
+            float cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+            float nn = 0.0f;
+            nn += cx * cx;
+            nn += cy * cy;
+            nn += cz * cz;
+            float area = 0.5f * std::sqrt(nn);
+            cum += area;
+            tri_cum.push_back(cum);
+            ltris.push_back({a.x, a.y, a.z, 0});
+            ltris.push_back({b.x, b.y, b.z, 0});
+            ltris.push_back({c.x, c.y, c.z, 0});
+            ltris.push_back({na.x, na.y, na.z, 0});
+            ltris.push_back({nb.x, nb.y, nb.z, 0});
+            ltris.push_back({nc.x, nc.y, nc.z, 0});
+        }
+        d.area = cum;
+        float k = static_cast<float>(M_PI) * cum;
+        float px = k * L.rad[0], py = k * L.rad[1], pz = k * L.rad[2];
+        float pn = 0.0f;
+        pn += px * px;
+        pn += py * py;
+        pn += pz * pz;
+        total_power += std::sqrt(pn);
+        cum_power.push_back(total_power);
+        d.rad[0] = L.rad[0];
+        d.rad[1] = L.rad[1];
+        d.rad[2] = L.rad[2];
+        d.mesh = static_cast<uint32_t>(L.mesh);
+        d.prob = std::sqrt(pn); // divided by the total below
+        dl.push_back(d);
+    }
+    for (auto &d : dl)
+        d.prob = d.prob / total_power;
+
+    std::string e;
+    buf.bytes = 0;
+    if (!(e = device_copy(&buf.nodes, nodes, &buf.bytes)).empty() ||
+        !(e = device_copy(&buf.tris, tris, &buf.bytes)).empty() ||
+        !(e = device_copy(&buf.tri_attr, attr, &buf.bytes)).empty() ||
+        !(e = device_copy(&buf.mats, materials, &buf.bytes)).empty() ||
+        !(e = device_copy(&buf.lights, dl, &buf.bytes)).empty() ||
+        !(e = device_copy(&buf.light_cum_power, cum_power, &buf.bytes)).empty() ||
+        !(e = device_copy(&buf.light_tris, ltris, &buf.bytes)).empty() ||
+        !(e = device_copy(&buf.light_tri_cum, tri_cum, &buf.bytes)).empty())
+        return e;
+
+    dev.nodes = buf.nodes;
+    dev.tris = buf.tris;
+    dev.tri_attr = buf.tri_attr;
+    dev.mats = static_cast<const DevMaterial *>(buf.mats);
+    dev.lights = static_cast<const DevLight *>(buf.lights);
+    dev.light_cum_power = static_cast<const float *>(buf.light_cum_power);
+    dev.light_tris = buf.light_tris;
+    dev.light_tri_cum = static_cast<const float *>(buf.light_tri_cum);
+    dev.root_ref = flat.empty() ? kEmptyRef : ref_of(0);
+    dev.num_lights = static_cast<uint32_t>(lights.size());
+    dev.num_mats = static_cast<uint32_t>(materials.size());
+    dev.total_power = total_power;
+    dev.max_stack = max_leaf_depth + 1;
+    dev.num_prims = static_cast<uint32_t>(prims.size());
+    dev.num_interior = num_interior;
+    return "";
+}
+
+} // namespace x3d

@@ -1,0 +1,77 @@
+// Device-resident scene layout shared by the host packer (scene.cpp) and the HIP kernels.
+//
+// HBM layout (all arrays 16-byte aligned):
+//   nodes     4 x float4 per interior BVH node (64 B): the two child boxes and child refs
+//             f0 = (c0.min.x, c0.max.x, c0.min.y, c0.max.y)
+//             f1 = (c1.min.x, c1.max.x, c1.min.y, c1.max.y)
+//             f2 = (c0.min.z, c0.max.z, c1.min.z, c1.max.z)
+//             f3 = (ref c0, ref c1, -, -) as uint32 bits; c0 = left child (flattened n + 1),
+//             c1 = right child (flattened next_child) of the reference layout.
+//   tris      3 x float4 per primitive, in BVH primitive order: v0 (w = mesh id bits),
+//             v1 (w = triangle index bits), v2 (w = 0).
+//   tri_attr  3 x float4 per primitive: n0 (w = material id bits), n1 (w = light id bits or
+//             -1), n2 (w = 0) — the shading normals read once per hit.
+//   light_tris 6 x float4 per emitter triangle: p0, p1, p2, n0, n1, n2.
+#pragma once
+
+#include <stdint.h>
+
+namespace x3d {
+
+constexpr int32_t kMatOrenNayar = 0;
+constexpr int32_t kMatCookTorr = 1;
+constexpr int32_t kMatMixture = 2;
+constexpr int32_t kMatFailSafe = 3;
+
+struct DevMaterial { // 64 B
+    int32_t kind;
+    float albedo[3];
+    float on_a, on_b; // Oren-Nayar A and B (material.cpp:116-118)
+    float alpha2;     // cook_torr alpha^2 = 2 r^2 (material.cpp:379)
+    float ior_re, ior_im;
+    float ratio;
+    int32_t sub0, sub1;
+    float roughness, sigma;
+    int32_t pad[2];
+};
+static_assert(sizeof(DevMaterial) == 64, "DevMaterial layout");
+
+struct DevLight { // 32 B
+    uint32_t tri_begin, tri_count;
+    float area; // sum of triangle areas (geometry.cpp:105-118)
+    float prob; // |power| / total power (lightsources.cpp:73)
+    float rad[3];
+    uint32_t mesh;
+};
+static_assert(sizeof(DevLight) == 32, "DevLight layout");
+
+// Child reference encoding: interior node index, or a leaf:
+// bit 31 = leaf, bits 28..29 = primitive count - 1 (1..4 primitives), bits 0..27 = first prim.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmptyRef = 0xFFFFFFFFu; // scene without triangles
+constexpr uint32_t kMaxPrims = 1u << 28;
+
+inline constexpr uint32_t leaf_ref(uint32_t start, uint32_t count) {
+    return kLeafBit | ((count - 1u) << 28) | start;
+}
+
+struct DevScene {
+    const void *nodes;      // float4[4 * num_interior]
+    const void *tris;       // float4[3 * num_prims]
+    const void *tri_attr;   // float4[3 * num_prims]
+    const DevMaterial *mats;
+    const DevLight *lights;
+    const float *light_cum_power; // cumulative |power| per light
+    const void *light_tris;       // float4[6 * total emitter triangles]
+    const float *light_tri_cum;   // cumulative area per emitter triangle (per light)
+    uint32_t root_ref;
+    uint32_t num_lights;
+    uint32_t num_mats;
+    uint32_t fail_safe_mat; // index of the fail-safe Lambertian 0.8
+    float total_power;
+    uint32_t max_stack;     // deepest root-to-leaf path (stack entries needed)
+    uint32_t num_prims;
+    uint32_t num_interior;
+};
+
+} // namespace x3d

@@ -1,0 +1,221 @@
+// BVH traversal on gfx950: closest hit and any hit over the reference layout.
+//
+// One lane per ray. Each interior visit reads one 64-byte node holding both child boxes, tests
+// both with the reference's slab test, descends into the nearer hit child and pushes the other
+// (with its entry distance) on a per-lane stack. The stack's first LDS_DEPTH entries live in
+// LDS, entry-major ([entry][lane]) so that lanes of a wave hit distinct banks; deeper entries
+// spill to a per-thread slice of global memory sized from the tree depth.
+//
+// Closest hit returns exactly the reference's winner (bvh_path_space_layout::intersect,
+// pathspace.cpp:330-413): the reference visits leaves in decreasing flattened order and keeps
+// the first of equal-distance hits, i.e. the hit in the leaf with the largest prim_start, then
+// the lowest primitive inside that leaf. That rule is applied explicitly, so the near-first order
+// and the distance cull (boxes starting beyond best * (1 + 2^-12) are skipped; a relative slack
+// keeps a hit whose FP32 distance falls slightly below its box's entry distance) do not change
+// the result. The reference order (right child first, no culling) is available for checking.
+#pragma once
+
+#include "device_math.h"
+
+namespace x3d {
+
+struct ClosestHit {
+    int32_t prim;
+    float t, b1, b2;
+};
+
+template <int WG, int LDS_DEPTH> struct TravStack {
+    uint2 *lds;   // this lane's column of the workgroup's [LDS_DEPTH][WG] array
+    uint2 *spill; // this thread's global overflow slice
+    int sp;
+    X3D_DI void push(uint32_t ref, float t) {
+        uint2 e = make_uint2(ref, __float_as_uint(t));
+        if (sp < LDS_DEPTH)
+            lds[sp * WG] = e;
+        else
+            spill[sp - LDS_DEPTH] = e;
+        sp++;
+    }
+    X3D_DI uint2 pop() {
+        sp--;
+        return sp < LDS_DEPTH ? lds[sp * WG] : spill[sp - LDS_DEPTH];
+    }
+};
+
+X3D_DI bool is_leaf(uint32_t ref) { return (ref & kLeafBit) != 0u; }
+X3D_DI uint32_t leaf_start(uint32_t ref) { return ref & 0x0FFFFFFFu; }
+X3D_DI uint32_t leaf_count(uint32_t ref) { return ((ref >> 28) & 3u) + 1u; }
+
+X3D_DI void load_node(DevScene const &S, uint32_t idx, float4 &f0, float4 &f1, float4 &f2, float4 &f3) {
+    const float4 *n = static_cast<const float4 *>(S.nodes) + 4u * idx;
+    f0 = n[0];
+    f1 = n[1];
+    f2 = n[2];
+    f3 = n[3];
+}
+
+// REF_ORDER: the reference's traversal order without culling.
+template <int WG, int LDS_DEPTH, bool REF_ORDER>
+X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_DEPTH> &st,
+                              uint32_t &pops, uint32_t &tests) {
+    ClosestHit h{-1, __builtin_inff(), 0.0f, 0.0f};
+    if (S.root_ref == kEmptyRef)
+        return h;
+    f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
+    const float4 *tris = static_cast<const float4 *>(S.tris);
+    uint32_t best_leaf = 0;
+    uint32_t ref = S.root_ref;
+    st.sp = 0;
+    for (;;) {
+        if (is_leaf(ref)) {
+            uint32_t s0 = leaf_start(ref), cnt = leaf_count(ref);
+            for (uint32_t i = s0; i < s0 + cnt; i++) {
+                float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
+                float c1, c2, t0;
+                tests++;
+                if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), kTMin, kTMax, &c1, &c2, &t0) &&
+                    (t0 < h.t || (t0 == h.t && s0 > best_leaf))) {
+                    h.t = t0;
+                    h.b1 = c1;
+                    h.b2 = c2;
+                    h.prim = static_cast<int32_t>(i);
+                    best_leaf = s0;
+                }
+            }
+        } else {
+            pops++;
+            float4 f0, f1, f2, f3n;
+            load_node(S, ref, f0, f1, f2, f3n);
+            float tl = 0.0f, tr = 0.0f;
+            bool hl = slab(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, kTMin, kTMax, &tl);
+            bool hr = slab(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, kTMin, kTMax, &tr);
+            uint32_t rl = __float_as_uint(f3n.x), rr = __float_as_uint(f3n.y);
+            if (!REF_ORDER) {
+                float cut = h.t + h.t * kCullSlack;
+                hl = hl && !(tl > cut);
+                hr = hr && !(tr > cut);
+            }
+            if (hl && hr) {
+                if (REF_ORDER || !(tr < tl)) {
+                    // reference order: right child first; culled order: nearer first
+                    if (REF_ORDER) {
+                        st.push(rl, tl);
+                        ref = rr;
+                    } else {
+                        st.push(rr, tr);
+                        ref = rl;
+                    }
+                } else {
+                    st.push(rl, tl);
+                    ref = rr;
+                }
+                continue;
+            }
+            if (hl) {
+                ref = rl;
+                continue;
+            }
+            if (hr) {
+                ref = rr;
+                continue;
+            }
+        }
+        // pop the next candidate
+        bool found = false;
+        while (st.sp > 0) {
+            uint2 e = st.pop();
+            if (!REF_ORDER) {
+                float te = __uint_as_float(e.y);
+                if (te > h.t + h.t * kCullSlack)
+                    continue;
+            }
+            ref = e.x;
+            found = true;
+            break;
+        }
+        if (!found)
+            break;
+    }
+    return h;
+}
+
+// has_intersect (pathspace.cpp:415-456): any triangle hit in [tmin, tmax] under boxes clipped to
+// the same interval; the result does not depend on the visiting order.
+template <int WG, int LDS_DEPTH>
+X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravStack<WG, LDS_DEPTH> &st,
+                    uint32_t &pops, uint32_t &tests) {
+    if (S.root_ref == kEmptyRef)
+        return false;
+    f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
+    const float4 *tris = static_cast<const float4 *>(S.tris);
+    uint32_t ref = S.root_ref;
+    st.sp = 0;
+    for (;;) {
+        if (is_leaf(ref)) {
+            uint32_t s0 = leaf_start(ref), cnt = leaf_count(ref);
+            for (uint32_t i = s0; i < s0 + cnt; i++) {
+                float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
+                float c1, c2, t0;
+                tests++;
+                if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), tmin, tmax, &c1, &c2, &t0))
+                    return true;
+            }
+        } else {
+            pops++;
+            float4 f0, f1, f2, f3n;
+            load_node(S, ref, f0, f1, f2, f3n);
+            float tl = 0.0f, tr = 0.0f;
+            bool hl = slab(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, tmin, tmax, &tl);
+            bool hr = slab(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, tmin, tmax, &tr);
+            uint32_t rl = __float_as_uint(f3n.x), rr = __float_as_uint(f3n.y);
+            if (hl && hr) {
+                if (tr < tl) {
+                    st.push(rl, tl);
+                    ref = rr;
+                } else {
+                    st.push(rr, tr);
+                    ref = rl;
+                }
+                continue;
+            }
+            if (hl) {
+                ref = rl;
+                continue;
+            }
+            if (hr) {
+                ref = rr;
+                continue;
+            }
+        }
+        if (st.sp == 0)
+            break;
+        ref = st.pop().x;
+    }
+    return false;
+}
+
+// Hit record of pathspace.cpp:385-409 plus the material / light ids of the hit mesh.
+struct HitRec {
+    f3 p, n;
+    float t;
+    int32_t geo, mat, light;
+};
+
+X3D_DI HitRec finish_hit(DevScene const &S, ClosestHit const &h) {
+    const float4 *tris = static_cast<const float4 *>(S.tris);
+    const float4 *attr = static_cast<const float4 *>(S.tri_attr);
+    uint32_t i = static_cast<uint32_t>(h.prim);
+    float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
+    float4 na = attr[3 * i], nb = attr[3 * i + 1], nc = attr[3 * i + 2];
+    float b0 = 1.0f - h.b1 - h.b2;
+    HitRec r;
+    r.p = b0 * xyz(a) + h.b1 * xyz(b) + h.b2 * xyz(c);
+    r.n = normalize(b0 * xyz(na) + h.b1 * xyz(nb) + h.b2 * xyz(nc));
+    r.t = h.t;
+    r.geo = static_cast<int32_t>(__float_as_uint(a.w));
+    r.mat = static_cast<int32_t>(__float_as_uint(na.w));
+    r.light = static_cast<int32_t>(__float_as_uint(nb.w));
+    return r;
+}
+
+} // namespace x3d

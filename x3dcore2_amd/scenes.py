@@ -1,0 +1,363 @@
+"""Seeded procedural scenes for the renderer's workloads (flat world-space triangle meshes).
+
+A scene is plain data: materials, meshes (positions, shading normals, triangles, material
+index), area lights and a pinhole camera. The same arrays are loaded into the HIP renderer and
+into the CPU checkers, so every backend sees bit-identical inputs.
+
+Coordinates: the room is the unit cube x, y, z in [0, 1] with the open side at z = 1; y is up.
+The camera sits in front of the opening and looks down -z, as the reference's pinhole camera
+does in its own frame (corelib/src/camera.cpp:38-48).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+# Material kinds, matching include/x3dpt.h.
+OREN_NAYAR = 0  # reference oren_nayar (material.cpp:108-169); sigma = 0 is Lambertian
+COOK_TORR = 1  # reference cook_torr (material.cpp:171-252), GGX
+MIXTURE = 2  # reference mat_mixture (material.cpp:78-106)
+
+GLOSSY_IOR = (2.93, 3.0)  # the reference's glossy metal (corelib/test/tst_renderer.cpp:49-50)
+
+
+@dataclasses.dataclass
+class Material:
+    kind: int = OREN_NAYAR
+    albedo: Tuple[float, float, float] = (0.8, 0.8, 0.8)
+    sigma: float = 0.0
+    roughness: float = 0.0
+    ior: Tuple[float, float] = (1.0, 0.0)
+    sub0: int = -1
+    sub1: int = -1
+    ratio: float = 0.5
+
+
+@dataclasses.dataclass
+class Mesh:
+    verts: np.ndarray  # float32 [n, 3]
+    normals: np.ndarray  # float32 [n, 3]
+    tris: np.ndarray  # uint32 [m, 3]
+    material: int = -1
+
+
+@dataclasses.dataclass
+class Camera:
+    sensor: float
+    focal: float
+    aspect: float
+    transform: np.ndarray  # float32 [16], column-major 4x4 (element (i, j) at 4*j + i)
+
+    def packed(self) -> np.ndarray:
+        return np.concatenate([[self.sensor, self.focal, self.aspect], self.transform]).astype(np.float32)
+
+
+@dataclasses.dataclass
+class SceneDesc:
+    name: str
+    materials: List[Material]
+    meshes: List[Mesh]
+    lights: List[Tuple[int, Tuple[float, float, float]]]
+    camera: Camera
+    width: int = 256
+    height: int = 256
+    spp: int = 16
+    max_depth: int = 5
+
+    @property
+    def num_tris(self) -> int:
+        return int(sum(m.tris.shape[0] for m in self.meshes))
+
+
+def translate_camera(x: float, y: float, z: float, sensor=0.036, focal=0.035, aspect=1.0) -> Camera:
+    m = np.eye(4, dtype=np.float32)
+    m[0, 3], m[1, 3], m[2, 3] = x, y, z
+    return Camera(sensor, focal, aspect, m.T.reshape(-1).copy())
+
+
+# ------------------------------------------------------------------------------------------
+# Primitives
+# ------------------------------------------------------------------------------------------
+def quad(p0, p1, p2, p3, normal, material) -> Mesh:
+    v = np.array([p0, p1, p2, p3], dtype=np.float32)
+    n = np.repeat(np.array([normal], dtype=np.float32), 4, axis=0)
+    t = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32)
+    return Mesh(v, n, t, material)
+
+
+def box(center, size, angle_y, material) -> Mesh:
+    """Axis-aligned box rotated by angle_y (radians) about the vertical axis through center."""
+    cx, cy, cz = center
+    sx, sy, sz = size
+    c, s = math.cos(angle_y), math.sin(angle_y)
+
+    def rot(p):
+        x, y, z = p
+        return (c * x + s * z, y, -s * x + c * z)
+
+    faces = [  # (normal, 4 corners as unit-cube offsets in [-1, 1])
+        ((1, 0, 0), [(1, -1, -1), (1, 1, -1), (1, 1, 1), (1, -1, 1)]),
+        ((-1, 0, 0), [(-1, -1, 1), (-1, 1, 1), (-1, 1, -1), (-1, -1, -1)]),
+        ((0, 1, 0), [(-1, 1, -1), (-1, 1, 1), (1, 1, 1), (1, 1, -1)]),
+        ((0, -1, 0), [(-1, -1, 1), (-1, -1, -1), (1, -1, -1), (1, -1, 1)]),
+        ((0, 0, 1), [(-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)]),
+        ((0, 0, -1), [(1, -1, -1), (-1, -1, -1), (-1, 1, -1), (1, 1, -1)]),
+    ]
+    verts, norms, tris = [], [], []
+    for nrm, corners in faces:
+        base = len(verts)
+        for o in corners:
+            p = rot((o[0] * sx / 2, o[1] * sy / 2, o[2] * sz / 2))
+            verts.append((p[0] + cx, p[1] + cy, p[2] + cz))
+            norms.append(rot(nrm))
+        tris += [(base, base + 1, base + 2), (base, base + 2, base + 3)]
+    return Mesh(np.array(verts, np.float32), np.array(norms, np.float32), np.array(tris, np.uint32), material)
+
+
+def icosphere(level: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere with 20 * 4**level faces (vertices on the sphere, shared)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = np.array(v, dtype=np.float64)
+    verts /= np.linalg.norm(verts, axis=1, keepdims=True)
+    faces = np.array(f, dtype=np.int64)
+    for _ in range(level):
+        nv = verts.shape[0]
+        e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+        e = np.sort(e, axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mids = verts[uniq[:, 0]] + verts[uniq[:, 1]]
+        mids /= np.linalg.norm(mids, axis=1, keepdims=True)
+        verts = np.concatenate([verts, mids])
+        nf = faces.shape[0]
+        a, b, c = faces[:, 0], faces[:, 1], faces[:, 2]
+        ab, bc, ca = inv[:nf] + nv, inv[nf:2 * nf] + nv, inv[2 * nf:] + nv
+        faces = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1),
+                                np.stack([c, ca, bc], 1), np.stack([ab, bc, ca], 1)])
+    return verts, faces
+
+
+def _value_noise(p: np.ndarray, seed: int) -> np.ndarray:
+    """Trilinear value noise in [-1, 1] on the integer lattice, hashed from `seed`."""
+    base = np.floor(p).astype(np.int64)
+    f = p - base
+    f = f * f * (3 - 2 * f)
+
+    def h(ix, iy, iz):
+        x = (ix * 73856093) ^ (iy * 19349663) ^ (iz * 83492791) ^ (seed * 2654435761)
+        x = (x ^ (x >> 13)) * 1274126177
+        x = x ^ (x >> 16)
+        return ((x & 0xFFFF).astype(np.float64) / 65535.0) * 2.0 - 1.0
+
+    out = np.zeros(p.shape[0])
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                w = ((f[:, 0] if dx else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1])
+                     * (f[:, 2] if dz else 1 - f[:, 2]))
+                out += w * h(base[:, 0] + dx, base[:, 1] + dy, base[:, 2] + dz)
+    return out
+
+
+def vertex_normals(verts: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    """Area-weighted vertex normals of a closed mesh."""
+    fn = np.cross(verts[faces[:, 1]] - verts[faces[:, 0]], verts[faces[:, 2]] - verts[faces[:, 0]])
+    n = np.zeros_like(verts)
+    for k in range(3):
+        np.add.at(n, faces[:, k], fn)
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+def displaced_sphere(center, radius, level, amplitude, seed, material) -> Mesh:
+    verts, faces = icosphere(level)
+    disp = 1.0 + amplitude * _value_noise(verts * 4.0 + 17.0 * (seed % 7), seed)
+    v = verts * (radius * disp)[:, None] + np.asarray(center)[None, :]
+    n = vertex_normals(v, faces)
+    return Mesh(v.astype(np.float32), n.astype(np.float32), faces.astype(np.uint32), material)
+
+
+# ------------------------------------------------------------------------------------------
+# Workloads
+# ------------------------------------------------------------------------------------------
+def _room(mats: List[Material], meshes: List[Mesh], ceiling_light=True):
+    """Five inward quads of the unit room; returns the material indices (white, red, green)."""
+    white = len(mats)
+    mats.append(Material(OREN_NAYAR, (0.73, 0.73, 0.73)))
+    red = len(mats)
+    mats.append(Material(OREN_NAYAR, (0.63, 0.07, 0.05)))
+    green = len(mats)
+    mats.append(Material(OREN_NAYAR, (0.14, 0.45, 0.09)))
+    meshes.append(quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), white))  # floor
+    meshes.append(quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0), (0, -1, 0), white))  # ceiling
+    meshes.append(quad((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0), (0, 0, 1), white))  # back
+    meshes.append(quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 0, 0), red))  # left
+    meshes.append(quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (-1, 0, 0), green))  # right
+    return white, red, green
+
+
+def _ceiling_emitter(mats, meshes, lights, radiance=(17.0, 12.0, 4.0), half=0.125, y=0.999):
+    black = len(mats)
+    mats.append(Material(OREN_NAYAR, (0.0, 0.0, 0.0)))
+    lo, hi = 0.5 - half, 0.5 + half
+    meshes.append(quad((lo, y, lo), (hi, y, lo), (hi, y, hi), (lo, y, hi), (0, -1, 0), black))
+    lights.append((len(meshes) - 1, radiance))
+
+
+def box_scene(width=256, height=256, spp=16, max_depth=5) -> SceneDesc:
+    """Lambertian room with two blocks rotated +-17 degrees and a 0.25 x 0.25 ceiling emitter
+    of radiance (17, 12, 4): 36 triangles."""
+    mats: List[Material] = []
+    meshes: List[Mesh] = []
+    lights = []
+    white, _, _ = _room(mats, meshes)
+    meshes.append(box((0.33, 0.15, 0.62), (0.3, 0.3, 0.3), math.radians(-17.0), white))
+    meshes.append(box((0.66, 0.3, 0.35), (0.3, 0.6, 0.3), math.radians(17.0), white))
+    _ceiling_emitter(mats, meshes, lights)
+    return SceneDesc("box", mats, meshes, lights, translate_camera(0.5, 0.5, 2.0), width, height, spp, max_depth)
+
+
+def sphere_levels(rng: np.random.Generator, count: int, target_tris: int) -> np.ndarray:
+    """Subdivision levels in {3, 4, 5} whose face total is closest to target_tris (seeded)."""
+    levels = rng.choice([3, 4, 5], size=count, p=[0.48, 0.30, 0.22])
+    faces = lambda lv: int(np.sum(20 * 4 ** lv.astype(np.int64)))
+    # Greedy seeded adjustment towards the target.
+    order = rng.permutation(count)
+    for i in order:
+        cur = faces(levels)
+        if abs(cur - target_tris) < 20 * 4 ** 3:
+            break
+        if cur > target_tris and levels[i] > 3:
+            levels[i] -= 1
+        elif cur < target_tris and levels[i] < 5:
+            levels[i] += 1
+    return levels
+
+
+def spheres_scene(width=1024, height=1024, spp=64, max_depth=8, count=200, target_tris=1_300_000,
+                  seed=0x5EED0002) -> SceneDesc:
+    """Room plus `count` displaced icospheres (about 1.3M triangles at the default count): centres
+    uniform in the room, radii log-uniform in [0.02, 0.15], displacement 0-5% from value noise,
+    per-object GGX roughness U[0.05, 0.8] and albedo U[0.1, 0.9]^3."""
+    rng = np.random.default_rng(seed)
+    mats: List[Material] = []
+    meshes: List[Mesh] = []
+    lights = []
+    _room(mats, meshes)
+    levels = sphere_levels(rng, count, target_tris)
+    for k in range(count):
+        r = float(math.exp(rng.uniform(math.log(0.02), math.log(0.15))))
+        c = rng.uniform(r, 1.0 - r, size=3)
+        amp = float(rng.uniform(0.0, 0.05))
+        rough = float(rng.uniform(0.05, 0.8))
+        alb = tuple(float(a) for a in rng.uniform(0.1, 0.9, size=3))
+        mats.append(Material(COOK_TORR, alb, roughness=rough, ior=GLOSSY_IOR))
+        meshes.append(displaced_sphere(c, r, int(levels[k]), amp, seed + k, len(mats) - 1))
+    _ceiling_emitter(mats, meshes, lights)
+    return SceneDesc("spheres", mats, meshes, lights, translate_camera(0.5, 0.5, 2.0), width, height, spp,
+                     max_depth)
+
+
+def indirect_scene(width=512, height=512, spp=32, max_path_len=6, level=5) -> SceneDesc:
+    """Room whose emitter faces the ceiling above a 0.5 x 0.5 occluding slab, so light reaches
+    the scene only after bouncing; four GGX icospheres of roughness 0.05 / 0.2 / 0.4 / 0.7."""
+    mats: List[Material] = []
+    meshes: List[Mesh] = []
+    lights = []
+    white, _, _ = _room(mats, meshes)
+    meshes.append(box((0.5, 0.85, 0.5), (0.5, 0.02, 0.5), 0.0, white))  # occluder below the emitter
+    black = len(mats)
+    mats.append(Material(OREN_NAYAR, (0.0, 0.0, 0.0)))
+    lo, hi, y = 0.375, 0.625, 0.87
+    meshes.append(quad((lo, y, lo), (lo, y, hi), (hi, y, hi), (hi, y, lo), (0, 1, 0), black))
+    lights.append((len(meshes) - 1, (17.0, 12.0, 4.0)))
+    for k, (rough, cx) in enumerate(zip((0.05, 0.2, 0.4, 0.7), (0.17, 0.39, 0.61, 0.83))):
+        mats.append(Material(COOK_TORR, (0.9, 0.9, 0.9), roughness=rough, ior=GLOSSY_IOR))
+        meshes.append(displaced_sphere((cx, 0.12, 0.5), 0.1, level, 0.0, 7 + k, len(mats) - 1))
+    desc = SceneDesc("indirect", mats, meshes, lights, translate_camera(0.5, 0.5, 2.0), width, height, spp,
+                     max_path_len)
+    return desc
+
+
+def furnace_sphere(radius=10.0, res=30, albedo=0.7, radiance=1.0) -> SceneDesc:
+    """Inward uv-sphere that is both an Oren-Nayar(sigma=0) reflector and an area light, the
+    closed-form furnace of corelib/test/tst_pathtracer/tst_pathtracer.cpp:24-56
+    (expected radiance L / (1 - albedo) per channel). Geometry follows geometry.cpp:162-241."""
+    verts, norms = [], []
+    verts.append((0.0, 0.0, -radius))
+    norms.append((0.0, 0.0, 1.0))
+    for j in range(1, res - 1):
+        u = np.float32(j) / np.float32(res)
+        phi = (np.float32(1) - u) * np.float32(math.pi)
+        for i in range(res):
+            v = np.float32(i) / np.float32(res)
+            theta = v * np.float32(2 * math.pi)
+            x = np.float32(math.sin(phi) * math.cos(theta))
+            y = np.float32(math.sin(phi) * math.sin(theta))
+            z = np.float32(math.cos(phi))
+            verts.append((radius * x, radius * y, radius * z))
+            norms.append((-x, -y, -z))
+    verts.append((0.0, 0.0, radius))
+    norms.append((0.0, 0.0, -1.0))
+    tris = []
+    for i in range(1, res + 1):
+        tris.append((i % res + 1, i, 0))
+    for j in range(1, res - 2):
+        for i in range(1, res + 1):
+            tris.append((i % res + 1 + j * res, i + j * res, i + (j - 1) * res))
+            tris.append((i % res + 1 + (j - 1) * res, i % res + 1 + j * res, i + (j - 1) * res))
+    for i in range(1, res + 1):
+        tris.append((i % res + 1 + (res - 3) * res, 1 + (res - 2) * res, i + (res - 3) * res))
+    mats = [Material(OREN_NAYAR, (albedo, albedo, albedo))]
+    mesh = Mesh(np.array(verts, np.float32), np.array(norms, np.float32), np.array(tris, np.uint32), 0)
+    return SceneDesc("furnace", mats, [mesh], [(0, (radiance, radiance, radiance))],
+                     translate_camera(0.0, 0.0, 0.0, aspect=1.0), 16, 16, 64, 5)
+
+
+def random_soup(num_tris: int, seed: int, extent=1.0) -> SceneDesc:
+    """Random triangles in a cube (for hit-id parity on non-axis-aligned geometry)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, extent, size=(num_tris, 1, 3))
+    v = (c + rng.normal(0, 0.05 * extent, size=(num_tris, 3, 3))).reshape(-1, 3).astype(np.float32)
+    fn = np.cross(v[1::3] - v[0::3], v[2::3] - v[0::3])
+    fn /= np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-12)
+    n = np.repeat(fn, 3, axis=0).astype(np.float32)
+    t = np.arange(3 * num_tris, dtype=np.uint32).reshape(-1, 3)
+    mats = [Material(OREN_NAYAR, (0.5, 0.5, 0.5))]
+    return SceneDesc("soup", mats, [Mesh(v, n, t, 0)], [], translate_camera(0.5, 0.5, 2.0))
+
+
+def seeded_rays(desc: SceneDesc, n: int, seed: int, inside=True) -> np.ndarray:
+    """Rays as float32 [n, 6] (origin, unit direction): origins uniform in the room (or
+    around the scene bounds), directions uniform on the sphere, plus a share of camera-like rays
+    and rays aimed at triangle vertices and edges (where ties between triangles occur)."""
+    rng = np.random.default_rng(seed)
+    allv = np.concatenate([m.verts for m in desc.meshes])
+    lo, hi = allv.min(0), allv.max(0)
+    pad = 0.05 * (hi - lo + 1e-6)
+    o = rng.uniform(lo + pad, hi - pad, size=(n, 3)) if inside else rng.uniform(lo - 2 * pad, hi + 2 * pad,
+                                                                                    size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    # A quarter of the rays aim at vertices or edge midpoints of random triangles.
+    k = n // 4
+    mesh_ids = rng.integers(0, len(desc.meshes), size=k)
+    tgt = np.empty((k, 3))
+    for j, mi in enumerate(mesh_ids):
+        m = desc.meshes[mi]
+        t = m.tris[rng.integers(0, m.tris.shape[0])]
+        w = rng.choice(4)
+        if w < 3:
+            tgt[j] = m.verts[t[w]]
+        else:
+            tgt[j] = 0.5 * (m.verts[t[0]].astype(np.float64) + m.verts[t[1]])
+    d[:k] = tgt - o[:k]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d], axis=1).astype(np.float32)

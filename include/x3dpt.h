@@ -193,7 +193,15 @@ typedef struct x3d_render_stats {
     uint64_t num_samples;       /* w*h*spp */
     uint64_t closest_queries;   /* closest-hit rays traced (including the first-hit pass) */
     uint64_t anyhit_queries;    /* shadow rays traced */
-    double kernel_ms;           /* device time of the render kernels (HIP events) */
+    uint64_t closest_nodes;     /* interior nodes visited by closest-hit queries */
+    uint64_t closest_tests;     /* ray-triangle tests of closest-hit queries */
+    uint64_t anyhit_nodes;      /* interior nodes visited by any-hit queries */
+    uint64_t anyhit_tests;      /* ray-triangle tests of any-hit queries */
+    uint64_t first_hit_nodes;   /* ... of which in the first-hit pass (closest_nodes) */
+    uint64_t first_hit_tests;   /* ... of which in the first-hit pass (closest_tests) */
+    double kernel_ms;           /* device time of all render kernels (HIP events) */
+    double first_hit_ms;        /* device time of the first-hit pass */
+    double trace_ms;            /* device time of the sample kernels (bounce loop) */
     double total_ms;            /* wall time of the call including copies */
 } x3d_render_stats;
 

@@ -68,7 +68,10 @@ class RenderOptions(C.Structure):
 
 class RenderStats(C.Structure):
     _fields_ = [("num_samples", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
-                ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+                ("closest_nodes", C.c_uint64), ("closest_tests", C.c_uint64), ("anyhit_nodes", C.c_uint64),
+                ("anyhit_tests", C.c_uint64), ("first_hit_nodes", C.c_uint64), ("first_hit_tests", C.c_uint64),
+                ("kernel_ms", C.c_double), ("first_hit_ms", C.c_double),
+                ("trace_ms", C.c_double), ("total_ms", C.c_double)]
 
 
 _lib: Optional[C.CDLL] = None

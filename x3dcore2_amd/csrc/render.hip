@@ -198,6 +198,11 @@ __global__ void k_camera_rays(CamParams c, uint32_t w, uint32_t h, float *rays) 
     r[5] = v.z;
 }
 
+__global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x)
+        p[k] = 0;
+}
+
 __global__ void k_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out) {
     if (blockIdx.x != 0 || threadIdx.x != 0)
         return;
@@ -410,8 +415,10 @@ struct x3d_renderer {
     x3d_render_options opt;
     x3d::DevBuf first, acc, out, spill, counters;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
     ~x3d_renderer() {
+        if (evm)
+            (void)hipEventDestroy(evm);
         if (ev0)
             (void)hipEventDestroy(ev0);
         if (ev1)
@@ -492,7 +499,7 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     uint64_t npix = static_cast<uint64_t>(width) * height;
     X3D_HIP(r->first.ensure(npix * 3 * sizeof(float4)));
     X3D_HIP(r->acc.ensure(npix * sizeof(float4)));
-    X3D_HIP(r->counters.ensure(sizeof(Counters)));
+    X3D_HIP(r->counters.ensure(2 * sizeof(Counters)));
     P.first = static_cast<float4 *>(r->first.p);
     P.acc = static_cast<float4 *>(r->acc.p);
     P.out = reinterpret_cast<float4 *>(d_out);
@@ -510,10 +517,15 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     uint32_t grid = grid_first > grid_render ? grid_first : grid_render;
     X3D_HIP(r->spill.ensure(static_cast<size_t>(grid) * kWG * P.spill_per_thread * sizeof(uint2)));
     P.spill = static_cast<uint2 *>(r->spill.p);
-    X3D_HIP(hipMemsetAsync(P.cnt, 0, sizeof(Counters), r->stream));
+    // Counter slots: [0] first-hit pass, [1] sample kernels; cleared by a kernel, not a memset,
+    // so the sequence stays capturable in a graph.
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream,
+                       reinterpret_cast<unsigned long long *>(P.cnt), static_cast<uint32_t>(2 * sizeof(Counters) / 8));
     X3D_HIP(hipEventRecord(r->ev0, r->stream));
     hipLaunchKernelGGL(k_first_hit, dim3(grid_first), dim3(kWG), 0, r->stream, P);
     X3D_HIP(hipGetLastError());
+    X3D_HIP(hipEventRecord(r->evm, r->stream));
+    P.cnt = P.cnt + 1;
     P.s_begin = 0;
     P.s_end = spp;
     P.finalize = 1;
@@ -528,14 +540,28 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     X3D_HIP(hipEventRecord(r->ev1, r->stream));
     X3D_HIP(hipEventSynchronize(r->ev1));
     if (stats) {
-        Counters c;
-        X3D_HIP(hipMemcpy(&c, P.cnt, sizeof c, hipMemcpyDeviceToHost));
-        float ms = 0;
+        Counters cc[2];
+        X3D_HIP(hipMemcpy(cc, r->counters.p, sizeof cc, hipMemcpyDeviceToHost));
+        Counters c = cc[1];
+        c.closest += cc[0].closest;
+        c.pops += cc[0].pops;
+        c.tests += cc[0].tests;
+        float ms = 0, ms_first = 0, ms_trace = 0;
         X3D_HIP(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+        X3D_HIP(hipEventElapsedTime(&ms_first, r->ev0, r->evm));
+        X3D_HIP(hipEventElapsedTime(&ms_trace, r->evm, r->ev1));
         stats->num_samples = npix * spp;
         stats->closest_queries = c.closest;
         stats->anyhit_queries = c.anyhit;
+        stats->closest_nodes = c.pops;
+        stats->closest_tests = c.tests;
+        stats->anyhit_nodes = c.spops;
+        stats->anyhit_tests = c.stests;
+        stats->first_hit_nodes = cc[0].pops;
+        stats->first_hit_tests = cc[0].tests;
         stats->kernel_ms = ms;
+        stats->first_hit_ms = ms_first;
+        stats->trace_ms = ms_trace;
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return X3D_OK;
@@ -949,6 +975,7 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
     X3D_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     X3D_HIP(hipEventCreate(&r->ev0));
     X3D_HIP(hipEventCreate(&r->ev1));
+    X3D_HIP(hipEventCreate(&r->evm));
     *out = r.release();
     return X3D_OK;
 }

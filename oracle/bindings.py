@@ -100,6 +100,8 @@ class Oracle(_Backend):
         lib.orc_bsdf_pdf.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _f32p, _f32p, _f32p, C.c_int, _f64p]
         lib.orc_bsdf_sample.argtypes = [C.c_void_p, C.c_int, C.c_uint64, C.c_uint32, _f32p, _f32p, C.c_int,
                                         C.c_int, _f64p, _f64p]
+        lib.orc_bsdf_sample_u.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _f32p, _f32p, _f32p, C.c_uint32,
+                                          C.c_int, _f64p, _f64p]
         lib.orc_pcg_draws.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _u32p]
         lib.orc_sincos_2pi.argtypes = [C.c_uint32, _f32p, _f32p, _f32p]
         lib.orc_camera_rays.argtypes = [C.c_float, C.c_float, C.c_float, _f32p, C.c_uint32, C.c_uint32, _f32p]
@@ -154,6 +156,15 @@ class Oracle(_Backend):
         i = np.zeros((n, 3), np.float64)
         d = np.zeros(n, np.float64)
         self.lib.orc_bsdf_sample(self.h, mat, seed, n, _f32(nrm), _f32(o), prec, int(ref_mixture), i, d)
+        return i, d
+
+    def bsdf_sample_u(self, mat, nrm, o, u, ref_mixture=True):
+        """Sample k consumes u[k, :] in order (u: float32 [n, per_sample])."""
+        n = nrm.shape[0]
+        u = _f32(u)
+        i = np.zeros((n, 3), np.float64)
+        d = np.zeros(n, np.float64)
+        self.lib.orc_bsdf_sample_u(self.h, mat, n, _f32(nrm), _f32(o), u, u.shape[1], int(ref_mixture), i, d)
         return i, d
 
     def pcg_draws(self, seed, pixel, sample, n):

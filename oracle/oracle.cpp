@@ -627,8 +627,15 @@ template <class T> T bsdf_pdf(Scene const &s, int mat, V3<T> n, V3<T> o, V3<T> i
 // Draws consumed: oren_nayar / fail-safe / cook_torr 2, mixture 1 + the chosen lobe's.
 // With ref_mixture the mixture density is the chosen lobe's density times its selection
 // probability (material.cpp:94-106); otherwise it is the full mixture density.
-template <class T>
-V3<T> bsdf_sample(Scene const &s, int mat, V3<T> n, V3<T> o, Pcg &g, T *dens, bool ref_mixture) {
+// Uniform source replaying given numbers (to feed the reference's own rng draws).
+struct Replay {
+    const float *u;
+    uint32_t k;
+    float draw() { return u[k++]; }
+};
+
+template <class T, class R>
+V3<T> bsdf_sample(Scene const &s, int mat, V3<T> n, V3<T> o, R &g, T *dens, bool ref_mixture) {
     if (mat < 0 || s.mats[mat].kind == MAT_OREN_NAYAR) { // material.cpp:163-169
         T e0 = T(g.draw());
         T e1 = T(g.draw());
@@ -667,10 +674,10 @@ V3<T> bsdf_sample(Scene const &s, int mat, V3<T> n, V3<T> o, Pcg &g, T *dens, bo
     V3<T> i;
     T d;
     if (sel < r) {
-        i = bsdf_sample<T>(s, m.sub0, n, o, g, &d, ref_mixture);
+        i = bsdf_sample<T, R>(s, m.sub0, n, o, g, &d, ref_mixture);
         d = ref_mixture ? d * r : d;
     } else {
-        i = bsdf_sample<T>(s, m.sub1, n, o, g, &d, ref_mixture);
+        i = bsdf_sample<T, R>(s, m.sub1, n, o, g, &d, ref_mixture);
         d = ref_mixture ? d * (T(1) - r) : d;
     }
     if (!ref_mixture && d != T(0))
@@ -683,9 +690,9 @@ template V3<float> bsdf_eval<float>(Scene const &, int, V3f, V3f, V3f);
 template V3<double> bsdf_eval<double>(Scene const &, int, V3<double>, V3<double>, V3<double>);
 template float bsdf_pdf<float>(Scene const &, int, V3f, V3f, V3f);
 template double bsdf_pdf<double>(Scene const &, int, V3<double>, V3<double>, V3<double>);
-template V3<float> bsdf_sample<float>(Scene const &, int, V3f, V3f, Pcg &, float *, bool);
-template V3<double> bsdf_sample<double>(Scene const &, int, V3<double>, V3<double>, Pcg &,
-                                        double *, bool);
+template V3<float> bsdf_sample<float, Pcg>(Scene const &, int, V3f, V3f, Pcg &, float *, bool);
+template V3<double> bsdf_sample<double, Pcg>(Scene const &, int, V3<double>, V3<double>, Pcg &,
+                                             double *, bool);
 
 // ---------------------------------------------------------------------------------------------
 // Lights.
@@ -1175,6 +1182,21 @@ void orc_bsdf_sample(void *h, int mat, uint64_t seed, uint32_t n, float const *n
     }
 }
 
+// Like orc_bsdf_sample, but sample k consumes the uniforms u[per_sample*k ...] in order.
+void orc_bsdf_sample_u(void *h, int mat, uint32_t n, float const *nrm, float const *o,
+                       float const *u, uint32_t per_sample, int ref_mixture, double *i_out,
+                       double *dens) {
+    Scene *s = static_cast<Scene *>(h);
+    for (uint32_t k = 0; k < n; k++) {
+        Replay g{u + static_cast<size_t>(per_sample) * k, 0};
+        float d;
+        V3f r = bsdf_sample<float, Replay>(*s, mat, ld3(nrm + 3 * k), ld3(o + 3 * k), g, &d,
+                                           ref_mixture);
+        st3d(i_out + 3 * k, r);
+        dens[k] = d;
+    }
+}
+
 void orc_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out) {
     Pcg g = pcg_for(seed, pixel, sample);
     for (uint32_t k = 0; k < n; k++)
@@ -1276,4 +1298,19 @@ void orc_render(void *h, float const *cam, int tracer, uint32_t max_depth, uint3
                 stats[i] += ts[i];
 }
 
+} // extern "C"
+
+extern "C" {
+// pcg32_srandom_r(initstate, initseq) followed by n pcg32_random_r draws (for pinning the
+// generator against its published reference outputs).
+void orc_pcg_raw(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t *out) {
+    orc::Pcg g;
+    g.state = 0;
+    g.inc = (initseq << 1u) | 1u;
+    g.next();
+    g.state += initstate;
+    g.next();
+    for (uint32_t k = 0; k < n; k++)
+        out[k] = g.next();
+}
 } // extern "C"

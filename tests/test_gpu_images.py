@@ -1,0 +1,109 @@
+"""Converged-image parity of the HIP renderer.
+
+- Full box workload (256 x 256, 16 spp, 5 bounces, MIS): the film is bit-identical to the FP32
+  oracle's.
+- Against an FP64 oracle (same estimator, double precision, independent seeds) on a seeded
+  pixel subset: per-pixel z-scores of the combined Monte Carlo error must stay within 3 sigma
+  except for the expected tail, and the mean z (a bias test) must be ~0.
+- Against the reference renderer itself (oracle/_ref, its unidirect_lt1 tracer and minstd
+  streams): same per-pixel test for the GPU's reference-mode tracer.
+Relative MSE against the FP64 oracle is printed for the record.
+"""
+
+import numpy as np
+import pytest
+
+from x3dcore2_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def gpu_replicates(x3d, desc, pt, w, h, spp, reps, **kw):
+    """Per-pixel mean and standard error from `reps` independently seeded renders."""
+    scene = x3d.Scene(desc)
+    imgs = []
+    for s in range(reps):
+        r = x3d.Renderer(pt, seed=1000 + s, **kw)
+        img, _ = r.render(scene, desc.camera, w, h, spp)
+        imgs.append(img[:, :3].astype(np.float64))
+    imgs = np.stack(imgs)
+    return imgs.mean(0), imgs.std(0, ddof=1) / np.sqrt(reps), imgs
+
+
+def z_report(a, sa, b, sb, mask=None):
+    se = np.sqrt(sa ** 2 + sb ** 2)
+    ok = se > 1e-9
+    if mask is not None:
+        ok &= mask
+    z = (a - b)[ok] / se[ok]
+    return z, ((a - b) ** 2)[ok].sum() / max((b ** 2)[ok].sum(), 1e-30)
+
+
+def test_box_workload_bit_exact(x3d, oracle_mod):
+    desc = scenes.box_scene()
+    scene = x3d.Scene(desc)
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=desc.max_depth, seed=1)
+    img, st = r.render(scene, desc.camera, desc.width, desc.height, desc.spp)
+    ref = oracle_mod.Oracle(desc).render(oracle_mod.TR_UNIDIRECT_MIS, desc.width, desc.height, desc.spp, seed=1,
+                                         max_depth=desc.max_depth)
+    assert np.array_equal(_bits(img), _bits(ref))
+    print(f"[gpu] box 256x256x16: {st.kernel_ms:.2f} ms, {st.closest_queries + st.anyhit_queries} rays")
+
+
+def test_box_vs_fp64_oracle(x3d, oracle_mod):
+    desc = scenes.box_scene()
+    w, h = 64, 64
+    mean, se, _ = gpu_replicates(x3d, desc, x3d.PT_UNIDIRECT_MIS, w, h, 64, 8, max_depth=5)
+    pix = np.random.default_rng(3).choice(w * h, size=1024, replace=False).astype(np.uint32)
+    orc = oracle_mod.Oracle(desc)
+    reps = [orc.render(oracle_mod.TR_UNIDIRECT_MIS, w, h, 128, seed=500 + s, max_depth=5, pixels=pix, prec=1)
+            [:, :3] for s in range(4)]
+    reps = np.stack(reps)
+    om, ose = reps.mean(0), reps.std(0, ddof=1) / np.sqrt(len(reps))
+    z, rel_mse = z_report(mean[pix], se[pix], om, ose)
+    print(f"[gpu] box vs FP64 oracle: rel MSE {rel_mse:.2e}, |z|>3 in {np.mean(np.abs(z) > 3):.4f}, mean z {z.mean():.3f}")
+    assert np.mean(np.abs(z) > 3) < 0.02
+    assert abs(z.mean()) < 4 / np.sqrt(z.size) + 0.05
+    assert rel_mse < 1e-2
+
+
+def test_spheres_vs_fp64_oracle(x3d, oracle_mod):
+    """GGX spheres workload (reduced count and resolution so the FP64 oracle finishes)."""
+    desc = scenes.spheres_scene(count=40, target_tris=120000, seed=91)
+    w, h = 64, 64
+    mean, se, _ = gpu_replicates(x3d, desc, x3d.PT_UNIDIRECT_MIS, w, h, 32, 8, max_depth=8)
+    pix = np.random.default_rng(4).choice(w * h, size=768, replace=False).astype(np.uint32)
+    orc = oracle_mod.Oracle(desc)
+    reps = np.stack([orc.render(oracle_mod.TR_UNIDIRECT_MIS, w, h, 64, seed=700 + s, max_depth=8, pixels=pix,
+                                prec=1)[:, :3] for s in range(4)])
+    om, ose = reps.mean(0), reps.std(0, ddof=1) / np.sqrt(len(reps))
+    z, rel_mse = z_report(mean[pix], se[pix], om, ose)
+    print(f"[gpu] spheres vs FP64 oracle: rel MSE {rel_mse:.2e}, |z|>3 in {np.mean(np.abs(z) > 3):.4f}, "
+          f"mean z {z.mean():.3f}")
+    assert np.mean(np.abs(z) > 3) < 0.03
+    assert abs(z.mean()) < 4 / np.sqrt(z.size) + 0.05
+
+
+def test_reference_mode_vs_reference_renderer(x3d, ref_mod):
+    """GPU unidirect_lt1 (PCG streams) against the reference renderer's own unidirect_lt1 (one
+    minstd stream per task, pathtracer.cpp:587-653, renderer.cpp:96-160)."""
+    desc = scenes.box_scene()
+    w, h = 48, 48
+    mean, se, imgs = gpu_replicates(x3d, desc, x3d.PT_UNIDIRECT_LT1, w, h, 64, 8)
+    ref = ref_mod.Reference(desc)
+    tasks = 64
+    img, taken = ref.render(ref_mod.TR_UNIDIRECT_LT1, w, h, 4 * tasks, threads=tasks)
+    assert taken == 4 * tasks
+    # same estimator => same per-sample variance: scale the GPU's per-render spread to the
+    # reference's sample count.
+    per_sample_sd = imgs.std(0, ddof=1) * np.sqrt(64)
+    ref_se = per_sample_sd / np.sqrt(taken)
+    z, rel_mse = z_report(mean, se, img[:, :3].astype(np.float64), ref_se)
+    print(f"[gpu] lt1 vs reference renderer: rel MSE {rel_mse:.2e}, |z|>3 in {np.mean(np.abs(z) > 3):.4f}, "
+          f"mean z {z.mean():.3f}")
+    assert np.mean(np.abs(z) > 3) < 0.03
+    assert abs(z.mean()) < 4 / np.sqrt(z.size) + 0.05

@@ -91,7 +91,17 @@ def main():
     ap.add_argument("--form", default="plain", choices=["plain", "wavefront"])
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (to calibrate rocprof byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from x3dcore2_amd.device import DeviceArray, copy_device, synchronize
+
+        a, b = DeviceArray(args.copy_calibration), DeviceArray(args.copy_calibration)
+        copy_device(b, a, args.copy_calibration)
+        synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the path tracer runs on a single MI355X in this build "
               "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
@@ -143,9 +153,11 @@ def main():
     achieved_gbs = alg_bytes / (trace_ms * 1e-3) / 1e9
     achieved_tflops = alg_flops / (trace_ms * 1e-3) / 1e12
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{desc.name}_{args.form}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{desc.name}_{args.form}.md")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        for line in open(tpath):
+            if line.startswith("hbm_bytes_per_launch:"):
+                traffic = float(line.split(":", 1)[1])
 
     out = {
         "metric": "Msamples/s & Mrays/s at 1024^2 vs plain GPU path at equal RMSE; grad evals/s",

@@ -29,6 +29,11 @@ def _check(rc: int, what: str):
         raise RuntimeError(f"{what} failed: {hip().hipGetErrorString(rc).decode()}")
 
 
+def copy_device(dst: "DeviceArray", src: "DeviceArray", nbytes: int):
+    """Device-to-device copy of nbytes (hipMemcpyDeviceToDevice)."""
+    _check(hip().hipMemcpy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), int(nbytes), 3), "hipMemcpy D2D")
+
+
 def synchronize():
     _check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 

@@ -181,7 +181,7 @@ def main():
         "kernel_ms_per_step": float(np.mean([s.kernel_ms for s in stats])),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_render_plain<UNIDIRECT_MIS>" if args.form == "plain" else "wavefront",
+            "kernel": "k_render_mis" if args.form == "plain" else "wavefront",
             "achieved": achieved_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

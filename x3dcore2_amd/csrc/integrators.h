@@ -68,6 +68,15 @@ X3D_DI f3 light_radiance(DevScene const &S, int32_t l, f3 w, f3 n) {
     return dot(n, w) > 0.0f ? light_rad(S, l) : mk(0.0f, 0.0f, 0.0f);
 }
 
+// State of one MIS path between bounces (see Tracer::mis_begin / mis_bounce).
+struct PathState {
+    HitRec x;
+    f3 wo, beta, L;
+    Pcg g;
+    uint32_t k;
+    bool alive;
+};
+
 template <int WG, int LDSD> struct Tracer {
     DevScene const &S;
     TravStack<WG, LDSD> &st;
@@ -113,47 +122,65 @@ template <int WG, int LDSD> struct Tracer {
 
     // Fixed-depth unidirectional path tracing: next-event estimation and BSDF sampling combined
     // with the power heuristic (Veach 1997); up to max_depth bounces after the first hit.
-    X3D_DI f3 li_mis(HitRec const &first, bool valid, f3 ray_v, Pcg &g, uint32_t max_depth) {
-        f3 L = mk(0.0f, 0.0f, 0.0f), beta = mk(1.0f, 1.0f, 1.0f);
+    // mis_begin starts the path of sample `s` at the cached first hit; each mis_bounce adds one
+    // vertex (light sample + shadow ray, BSDF sample + closest hit, MIS-weighted emission).
+    X3D_DI void mis_begin(PathState &ps, HitRec const &first, bool valid, f3 ray_v, Pcg g, uint32_t max_depth) {
+        ps.g = g;
+        ps.L = mk(0.0f, 0.0f, 0.0f);
+        ps.beta = mk(1.0f, 1.0f, 1.0f);
+        ps.k = 0;
+        ps.alive = valid && max_depth > 0;
         if (!valid)
-            return L;
-        HitRec x = first;
-        f3 wo = -ray_v;
-        if (x.light >= 0)
-            L = L + light_radiance(S, x.light, wo, x.n);
-        for (uint32_t k = 0; k < max_depth; k++) {
-            if (S.num_lights > 0) {
-                LightSample ls = sample_light(S, g);
-                f3 d, wi;
-                float r2, cl;
-                if (direct(ls, x, wo, d, wi, r2, cl)) {
-                    float pl = ls.area_dens * r2 / cl;
-                    float pb = bsdf_pdf(S.mats, x.mat, x.n, wo, wi);
-                    float w = pl * pl / (pl * pl + pb * pb);
-                    L = L + mul(beta, d * w);
-                }
+            return;
+        ps.x = first;
+        ps.wo = -ray_v;
+        if (first.light >= 0)
+            ps.L = ps.L + light_radiance(S, first.light, ps.wo, first.n);
+    }
+
+    X3D_DI void mis_bounce(PathState &ps, uint32_t max_depth) {
+        HitRec const &x = ps.x;
+        if (S.num_lights > 0) {
+            LightSample ls = sample_light(S, ps.g);
+            f3 d, wi;
+            float r2, cl;
+            if (direct(ls, x, ps.wo, d, wi, r2, cl)) {
+                float pl = ls.area_dens * r2 / cl;
+                float pb = bsdf_pdf(S.mats, x.mat, x.n, ps.wo, wi);
+                float w = pl * pl / (pl * pl + pb * pb);
+                ps.L = ps.L + mul(ps.beta, d * w);
             }
-            float dens;
-            f3 wi = bsdf_sample(S.mats, x.mat, x.n, wo, g, &dens, false);
-            if (!(dens > 0.0f))
-                break;
-            HitRec nx;
-            if (!trace(x.p, wi, nx) || dot(nx.n, -wi) <= 0.0f)
-                break;
-            float cw = dot(x.n, wi);
-            f3 f = bsdf_eval(S.mats, x.mat, x.n, wo, wi);
-            beta = mul(beta, f) * cw / dens;
-            if (nx.light >= 0) {
-                float cl = dot(nx.n, -wi);
-                DevLight const &Ln = S.lights[nx.light];
-                float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
-                float w = dens * dens / (dens * dens + pl * pl);
-                L = L + mul(beta, light_radiance(S, nx.light, -wi, nx.n) * w);
-            }
-            x = nx;
-            wo = -wi;
         }
-        return L;
+        float dens;
+        f3 wi = bsdf_sample(S.mats, x.mat, x.n, ps.wo, ps.g, &dens, false);
+        HitRec nx;
+        if (!(dens > 0.0f) || !trace(x.p, wi, nx) || dot(nx.n, -wi) <= 0.0f) {
+            ps.alive = false;
+            return;
+        }
+        float cw = dot(x.n, wi);
+        f3 f = bsdf_eval(S.mats, x.mat, x.n, ps.wo, wi);
+        ps.beta = mul(ps.beta, f) * cw / dens;
+        if (nx.light >= 0) {
+            float cl = dot(nx.n, -wi);
+            DevLight const &Ln = S.lights[nx.light];
+            float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
+            float w = dens * dens / (dens * dens + pl * pl);
+            ps.L = ps.L + mul(ps.beta, light_radiance(S, nx.light, -wi, nx.n) * w);
+        }
+        ps.x = nx;
+        ps.wo = -wi;
+        ps.k++;
+        if (ps.k >= max_depth)
+            ps.alive = false;
+    }
+
+    X3D_DI f3 li_mis(HitRec const &first, bool valid, f3 ray_v, Pcg &g, uint32_t max_depth) {
+        PathState ps;
+        mis_begin(ps, first, valid, ray_v, g, max_depth);
+        while (ps.alive)
+            mis_bounce(ps, max_depth);
+        return ps.L;
     }
 
     // unidirect_lt1_path_tracer (pathtracer.cpp:587-653): NEE without MIS, Russian roulette

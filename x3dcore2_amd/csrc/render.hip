@@ -315,6 +315,81 @@ template <int TRACER> __global__ __launch_bounds__(kWG) void k_render_plain(Rend
     }
 }
 
+X3D_DI HitRec load_first(RenderParams const &P, uint32_t pix, bool &valid) {
+    float4 a = P.first[3 * pix], b = P.first[3 * pix + 1], c = P.first[3 * pix + 2];
+    HitRec first;
+    first.p = mk(a.x, a.y, a.z);
+    first.t = a.w;
+    first.n = mk(b.x, b.y, b.z);
+    first.geo = static_cast<int32_t>(__float_as_uint(b.w));
+    first.mat = static_cast<int32_t>(__float_as_uint(c.x));
+    first.light = static_cast<int32_t>(__float_as_uint(c.y));
+    valid = c.z != 0.0f;
+    return first;
+}
+
+// MIS tracer of the plain form with path regeneration: every loop iteration advances each lane's
+// path by one bounce; a lane whose path ends adds it to its pixel's sum and starts the pixel's
+// next sample, and a lane whose pixel is complete writes it and takes the next unclaimed pixel
+// from a global counter. Lanes therefore stay busy while paths of different lengths finish,
+// and each pixel's samples are still summed in order, as in the oracle.
+__global__ __launch_bounds__(kWG) void k_render_mis(RenderParams P, uint32_t *next_pixel) {
+    __shared__ uint2 lds[kLdsDepth * kWG];
+    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
+    Tracer<kWG, kLdsDepth> tr{P.S, st};
+    uint32_t npix = P.width * P.height;
+    uint32_t grid_threads = gridDim.x * kWG;
+    float scale = 1.0f / static_cast<float>(P.spp_total);
+    uint32_t pix = blockIdx.x * kWG + threadIdx.x;
+    if (pix < npix && P.s_begin < P.s_end) {
+        f3 o, v;
+        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        bool valid;
+        HitRec first = load_first(P, pix, valid);
+        f3 acc = mk(0.0f, 0.0f, 0.0f);
+        if (P.s_begin > 0) {
+            float4 q = P.acc[pix];
+            acc = mk(q.x, q.y, q.z);
+        }
+        uint32_t s = P.s_begin;
+        PathState ps;
+        tr.mis_begin(ps, first, valid, v, pcg_for(P.seed, pix, s), P.max_depth);
+        for (;;) {
+            if (ps.alive)
+                tr.mis_bounce(ps, P.max_depth);
+            if (!ps.alive) {
+                acc = acc + ps.L;
+                if (++s == P.s_end) {
+                    if (P.finalize)
+                        P.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
+                    else
+                        P.acc[pix] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+                    pix = grid_threads + atomicAdd(next_pixel, 1u);
+                    if (pix >= npix)
+                        break;
+                    camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+                    first = load_first(P, pix, valid);
+                    acc = mk(0.0f, 0.0f, 0.0f);
+                    if (P.s_begin > 0) {
+                        float4 q = P.acc[pix];
+                        acc = mk(q.x, q.y, q.z);
+                    }
+                    s = P.s_begin;
+                }
+                tr.mis_begin(ps, first, valid, v, pcg_for(P.seed, pix, s), P.max_depth);
+            }
+        }
+    }
+    if (P.cnt) {
+        flush_counter(&P.cnt->closest, tr.nclosest);
+        flush_counter(&P.cnt->anyhit, tr.nany);
+        flush_counter(&P.cnt->pops, tr.pops);
+        flush_counter(&P.cnt->tests, tr.tests);
+        flush_counter(&P.cnt->spops, tr.spops);
+        flush_counter(&P.cnt->stests, tr.stests);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------------------------
@@ -499,7 +574,8 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     uint64_t npix = static_cast<uint64_t>(width) * height;
     X3D_HIP(r->first.ensure(npix * 3 * sizeof(float4)));
     X3D_HIP(r->acc.ensure(npix * sizeof(float4)));
-    X3D_HIP(r->counters.ensure(2 * sizeof(Counters)));
+    // counters: [0] first-hit pass, [1] sample kernels, then the pixel work counter
+    X3D_HIP(r->counters.ensure(2 * sizeof(Counters) + 64));
     P.first = static_cast<float4 *>(r->first.p);
     P.acc = static_cast<float4 *>(r->acc.p);
     P.out = reinterpret_cast<float4 *>(d_out);
@@ -508,7 +584,7 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     uint32_t grid_first = persistent_grid(k_first_hit, npix);
     uint32_t grid_render = 1;
     switch (tracer) {
-    case X3D_PT_UNIDIRECT_MIS: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT_MIS>, npix); break;
+    case X3D_PT_UNIDIRECT_MIS: grid_render = persistent_grid(k_render_mis, npix); break;
     case X3D_PT_UNIDIRECT_LT1: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT_LT1>, npix); break;
     case X3D_PT_UNIDIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT>, npix); break;
     case X3D_PT_DIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_DIRECT>, npix); break;
@@ -517,10 +593,10 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     uint32_t grid = grid_first > grid_render ? grid_first : grid_render;
     X3D_HIP(r->spill.ensure(static_cast<size_t>(grid) * kWG * P.spill_per_thread * sizeof(uint2)));
     P.spill = static_cast<uint2 *>(r->spill.p);
-    // Counter slots: [0] first-hit pass, [1] sample kernels; cleared by a kernel, not a memset,
-    // so the sequence stays capturable in a graph.
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream,
-                       reinterpret_cast<unsigned long long *>(P.cnt), static_cast<uint32_t>(2 * sizeof(Counters) / 8));
+    // Counters are cleared by a kernel, not a memset, so the sequence stays capturable in a graph.
+    uint32_t *next_pixel = reinterpret_cast<uint32_t *>(P.cnt + 2);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream, reinterpret_cast<unsigned long long *>(P.cnt),
+                       static_cast<uint32_t>((2 * sizeof(Counters) + 64) / 8));
     X3D_HIP(hipEventRecord(r->ev0, r->stream));
     hipLaunchKernelGGL(k_first_hit, dim3(grid_first), dim3(kWG), 0, r->stream, P);
     X3D_HIP(hipGetLastError());
@@ -530,7 +606,9 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     P.s_end = spp;
     P.finalize = 1;
     switch (tracer) {
-    case X3D_PT_UNIDIRECT_MIS: launch_plain<X3D_PT_UNIDIRECT_MIS>(grid_render, r->stream, P); break;
+    case X3D_PT_UNIDIRECT_MIS:
+        hipLaunchKernelGGL(k_render_mis, dim3(grid_render), dim3(kWG), 0, r->stream, P, next_pixel);
+        break;
     case X3D_PT_UNIDIRECT_LT1: launch_plain<X3D_PT_UNIDIRECT_LT1>(grid_render, r->stream, P); break;
     case X3D_PT_UNIDIRECT: launch_plain<X3D_PT_UNIDIRECT>(grid_render, r->stream, P); break;
     case X3D_PT_DIRECT: launch_plain<X3D_PT_DIRECT>(grid_render, r->stream, P); break;

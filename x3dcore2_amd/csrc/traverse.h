@@ -6,13 +6,19 @@
 // LDS, entry-major ([entry][lane]) so that lanes of a wave hit distinct banks; deeper entries
 // spill to a per-thread slice of global memory sized from the tree depth.
 //
+// Loop structure (while-while with postponed leaves, Aila & Laine, HPG 2009): a lane that reaches
+// a leaf parks it and keeps descending until every active lane of the wave holds a leaf; then
+// all lanes test their leaves' triangles together, so the expensive triangle tests run with the
+// wave's lanes converged.
+//
 // Closest hit returns exactly the reference's winner (bvh_path_space_layout::intersect,
 // pathspace.cpp:330-413): the reference visits leaves in decreasing flattened order and keeps
 // the first of equal-distance hits, i.e. the hit in the leaf with the largest prim_start, then
-// the lowest primitive inside that leaf. That rule is applied explicitly, so the near-first order
-// and the distance cull (boxes starting beyond best * (1 + 2^-12) are skipped; a relative slack
-// keeps a hit whose FP32 distance falls slightly below its box's entry distance) do not change
-// the result. The reference order (right child first, no culling) is available for checking.
+// the lowest primitive inside that leaf. That rule is applied explicitly, so the visiting order
+// and the distance cull (boxes starting beyond best * (1 + 2^-12) are skipped; the relative
+// slack keeps a hit whose FP32 distance falls slightly below its box's entry distance) do not
+// change the result. The reference order (right child first, no culling) is available for
+// checking.
 #pragma once
 
 #include "device_math.h"
@@ -23,6 +29,8 @@ struct ClosestHit {
     int32_t prim;
     float t, b1, b2;
 };
+
+constexpr uint32_t kDone = 0xFFFFFFFFu; // no node left (same bits as kEmptyRef)
 
 template <int WG, int LDS_DEPTH> struct TravStack {
     uint2 *lds;   // this lane's column of the workgroup's [LDS_DEPTH][WG] array
@@ -40,9 +48,19 @@ template <int WG, int LDS_DEPTH> struct TravStack {
         sp--;
         return sp < LDS_DEPTH ? lds[sp * WG] : spill[sp - LDS_DEPTH];
     }
+    // Next stacked node whose entry distance does not exceed `cut`, or kDone.
+    X3D_DI uint32_t pop_next(float cut) {
+        while (sp > 0) {
+            uint2 e = pop();
+            if (!(__uint_as_float(e.y) > cut))
+                return e.x;
+        }
+        return kDone;
+    }
 };
 
-X3D_DI bool is_leaf(uint32_t ref) { return (ref & kLeafBit) != 0u; }
+X3D_DI bool is_interior(uint32_t ref) { return ref < kLeafBit; }
+X3D_DI bool is_leaf_ref(uint32_t ref) { return ref >= kLeafBit && ref != kDone; }
 X3D_DI uint32_t leaf_start(uint32_t ref) { return ref & 0x0FFFFFFFu; }
 X3D_DI uint32_t leaf_count(uint32_t ref) { return ((ref >> 28) & 3u) + 1u; }
 
@@ -52,6 +70,38 @@ X3D_DI void load_node(DevScene const &S, uint32_t idx, float4 &f0, float4 &f1, f
     f1 = n[1];
     f2 = n[2];
     f3 = n[3];
+}
+
+// Visits interior node `ref`: tests both child boxes against [tmin, tmax], drops children
+// entering beyond `cut`, pushes the farther of two hit children and returns the one to descend
+// into (or the next stacked node, or kDone). REF_ORDER pushes left, descends right
+// (pathspace.cpp:371-381 with its LIFO stack).
+template <bool REF_ORDER, int WG, int LDS_DEPTH>
+X3D_DI uint32_t visit_interior(DevScene const &S, uint32_t ref, f3 o, f3 vinv, float tmin, float tmax, float cut,
+                               TravStack<WG, LDS_DEPTH> &st) {
+    float4 f0, f1, f2, f3n;
+    load_node(S, ref, f0, f1, f2, f3n);
+    float tl = 0.0f, tr = 0.0f;
+    bool hl = slab(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, tmin, tmax, &tl);
+    bool hr = slab(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, tmin, tmax, &tr);
+    uint32_t rl = __float_as_uint(f3n.x), rr = __float_as_uint(f3n.y);
+    if (!REF_ORDER) {
+        hl = hl && !(tl > cut);
+        hr = hr && !(tr > cut);
+    }
+    if (hl && hr) {
+        if (REF_ORDER || tr < tl) {
+            st.push(rl, tl);
+            return rr;
+        }
+        st.push(rr, tr);
+        return rl;
+    }
+    if (hl)
+        return rl;
+    if (hr)
+        return rr;
+    return st.pop_next(REF_ORDER ? __builtin_inff() : cut);
 }
 
 // REF_ORDER: the reference's traversal order without culling.
@@ -64,11 +114,28 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
     const float4 *tris = static_cast<const float4 *>(S.tris);
     uint32_t best_leaf = 0;
-    uint32_t ref = S.root_ref;
+    uint32_t ref = S.root_ref, leaf = kDone;
+    if (!is_interior(ref)) {
+        leaf = ref;
+        ref = kDone;
+    }
     st.sp = 0;
     for (;;) {
-        if (is_leaf(ref)) {
-            uint32_t s0 = leaf_start(ref), cnt = leaf_count(ref);
+        // Descend; park the first leaf met and keep going until all active lanes hold one.
+        while (is_interior(ref)) {
+            pops++;
+            float cut = REF_ORDER ? __builtin_inff() : h.t + h.t * kCullSlack;
+            ref = visit_interior<REF_ORDER>(S, ref, o, vinv, kTMin, kTMax, cut, st);
+            if (is_leaf_ref(ref) && leaf == kDone) {
+                leaf = ref;
+                ref = st.pop_next(cut);
+            }
+            if (__all(leaf != kDone))
+                break;
+        }
+        // Test the parked leaf, then any leaves that follow on the stack.
+        while (leaf != kDone) {
+            uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
             for (uint32_t i = s0; i < s0 + cnt; i++) {
                 float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
                 float c1, c2, t0;
@@ -82,58 +149,14 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
                     best_leaf = s0;
                 }
             }
-        } else {
-            pops++;
-            float4 f0, f1, f2, f3n;
-            load_node(S, ref, f0, f1, f2, f3n);
-            float tl = 0.0f, tr = 0.0f;
-            bool hl = slab(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, kTMin, kTMax, &tl);
-            bool hr = slab(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, kTMin, kTMax, &tr);
-            uint32_t rl = __float_as_uint(f3n.x), rr = __float_as_uint(f3n.y);
-            if (!REF_ORDER) {
-                float cut = h.t + h.t * kCullSlack;
-                hl = hl && !(tl > cut);
-                hr = hr && !(tr > cut);
-            }
-            if (hl && hr) {
-                if (REF_ORDER || !(tr < tl)) {
-                    // reference order: right child first; culled order: nearer first
-                    if (REF_ORDER) {
-                        st.push(rl, tl);
-                        ref = rr;
-                    } else {
-                        st.push(rr, tr);
-                        ref = rl;
-                    }
-                } else {
-                    st.push(rl, tl);
-                    ref = rr;
-                }
-                continue;
-            }
-            if (hl) {
-                ref = rl;
-                continue;
-            }
-            if (hr) {
-                ref = rr;
-                continue;
+            if (is_leaf_ref(ref)) {
+                leaf = ref;
+                ref = st.pop_next(REF_ORDER ? __builtin_inff() : h.t + h.t * kCullSlack);
+            } else {
+                leaf = kDone;
             }
         }
-        // pop the next candidate
-        bool found = false;
-        while (st.sp > 0) {
-            uint2 e = st.pop();
-            if (!REF_ORDER) {
-                float te = __uint_as_float(e.y);
-                if (te > h.t + h.t * kCullSlack)
-                    continue;
-            }
-            ref = e.x;
-            found = true;
-            break;
-        }
-        if (!found)
+        if (ref == kDone)
             break;
     }
     return h;
@@ -148,11 +171,25 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
         return false;
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
     const float4 *tris = static_cast<const float4 *>(S.tris);
-    uint32_t ref = S.root_ref;
+    uint32_t ref = S.root_ref, leaf = kDone;
+    if (!is_interior(ref)) {
+        leaf = ref;
+        ref = kDone;
+    }
     st.sp = 0;
     for (;;) {
-        if (is_leaf(ref)) {
-            uint32_t s0 = leaf_start(ref), cnt = leaf_count(ref);
+        while (is_interior(ref)) {
+            pops++;
+            ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st);
+            if (is_leaf_ref(ref) && leaf == kDone) {
+                leaf = ref;
+                ref = st.pop_next(__builtin_inff());
+            }
+            if (__all(leaf != kDone))
+                break;
+        }
+        while (leaf != kDone) {
+            uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
             for (uint32_t i = s0; i < s0 + cnt; i++) {
                 float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
                 float c1, c2, t0;
@@ -160,36 +197,15 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
                 if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), tmin, tmax, &c1, &c2, &t0))
                     return true;
             }
-        } else {
-            pops++;
-            float4 f0, f1, f2, f3n;
-            load_node(S, ref, f0, f1, f2, f3n);
-            float tl = 0.0f, tr = 0.0f;
-            bool hl = slab(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, tmin, tmax, &tl);
-            bool hr = slab(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, tmin, tmax, &tr);
-            uint32_t rl = __float_as_uint(f3n.x), rr = __float_as_uint(f3n.y);
-            if (hl && hr) {
-                if (tr < tl) {
-                    st.push(rl, tl);
-                    ref = rr;
-                } else {
-                    st.push(rr, tr);
-                    ref = rl;
-                }
-                continue;
-            }
-            if (hl) {
-                ref = rl;
-                continue;
-            }
-            if (hr) {
-                ref = rr;
-                continue;
+            if (is_leaf_ref(ref)) {
+                leaf = ref;
+                ref = st.pop_next(__builtin_inff());
+            } else {
+                leaf = kDone;
             }
         }
-        if (st.sp == 0)
+        if (ref == kDone)
             break;
-        ref = st.pop().x;
     }
     return false;
 }

@@ -103,8 +103,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes": args.copy_calibration}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the path tracer runs on a single MI355X in this build "
-              "(multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print("bench.py measures one GPU only: the renderer runs on a single MI355X "
+              "(sample-split sharding across GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import x3dcore2_amd as x3d

@@ -179,9 +179,16 @@ typedef struct x3d_render_options {
     uint32_t max_depth;    /* UNIDIRECT_MIS: bounces after the first hit */
     uint32_t max_path_len; /* BIDIRECT_MIS: vertices per subpath (reference: 8) */
     uint64_t seed;         /* PCG seed; sample s of pixel p uses stream (seed, p, s) */
-    uint32_t workgroup_size; /* 0 = library default / autotuned */
-    uint32_t reserved[7];
+    uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default */
+    uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default (4M) */
+    int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
+    uint32_t reserved[5];
 } x3d_render_options;
+
+/* Wavefront sort keys: group active paths by material kind (default) or keep append order. */
+#define X3D_SORT_DEFAULT 0
+#define X3D_SORT_MATERIAL 1
+#define X3D_SORT_NONE 2
 
 /* Fills *opts with the defaults for pt_type. */
 int x3d_render_options_default(int32_t pt_type, x3d_render_options *opts);

@@ -167,3 +167,26 @@ def test_render_deterministic_and_shape_independent(x3d):
     a, _ = r.render(gs, desc.camera, 96, 64, 8)
     b, _ = r.render(gs, desc.camera, 96, 64, 8)
     assert np.array_equal(_bits(a), _bits(b))
+
+
+WAVEFRONT_CASES = [
+    dict(),                                      # defaults: one generation, material sort, graph
+    dict(workgroup_size=64, queue_capacity=3000),   # several generations, partial last one
+    dict(workgroup_size=128, sort_key=2),           # no material sort
+]
+
+
+@pytest.mark.parametrize("kw", WAVEFRONT_CASES)
+@pytest.mark.parametrize("scene_name", ["box", "spheres_small"])
+def test_wavefront_film_identical_to_plain(x3d, scene_name, kw):
+    desc = QUERY_SCENES[scene_name]()
+    print(f"[gpu] wavefront {scene_name} {kw}", flush=True)
+    gs = x3d.Scene(desc)
+    w, h, spp = 48, 40, 5
+    plain, sp = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=17).render(gs, desc.camera, w, h, spp)
+    wf = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=6, seed=17, **kw)
+    for _ in range(2):  # second call replays the cached graph
+        img, st = wf.render(gs, desc.camera, w, h, spp)
+        assert np.array_equal(_bits(img), _bits(plain)), np.abs(img - plain).max()
+        assert st.closest_queries == sp.closest_queries
+        assert st.anyhit_queries == sp.anyhit_queries

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libx3dpt.so")
 PT_NORMAL, PT_POSITION, PT_DIRECT, PT_UNIDIRECT, PT_UNIDIRECT_LT1, PT_BIDIRECT_LT2, PT_BIDIRECT_MIS = range(7)
 PT_UNIDIRECT_MIS = 16
 FORM_PLAIN, FORM_WAVEFRONT = 0, 1
+SORT_DEFAULT, SORT_MATERIAL, SORT_NONE = 0, 1, 2
 TRAVERSAL_CULLED, TRAVERSAL_REFERENCE = 0, 1
 
 
@@ -63,7 +64,7 @@ class CameraDesc(C.Structure):
 class RenderOptions(C.Structure):
     _fields_ = [("pt_type", C.c_int32), ("form", C.c_int32), ("max_depth", C.c_uint32),
                 ("max_path_len", C.c_uint32), ("seed", C.c_uint64), ("workgroup_size", C.c_uint32),
-                ("reserved", C.c_uint32 * 7)]
+                ("queue_capacity", C.c_uint32), ("sort_key", C.c_int32), ("reserved", C.c_uint32 * 5)]
 
 
 class RenderStats(C.Structure):
@@ -255,11 +256,13 @@ class Scene:
 class Renderer:
     """pt_image_renderer equivalent (x3d_renderer)."""
 
-    def __init__(self, pt_type=PT_UNIDIRECT_MIS, form=FORM_PLAIN, max_depth=5, max_path_len=8, seed=1):
+    def __init__(self, pt_type=PT_UNIDIRECT_MIS, form=FORM_PLAIN, max_depth=5, max_path_len=8, seed=1,
+                 workgroup_size=0, queue_capacity=0, sort_key=0):
         L = lib()
         o = RenderOptions()
         _check(L.x3d_render_options_default(pt_type, C.byref(o)))
         o.form, o.max_depth, o.max_path_len, o.seed = form, max_depth, max_path_len, seed
+        o.workgroup_size, o.queue_capacity, o.sort_key = workgroup_size, queue_capacity, sort_key
         h = C.c_void_p()
         _check(L.x3d_renderer_create(C.byref(o), C.byref(h)))
         self.h = h

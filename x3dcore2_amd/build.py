@@ -41,11 +41,14 @@ def build_library(verbose: bool = True) -> str:
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     gxx = shutil.which("g++") or "g++"
     scene_o = os.path.join(BUILD, "scene.o")
-    render_o = os.path.join(BUILD, "render.o")
     _run([gxx, *GXX_FLAGS, "-c", os.path.join(CSRC, "scene.cpp"), "-o", scene_o])
-    _run([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, "render.hip"), "-o", render_o])
+    objs = [scene_o]
+    for unit in ("render", "wavefront"):
+        o = os.path.join(BUILD, unit + ".o")
+        _run([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, unit + ".hip"), "-o", o])
+        objs.append(o)
     tmp = LIB + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", scene_o, render_o, "-o", tmp,
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
           f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
     return LIB

@@ -17,60 +17,11 @@
 #include <vector>
 
 #include "../../include/x3dpt.h"
-#include "integrators.h"
+#include "render_common.h"
 #include "scene.h"
+#include "wavefront.h"
 
 namespace x3d {
-
-constexpr int kWG = 256;       // lanes per workgroup (4 waves)
-constexpr int kLdsDepth = 16;  // LDS stack entries per lane (16 x 8 B x 256 = 32 KiB per WG)
-
-struct Counters { // device-side totals (uint64)
-    unsigned long long closest, anyhit, pops, tests, spops, stests;
-};
-
-struct CamParams {
-    float sensor, focal, aspect;
-    float m[16];
-};
-
-// pinhole_camera::sample (camera.cpp:38-48) after pinhole_camera::transform (camera.cpp:63-71).
-X3D_DI void camera_ray(CamParams const &c, uint32_t x, uint32_t y, uint32_t w, uint32_t h, f3 &o, f3 &v) {
-    float s = c.sensor;
-    float dx = (static_cast<float>(x) / static_cast<float>(w - 1) * s - s / 2.0f) * c.aspect;
-    float dy = (static_cast<float>(h - 1) - static_cast<float>(y)) / static_cast<float>(h - 1) * s - s / 2.0f;
-    float dz = -c.focal;
-    float r[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        float a = 0.0f;
-        a = a + c.m[0 * 4 + i] * dx;
-        a = a + c.m[1 * 4 + i] * dy;
-        a = a + c.m[2 * 4 + i] * dz;
-        a = a + 0.0f * 0.0f;
-        r[i] = a;
-    }
-    o = mk(c.m[12], c.m[13], c.m[14]);
-    v = normalize(mk(r[0], r[1], r[2]));
-}
-
-// Wave-level sum then one atomic per wave.
-X3D_DI void flush_counter(unsigned long long *dst, uint32_t v) {
-    unsigned long long x = v;
-    for (int off = 32; off > 0; off >>= 1)
-        x += __shfl_xor(x, off, 64);
-    if ((threadIdx.x & 63) == 0 && x != 0)
-        atomicAdd(dst, x);
-}
-
-template <int WG> X3D_DI TravStack<WG, kLdsDepth> make_stack(uint2 *lds, uint2 *spill, uint32_t spill_per_thread) {
-    TravStack<WG, kLdsDepth> st;
-    st.lds = lds + threadIdx.x;
-    uint32_t gtid = blockIdx.x * WG + threadIdx.x;
-    st.spill = spill + static_cast<size_t>(gtid) * spill_per_thread;
-    st.sp = 0;
-    return st;
-}
 
 // ---------------------------------------------------------------------------------------------
 // Ray queries.
@@ -214,22 +165,6 @@ __global__ void k_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint
 // ---------------------------------------------------------------------------------------------
 // Rendering.
 // ---------------------------------------------------------------------------------------------
-struct RenderParams {
-    DevScene S;
-    CamParams cam;
-    uint32_t width, height;
-    uint32_t s_begin, s_end, spp_total;
-    uint64_t seed;
-    uint32_t max_depth, max_path_len;
-    float4 *first; // 3 x float4 per pixel: (p, t), (n, geo), (mat, light, valid, -)
-    float4 *acc;   // running rgb sum per pixel
-    float4 *out;   // final rgba per pixel (written when finalize != 0)
-    uint2 *spill;
-    uint32_t spill_per_thread;
-    int32_t finalize;
-    Counters *cnt;
-};
-
 __global__ __launch_bounds__(kWG) void k_first_hit(RenderParams P) {
     __shared__ uint2 lds[kLdsDepth * kWG];
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
@@ -313,19 +248,6 @@ template <int TRACER> __global__ __launch_bounds__(kWG) void k_render_plain(Rend
         flush_counter(&P.cnt->spops, tr.spops);
         flush_counter(&P.cnt->stests, tr.stests);
     }
-}
-
-X3D_DI HitRec load_first(RenderParams const &P, uint32_t pix, bool &valid) {
-    float4 a = P.first[3 * pix], b = P.first[3 * pix + 1], c = P.first[3 * pix + 2];
-    HitRec first;
-    first.p = mk(a.x, a.y, a.z);
-    first.t = a.w;
-    first.n = mk(b.x, b.y, b.z);
-    first.geo = static_cast<int32_t>(__float_as_uint(b.w));
-    first.mat = static_cast<int32_t>(__float_as_uint(c.x));
-    first.light = static_cast<int32_t>(__float_as_uint(c.y));
-    valid = c.z != 0.0f;
-    return first;
 }
 
 // MIS tracer of the plain form with path regeneration: every loop iteration advances each lane's
@@ -476,9 +398,7 @@ struct DevBuf {
     }
 };
 
-uint32_t spill_entries(DevScene const &S) {
-    return S.max_stack > static_cast<uint32_t>(kLdsDepth) ? S.max_stack - kLdsDepth + 1 : 1;
-}
+uint32_t spill_entries(DevScene const &S) { return spill_entries_for(S, kLdsDepth); }
 
 } // namespace x3d
 
@@ -491,7 +411,10 @@ struct x3d_renderer {
     x3d::DevBuf first, acc, out, spill, counters;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
+    x3d::WavefrontEngine *wf = nullptr;
     ~x3d_renderer() {
+        if (wf)
+            x3d::wf_destroy(wf);
         if (evm)
             (void)hipEventDestroy(evm);
         if (ev0)
@@ -605,7 +528,20 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     P.s_begin = 0;
     P.s_end = spp;
     P.finalize = 1;
+    if (r->wf) {
+        x3d::WavefrontTuning tune;
+        if (r->opt.workgroup_size)
+            tune.trace_wg = static_cast<int>(r->opt.workgroup_size);
+        if (r->opt.queue_capacity)
+            tune.paths_in_flight = r->opt.queue_capacity;
+        tune.sort_by_material = r->opt.sort_key != X3D_SORT_NONE;
+        std::string e = x3d::wf_render(r->wf, P, tune, r->stream);
+        if (!e.empty())
+            return fail(X3D_ERR_DEVICE, e);
+        tracer = -1; // launched
+    }
     switch (tracer) {
+    case -1: break;
     case X3D_PT_UNIDIRECT_MIS:
         hipLaunchKernelGGL(k_render_mis, dim3(grid_render), dim3(kWG), 0, r->stream, P, next_pixel);
         break;
@@ -1041,8 +977,15 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
     default:
         return fail(X3D_ERR_UNSUPPORTED, "tracer type " + std::to_string(opts->pt_type) + " is not implemented");
     }
-    if (opts->form != X3D_FORM_PLAIN)
-        return fail(X3D_ERR_UNSUPPORTED, "render form " + std::to_string(opts->form) + " is not implemented");
+    if (opts->form != X3D_FORM_PLAIN && opts->form != X3D_FORM_WAVEFRONT)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "unknown render form " + std::to_string(opts->form));
+    if (opts->form == X3D_FORM_WAVEFRONT && opts->pt_type != X3D_PT_UNIDIRECT_MIS)
+        return fail(X3D_ERR_UNSUPPORTED, "the wavefront form runs the UNIDIRECT_MIS tracer only");
+    if (opts->workgroup_size != 0 && opts->workgroup_size != 64 && opts->workgroup_size != 128 &&
+        opts->workgroup_size != 256)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "workgroup_size must be 0, 64, 128 or 256");
+    if (opts->sort_key < X3D_SORT_DEFAULT || opts->sort_key > X3D_SORT_NONE)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "unknown sort key");
     int rc = require_device();
     if (rc)
         return rc;
@@ -1054,6 +997,8 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
     X3D_HIP(hipEventCreate(&r->ev0));
     X3D_HIP(hipEventCreate(&r->ev1));
     X3D_HIP(hipEventCreate(&r->evm));
+    if (opts->form == X3D_FORM_WAVEFRONT)
+        r->wf = x3d::wf_create();
     *out = r.release();
     return X3D_OK;
 }

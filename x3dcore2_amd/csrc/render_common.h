@@ -1,0 +1,96 @@
+// Pieces shared by the plain and wavefront render kernels: camera rays, counters, traversal
+// stacks and the per-render parameter block.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "integrators.h"
+
+namespace x3d {
+
+constexpr int kWG = 256;      // lanes per workgroup (4 waves)
+constexpr int kLdsDepth = 16; // LDS stack entries per lane (16 x 8 B x 256 = 32 KiB per WG)
+
+struct Counters { // device-side totals (uint64)
+    unsigned long long closest, anyhit, pops, tests, spops, stests;
+};
+
+struct CamParams {
+    float sensor, focal, aspect;
+    float m[16];
+};
+
+// pinhole_camera::sample (camera.cpp:38-48) after pinhole_camera::transform (camera.cpp:63-71).
+X3D_DI void camera_ray(CamParams const &c, uint32_t x, uint32_t y, uint32_t w, uint32_t h, f3 &o, f3 &v) {
+    float s = c.sensor;
+    float dx = (static_cast<float>(x) / static_cast<float>(w - 1) * s - s / 2.0f) * c.aspect;
+    float dy = (static_cast<float>(h - 1) - static_cast<float>(y)) / static_cast<float>(h - 1) * s - s / 2.0f;
+    float dz = -c.focal;
+    float r[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float a = 0.0f;
+        a = a + c.m[0 * 4 + i] * dx;
+        a = a + c.m[1 * 4 + i] * dy;
+        a = a + c.m[2 * 4 + i] * dz;
+        a = a + 0.0f * 0.0f;
+        r[i] = a;
+    }
+    o = mk(c.m[12], c.m[13], c.m[14]);
+    v = normalize(mk(r[0], r[1], r[2]));
+}
+
+// Wave-level sum then one atomic per wave (all lanes of the wave must be converged).
+X3D_DI void flush_counter(unsigned long long *dst, uint32_t v) {
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1)
+        x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0 && x != 0)
+        atomicAdd(dst, x);
+}
+
+template <int WG, int LDSD = kLdsDepth>
+X3D_DI TravStack<WG, LDSD> make_stack(uint2 *lds, uint2 *spill, uint32_t spill_per_thread) {
+    TravStack<WG, LDSD> st;
+    st.lds = lds + threadIdx.x;
+    uint32_t gtid = blockIdx.x * WG + threadIdx.x;
+    st.spill = spill + static_cast<size_t>(gtid) * spill_per_thread;
+    st.sp = 0;
+    return st;
+}
+
+// Stack entries spilled to global memory per lane for a tree of the scene's depth.
+inline uint32_t spill_entries_for(DevScene const &S, int lds_depth) {
+    return S.max_stack > static_cast<uint32_t>(lds_depth) ? S.max_stack - lds_depth + 1 : 1;
+}
+
+struct RenderParams {
+    DevScene S;
+    CamParams cam;
+    uint32_t width, height;
+    uint32_t s_begin, s_end, spp_total;
+    uint64_t seed;
+    uint32_t max_depth, max_path_len;
+    float4 *first; // 3 x float4 per pixel: (p, t), (n, geo), (mat, light, valid, -)
+    float4 *acc;   // running rgb sum per pixel
+    float4 *out;   // final rgba per pixel (written when finalize != 0)
+    uint2 *spill;
+    uint32_t spill_per_thread;
+    int32_t finalize;
+    Counters *cnt;
+};
+
+X3D_DI HitRec load_first(RenderParams const &P, uint32_t pix, bool &valid) {
+    float4 a = P.first[3 * pix], b = P.first[3 * pix + 1], c = P.first[3 * pix + 2];
+    HitRec first;
+    first.p = mk(a.x, a.y, a.z);
+    first.t = a.w;
+    first.n = mk(b.x, b.y, b.z);
+    first.geo = static_cast<int32_t>(__float_as_uint(b.w));
+    first.mat = static_cast<int32_t>(__float_as_uint(c.x));
+    first.light = static_cast<int32_t>(__float_as_uint(c.y));
+    valid = c.z != 0.0f;
+    return first;
+}
+
+} // namespace x3d

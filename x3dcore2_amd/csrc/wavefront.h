@@ -1,0 +1,29 @@
+// Wavefront form of the MIS tracer (host interface).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "render_common.h"
+
+namespace x3d {
+
+struct WavefrontEngine;
+
+struct WavefrontTuning {
+    uint32_t paths_in_flight = 1u << 22; // queue capacity (paths of one generation)
+    int trace_wg = 256;                  // workgroup size of the traversal kernels (64, 128, 256)
+    int sort_by_material = 1;            // group active paths by material kind before shading
+    int use_graph = 1;                   // replay one generation's launches as a hipGraph
+};
+
+WavefrontEngine *wf_create();
+void wf_destroy(WavefrontEngine *e);
+
+// Runs the sample part of a render whose first-hit pass has already filled P.first:
+// samples [P.s_begin, P.s_end) of every pixel, in generations of `paths_in_flight / pixels`
+// samples; writes P.out (P.finalize) or P.acc. Returns an empty string or an error message.
+std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, hipStream_t stream);
+
+} // namespace x3d

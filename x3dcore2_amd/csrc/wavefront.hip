@@ -1,0 +1,656 @@
+// Wavefront form of the fixed-depth MIS tracer.
+//
+// Paths live in HBM as structure-of-arrays state; each bounce runs as separate kernels:
+//   reset      clear the queue counters of this bounce
+//   nee_bsdf   per active path: light sample (shadow ray + MIS-weighted contribution queued),
+//              BSDF sample (extension ray queued, or the path finishes)
+//   shadow     persistent any-hit traversal over the shadow queue; adds unoccluded contributions
+//   extend     persistent closest-hit traversal over the extension queue
+//   shade      per extension result: throughput update, MIS-weighted emission, next vertex;
+//              surviving paths are appended to per-material-kind active lists (material sort)
+// The traversal kernels keep every lane busy: a lane whose ray finishes writes its result and
+// takes the next ray (one atomic per wave for all lanes that need one) without waiting for the
+// rest of its wave. All paths of a generation (a fixed number of samples for every pixel) start
+// together, so they all sit at the same depth; one generation's launches are captured once as
+// a hipGraph and replayed. Each path performs exactly the operations of the plain kernel in the
+// same order (same PCG stream, same float expressions), and each pixel's samples are summed in
+// sample order after their generation, so the film is bit-identical to the plain form's.
+
+#include "wavefront.h"
+
+#include <cstring>
+#include <vector>
+
+namespace x3d {
+
+constexpr int kKinds = 3; // diffuse (Oren-Nayar / fail-safe), GGX, mixture
+
+struct WfCtl {
+    uint32_t act_count[2][4];
+    uint32_t sq_count, sq_head, eq_count, eq_head, fin_count;
+    uint32_t gen;
+    uint32_t pad[2];
+};
+
+struct WfArgs {
+    DevScene S;
+    CamParams cam;
+    uint32_t width, height, npix;
+    uint32_t s_begin, s_end, spp_total, gen_samples, capacity;
+    uint64_t seed;
+    uint32_t max_depth;
+    int32_t finalize;
+    float4 *first, *acc, *out;
+    // path state, index = path id = s_local * npix + pixel
+    float4 *xp, *xn, *wo, *beta, *L, *wi;
+    uint64_t *rng;
+    float4 *lbuf;
+    // queues
+    uint32_t *act[2];     // [kind * capacity + j]
+    float4 *sq;           // 3 x float4 per shadow ray: (o, tmax), (d, path), (contribution, -)
+    uint32_t *eq, *fin;   // extension / finishing path ids
+    ClosestHit *eh;       // extension results
+    WfCtl *ctl;
+    uint2 *spill;
+    uint32_t spill_per_thread;
+    Counters *cnt;
+    int sort_by_material;
+};
+
+X3D_DI int kind_of(DevScene const &S, int32_t mat) {
+    int32_t k = S.mats[mat].kind;
+    return k == kMatCookTorr ? 1 : (k == kMatMixture ? 2 : 0);
+}
+
+X3D_DI void push_active(WfArgs const &A, int parity, int kind, uint32_t path) {
+    int k = A.sort_by_material ? kind : 0;
+    uint32_t j = atomicAdd(&A.ctl->act_count[parity][k], 1u);
+    A.act[parity][static_cast<size_t>(k) * A.capacity + j] = path;
+}
+
+// Lanes of the calling (possibly partial) wave take consecutive items with one atomic.
+X3D_DI uint32_t wave_fetch(uint32_t *head) {
+    unsigned long long mask = __ballot(1);
+    uint32_t lane = __lane_id();
+    uint32_t leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(head, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    return base + rank;
+}
+
+__global__ void k_wf_begin_render(WfCtl *ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        ctl->gen = 0;
+}
+
+__global__ void k_wf_next_gen(WfCtl *ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        ctl->gen++;
+}
+
+// parity < 0: start of a generation (clear both active lists).
+__global__ void k_wf_reset(WfCtl *ctl, int parity) {
+    if (blockIdx.x != 0)
+        return;
+    uint32_t t = threadIdx.x;
+    if (t < 4) {
+        if (parity < 0) {
+            ctl->act_count[0][t] = 0;
+            ctl->act_count[1][t] = 0;
+        } else {
+            ctl->act_count[parity ^ 1][t] = 0;
+        }
+    }
+    if (t == 0) {
+        ctl->sq_count = 0;
+        ctl->sq_head = 0;
+        ctl->eq_count = 0;
+        ctl->eq_head = 0;
+        ctl->fin_count = 0;
+    }
+}
+
+X3D_DI uint32_t gen_count(WfArgs const &A, uint32_t gen) {
+    uint32_t first = A.s_begin + gen * A.gen_samples;
+    uint32_t left = A.s_end > first ? A.s_end - first : 0;
+    return left < A.gen_samples ? left : A.gen_samples;
+}
+
+__global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
+    uint32_t gen = A.ctl->gen;
+    uint32_t n = gen_count(A, gen) * A.npix;
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) {
+        uint32_t s_local = id / A.npix, pix = id - s_local * A.npix;
+        uint32_t s = A.s_begin + gen * A.gen_samples + s_local;
+        f3 o, v;
+        camera_ray(A.cam, pix % A.width, pix / A.width, A.width, A.height, o, v);
+        RenderParams RP;
+        RP.first = A.first;
+        bool valid;
+        HitRec x = load_first(RP, pix, valid);
+        // Tracer::mis_begin
+        Pcg g = pcg_for(A.seed, pix, s);
+        f3 L = mk(0.0f, 0.0f, 0.0f);
+        f3 wo = -v;
+        if (valid && x.light >= 0)
+            L = L + light_radiance(A.S, x.light, wo, x.n);
+        if (!(valid && A.max_depth > 0)) {
+            A.lbuf[id] = make_float4(L.x, L.y, L.z, 0.0f);
+            continue;
+        }
+        A.xp[id] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
+        A.xn[id] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
+        A.wo[id] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
+        A.beta[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+        A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
+        A.rng[id] = g.state;
+        push_active(A, 0, kind_of(A.S, x.mat), id);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
+    uint32_t c0 = A.ctl->act_count[parity][0], c1 = A.ctl->act_count[parity][1],
+             c2 = A.ctl->act_count[parity][2];
+    uint32_t total = c0 + c1 + c2;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        uint32_t kind = t < c0 ? 0u : (t < c0 + c1 ? 1u : 2u);
+        uint32_t j = t - (kind == 0 ? 0u : (kind == 1 ? c0 : c0 + c1));
+        uint32_t path = A.act[parity][static_cast<size_t>(kind) * A.capacity + j];
+        float4 xp = A.xp[path], xn = A.xn[path], wo4 = A.wo[path], b4 = A.beta[path];
+        HitRec x;
+        x.p = xyz(xp);
+        x.n = xyz(xn);
+        x.mat = static_cast<int32_t>(__float_as_uint(xp.w));
+        x.light = static_cast<int32_t>(__float_as_uint(xn.w));
+        f3 wo = xyz(wo4), beta = xyz(b4);
+        uint32_t pix = __float_as_uint(wo4.w);
+        Pcg g;
+        g.state = A.rng[path];
+        g.inc = (static_cast<uint64_t>(pix) << 1u) | 1u;
+        // next-event estimation (Tracer::direct + the MIS weight of Tracer::mis_bounce)
+        if (A.S.num_lights > 0) {
+            LightSample ls = sample_light(A.S, g);
+            f3 l = x.p - ls.p;
+            float r2 = dot(l, l);
+            f3 in = l / __builtin_sqrtf(r2);
+            float co = dot(ls.n, in);
+            float ci = dot(x.n, -in);
+            if (co > 0 && ci > 0) {
+                f3 illum = ((light_rad(A.S, ls.light) * ci) * co) / r2;
+                if (!near0(illum)) {
+                    float dist = norm(l);
+                    f3 wl = -l / dist;
+                    f3 f = bsdf_eval(A.S.mats, x.mat, x.n, wo, wl);
+                    f3 d = mul(illum, f) / ls.area_dens;
+                    float cl = dot(ls.n, -wl);
+                    float pl = ls.area_dens * r2 / cl;
+                    float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
+                    float w = pl * pl / (pl * pl + pb * pb);
+                    f3 c = mul(beta, d * w);
+                    uint32_t q = atomicAdd(&A.ctl->sq_count, 1u);
+                    A.sq[3 * q] = make_float4(x.p.x, x.p.y, x.p.z, dist - 1e-3f);
+                    A.sq[3 * q + 1] = make_float4(wl.x, wl.y, wl.z, __uint_as_float(path));
+                    A.sq[3 * q + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+                }
+            }
+        }
+        float dens;
+        f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, false);
+        A.rng[path] = g.state;
+        if (!(dens > 0.0f)) {
+            uint32_t q = atomicAdd(&A.ctl->fin_count, 1u);
+            A.fin[q] = path;
+        } else {
+            A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
+            uint32_t q = atomicAdd(&A.ctl->eq_count, 1u);
+            A.eq[q] = path;
+        }
+    }
+}
+
+template <int WG>
+__global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
+    __shared__ uint2 lds[kLdsDepth * WG];
+    TravStack<WG, kLdsDepth> st = make_stack<WG>(lds, A.spill, A.spill_per_thread);
+    DevScene const &S = A.S;
+    const float4 *tris = static_cast<const float4 *>(S.tris);
+    const uint32_t n = A.ctl->sq_count;
+    uint32_t pops = 0, tests = 0;
+    uint32_t i = wave_fetch(&A.ctl->sq_head);
+    f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
+    float tmax = 0.0f;
+    uint32_t ref = kDone, leaf = kDone;
+    bool blocked = false;
+    auto init = [&](uint32_t k) {
+        float4 a = A.sq[3 * k], b = A.sq[3 * k + 1];
+        o = xyz(a);
+        tmax = a.w;
+        v = xyz(b);
+        vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
+        blocked = false;
+        st.sp = 0;
+        ref = S.root_ref;
+        leaf = kDone;
+        if (ref != kEmptyRef && !is_interior(ref)) {
+            leaf = ref;
+            ref = kDone;
+        }
+    };
+    if (i < n)
+        init(i);
+    while (i < n) {
+        while (is_interior(ref)) {
+            pops++;
+            ref = visit_interior<false>(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st);
+            if (is_leaf_ref(ref) && leaf == kDone) {
+                leaf = ref;
+                ref = st.pop_next(__builtin_inff());
+            }
+            if (__all(leaf != kDone))
+                break;
+        }
+        while (leaf != kDone) {
+            uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
+            for (uint32_t p = s0; p < s0 + cnt; p++) {
+                float4 a = tris[3 * p], b = tris[3 * p + 1], c = tris[3 * p + 2];
+                float c1, c2, t0;
+                tests++;
+                if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), 1e-4f, tmax, &c1, &c2, &t0)) {
+                    blocked = true;
+                    break;
+                }
+            }
+            if (blocked) {
+                leaf = kDone;
+                ref = kDone;
+            } else if (is_leaf_ref(ref)) {
+                leaf = ref;
+                ref = st.pop_next(__builtin_inff());
+            } else {
+                leaf = kDone;
+            }
+        }
+        if (ref == kDone) {
+            if (!blocked) {
+                uint32_t path = __float_as_uint(A.sq[3 * i + 1].w);
+                float4 c = A.sq[3 * i + 2];
+                float4 L = A.L[path];
+                f3 nl = xyz(L) + xyz(c);
+                A.L[path] = make_float4(nl.x, nl.y, nl.z, L.w);
+            }
+            i = wave_fetch(&A.ctl->sq_head);
+            if (i < n)
+                init(i);
+        }
+    }
+    if (A.cnt) {
+        flush_counter(&A.cnt->spops, pops);
+        flush_counter(&A.cnt->stests, tests);
+    }
+}
+
+template <int WG>
+__global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
+    __shared__ uint2 lds[kLdsDepth * WG];
+    TravStack<WG, kLdsDepth> st = make_stack<WG>(lds, A.spill, A.spill_per_thread);
+    DevScene const &S = A.S;
+    const float4 *tris = static_cast<const float4 *>(S.tris);
+    const uint32_t n = A.ctl->eq_count;
+    uint32_t pops = 0, tests = 0;
+    uint32_t i = wave_fetch(&A.ctl->eq_head);
+    f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
+    ClosestHit h{-1, __builtin_inff(), 0.0f, 0.0f};
+    uint32_t best_leaf = 0, ref = kDone, leaf = kDone;
+    auto init = [&](uint32_t k) {
+        uint32_t path = A.eq[k];
+        o = xyz(A.xp[path]);
+        v = xyz(A.wi[path]);
+        vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
+        h = ClosestHit{-1, __builtin_inff(), 0.0f, 0.0f};
+        best_leaf = 0;
+        st.sp = 0;
+        ref = S.root_ref;
+        leaf = kDone;
+        if (ref != kEmptyRef && !is_interior(ref)) {
+            leaf = ref;
+            ref = kDone;
+        }
+    };
+    if (i < n)
+        init(i);
+    while (i < n) {
+        while (is_interior(ref)) {
+            pops++;
+            float cut = h.t + h.t * kCullSlack;
+            ref = visit_interior<false>(S, ref, o, vinv, kTMin, kTMax, cut, st);
+            if (is_leaf_ref(ref) && leaf == kDone) {
+                leaf = ref;
+                ref = st.pop_next(cut);
+            }
+            if (__all(leaf != kDone))
+                break;
+        }
+        while (leaf != kDone) {
+            uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
+            for (uint32_t p = s0; p < s0 + cnt; p++) {
+                float4 a = tris[3 * p], b = tris[3 * p + 1], c = tris[3 * p + 2];
+                float c1, c2, t0;
+                tests++;
+                if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), kTMin, kTMax, &c1, &c2, &t0) &&
+                    (t0 < h.t || (t0 == h.t && s0 > best_leaf))) {
+                    h.t = t0;
+                    h.b1 = c1;
+                    h.b2 = c2;
+                    h.prim = static_cast<int32_t>(p);
+                    best_leaf = s0;
+                }
+            }
+            if (is_leaf_ref(ref)) {
+                leaf = ref;
+                ref = st.pop_next(h.t + h.t * kCullSlack);
+            } else {
+                leaf = kDone;
+            }
+        }
+        if (ref == kDone) {
+            A.eh[i] = h;
+            i = wave_fetch(&A.ctl->eq_head);
+            if (i < n)
+                init(i);
+        }
+    }
+    if (A.cnt) {
+        flush_counter(&A.cnt->pops, pops);
+        flush_counter(&A.cnt->tests, tests);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t bounce) {
+    uint32_t ne = A.ctl->eq_count, nf = A.ctl->fin_count;
+    uint32_t total = ne + nf;
+    uint32_t nq = 0, na = 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        if (t >= ne) {
+            uint32_t path = A.fin[t - ne];
+            A.lbuf[path] = A.L[path];
+            continue;
+        }
+        uint32_t path = A.eq[t];
+        ClosestHit h = A.eh[t];
+        float4 L4 = A.L[path];
+        f3 L = xyz(L4);
+        if (h.prim < 0) {
+            A.lbuf[path] = L4;
+            continue;
+        }
+        HitRec nx = finish_hit(A.S, h);
+        float4 wi4 = A.wi[path];
+        f3 wi = xyz(wi4);
+        if (dot(nx.n, -wi) <= 0.0f) {
+            A.lbuf[path] = L4;
+            continue;
+        }
+        float dens = wi4.w;
+        float4 xp = A.xp[path], xn = A.xn[path], wo4 = A.wo[path], b4 = A.beta[path];
+        f3 xpos = xyz(xp), xnrm = xyz(xn), wo = xyz(wo4), beta = xyz(b4);
+        int32_t xmat = static_cast<int32_t>(__float_as_uint(xp.w));
+        float cw = dot(xnrm, wi);
+        f3 f = bsdf_eval(A.S.mats, xmat, xnrm, wo, wi);
+        beta = mul(beta, f) * cw / dens;
+        if (nx.light >= 0) {
+            float cl = dot(nx.n, -wi);
+            DevLight const &Ln = A.S.lights[nx.light];
+            float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
+            float w = dens * dens / (dens * dens + pl * pl);
+            L = L + mul(beta, light_radiance(A.S, nx.light, -wi, nx.n) * w);
+        }
+        (void)xpos;
+        if (bounce + 1 >= A.max_depth) {
+            A.lbuf[path] = make_float4(L.x, L.y, L.z, 0.0f);
+            continue;
+        }
+        A.xp[path] = make_float4(nx.p.x, nx.p.y, nx.p.z, __uint_as_float(static_cast<uint32_t>(nx.mat)));
+        A.xn[path] = make_float4(nx.n.x, nx.n.y, nx.n.z, __uint_as_float(static_cast<uint32_t>(nx.light)));
+        A.wo[path] = make_float4(-wi.x, -wi.y, -wi.z, wo4.w);
+        A.beta[path] = make_float4(beta.x, beta.y, beta.z, b4.w);
+        A.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+        push_active(A, parity ^ 1, kind_of(A.S, nx.mat), path);
+        na++;
+    }
+    (void)nq;
+    (void)na;
+}
+
+__global__ __launch_bounds__(256) void k_wf_accumulate(WfArgs A) {
+    uint32_t gen = A.ctl->gen;
+    uint32_t cnt = gen_count(A, gen);
+    bool last = A.s_begin + gen * A.gen_samples + cnt >= A.s_end;
+    float scale = 1.0f / static_cast<float>(A.spp_total);
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < A.npix; pix += gridDim.x * blockDim.x) {
+        f3 acc = mk(0.0f, 0.0f, 0.0f);
+        if (gen > 0 || A.s_begin > 0) {
+            float4 q = A.acc[pix];
+            acc = mk(q.x, q.y, q.z);
+        }
+        for (uint32_t s = 0; s < cnt; s++)
+            acc = acc + xyz(A.lbuf[static_cast<size_t>(s) * A.npix + pix]);
+        if (last && A.finalize)
+            A.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
+        else
+            A.acc[pix] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+    }
+}
+
+// Ray-query counts (one closest-hit per extension ray, one any-hit per shadow ray).
+__global__ void k_wf_count(WfCtl *ctl, Counters *cnt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && cnt) {
+        atomicAdd(&cnt->closest, static_cast<unsigned long long>(ctl->eq_count));
+        atomicAdd(&cnt->anyhit, static_cast<unsigned long long>(ctl->sq_count));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------------------------
+struct WfBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~WfBuf() {
+        if (p)
+            (void)hipFree(p);
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= bytes)
+            return hipSuccess;
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess)
+            bytes = n;
+        return e;
+    }
+};
+
+struct WavefrontEngine {
+    WfBuf xp, xn, wo, beta, L, wi, rng, lbuf, act0, act1, sq, eq, fin, eh, ctl, spill;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    WfArgs key{};
+    WavefrontTuning key_tune{};
+    bool have_graph = false;
+    ~WavefrontEngine() {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+    }
+};
+
+WavefrontEngine *wf_create() { return new WavefrontEngine(); }
+void wf_destroy(WavefrontEngine *e) { delete e; }
+
+namespace {
+
+template <class K> uint32_t grid_for(K kernel, int wg) {
+    int per_cu = 0, dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, wg, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return static_cast<uint32_t>(cus * per_cu);
+}
+
+template <int WG> void launch_trace(WfArgs const &A, hipStream_t s, uint32_t grid_shadow, uint32_t grid_extend) {
+    hipLaunchKernelGGL(k_wf_shadow<WG>, dim3(grid_shadow), dim3(WG), 0, s, A);
+    hipLaunchKernelGGL(k_wf_extend<WG>, dim3(grid_extend), dim3(WG), 0, s, A);
+}
+
+bool same_config(WfArgs const &a, WfArgs const &b, WavefrontTuning const &ta, WavefrontTuning const &tb) {
+    return std::memcmp(&a, &b, sizeof(WfArgs)) == 0 && ta.paths_in_flight == tb.paths_in_flight &&
+           ta.trace_wg == tb.trace_wg && ta.sort_by_material == tb.sort_by_material;
+}
+
+} // namespace
+
+std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, hipStream_t stream) {
+#define WF_HIP(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t err_ = (call);                                                                    \
+        if (err_ != hipSuccess)                                                                      \
+            return std::string(#call) + ": " + hipGetErrorString(err_);                              \
+    } while (0)
+    uint32_t npix = P.width * P.height;
+    if (P.s_end <= P.s_begin)
+        return "";
+    uint32_t gen_samples = tune.paths_in_flight / npix;
+    if (gen_samples < 1)
+        gen_samples = 1;
+    if (gen_samples > P.s_end - P.s_begin)
+        gen_samples = P.s_end - P.s_begin;
+    uint32_t cap = gen_samples * npix;
+    size_t f4 = sizeof(float4) * static_cast<size_t>(cap);
+    WF_HIP(e->xp.ensure(f4));
+    WF_HIP(e->xn.ensure(f4));
+    WF_HIP(e->wo.ensure(f4));
+    WF_HIP(e->beta.ensure(f4));
+    WF_HIP(e->L.ensure(f4));
+    WF_HIP(e->wi.ensure(f4));
+    WF_HIP(e->lbuf.ensure(f4));
+    WF_HIP(e->rng.ensure(sizeof(uint64_t) * cap));
+    WF_HIP(e->act0.ensure(sizeof(uint32_t) * kKinds * static_cast<size_t>(cap)));
+    WF_HIP(e->act1.ensure(sizeof(uint32_t) * kKinds * static_cast<size_t>(cap)));
+    WF_HIP(e->sq.ensure(3 * f4));
+    WF_HIP(e->eq.ensure(sizeof(uint32_t) * cap));
+    WF_HIP(e->fin.ensure(sizeof(uint32_t) * cap));
+    WF_HIP(e->eh.ensure(sizeof(ClosestHit) * cap));
+    WF_HIP(e->ctl.ensure(sizeof(WfCtl)));
+
+    int wg = tune.trace_wg == 64 || tune.trace_wg == 128 ? tune.trace_wg : 256;
+    uint32_t g_shadow = 0, g_extend = 0;
+    switch (wg) {
+    case 64: g_shadow = grid_for(k_wf_shadow<64>, 64); g_extend = grid_for(k_wf_extend<64>, 64); break;
+    case 128: g_shadow = grid_for(k_wf_shadow<128>, 128); g_extend = grid_for(k_wf_extend<128>, 128); break;
+    default: g_shadow = grid_for(k_wf_shadow<256>, 256); g_extend = grid_for(k_wf_extend<256>, 256); break;
+    }
+    uint32_t trace_threads = (g_shadow > g_extend ? g_shadow : g_extend) * wg;
+    uint32_t spill_per_thread = spill_entries_for(P.S, kLdsDepth);
+    WF_HIP(e->spill.ensure(sizeof(uint2) * static_cast<size_t>(trace_threads) * spill_per_thread));
+
+    WfArgs A{};
+    A.S = P.S;
+    A.cam = P.cam;
+    A.width = P.width;
+    A.height = P.height;
+    A.npix = npix;
+    A.s_begin = P.s_begin;
+    A.s_end = P.s_end;
+    A.spp_total = P.spp_total;
+    A.gen_samples = gen_samples;
+    A.capacity = cap;
+    A.seed = P.seed;
+    A.max_depth = P.max_depth;
+    A.finalize = P.finalize;
+    A.first = P.first;
+    A.acc = P.acc;
+    A.out = P.out;
+    A.xp = static_cast<float4 *>(e->xp.p);
+    A.xn = static_cast<float4 *>(e->xn.p);
+    A.wo = static_cast<float4 *>(e->wo.p);
+    A.beta = static_cast<float4 *>(e->beta.p);
+    A.L = static_cast<float4 *>(e->L.p);
+    A.wi = static_cast<float4 *>(e->wi.p);
+    A.rng = static_cast<uint64_t *>(e->rng.p);
+    A.lbuf = static_cast<float4 *>(e->lbuf.p);
+    A.act[0] = static_cast<uint32_t *>(e->act0.p);
+    A.act[1] = static_cast<uint32_t *>(e->act1.p);
+    A.sq = static_cast<float4 *>(e->sq.p);
+    A.eq = static_cast<uint32_t *>(e->eq.p);
+    A.fin = static_cast<uint32_t *>(e->fin.p);
+    A.eh = static_cast<ClosestHit *>(e->eh.p);
+    A.ctl = static_cast<WfCtl *>(e->ctl.p);
+    A.spill = static_cast<uint2 *>(e->spill.p);
+    A.spill_per_thread = spill_per_thread;
+    A.cnt = P.cnt;
+    A.sort_by_material = tune.sort_by_material;
+
+    int dev = 0, cus = 0;
+    WF_HIP(hipGetDevice(&dev));
+    WF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    uint32_t g_item = static_cast<uint32_t>(cus) * 8u; // grid-strided per-path kernels
+    uint32_t generations = (P.s_end - P.s_begin + gen_samples - 1) / gen_samples;
+
+    auto record_generation = [&](hipStream_t s) {
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, -1);
+        hipLaunchKernelGGL(k_wf_generate, dim3(g_item), dim3(256), 0, s, A);
+        for (uint32_t k = 0; k < P.max_depth; k++) {
+            int par = static_cast<int>(k & 1u);
+            hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, par);
+            hipLaunchKernelGGL(k_wf_nee_bsdf, dim3(g_item), dim3(256), 0, s, A, par);
+            switch (wg) {
+            case 64: launch_trace<64>(A, s, g_shadow, g_extend); break;
+            case 128: launch_trace<128>(A, s, g_shadow, g_extend); break;
+            default: launch_trace<256>(A, s, g_shadow, g_extend); break;
+            }
+            hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, s, A.ctl, A.cnt);
+            hipLaunchKernelGGL(k_wf_shade, dim3(g_item), dim3(256), 0, s, A, par, k);
+        }
+        hipLaunchKernelGGL(k_wf_accumulate, dim3(g_item), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_wf_next_gen, dim3(1), dim3(64), 0, s, A.ctl);
+    };
+
+    hipLaunchKernelGGL(k_wf_begin_render, dim3(1), dim3(64), 0, stream, A.ctl);
+    WF_HIP(hipGetLastError());
+    if (tune.use_graph) {
+        if (!e->have_graph || !same_config(A, e->key, tune, e->key_tune)) {
+            if (e->exec)
+                WF_HIP(hipGraphExecDestroy(e->exec));
+            if (e->graph)
+                WF_HIP(hipGraphDestroy(e->graph));
+            e->exec = nullptr;
+            e->graph = nullptr;
+            e->have_graph = false;
+            WF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            record_generation(stream);
+            WF_HIP(hipStreamEndCapture(stream, &e->graph));
+            WF_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+            std::memcpy(&e->key, &A, sizeof A);
+            e->key_tune = tune;
+            e->have_graph = true;
+        }
+        for (uint32_t g = 0; g < generations; g++)
+            WF_HIP(hipGraphLaunch(e->exec, stream));
+    } else {
+        for (uint32_t g = 0; g < generations; g++)
+            record_generation(stream);
+    }
+    WF_HIP(hipGetLastError());
+    return "";
+#undef WF_HIP
+}
+
+} // namespace x3d

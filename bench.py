@@ -5,10 +5,13 @@ Workload (default): unidirectional MIS path tracing (light + BSDF sampling, powe
 the room with 200 displaced GGX icospheres (about 1.3M triangles), 1024 x 1024 pixels, 64 samples
 per pixel, at most 8 bounces, FP32. One step = one render of the whole film; the scene (BVH,
 triangles, materials, lights) is resident in HBM before the timed region and the film stays in
-HBM. Prints one JSON line.
+HBM. Both execution forms are timed on the same workload — the plain one-thread-per-path kernel
+(the drop-in) and the autotuned wavefront form — and their films are checked to be bit-identical
+(so the RMSE against any reference is equal). `value` is the wavefront form's samples/s. Prints
+one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload spheres|box|indirect] [--form plain]
-                    [--spp N] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--workload spheres|box|indirect] [--spp N]
+                    [--no-cpu-baseline] [--no-autotune] [--form both|plain|wavefront]
 """
 
 from __future__ import annotations
@@ -31,6 +34,10 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (dense)
 # vertices (36 B) and an id (4 B) and does the 53 flops of ray::intersect (tensor.h:688-725).
 BYTES_PER_NODE, BYTES_PER_TEST = 56, 40
 FLOPS_PER_NODE, FLOPS_PER_TEST = 24, 53
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def workload(name, spp):
@@ -64,8 +71,7 @@ def cpu_baseline(desc, budget_s=12.0):
     t0 = time.perf_counter()
     _, taken = ref.render(bindings.TR_UNIDIRECT_LT1, side, side, threads, threads=threads)
     dt = time.perf_counter() - t0
-    # scale the pixel count to spend about budget_s seconds
-    scale = max(1.0, budget_s / max(dt, 1e-3))
+    scale = max(1.0, budget_s / max(dt, 1e-3))  # scale the pixel count to ~budget_s seconds
     side = int(min(desc.width, max(32, side * scale ** 0.5)))
     t0 = time.perf_counter()
     _, taken = ref.render(bindings.TR_UNIDIRECT_LT1, side, side, threads, threads=threads)
@@ -82,15 +88,43 @@ def cpu_baseline(desc, budget_s=12.0):
     }
 
 
+def timed_renders(x3d, r, scene, desc, film, steps, warmup, name):
+    from x3dcore2_amd.device import synchronize
+
+    w, h, spp = desc.width, desc.height, desc.spp
+    for k in range(warmup):
+        st = r.render_device(scene, desc.camera, w, h, spp, film.ptr)
+        log(f"{name} warmup {k}: {st.kernel_ms:.1f} ms")
+    synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st = r.render_device(scene, desc.camera, w, h, spp, film.ptr)
+        stats.append(st)
+        log(f"{name} step {k}: {st.kernel_ms:.1f} ms (sample kernels {st.trace_ms:.1f} ms)")
+    synchronize()
+    return time.perf_counter() - t0, stats
+
+
+def traffic_from_profile(name):
+    tpath = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(tpath):
+        for line in open(tpath):
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="spheres")
-    ap.add_argument("--form", default="plain", choices=["plain", "wavefront"])
+    ap.add_argument("--form", default="both", choices=["both", "plain", "wavefront"])
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (to calibrate rocprof byte counters)")
     args = ap.parse_args()
@@ -108,93 +142,117 @@ def main():
         sys.exit(2)
 
     import x3dcore2_amd as x3d
-    from x3dcore2_amd.device import DeviceArray, synchronize
+    from x3dcore2_amd.device import DeviceArray
 
     desc = workload(args.workload, args.spp)
     w, h, spp, depth = desc.width, desc.height, desc.spp, desc.max_depth
-    print(f"[bench] workload {desc.name}: {desc.num_tris} tris, {w}x{h}, {spp} spp, depth {depth}", file=sys.stderr,
-          flush=True)
+    log(f"workload {desc.name}: {desc.num_tris} tris, {w}x{h}, {spp} spp, depth {depth}")
     scene = x3d.Scene(desc)
     info = scene.info()
-    print(f"[bench] BVH build {info.build_seconds:.2f} s, upload {info.upload_seconds:.2f} s, "
-          f"{info.device_bytes / 2**20:.1f} MiB in HBM, max leaf depth {info.max_leaf_depth}", file=sys.stderr,
-          flush=True)
-    form = x3d.FORM_PLAIN if args.form == "plain" else x3d.FORM_WAVEFRONT
-    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=form, max_depth=depth, seed=1)
-    film = DeviceArray(w * h * 16)
-    for k in range(args.warmup):
-        st = r.render_device(scene, desc.camera, w, h, spp, film.ptr)
-        print(f"[bench] warmup {k}: {st.kernel_ms:.1f} ms", file=sys.stderr, flush=True)
-    synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        st = r.render_device(scene, desc.camera, w, h, spp, film.ptr)
-        stats.append(st)
-        print(f"[bench] step {k}: {st.kernel_ms:.1f} ms (trace {st.trace_ms:.1f} ms)", file=sys.stderr, flush=True)
-    synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    log(f"BVH build {info.build_seconds:.2f} s, upload {info.upload_seconds:.2f} s, "
+        f"{info.device_bytes / 2**20:.1f} MiB in HBM, max leaf depth {info.max_leaf_depth}")
     samples = w * h * spp * args.steps
-    rays = sum(s.closest_queries + s.anyhit_queries for s in stats)
-    img = film.to_host(np.float32, (h * w, 4))
-    if not np.isfinite(img).all():
-        raise SystemExit("non-finite film")
-
-    # Dominant kernel: the sample kernel (bounce loop). Its algorithmic bytes / flops per launch
-    # come from the traversal counters of the same launch; its duration from HIP events on its
-    # stream.
-    s0 = stats[-1]
-    nodes = (s0.closest_nodes - s0.first_hit_nodes) + s0.anyhit_nodes
-    tests = (s0.closest_tests - s0.first_hit_tests) + s0.anyhit_tests
-    trace_ms = float(np.mean([s.trace_ms for s in stats]))
-    alg_bytes = BYTES_PER_NODE * nodes + BYTES_PER_TEST * tests
-    alg_flops = FLOPS_PER_NODE * nodes + FLOPS_PER_TEST * tests
-    achieved_gbs = alg_bytes / (trace_ms * 1e-3) / 1e9
-    achieved_tflops = alg_flops / (trace_ms * 1e-3) / 1e12
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{desc.name}_{args.form}.md")
-    if os.path.exists(tpath):
-        for line in open(tpath):
-            if line.startswith("hbm_bytes_per_launch:"):
-                traffic = float(line.split(":", 1)[1])
-
     out = {
         "metric": "Msamples/s & Mrays/s at 1024^2 vs plain GPU path at equal RMSE; grad evals/s",
-        "value": samples / elapsed / 1e6,
         "unit": "Msamples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": f"{desc.name}: room + 200 displaced GGX icospheres" if desc.name == "spheres"
-                   else desc.name, "triangles": desc.num_tris, "width": w, "height": h, "spp": spp,
-                   "max_depth": depth, "tracer": "unidirectional MIS (NEE + BSDF, power heuristic)",
-                   "form": args.form},
-        "mrays_per_s": rays / elapsed / 1e6,
-        "rays_per_sample": rays / samples,
-        "kernel_ms_per_step": float(np.mean([s.kernel_ms for s in stats])),
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_render_mis" if args.form == "plain" else "wavefront",
-            "achieved": achieved_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "alg_bytes_per_launch": alg_bytes,
-            "launch_ms": trace_ms,
-            "valu_tflops": achieved_tflops,
-            "valu_frac": achieved_tflops / FP32_PEAK_TFLOPS,
-            "node_visits_per_launch": nodes,
-            "tri_tests_per_launch": tests,
-        },
+        "config": {"workload": (f"{desc.name}: room + 200 displaced GGX icospheres" if desc.name == "spheres"
+                                else desc.name), "triangles": desc.num_tris, "width": w, "height": h, "spp": spp,
+                   "max_depth": depth, "tracer": "unidirectional MIS (NEE + BSDF, power heuristic)"},
         "bvh_build_s": info.build_seconds,
+    }
+    films = {}
+    results = {}
+    forms = ["plain", "wavefront"] if args.form == "both" else [args.form]
+    for form in forms:
+        film = DeviceArray(w * h * 16)
+        if form == "plain":
+            r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=depth, seed=1)
+        else:
+            r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=depth, seed=1)
+            if not args.no_autotune:
+                rep = r.autotune(scene, desc.camera, w, h, 2)
+                out["autotune"] = {"workgroup_size": rep.workgroup_size, "stack_depth": rep.stack_depth,
+                                   "sort_key": ["default", "material", "none"][rep.sort_key],
+                                   "queue_capacity": rep.queue_capacity, "candidates": rep.candidates,
+                                   "probe_ms_best": rep.best_ms, "probe_ms_default": rep.default_ms}
+                log(f"autotune: {out['autotune']}")
+        elapsed, stats = timed_renders(x3d, r, scene, desc, film, args.steps, args.warmup, form)
+        films[form] = film.to_host(np.float32, (h * w, 4))
+        if not np.isfinite(films[form]).all():
+            raise SystemExit(f"non-finite film ({form})")
+        rays = sum(s.closest_queries + s.anyhit_queries for s in stats)
+        results[form] = dict(elapsed=elapsed, stats=stats, rays=rays, renderer=r, film=film)
+        out[f"{form}_msamples_per_s"] = samples / elapsed / 1e6
+        out[f"{form}_mrays_per_s"] = rays / elapsed / 1e6
+        out[f"{form}_ms_per_step"] = elapsed * 1e3 / args.steps
+
+    best = "wavefront" if "wavefront" in results else "plain"
+    R = results[best]
+    out["value"] = samples / R["elapsed"] / 1e6
+    out["ms_per_step"] = R["elapsed"] * 1e3 / args.steps
+    out["mrays_per_s"] = R["rays"] / R["elapsed"] / 1e6
+    out["rays_per_sample"] = R["rays"] / samples
+    out["config"]["form"] = best
+    if len(results) == 2:
+        out["speedup_vs_plain"] = results["plain"]["elapsed"] / results["wavefront"]["elapsed"]
+        out["films_bit_identical"] = bool(np.array_equal(films["plain"].view(np.uint32),
+                                                         films["wavefront"].view(np.uint32)))
+        if not out["films_bit_identical"]:
+            raise SystemExit("plain and wavefront films differ")
+
+    # Roofline of the dominant kernel. Algorithmic bytes / flops per launch come from the
+    # traversal counters of the same launches (56 B + 24 flop per node visit, 40 B + 53 flop per
+    # triangle test); the duration from HIP events on the kernel's stream.
+    s0 = R["stats"][-1]
+    if best == "wavefront":
+        # closest-hit traversal kernel: one profiled render with the same configuration (events
+        # around each launch, no graph)
+        tuned = out.get("autotune", {})
+        wg, sd = tuned.get("workgroup_size", 0), tuned.get("stack_depth", 0)
+        sort = {"default": 0, "material": 1, "none": 2}[tuned.get("sort_key", "default")]
+        rp = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=depth, seed=1, workgroup_size=wg,
+                          queue_capacity=tuned.get("queue_capacity", 0), sort_key=sort, stack_depth=sd, profile=1)
+        sp = rp.render_device(scene, desc.camera, w, h, spp, R["film"].ptr)
+        launches = max(sp.trace_launches, 1)
+        nodes = (sp.closest_nodes - sp.first_hit_nodes) / launches
+        tests = (sp.closest_tests - sp.first_hit_tests) / launches
+        launch_ms = sp.extend_ms / launches
+        kernel = f"k_wf_extend<{wg or 256},{sd or 16}>"
+        traffic = traffic_from_profile(f"traffic_{desc.name}_wavefront.md")
+        out["wavefront_kernel_ms"] = {"extend": sp.extend_ms, "shadow": sp.shadow_ms, "launches": sp.trace_launches,
+                                      "all_sample_kernels": sp.trace_ms}
+    else:
+        nodes = (s0.closest_nodes - s0.first_hit_nodes) + s0.anyhit_nodes
+        tests = (s0.closest_tests - s0.first_hit_tests) + s0.anyhit_tests
+        launch_ms = float(np.mean([s.trace_ms for s in R["stats"]]))
+        kernel = "k_render_mis"
+        traffic = traffic_from_profile(f"traffic_{desc.name}_plain.md")
+    alg_bytes = BYTES_PER_NODE * nodes + BYTES_PER_TEST * tests
+    alg_flops = FLOPS_PER_NODE * nodes + FLOPS_PER_TEST * tests
+    achieved_gbs = alg_bytes / (launch_ms * 1e-3) / 1e9
+    achieved_tflops = alg_flops / (launch_ms * 1e-3) / 1e12
+    out["roofline"] = {
+        "bound": "hbm",
+        "kernel": kernel,
+        "achieved": achieved_gbs,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved_gbs / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "alg_bytes_per_launch": alg_bytes,
+        "launch_ms": launch_ms,
+        "valu_tflops": achieved_tflops,
+        "valu_frac": achieved_tflops / FP32_PEAK_TFLOPS,
+        "node_visits_per_launch": nodes,
+        "tri_tests_per_launch": tests,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(desc)

@@ -182,7 +182,9 @@ typedef struct x3d_render_options {
     uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default */
     uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default (4M) */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
-    uint32_t reserved[5];
+    uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16 */
+    int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */
+    uint32_t reserved[3];
 } x3d_render_options;
 
 /* Wavefront sort keys: group active paths by material kind (default) or keep append order. */
@@ -210,7 +212,28 @@ typedef struct x3d_render_stats {
     double first_hit_ms;        /* device time of the first-hit pass */
     double trace_ms;            /* device time of the sample kernels (bounce loop) */
     double total_ms;            /* wall time of the call including copies */
+    double extend_ms;           /* WAVEFRONT + profile: closest-hit traversal kernels, summed */
+    double shadow_ms;           /* WAVEFRONT + profile: any-hit traversal kernels, summed */
+    uint32_t trace_launches;    /* WAVEFRONT + profile: launches of each traversal kernel */
+    uint32_t reserved;
 } x3d_render_stats;
+
+/* Autotuner (no reference counterpart): renders `probe_spp` samples per pixel of the given
+ * workload with every candidate wavefront configuration (traversal workgroup size, LDS stack
+ * depth, sort key, queue capacity), keeps the fastest in the renderer's options and reports it.
+ * The film does not depend on the configuration. */
+typedef struct x3d_autotune_report {
+    uint32_t workgroup_size, stack_depth, queue_capacity;
+    int32_t sort_key;
+    double best_ms;      /* trace time of the probe render with the chosen configuration */
+    double default_ms;   /* ... with the default configuration */
+    uint32_t candidates; /* configurations tried */
+    uint32_t reserved;
+} x3d_autotune_report;
+
+int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera,
+                          uint32_t width, uint32_t height, uint32_t probe_spp,
+                          x3d_autotune_report *report);
 
 /* Renders the film: rgba float[4*w*h], pixel (x, y) at index x + y*w, row 0 = top; rgb is the
  * mean radiance over spp samples, alpha = 1/spp (renderer.cpp:270-295 with one task).

@@ -190,3 +190,21 @@ def test_wavefront_film_identical_to_plain(x3d, scene_name, kw):
         assert np.array_equal(_bits(img), _bits(plain)), np.abs(img - plain).max()
         assert st.closest_queries == sp.closest_queries
         assert st.anyhit_queries == sp.anyhit_queries
+
+
+def test_wavefront_autotune_and_profile(x3d):
+    desc = QUERY_SCENES["spheres_small"]()
+    gs = x3d.Scene(desc)
+    w, h, spp = 64, 48, 3
+    plain, _ = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=8, seed=5).render(gs, desc.camera, w, h, spp)
+    wf = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=8, seed=5)
+    rep = wf.autotune(gs, desc.camera, w, h, 1)
+    assert rep.candidates == 36 and rep.best_ms > 0 and rep.best_ms <= rep.default_ms * 1.5
+    assert rep.workgroup_size in (64, 128, 256) and rep.stack_depth in (8, 16)
+    img, _ = wf.render(gs, desc.camera, w, h, spp)
+    assert np.array_equal(_bits(img), _bits(plain))
+    prof = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=8, seed=5, profile=1,
+                        stack_depth=8, workgroup_size=64)
+    img, st = prof.render(gs, desc.camera, w, h, spp)
+    assert np.array_equal(_bits(img), _bits(plain))
+    assert st.trace_launches == 8 and st.extend_ms > 0 and st.shadow_ms > 0

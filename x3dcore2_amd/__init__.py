@@ -64,7 +64,8 @@ class CameraDesc(C.Structure):
 class RenderOptions(C.Structure):
     _fields_ = [("pt_type", C.c_int32), ("form", C.c_int32), ("max_depth", C.c_uint32),
                 ("max_path_len", C.c_uint32), ("seed", C.c_uint64), ("workgroup_size", C.c_uint32),
-                ("queue_capacity", C.c_uint32), ("sort_key", C.c_int32), ("reserved", C.c_uint32 * 5)]
+                ("queue_capacity", C.c_uint32), ("sort_key", C.c_int32), ("stack_depth", C.c_uint32),
+                ("profile", C.c_int32), ("reserved", C.c_uint32 * 3)]
 
 
 class RenderStats(C.Structure):
@@ -72,7 +73,14 @@ class RenderStats(C.Structure):
                 ("closest_nodes", C.c_uint64), ("closest_tests", C.c_uint64), ("anyhit_nodes", C.c_uint64),
                 ("anyhit_tests", C.c_uint64), ("first_hit_nodes", C.c_uint64), ("first_hit_tests", C.c_uint64),
                 ("kernel_ms", C.c_double), ("first_hit_ms", C.c_double),
-                ("trace_ms", C.c_double), ("total_ms", C.c_double)]
+                ("trace_ms", C.c_double), ("total_ms", C.c_double), ("extend_ms", C.c_double),
+                ("shadow_ms", C.c_double), ("trace_launches", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class AutotuneReport(C.Structure):
+    _fields_ = [("workgroup_size", C.c_uint32), ("stack_depth", C.c_uint32), ("queue_capacity", C.c_uint32),
+                ("sort_key", C.c_int32), ("best_ms", C.c_double), ("default_ms", C.c_double),
+                ("candidates", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -84,6 +92,7 @@ EXPORTS = [
     "x3d_scene_dump_bvh", "x3d_intersect", "x3d_occluded", "x3d_intersect_device", "x3d_occluded_device",
     "x3d_bsdf_eval", "x3d_bsdf_pdf", "x3d_bsdf_sample", "x3d_camera_rays", "x3d_render_options_default",
     "x3d_renderer_create", "x3d_renderer_destroy", "x3d_render", "x3d_render_device", "x3d_pcg_draws",
+    "x3d_renderer_autotune",
 ]
 
 
@@ -126,6 +135,7 @@ def lib() -> C.CDLL:
     L.x3d_render.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, i32, vp, C.POINTER(RenderStats)]
     L.x3d_render_device.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp, C.POINTER(RenderStats)]
     L.x3d_pcg_draws.argtypes = [u64, u64, u32, u32, vp]
+    L.x3d_renderer_autotune.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, C.POINTER(AutotuneReport)]
     _lib = L
     return L
 
@@ -257,12 +267,13 @@ class Renderer:
     """pt_image_renderer equivalent (x3d_renderer)."""
 
     def __init__(self, pt_type=PT_UNIDIRECT_MIS, form=FORM_PLAIN, max_depth=5, max_path_len=8, seed=1,
-                 workgroup_size=0, queue_capacity=0, sort_key=0):
+                 workgroup_size=0, queue_capacity=0, sort_key=0, stack_depth=0, profile=0):
         L = lib()
         o = RenderOptions()
         _check(L.x3d_render_options_default(pt_type, C.byref(o)))
         o.form, o.max_depth, o.max_path_len, o.seed = form, max_depth, max_path_len, seed
         o.workgroup_size, o.queue_capacity, o.sort_key = workgroup_size, queue_capacity, sort_key
+        o.stack_depth, o.profile = stack_depth, profile
         h = C.c_void_p()
         _check(L.x3d_renderer_create(C.byref(o), C.byref(h)))
         self.h = h
@@ -285,6 +296,12 @@ class Renderer:
         _check(lib().x3d_render_device(self.h, scene.h, C.byref(camera_desc(camera)), width, height, spp,
                                        C.c_void_p(d_rgba), C.byref(st)))
         return st
+
+    def autotune(self, scene: Scene, camera, width, height, probe_spp) -> AutotuneReport:
+        rep = AutotuneReport()
+        _check(lib().x3d_renderer_autotune(self.h, scene.h, C.byref(camera_desc(camera)), width, height, probe_spp,
+                                           C.byref(rep)))
+        return rep
 
 
 def camera_rays(camera, width, height) -> np.ndarray:

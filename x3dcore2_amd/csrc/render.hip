@@ -534,7 +534,10 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
             tune.trace_wg = static_cast<int>(r->opt.workgroup_size);
         if (r->opt.queue_capacity)
             tune.paths_in_flight = r->opt.queue_capacity;
+        if (r->opt.stack_depth)
+            tune.stack_depth = static_cast<int>(r->opt.stack_depth);
         tune.sort_by_material = r->opt.sort_key != X3D_SORT_NONE;
+        tune.profile = r->opt.profile;
         std::string e = x3d::wf_render(r->wf, P, tune, r->stream);
         if (!e.empty())
             return fail(X3D_ERR_DEVICE, e);
@@ -577,6 +580,10 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         stats->first_hit_ms = ms_first;
         stats->trace_ms = ms_trace;
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        stats->extend_ms = stats->shadow_ms = 0.0;
+        stats->trace_launches = 0;
+        if (r->wf && r->opt.profile)
+            x3d::wf_last_timing(r->wf, &stats->shadow_ms, &stats->extend_ms, &stats->trace_launches);
     }
     return X3D_OK;
 }
@@ -986,6 +993,8 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
         return fail(X3D_ERR_INVALID_ARGUMENT, "workgroup_size must be 0, 64, 128 or 256");
     if (opts->sort_key < X3D_SORT_DEFAULT || opts->sort_key > X3D_SORT_NONE)
         return fail(X3D_ERR_INVALID_ARGUMENT, "unknown sort key");
+    if (opts->stack_depth != 0 && opts->stack_depth != 8 && opts->stack_depth != 16)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "stack_depth must be 0, 8 or 16");
     int rc = require_device();
     if (rc)
         return rc;
@@ -1037,6 +1046,74 @@ int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera
     X3D_HIP(hipMemcpy(rgba, r->out.p, npix * sizeof(float4), hipMemcpyDeviceToHost));
     if (stats)
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return X3D_OK;
+}
+
+int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                          uint32_t height, uint32_t probe_spp, x3d_autotune_report *report) {
+    int rc = check_render_args(r, scene, camera, width, height, probe_spp);
+    if (rc)
+        return rc;
+    if (!r->wf)
+        return fail(X3D_ERR_STATE, "autotuning applies to the wavefront form");
+    uint64_t npix = static_cast<uint64_t>(width) * height;
+    X3D_HIP(r->out.ensure(npix * sizeof(float4)));
+    x3d_render_options saved = r->opt;
+    r->opt.profile = 0;
+    auto time_config = [&](uint32_t wg, uint32_t depth, int32_t sort, uint32_t cap, double *ms) -> int {
+        r->opt.workgroup_size = wg;
+        r->opt.stack_depth = depth;
+        r->opt.sort_key = sort;
+        r->opt.queue_capacity = cap;
+        x3d_render_stats st;
+        double best = 1e300;
+        for (int k = 0; k < 2; k++) { // first call captures the graph
+            int e = render_impl(r, scene, camera, width, height, probe_spp, static_cast<float *>(r->out.p), &st);
+            if (e)
+                return e;
+            if (k > 0 && st.trace_ms < best)
+                best = st.trace_ms;
+        }
+        *ms = best;
+        return X3D_OK;
+    };
+    const uint32_t wgs[] = {64, 128, 256};
+    const uint32_t depths[] = {8, 16};
+    const int32_t sorts[] = {X3D_SORT_MATERIAL, X3D_SORT_NONE};
+    const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23};
+    x3d_autotune_report rep{};
+    rep.best_ms = 1e300;
+    rc = time_config(0, 0, X3D_SORT_DEFAULT, 0, &rep.default_ms);
+    if (rc) {
+        r->opt = saved;
+        return rc;
+    }
+    for (uint32_t wg : wgs)
+        for (uint32_t d : depths)
+            for (int32_t so : sorts)
+                for (uint32_t cap : caps) {
+                    double ms;
+                    rc = time_config(wg, d, so, cap, &ms);
+                    if (rc) {
+                        r->opt = saved;
+                        return rc;
+                    }
+                    rep.candidates++;
+                    if (ms < rep.best_ms) {
+                        rep.best_ms = ms;
+                        rep.workgroup_size = wg;
+                        rep.stack_depth = d;
+                        rep.sort_key = so;
+                        rep.queue_capacity = cap;
+                    }
+                }
+    r->opt = saved;
+    r->opt.workgroup_size = rep.workgroup_size;
+    r->opt.stack_depth = rep.stack_depth;
+    r->opt.sort_key = rep.sort_key;
+    r->opt.queue_capacity = rep.queue_capacity;
+    if (report)
+        *report = rep;
     return X3D_OK;
 }
 

@@ -14,12 +14,17 @@ struct WavefrontEngine;
 struct WavefrontTuning {
     uint32_t paths_in_flight = 1u << 22; // queue capacity (paths of one generation)
     int trace_wg = 256;                  // workgroup size of the traversal kernels (64, 128, 256)
+    int stack_depth = 16;                // LDS stack entries per lane of the traversal kernels (8, 16)
     int sort_by_material = 1;            // group active paths by material kind before shading
     int use_graph = 1;                   // replay one generation's launches as a hipGraph
+    int profile = 0;                     // time the traversal kernels with events (no graph)
 };
 
 WavefrontEngine *wf_create();
 void wf_destroy(WavefrontEngine *e);
+
+// Event-timed durations of the traversal kernels of the last profiled render.
+void wf_last_timing(WavefrontEngine const *e, double *shadow_ms, double *extend_ms, uint32_t *launches);
 
 // Runs the sample part of a render whose first-hit pass has already filled P.first:
 // samples [P.s_begin, P.s_end) of every pixel, in generations of `paths_in_flight / pixels`

@@ -62,10 +62,31 @@ X3D_DI int kind_of(DevScene const &S, int32_t mat) {
     return k == kMatCookTorr ? 1 : (k == kMatMixture ? 2 : 0);
 }
 
-X3D_DI void push_active(WfArgs const &A, int parity, int kind, uint32_t path) {
-    int k = A.sort_by_material ? kind : 0;
-    uint32_t j = atomicAdd(&A.ctl->act_count[parity][k], 1u);
-    A.act[parity][static_cast<size_t>(k) * A.capacity + j] = path;
+// Lanes of the calling wave with pred set take consecutive slots of the queue behind `counter`,
+// with one atomic per wave. Every active lane must call it (pred may be false).
+X3D_DI uint32_t wave_append(uint32_t *counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    if (mask == 0)
+        return 0;
+    uint32_t lane = __lane_id();
+    uint32_t leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    return base + rank;
+}
+
+// Appends `path` to the active list of its material kind (all kinds in list 0 without the
+// material sort). Every active lane must call it; `live` says whether this lane appends.
+X3D_DI void push_active(WfArgs const &A, int parity, int kind, uint32_t path, bool live) {
+    for (int k = 0; k < (A.sort_by_material ? kKinds : 1); k++) {
+        bool mine = live && (!A.sort_by_material || kind == k);
+        uint32_t j = wave_append(&A.ctl->act_count[parity][k], mine);
+        if (mine)
+            A.act[parity][static_cast<size_t>(k) * A.capacity + j] = path;
+    }
 }
 
 // Lanes of the calling (possibly partial) wave take consecutive items with one atomic.
@@ -137,17 +158,18 @@ __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
         f3 wo = -v;
         if (valid && x.light >= 0)
             L = L + light_radiance(A.S, x.light, wo, x.n);
-        if (!(valid && A.max_depth > 0)) {
+        bool live = valid && A.max_depth > 0;
+        if (!live) {
             A.lbuf[id] = make_float4(L.x, L.y, L.z, 0.0f);
-            continue;
+        } else {
+            A.xp[id] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
+            A.xn[id] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
+            A.wo[id] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
+            A.beta[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+            A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
+            A.rng[id] = g.state;
         }
-        A.xp[id] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
-        A.xn[id] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
-        A.wo[id] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
-        A.beta[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
-        A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
-        A.rng[id] = g.state;
-        push_active(A, 0, kind_of(A.S, x.mat), id);
+        push_active(A, 0, live ? kind_of(A.S, x.mat) : 0, id, live);
     }
 }
 
@@ -171,6 +193,9 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
         g.state = A.rng[path];
         g.inc = (static_cast<uint64_t>(pix) << 1u) | 1u;
         // next-event estimation (Tracer::direct + the MIS weight of Tracer::mis_bounce)
+        bool has_shadow = false;
+        f3 sc = mk(0.0f, 0.0f, 0.0f), sdir = mk(0.0f, 0.0f, 0.0f);
+        float sdist = 0.0f;
         if (A.S.num_lights > 0) {
             LightSample ls = sample_light(A.S, g);
             f3 l = x.p - ls.p;
@@ -189,32 +214,38 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
                     float pl = ls.area_dens * r2 / cl;
                     float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
                     float w = pl * pl / (pl * pl + pb * pb);
-                    f3 c = mul(beta, d * w);
-                    uint32_t q = atomicAdd(&A.ctl->sq_count, 1u);
-                    A.sq[3 * q] = make_float4(x.p.x, x.p.y, x.p.z, dist - 1e-3f);
-                    A.sq[3 * q + 1] = make_float4(wl.x, wl.y, wl.z, __uint_as_float(path));
-                    A.sq[3 * q + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+                    sc = mul(beta, d * w);
+                    sdist = dist;
+                    sdir = wl;
+                    has_shadow = true;
                 }
             }
         }
         float dens;
         f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, false);
         A.rng[path] = g.state;
-        if (!(dens > 0.0f)) {
-            uint32_t q = atomicAdd(&A.ctl->fin_count, 1u);
-            A.fin[q] = path;
-        } else {
+        bool ok = dens > 0.0f;
+        if (ok)
             A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
-            uint32_t q = atomicAdd(&A.ctl->eq_count, 1u);
-            A.eq[q] = path;
+        uint32_t q = wave_append(&A.ctl->sq_count, has_shadow);
+        if (has_shadow) {
+            A.sq[3 * q] = make_float4(x.p.x, x.p.y, x.p.z, sdist - 1e-3f);
+            A.sq[3 * q + 1] = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(path));
+            A.sq[3 * q + 2] = make_float4(sc.x, sc.y, sc.z, 0.0f);
         }
+        q = wave_append(&A.ctl->eq_count, ok);
+        if (ok)
+            A.eq[q] = path;
+        q = wave_append(&A.ctl->fin_count, !ok);
+        if (!ok)
+            A.fin[q] = path;
     }
 }
 
-template <int WG>
+template <int WG, int LDSD>
 __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
-    __shared__ uint2 lds[kLdsDepth * WG];
-    TravStack<WG, kLdsDepth> st = make_stack<WG>(lds, A.spill, A.spill_per_thread);
+    __shared__ uint2 lds[LDSD * WG];
+    TravStack<WG, LDSD> st = make_stack<WG, LDSD>(lds, A.spill, A.spill_per_thread);
     DevScene const &S = A.S;
     const float4 *tris = static_cast<const float4 *>(S.tris);
     const uint32_t n = A.ctl->sq_count;
@@ -292,10 +323,10 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     }
 }
 
-template <int WG>
+template <int WG, int LDSD>
 __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
-    __shared__ uint2 lds[kLdsDepth * WG];
-    TravStack<WG, kLdsDepth> st = make_stack<WG>(lds, A.spill, A.spill_per_thread);
+    __shared__ uint2 lds[LDSD * WG];
+    TravStack<WG, LDSD> st = make_stack<WG, LDSD>(lds, A.spill, A.spill_per_thread);
     DevScene const &S = A.S;
     const float4 *tris = static_cast<const float4 *>(S.tris);
     const uint32_t n = A.ctl->eq_count;
@@ -371,57 +402,60 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
 __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t bounce) {
     uint32_t ne = A.ctl->eq_count, nf = A.ctl->fin_count;
     uint32_t total = ne + nf;
-    uint32_t nq = 0, na = 0;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        bool live = false;
+        int kind = 0;
+        uint32_t path;
         if (t >= ne) {
-            uint32_t path = A.fin[t - ne];
+            path = A.fin[t - ne];
             A.lbuf[path] = A.L[path];
-            continue;
+        } else {
+            path = A.eq[t];
+            ClosestHit h = A.eh[t];
+            float4 L4 = A.L[path];
+            f3 L = xyz(L4);
+            float4 wi4 = A.wi[path];
+            f3 wi = xyz(wi4);
+            HitRec nx;
+            bool hit = h.prim >= 0;
+            if (hit) {
+                nx = finish_hit(A.S, h);
+                hit = !(dot(nx.n, -wi) <= 0.0f);
+            }
+            if (!hit) {
+                A.lbuf[path] = L4;
+            } else {
+                float dens = wi4.w;
+                float4 xn = A.xn[path], wo4 = A.wo[path], b4 = A.beta[path];
+                int32_t xmat = static_cast<int32_t>(__float_as_uint(A.xp[path].w));
+                f3 xnrm = xyz(xn), wo = xyz(wo4), beta = xyz(b4);
+                float cw = dot(xnrm, wi);
+                f3 f = bsdf_eval(A.S.mats, xmat, xnrm, wo, wi);
+                beta = mul(beta, f) * cw / dens;
+                if (nx.light >= 0) {
+                    float cl = dot(nx.n, -wi);
+                    DevLight const &Ln = A.S.lights[nx.light];
+                    float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
+                    float w = dens * dens / (dens * dens + pl * pl);
+                    L = L + mul(beta, light_radiance(A.S, nx.light, -wi, nx.n) * w);
+                }
+                if (bounce + 1 >= A.max_depth) {
+                    A.lbuf[path] = make_float4(L.x, L.y, L.z, 0.0f);
+                } else {
+                    A.xp[path] =
+                        make_float4(nx.p.x, nx.p.y, nx.p.z, __uint_as_float(static_cast<uint32_t>(nx.mat)));
+                    A.xn[path] =
+                        make_float4(nx.n.x, nx.n.y, nx.n.z, __uint_as_float(static_cast<uint32_t>(nx.light)));
+                    A.wo[path] = make_float4(-wi.x, -wi.y, -wi.z, wo4.w);
+                    A.beta[path] = make_float4(beta.x, beta.y, beta.z, b4.w);
+                    A.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+                    live = true;
+                    kind = kind_of(A.S, nx.mat);
+                }
+            }
         }
-        uint32_t path = A.eq[t];
-        ClosestHit h = A.eh[t];
-        float4 L4 = A.L[path];
-        f3 L = xyz(L4);
-        if (h.prim < 0) {
-            A.lbuf[path] = L4;
-            continue;
-        }
-        HitRec nx = finish_hit(A.S, h);
-        float4 wi4 = A.wi[path];
-        f3 wi = xyz(wi4);
-        if (dot(nx.n, -wi) <= 0.0f) {
-            A.lbuf[path] = L4;
-            continue;
-        }
-        float dens = wi4.w;
-        float4 xp = A.xp[path], xn = A.xn[path], wo4 = A.wo[path], b4 = A.beta[path];
-        f3 xpos = xyz(xp), xnrm = xyz(xn), wo = xyz(wo4), beta = xyz(b4);
-        int32_t xmat = static_cast<int32_t>(__float_as_uint(xp.w));
-        float cw = dot(xnrm, wi);
-        f3 f = bsdf_eval(A.S.mats, xmat, xnrm, wo, wi);
-        beta = mul(beta, f) * cw / dens;
-        if (nx.light >= 0) {
-            float cl = dot(nx.n, -wi);
-            DevLight const &Ln = A.S.lights[nx.light];
-            float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
-            float w = dens * dens / (dens * dens + pl * pl);
-            L = L + mul(beta, light_radiance(A.S, nx.light, -wi, nx.n) * w);
-        }
-        (void)xpos;
-        if (bounce + 1 >= A.max_depth) {
-            A.lbuf[path] = make_float4(L.x, L.y, L.z, 0.0f);
-            continue;
-        }
-        A.xp[path] = make_float4(nx.p.x, nx.p.y, nx.p.z, __uint_as_float(static_cast<uint32_t>(nx.mat)));
-        A.xn[path] = make_float4(nx.n.x, nx.n.y, nx.n.z, __uint_as_float(static_cast<uint32_t>(nx.light)));
-        A.wo[path] = make_float4(-wi.x, -wi.y, -wi.z, wo4.w);
-        A.beta[path] = make_float4(beta.x, beta.y, beta.z, b4.w);
-        A.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
-        push_active(A, parity ^ 1, kind_of(A.S, nx.mat), path);
-        na++;
+        push_active(A, parity ^ 1, kind, path, live);
     }
-    (void)nq;
-    (void)na;
 }
 
 __global__ __launch_bounds__(256) void k_wf_accumulate(WfArgs A) {
@@ -483,7 +517,13 @@ struct WavefrontEngine {
     WfArgs key{};
     WavefrontTuning key_tune{};
     bool have_graph = false;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    double shadow_ms = 0, extend_ms = 0;
+    uint32_t trace_launches = 0;
     ~WavefrontEngine() {
+        for (auto &x : ev)
+            if (x)
+                (void)hipEventDestroy(x);
         if (exec)
             (void)hipGraphExecDestroy(exec);
         if (graph)
@@ -492,6 +532,12 @@ struct WavefrontEngine {
 };
 
 WavefrontEngine *wf_create() { return new WavefrontEngine(); }
+
+void wf_last_timing(WavefrontEngine const *e, double *shadow_ms, double *extend_ms, uint32_t *launches) {
+    *shadow_ms = e->shadow_ms;
+    *extend_ms = e->extend_ms;
+    *launches = e->trace_launches;
+}
 void wf_destroy(WavefrontEngine *e) { delete e; }
 
 namespace {
@@ -505,14 +551,29 @@ template <class K> uint32_t grid_for(K kernel, int wg) {
     return static_cast<uint32_t>(cus * per_cu);
 }
 
-template <int WG> void launch_trace(WfArgs const &A, hipStream_t s, uint32_t grid_shadow, uint32_t grid_extend) {
-    hipLaunchKernelGGL(k_wf_shadow<WG>, dim3(grid_shadow), dim3(WG), 0, s, A);
-    hipLaunchKernelGGL(k_wf_extend<WG>, dim3(grid_extend), dim3(WG), 0, s, A);
+using TraceKernel = void (*)(WfArgs);
+
+struct TraceKernels {
+    TraceKernel shadow, extend;
+    int wg, ldsd;
+};
+
+template <int WG, int D> TraceKernels trace_kernels() { return {k_wf_shadow<WG, D>, k_wf_extend<WG, D>, WG, D}; }
+
+// Workgroup sizes 64 / 128 / 256 and LDS stack depths 8 / 16 (deeper entries spill to memory).
+TraceKernels pick_trace(int wg, int ldsd) {
+    bool shallow = ldsd == 8;
+    switch (wg) {
+    case 64: return shallow ? trace_kernels<64, 8>() : trace_kernels<64, 16>();
+    case 128: return shallow ? trace_kernels<128, 8>() : trace_kernels<128, 16>();
+    default: return shallow ? trace_kernels<256, 8>() : trace_kernels<256, 16>();
+    }
 }
 
 bool same_config(WfArgs const &a, WfArgs const &b, WavefrontTuning const &ta, WavefrontTuning const &tb) {
     return std::memcmp(&a, &b, sizeof(WfArgs)) == 0 && ta.paths_in_flight == tb.paths_in_flight &&
-           ta.trace_wg == tb.trace_wg && ta.sort_by_material == tb.sort_by_material;
+           ta.trace_wg == tb.trace_wg && ta.stack_depth == tb.stack_depth &&
+           ta.sort_by_material == tb.sort_by_material;
 }
 
 } // namespace
@@ -550,15 +611,11 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     WF_HIP(e->eh.ensure(sizeof(ClosestHit) * cap));
     WF_HIP(e->ctl.ensure(sizeof(WfCtl)));
 
-    int wg = tune.trace_wg == 64 || tune.trace_wg == 128 ? tune.trace_wg : 256;
-    uint32_t g_shadow = 0, g_extend = 0;
-    switch (wg) {
-    case 64: g_shadow = grid_for(k_wf_shadow<64>, 64); g_extend = grid_for(k_wf_extend<64>, 64); break;
-    case 128: g_shadow = grid_for(k_wf_shadow<128>, 128); g_extend = grid_for(k_wf_extend<128>, 128); break;
-    default: g_shadow = grid_for(k_wf_shadow<256>, 256); g_extend = grid_for(k_wf_extend<256>, 256); break;
-    }
+    TraceKernels tk = pick_trace(tune.trace_wg, tune.stack_depth);
+    int wg = tk.wg;
+    uint32_t g_shadow = grid_for(tk.shadow, wg), g_extend = grid_for(tk.extend, wg);
     uint32_t trace_threads = (g_shadow > g_extend ? g_shadow : g_extend) * wg;
-    uint32_t spill_per_thread = spill_entries_for(P.S, kLdsDepth);
+    uint32_t spill_per_thread = spill_entries_for(P.S, tk.ldsd);
     WF_HIP(e->spill.ensure(sizeof(uint2) * static_cast<size_t>(trace_threads) * spill_per_thread));
 
     WfArgs A{};
@@ -604,6 +661,7 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     uint32_t g_item = static_cast<uint32_t>(cus) * 8u; // grid-strided per-path kernels
     uint32_t generations = (P.s_end - P.s_begin + gen_samples - 1) / gen_samples;
 
+    bool profile = tune.profile != 0;
     auto record_generation = [&](hipStream_t s) {
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, -1);
         hipLaunchKernelGGL(k_wf_generate, dim3(g_item), dim3(256), 0, s, A);
@@ -611,10 +669,21 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
             int par = static_cast<int>(k & 1u);
             hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, par);
             hipLaunchKernelGGL(k_wf_nee_bsdf, dim3(g_item), dim3(256), 0, s, A, par);
-            switch (wg) {
-            case 64: launch_trace<64>(A, s, g_shadow, g_extend); break;
-            case 128: launch_trace<128>(A, s, g_shadow, g_extend); break;
-            default: launch_trace<256>(A, s, g_shadow, g_extend); break;
+            if (profile)
+                (void)hipEventRecord(e->ev[0], s);
+            hipLaunchKernelGGL(tk.shadow, dim3(g_shadow), dim3(wg), 0, s, A);
+            if (profile)
+                (void)hipEventRecord(e->ev[1], s);
+            hipLaunchKernelGGL(tk.extend, dim3(g_extend), dim3(wg), 0, s, A);
+            if (profile) {
+                (void)hipEventRecord(e->ev[2], s);
+                float a = 0, b = 0;
+                (void)hipEventSynchronize(e->ev[2]);
+                (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
+                (void)hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
+                e->shadow_ms += a;
+                e->extend_ms += b;
+                e->trace_launches++;
             }
             hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, s, A.ctl, A.cnt);
             hipLaunchKernelGGL(k_wf_shade, dim3(g_item), dim3(256), 0, s, A, par, k);
@@ -623,9 +692,14 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
         hipLaunchKernelGGL(k_wf_next_gen, dim3(1), dim3(64), 0, s, A.ctl);
     };
 
+    e->shadow_ms = e->extend_ms = 0.0;
+    e->trace_launches = 0;
+    if (profile && !e->ev[0])
+        for (auto &ev : e->ev)
+            WF_HIP(hipEventCreate(&ev));
     hipLaunchKernelGGL(k_wf_begin_render, dim3(1), dim3(64), 0, stream, A.ctl);
     WF_HIP(hipGetLastError());
-    if (tune.use_graph) {
+    if (tune.use_graph && !profile) {
         if (!e->have_graph || !same_config(A, e->key, tune, e->key_tune)) {
             if (e->exec)
                 WF_HIP(hipGraphExecDestroy(e->exec));

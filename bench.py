@@ -11,7 +11,8 @@ HBM. Both execution forms are timed on the same workload — the plain one-threa
 one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--workload spheres|box|indirect] [--spp N]
-                    [--no-cpu-baseline] [--no-autotune] [--form both|plain|wavefront]
+                    [--no-cpu-baseline] [--no-autotune | --wf-config WG,DEPTH,SORT,CAP]
+                    [--probe-spp N] [--form both|plain|wavefront]
 """
 
 from __future__ import annotations
@@ -106,13 +107,20 @@ def timed_renders(x3d, r, scene, desc, film, steps, warmup, name):
     return time.perf_counter() - t0, stats
 
 
-def traffic_from_profile(name):
+def traffic_from_profile(name, kernel):
+    """HBM bytes per launch recorded in profiles/<name> (copy-calibrated FETCH_SIZE + WRITE_SIZE),
+    used only when the summary was taken on the same kernel instance."""
     tpath = os.path.join(ROOT, "profiles", name)
-    if os.path.exists(tpath):
-        for line in open(tpath):
-            if line.startswith("hbm_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
-    return None
+    if not os.path.exists(tpath):
+        return None
+    fields = {}
+    for line in open(tpath):
+        if ":" in line and line.split(":", 1)[0] in ("kernel", "hbm_bytes_per_launch"):
+            k, v = line.split(":", 1)
+            fields[k] = v.strip()
+    if fields.get("kernel") != kernel or "hbm_bytes_per_launch" not in fields:
+        return None
+    return float(fields["hbm_bytes_per_launch"])
 
 
 def main():
@@ -125,6 +133,10 @@ def main():
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--wf-config", default="", metavar="WG,DEPTH,SORT,CAP",
+                    help="fixed wavefront configuration instead of autotuning (SORT: material|none)")
+    ap.add_argument("--probe-spp", type=int, default=16,
+                    help="samples per pixel of each autotuner probe render (16 at 1024^2 = 16M paths)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (to calibrate rocprof byte counters)")
     args = ap.parse_args()
@@ -177,8 +189,15 @@ def main():
             r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=depth, seed=1)
         else:
             r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=depth, seed=1)
-            if not args.no_autotune:
-                rep = r.autotune(scene, desc.camera, w, h, 2)
+            if args.wf_config:
+                wg, sd, so, cap = args.wf_config.split(",")
+                out["autotune"] = {"workgroup_size": int(wg), "stack_depth": int(sd), "sort_key": so,
+                                   "queue_capacity": int(cap), "candidates": 0}
+                r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=depth, seed=1,
+                                 workgroup_size=int(wg), stack_depth=int(sd), queue_capacity=int(cap),
+                                 sort_key={"material": 1, "none": 2}[so])
+            elif not args.no_autotune:
+                rep = r.autotune(scene, desc.camera, w, h, args.probe_spp)
                 out["autotune"] = {"workgroup_size": rep.workgroup_size, "stack_depth": rep.stack_depth,
                                    "sort_key": ["default", "material", "none"][rep.sort_key],
                                    "queue_capacity": rep.queue_capacity, "candidates": rep.candidates,
@@ -226,7 +245,7 @@ def main():
         tests = (sp.closest_tests - sp.first_hit_tests) / launches
         launch_ms = sp.extend_ms / launches
         kernel = f"k_wf_extend<{wg or 256},{sd or 16}>"
-        traffic = traffic_from_profile(f"traffic_{desc.name}_wavefront.md")
+        traffic = traffic_from_profile(f"traffic_{desc.name}_wavefront.md", kernel)
         out["wavefront_kernel_ms"] = {"extend": sp.extend_ms, "shadow": sp.shadow_ms, "launches": sp.trace_launches,
                                       "all_sample_kernels": sp.trace_ms}
     else:
@@ -234,7 +253,7 @@ def main():
         tests = (s0.closest_tests - s0.first_hit_tests) + s0.anyhit_tests
         launch_ms = float(np.mean([s.trace_ms for s in R["stats"]]))
         kernel = "k_render_mis"
-        traffic = traffic_from_profile(f"traffic_{desc.name}_plain.md")
+        traffic = traffic_from_profile(f"traffic_{desc.name}_plain.md", kernel)
     alg_bytes = BYTES_PER_NODE * nodes + BYTES_PER_TEST * tests
     alg_flops = FLOPS_PER_NODE * nodes + FLOPS_PER_TEST * tests
     achieved_gbs = alg_bytes / (launch_ms * 1e-3) / 1e9

@@ -180,7 +180,7 @@ typedef struct x3d_render_options {
     uint32_t max_path_len; /* BIDIRECT_MIS: vertices per subpath (reference: 8) */
     uint64_t seed;         /* PCG seed; sample s of pixel p uses stream (seed, p, s) */
     uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default */
-    uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default (4M) */
+    uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default (4M), max 64M */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
     uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16 */
     int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */
@@ -219,9 +219,12 @@ typedef struct x3d_render_stats {
 } x3d_render_stats;
 
 /* Autotuner (no reference counterpart): renders `probe_spp` samples per pixel of the given
- * workload with every candidate wavefront configuration (traversal workgroup size, LDS stack
- * depth, sort key, queue capacity), keeps the fastest in the renderer's options and reports it.
- * The film does not depend on the configuration. */
+ * workload with candidate wavefront configurations, keeps the fastest in the renderer's options
+ * and reports it. Stage 1 tries traversal workgroup size {64,128,256} x LDS stack depth {8,16} x
+ * sort key {material, none} at the default capacity; stage 2 tries queue capacities
+ * {2M, 8M, 16M} paths for the stage-1 winner, skipping any above width*height*probe_spp (a
+ * probe that small cannot tell them apart). Each candidate is timed as the best of two renders
+ * after one warm-up render. The film does not depend on the configuration. */
 typedef struct x3d_autotune_report {
     uint32_t workgroup_size, stack_depth, queue_capacity;
     int32_t sort_key;

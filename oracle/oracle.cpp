@@ -971,6 +971,131 @@ template <class T> V3<T> li_direct(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, 
     return L;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Bidirectional path tracing (bidirect_mis_path_tracer, pathtracer.cpp:12-137, 247-304,
+// 351-452, 754-803).
+// ---------------------------------------------------------------------------------------------
+constexpr uint32_t kMaxPathLen = 16; // upper bound of RenderOpts::max_path_len
+
+// sampled_pathlet (pathtracer.cpp:12-45): a subpath vertex, the unit vector from it towards the
+// previous vertex (towards_prev) and the prefix transport up to it (light_transport_info,
+// pathtracer.cpp:254-274), accumulated while the walk extends the subpath.
+template <class T> struct Pathlet {
+    Hit<T> x;
+    V3<T> v;
+    V3<T> transport;
+};
+
+// sample_path (pathtracer.cpp:70-92): extends path[0..len) by BSDF sampling (the reference's
+// mixture densities) until max_len vertices, a density within 1e-5 of zero, a miss or a
+// back-facing hit. IMPORTANCE selects projected_brdf (light subpaths) over
+// projected_adjoint_brdf (camera subpaths) for the prefix transport (pathtracer.cpp:54-68,
+// 259-270): transport_{k+1} = transport_k * (f * cos) / dens.
+template <class T, bool IMPORTANCE>
+uint32_t extend_walk(Ctx<T> &c, Pathlet<T> *path, uint32_t len, uint32_t max_len, Pcg &g) {
+    Scene const &s = c.s;
+    while (len < max_len) {
+        Pathlet<T> const &cur = path[len - 1];
+        int mat = mat_of<T>(s, cur.x.geo);
+        T dens;
+        V3<T> i = bsdf_sample<T>(s, mat, cur.x.n, cur.v, g, &dens, true);
+        if (near0(dens))
+            break;
+        Hit<T> nx = trace<T>(c, cur.x.p, i);
+        if (!nx.valid() || !(inner(nx.n, -i) > T(0)))
+            break;
+        T cos_w = inner(cur.x.n, i);
+        V3<T> f = IMPORTANCE ? bsdf_eval<T>(s, mat, cur.x.n, i, cur.v) : bsdf_eval<T>(s, mat, cur.x.n, cur.v, i);
+        path[len].x = nx;
+        path[len].v = -i;
+        path[len].transport = mod(cur.transport, (f * cos_w) / dens);
+        len++;
+    }
+    return len;
+}
+
+// bidirect_mis_path_tracer::sample (pathtracer.cpp:775-803) for one camera ray: a camera
+// subpath from the cached first hit, an emission sample (light by power, point by area,
+// cosine-weighted direction; pathtracer.cpp:754-767, light.cpp:26-32) and a light subpath,
+// then transport_all_connectible_subpaths (pathtracer.cpp:351-452): for each total length,
+// the average over its strategies with the reference's uniform weights — (0,0) and every
+// light_plen == 0 or connection strategy counts, occluded or not; cam_plen == 0 does not.
+// Draws: camera walk, then 6 for the emission sample, then the light walk.
+template <class T> V3<T> li_bdpt(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg &g) {
+    Scene const &s = c.s;
+    V3<T> rad;
+    uint32_t max_len = c.opt.max_path_len;
+    if (!first.valid() || max_len == 0 || s.lights.empty())
+        return rad;
+    Pathlet<T> cam[kMaxPathLen], lig[kMaxPathLen];
+    cam[0].x = first;
+    cam[0].v = -ray_v;
+    cam[0].transport = V3<T>(T(1));
+    uint32_t cam_len = extend_walk<T, false>(c, cam, 1, max_len, g);
+
+    LightSample<T> em = sample_light<T>(s, g);
+    T e0 = T(g.draw());
+    T e1 = T(g.draw());
+    V3<T> w = cos_hemisphere(em.n, e0, e1);
+    T w_dens = inner(em.n, w) / pi_v<T>();
+    Light const &light = s.lights[em.light];
+    uint32_t light_len = 0;
+    Hit<T> v0 = trace<T>(c, em.p, w);
+    if (v0.valid() && inner(v0.n, -w) > T(0)) {
+        lig[0].x = v0;
+        lig[0].v = -w;
+        lig[0].transport = V3<T>(T(1));
+        light_len = extend_walk<T, true>(c, lig, 1, max_len, g);
+    }
+    // light.projected_radiance(w, n) / (dens0 * area_dens) (pathtracer.cpp:418-420)
+    T cos_e = inner(em.n, w);
+    V3<T> emitted = cos_e > T(0) ? cvt<T>(light.rad) * cos_e : V3<T>();
+    V3<T> light_emission = emitted / (w_dens * em.area_dens);
+
+    for (uint32_t plen = 1; plen <= cam_len + light_len + 1; plen++) {
+        int cp = static_cast<int>(std::min(plen - 1, cam_len));
+        uint32_t lp = plen - 1 - static_cast<uint32_t>(cp);
+        V3<T> psum;
+        T wsum = T(0);
+        for (; cp >= 0 && lp <= light_len; lp++, cp--) {
+            if (lp == 0 && cp == 0) {
+                int l0 = s.meshes[cam[0].x.geo].light;
+                if (l0 >= 0)
+                    psum = psum + light_radiance<T>(s.lights[l0], cam[0].v, cam[0].x.n);
+                wsum = wsum + T(1);
+            } else if (lp == 0) {
+                Pathlet<T> const &cj = cam[cp - 1];
+                V3<T> wi;
+                T r2 = 0, cl = 0;
+                bool ok;
+                V3<T> tr = direct_from<T>(c, em, cj.x, cj.v, &wi, &r2, &cl, &ok);
+                psum = psum + mod(tr, cj.transport);
+                wsum = wsum + T(1);
+            } else if (cp > 0) {
+                Pathlet<T> const &lj = lig[lp - 1];
+                Pathlet<T> const &cj = cam[cp - 1];
+                V3<T> join = cj.x.p - lj.x.p;
+                T dist = norm(join);
+                join = join / dist;
+                T cos_wo = inner(lj.x.n, join);
+                T cos_wi = inner(cj.x.n, -join);
+                if (cos_wo > T(0) && cos_wi > T(0) && !shadow<T>(c, lj.x.p, join, T(1e-3f), dist - T(1e-3f))) {
+                    V3<T> sub = mod(light_emission, lj.transport);
+                    T geo = cos_wi * cos_wo / (dist * dist);
+                    V3<T> fl = bsdf_eval<T>(s, mat_of<T>(s, lj.x.geo), lj.x.n, join, lj.v);
+                    V3<T> fc = bsdf_eval<T>(s, mat_of<T>(s, cj.x.geo), cj.x.n, cj.v, -join);
+                    V3<T> tr = mod(mod(sub, fl), fc) * geo;
+                    psum = psum + mod(tr, cj.transport);
+                }
+                wsum = wsum + T(1);
+            }
+        }
+        if (wsum > T(0))
+            rad = rad + psum / wsum;
+    }
+    return rad;
+}
+
 } // namespace
 
 template <class T>
@@ -987,6 +1112,8 @@ V3<T> trace_sample(Scene const &s, RenderOpts const &opt, Hit<T> const &first, V
         return li_unidirect<T>(c, first, ray_v, g);
     case TR_DIRECT:
         return li_direct<T>(c, first, ray_v, g);
+    case TR_BIDIRECT_MIS:
+        return li_bdpt<T>(c, first, ray_v, g);
     case TR_NORMAL: // normal_tracer (pathtracer.cpp:492-506)
         return first.valid() ? (first.n + V3<T>(T(1.0f))) / T(2.0f) : V3<T>();
     default:

@@ -995,6 +995,8 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
         return fail(X3D_ERR_INVALID_ARGUMENT, "unknown sort key");
     if (opts->stack_depth != 0 && opts->stack_depth != 8 && opts->stack_depth != 16)
         return fail(X3D_ERR_INVALID_ARGUMENT, "stack_depth must be 0, 8 or 16");
+    if (opts->queue_capacity > x3d::kMaxQueueCapacity)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "queue_capacity exceeds " + std::to_string(x3d::kMaxQueueCapacity));
     int rc = require_device();
     if (rc)
         return rc;
@@ -1067,7 +1069,7 @@ int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_cam
         r->opt.queue_capacity = cap;
         x3d_render_stats st;
         double best = 1e300;
-        for (int k = 0; k < 2; k++) { // first call captures the graph
+        for (int k = 0; k < 3; k++) { // first call captures the graph; best of the other two
             int e = render_impl(r, scene, camera, width, height, probe_spp, static_cast<float *>(r->out.p), &st);
             if (e)
                 return e;
@@ -1077,36 +1079,46 @@ int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_cam
         *ms = best;
         return X3D_OK;
     };
+    auto consider = [&](x3d_autotune_report &rep, uint32_t wg, uint32_t d, int32_t so, uint32_t cap) -> int {
+        double ms;
+        int e = time_config(wg, d, so, cap, &ms);
+        if (e)
+            return e;
+        rep.candidates++;
+        if (ms < rep.best_ms) {
+            rep.best_ms = ms;
+            rep.workgroup_size = wg;
+            rep.stack_depth = d;
+            rep.sort_key = so;
+            rep.queue_capacity = cap;
+        }
+        return X3D_OK;
+    };
+    // Stage 1: workgroup size x LDS stack depth x sort key at the default capacity. Stage 2: the
+    // queue capacity for the winner. A capacity above the probe's path count is never tried: the
+    // probe would clamp it to one generation and could not tell it apart from a smaller one.
     const uint32_t wgs[] = {64, 128, 256};
     const uint32_t depths[] = {8, 16};
     const int32_t sorts[] = {X3D_SORT_MATERIAL, X3D_SORT_NONE};
-    const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23};
+    const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
+    const uint64_t probe_paths = npix * probe_spp;
     x3d_autotune_report rep{};
     rep.best_ms = 1e300;
     rc = time_config(0, 0, X3D_SORT_DEFAULT, 0, &rep.default_ms);
+    for (uint32_t wg : wgs)
+        for (uint32_t d : depths)
+            for (int32_t so : sorts)
+                if (!rc)
+                    rc = consider(rep, wg, d, so, x3d::kDefaultQueueCapacity);
+    const uint32_t wg1 = rep.workgroup_size, d1 = rep.stack_depth;
+    const int32_t so1 = rep.sort_key;
+    for (uint32_t cap : caps)
+        if (!rc && cap != x3d::kDefaultQueueCapacity && cap <= probe_paths)
+            rc = consider(rep, wg1, d1, so1, cap);
     if (rc) {
         r->opt = saved;
         return rc;
     }
-    for (uint32_t wg : wgs)
-        for (uint32_t d : depths)
-            for (int32_t so : sorts)
-                for (uint32_t cap : caps) {
-                    double ms;
-                    rc = time_config(wg, d, so, cap, &ms);
-                    if (rc) {
-                        r->opt = saved;
-                        return rc;
-                    }
-                    rep.candidates++;
-                    if (ms < rep.best_ms) {
-                        rep.best_ms = ms;
-                        rep.workgroup_size = wg;
-                        rep.stack_depth = d;
-                        rep.sort_key = so;
-                        rep.queue_capacity = cap;
-                    }
-                }
     r->opt = saved;
     r->opt.workgroup_size = rep.workgroup_size;
     r->opt.stack_depth = rep.stack_depth;

@@ -180,7 +180,8 @@ typedef struct x3d_render_options {
     uint32_t max_path_len; /* BIDIRECT_MIS: vertices per subpath (reference: 8) */
     uint64_t seed;         /* PCG seed; sample s of pixel p uses stream (seed, p, s) */
     uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default */
-    uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default (4M), max 64M */
+    uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default (4M), max 64M;
+                              BIDIRECT_MIS: (pixel, sample) items per generation (default 4M) */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
     uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16 */
     int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */

@@ -51,3 +51,22 @@ def test_scene_edit_errors_without_gpu(x3d):
         s.add_material(scenes.Material(scenes.COOK_TORR, roughness=0.0))
     with pytest.raises(x3d.X3DError):
         s.add_material(scenes.Material(scenes.MIXTURE, sub0=0, sub1=9))
+
+
+def test_render_option_errors_without_gpu(x3d):
+    """Renderer options are validated before the device is touched."""
+    import pytest
+
+    bad = [
+        dict(pt_type=x3d.PT_BIDIRECT_MIS, max_path_len=0),
+        dict(pt_type=x3d.PT_BIDIRECT_MIS, max_path_len=17),
+        dict(pt_type=x3d.PT_BIDIRECT_MIS, form=x3d.FORM_WAVEFRONT),
+        dict(pt_type=x3d.PT_BIDIRECT_LT2),
+        dict(pt_type=x3d.PT_UNIDIRECT_MIS, workgroup_size=96),
+        dict(pt_type=x3d.PT_UNIDIRECT_MIS, stack_depth=12),
+        dict(pt_type=x3d.PT_UNIDIRECT_MIS, queue_capacity=(1 << 26) + 1),
+    ]
+    for kw in bad:
+        with pytest.raises(x3d.X3DError) as e:
+            x3d.Renderer(**kw)
+        assert e.value.args, kw

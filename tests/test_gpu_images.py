@@ -7,6 +7,8 @@
   except for the expected tail, and the mean z (a bias test) must be ~0.
 - Against the reference renderer itself (oracle/_ref, its unidirect_lt1 tracer and minstd
   streams): same per-pixel test for the GPU's reference-mode tracer.
+- The bidirectional tracer against the FP64 oracle and against the reference renderer's own
+  bidirectional tracer.
 Relative MSE against the FP64 oracle is printed for the record.
 """
 
@@ -54,17 +56,24 @@ def test_box_workload_bit_exact(x3d, oracle_mod):
     print(f"[gpu] box 256x256x16: {st.kernel_ms:.2f} ms, {st.closest_queries + st.anyhit_queries} rays")
 
 
-def test_box_vs_fp64_oracle(x3d, oracle_mod):
-    desc = scenes.box_scene()
-    w, h = 64, 64
-    mean, se, _ = gpu_replicates(x3d, desc, x3d.PT_UNIDIRECT_MIS, w, h, 64, 8, max_depth=5)
-    pix = np.random.default_rng(3).choice(w * h, size=1024, replace=False).astype(np.uint32)
+def vs_fp64_oracle(x3d, oracle_mod, desc, pt, w, h, gpu_spp, oracle_spp, npix, seed, **kw):
+    """z-scores of 32 independently seeded GPU renders against one FP64 oracle render (same
+    estimator, other seeds) on a seeded pixel subset. Both standard errors come from the GPU
+    replicates' per-sample spread (same estimator, same per-sample variance), so they rest on
+    31 degrees of freedom."""
+    reps = 32
+    mean, se, imgs = gpu_replicates(x3d, desc, pt, w, h, gpu_spp, reps, **kw)
+    pix = np.random.default_rng(seed).choice(w * h, size=npix, replace=False).astype(np.uint32)
     orc = oracle_mod.Oracle(desc)
-    reps = [orc.render(oracle_mod.TR_UNIDIRECT_MIS, w, h, 128, seed=500 + s, max_depth=5, pixels=pix, prec=1)
-            [:, :3] for s in range(4)]
-    reps = np.stack(reps)
-    om, ose = reps.mean(0), reps.std(0, ddof=1) / np.sqrt(len(reps))
-    z, rel_mse = z_report(mean[pix], se[pix], om, ose)
+    om = orc.render(pt, w, h, oracle_spp, seed=500 + seed, pixels=pix, prec=1, threads=8,
+                    max_depth=kw.get("max_depth", 5), max_path_len=kw.get("max_path_len", 8))[:, :3]
+    per_sample_sd = imgs.std(0, ddof=1)[pix] * np.sqrt(gpu_spp)
+    return z_report(mean[pix], se[pix], om, per_sample_sd / np.sqrt(oracle_spp))
+
+
+def test_box_vs_fp64_oracle(x3d, oracle_mod):
+    z, rel_mse = vs_fp64_oracle(x3d, oracle_mod, scenes.box_scene(), x3d.PT_UNIDIRECT_MIS, 64, 64, 64, 512, 1024, 3,
+                                max_depth=5)
     print(f"[gpu] box vs FP64 oracle: rel MSE {rel_mse:.2e}, |z|>3 in {np.mean(np.abs(z) > 3):.4f}, mean z {z.mean():.3f}")
     assert np.mean(np.abs(z) > 3) < 0.02
     assert abs(z.mean()) < 4 / np.sqrt(z.size) + 0.05
@@ -74,15 +83,20 @@ def test_box_vs_fp64_oracle(x3d, oracle_mod):
 def test_spheres_vs_fp64_oracle(x3d, oracle_mod):
     """GGX spheres workload (reduced count and resolution so the FP64 oracle finishes)."""
     desc = scenes.spheres_scene(count=40, target_tris=120000, seed=91)
-    w, h = 64, 64
-    mean, se, _ = gpu_replicates(x3d, desc, x3d.PT_UNIDIRECT_MIS, w, h, 32, 8, max_depth=8)
-    pix = np.random.default_rng(4).choice(w * h, size=768, replace=False).astype(np.uint32)
-    orc = oracle_mod.Oracle(desc)
-    reps = np.stack([orc.render(oracle_mod.TR_UNIDIRECT_MIS, w, h, 64, seed=700 + s, max_depth=8, pixels=pix,
-                                prec=1)[:, :3] for s in range(4)])
-    om, ose = reps.mean(0), reps.std(0, ddof=1) / np.sqrt(len(reps))
-    z, rel_mse = z_report(mean[pix], se[pix], om, ose)
+    z, rel_mse = vs_fp64_oracle(x3d, oracle_mod, desc, x3d.PT_UNIDIRECT_MIS, 64, 64, 32, 256, 768, 4, max_depth=8)
     print(f"[gpu] spheres vs FP64 oracle: rel MSE {rel_mse:.2e}, |z|>3 in {np.mean(np.abs(z) > 3):.4f}, "
+          f"mean z {z.mean():.3f}")
+    assert np.mean(np.abs(z) > 3) < 0.03
+    assert abs(z.mean()) < 4 / np.sqrt(z.size) + 0.05
+
+
+def test_bdpt_indirect_vs_fp64_oracle(x3d, oracle_mod):
+    """Bidirectional tracer on the indirectly lit room with GGX spheres (subpaths of up to 6
+    vertices), 20k-triangle spheres as in the full workload, reduced resolution."""
+    desc = scenes.indirect_scene(level=5)
+    z, rel_mse = vs_fp64_oracle(x3d, oracle_mod, desc, x3d.PT_BIDIRECT_MIS, 64, 64, 32, 256, 512, 6,
+                                max_path_len=6)
+    print(f"[gpu] bdpt indirect vs FP64 oracle: rel MSE {rel_mse:.2e}, |z|>3 in {np.mean(np.abs(z) > 3):.4f}, "
           f"mean z {z.mean():.3f}")
     assert np.mean(np.abs(z) > 3) < 0.03
     assert abs(z.mean()) < 4 / np.sqrt(z.size) + 0.05
@@ -106,4 +120,24 @@ def test_reference_mode_vs_reference_renderer(x3d, ref_mod):
     print(f"[gpu] lt1 vs reference renderer: rel MSE {rel_mse:.2e}, |z|>3 in {np.mean(np.abs(z) > 3):.4f}, "
           f"mean z {z.mean():.3f}")
     assert np.mean(np.abs(z) > 3) < 0.03
+    assert abs(z.mean()) < 4 / np.sqrt(z.size) + 0.05
+
+
+def test_bdpt_vs_reference_renderer(x3d, ref_mod):
+    """GPU bidirectional tracer (PCG streams, subpaths of 8 vertices) against the reference
+    renderer's own bidirect_mis_path_tracer (pathtracer.cpp:754-803) on the indirect room, in
+    8 x 8-pixel blocks (the estimator's per-pixel tails are heavy: connections near creases)."""
+    desc = scenes.indirect_scene(level=3)
+    w, h, spp, reps = 48, 48, 64, 16
+    _, _, imgs = gpu_replicates(x3d, desc, x3d.PT_BIDIRECT_MIS, w, h, spp, reps, max_path_len=8)
+    ref = ref_mod.Reference(desc)
+    img, taken = ref.render(ref_mod.TR_BIDIRECT_MIS, w, h, 256, threads=64)
+    blk = lambda x: x.reshape(h // 8, 8, w // 8, 8, -1).mean((1, 3)).reshape(-1, x.shape[-1])
+    gb = np.stack([blk(x) for x in imgs])
+    rb = blk(img[:, :3].astype(np.float64))
+    se = gb.std(0, ddof=1) / np.sqrt(reps)
+    ref_se = gb.std(0, ddof=1) * np.sqrt(spp / taken)
+    z, rel_mse = z_report(gb.mean(0), se, rb, ref_se)
+    print(f"[gpu] bdpt vs reference renderer: rel MSE {rel_mse:.2e}, max |z| {np.abs(z).max():.2f}, mean z {z.mean():.3f}")
+    assert np.abs(z).max() < 4.5
     assert abs(z.mean()) < 4 / np.sqrt(z.size) + 0.05

@@ -1,7 +1,7 @@
 """GPU parity: the HIP library (through its C ABI) against the CPU oracle, bit for bit.
 
 Closest hits (primitive, t, barycentrics, vertex, shading normal) must equal the oracle's —
-which is pinned to the reference's own intersector in test_oracle_ref.py / test_golden.py — in
+which is pinned to the reference's own intersector in test_golden.py — in
 both traversal modes; any-hit answers, BSDF eval / pdf / sample, camera rays, PCG streams and
 whole films of every tracer must be identical as well.
 """
@@ -140,6 +140,11 @@ RENDER_CASES = [
     ("box", "NORMAL", {}),
     ("spheres_small", "UNIDIRECT_MIS", dict(max_depth=8)),
     ("spheres_small", "UNIDIRECT_LT1", {}),
+    ("box", "BIDIRECT_MIS", dict(max_path_len=8)),
+    ("box", "BIDIRECT_MIS", dict(max_path_len=1)),
+    ("indirect_l3", "BIDIRECT_MIS", dict(max_path_len=6)),
+    ("spheres_small", "BIDIRECT_MIS", dict(max_path_len=5)),
+    ("furnace", "BIDIRECT_MIS", dict(max_path_len=16)),
 ]
 
 
@@ -153,7 +158,7 @@ def test_render_bit_exact_vs_oracle(x3d, oracle_mod, scene_name, tracer, kw):
     r = x3d.Renderer(pt, seed=11, **kw)
     img, st = r.render(gs, desc.camera, w, h, spp)
     orc = oracle_mod.Oracle(desc)
-    ref = orc.render(pt, w, h, spp, seed=11, max_depth=kw.get("max_depth", 5))
+    ref = orc.render(pt, w, h, spp, seed=11, max_depth=kw.get("max_depth", 5), max_path_len=kw.get("max_path_len", 8))
     assert np.isfinite(img).all()
     assert img[:, :3].max() > 0
     assert np.array_equal(_bits(img), _bits(ref)), np.abs(img - ref).max()
@@ -209,3 +214,23 @@ def test_wavefront_autotune_and_profile(x3d):
     img, st = prof.render(gs, desc.camera, w, h, spp)
     assert np.array_equal(_bits(img), _bits(plain))
     assert st.trace_launches == 8 and st.extend_ms > 0 and st.shadow_ms > 0
+
+
+def test_bdpt_generations_and_edge_cases(x3d, oracle_mod):
+    """Bidirectional tracer split into several generations of samples (5 spp in generations of
+    2, 2, 1 samples) gives the oracle's film; a scene without lights renders black."""
+    desc = QUERY_SCENES["box"]()
+    w, h, spp = 48, 40, 5
+    gs = x3d.Scene(desc)
+    orc = oracle_mod.Oracle(desc)
+    ref = orc.render(x3d.PT_BIDIRECT_MIS, w, h, spp, seed=4, max_path_len=7)
+    for cap in (w * h * 2, w * h * 2 + 17, 1):
+        print(f"[gpu] bdpt generations, capacity {cap}", flush=True)
+        img, st = x3d.Renderer(x3d.PT_BIDIRECT_MIS, max_path_len=7, seed=4, queue_capacity=cap).render(
+            gs, desc.camera, w, h, spp)
+        assert np.array_equal(_bits(img), _bits(ref)), np.abs(img - ref).max()
+        assert st.num_samples == w * h * spp
+    dark = scenes.box_scene()
+    dark.lights = []
+    img, _ = x3d.Renderer(x3d.PT_BIDIRECT_MIS, seed=4).render(x3d.Scene(dark), dark.camera, 16, 16, 2)
+    assert not img[:, :3].any()

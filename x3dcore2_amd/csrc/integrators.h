@@ -68,6 +68,14 @@ X3D_DI f3 light_radiance(DevScene const &S, int32_t l, f3 w, f3 n) {
     return dot(n, w) > 0.0f ? light_rad(S, l) : mk(0.0f, 0.0f, 0.0f);
 }
 
+// Subpath vertex of the bidirectional tracer (sampled_pathlet, pathtracer.cpp:12-45): position,
+// shading normal, unit vector towards the previous vertex, prefix transport, material.
+constexpr uint32_t kMaxPathLen = 16; // upper bound of the per-subpath vertex count
+struct Pathlet {
+    f3 p, n, v, transport;
+    int32_t mat;
+};
+
 // State of one MIS path between bounces (see Tracer::mis_begin / mis_bounce).
 struct PathState {
     HitRec x;
@@ -98,13 +106,13 @@ template <int WG, int LDSD> struct Tracer {
     // transport_illum_source (pathtracer.cpp:152-176) divided by the light sample's area
     // density (pathtracer.cpp:236-239). On success returns true with the unweighted estimate,
     // the direction to the light, the squared distance and the cosine at the light.
-    X3D_DI bool direct(LightSample const &ls, HitRec const &x, f3 wo, f3 &contrib, f3 &wi, float &r2,
+    X3D_DI bool direct(LightSample const &ls, f3 xp, f3 xn, int32_t xmat, f3 wo, f3 &contrib, f3 &wi, float &r2,
                        float &cos_l) {
-        f3 l = x.p - ls.p;
+        f3 l = xp - ls.p;
         r2 = dot(l, l);
         f3 in = l / __builtin_sqrtf(r2);
         float co = dot(ls.n, in);
-        float ci = dot(x.n, -in);
+        float ci = dot(xn, -in);
         if (!(co > 0 && ci > 0))
             return false; // area_light::eval is zero (light.cpp:41-52)
         f3 illum = ((light_rad(S, ls.light) * ci) * co) / r2;
@@ -112,12 +120,16 @@ template <int WG, int LDSD> struct Tracer {
             return false;
         float dist = norm(l);
         wi = -l / dist;
-        if (occluded(x.p, wi, 1e-4f, dist - 1e-3f))
+        if (occluded(xp, wi, 1e-4f, dist - 1e-3f))
             return false;
-        f3 f = bsdf_eval(S.mats, x.mat, x.n, wo, wi);
+        f3 f = bsdf_eval(S.mats, xmat, xn, wo, wi);
         contrib = mul(illum, f) / ls.area_dens;
         cos_l = dot(ls.n, -wi);
         return true;
+    }
+    X3D_DI bool direct(LightSample const &ls, HitRec const &x, f3 wo, f3 &contrib, f3 &wi, float &r2,
+                       float &cos_l) {
+        return direct(ls, x.p, x.n, x.mat, wo, contrib, wi, r2, cos_l);
     }
 
     // Fixed-depth unidirectional path tracing: next-event estimation and BSDF sampling combined
@@ -253,6 +265,113 @@ template <int WG, int LDSD> struct Tracer {
             wo = -wi;
         }
         return L;
+    }
+
+    // sample_path (pathtracer.cpp:70-92): extends path[0..len) by BSDF sampling (the reference's
+    // mixture densities) until max_len vertices, a density within 1e-5 of zero, a miss or a
+    // back-facing hit, accumulating the prefix transport (f * cos) / dens of light_transport_info
+    // (pathtracer.cpp:254-274); IMPORTANCE: projected_brdf, else projected_adjoint_brdf.
+    template <bool IMPORTANCE> X3D_DI uint32_t extend_walk(Pathlet *path, uint32_t len, uint32_t max_len, Pcg &g) {
+        while (len < max_len) {
+            Pathlet const cur = path[len - 1];
+            float dens;
+            f3 i = bsdf_sample(S.mats, cur.mat, cur.n, cur.v, g, &dens, true);
+            if (near0(dens))
+                break;
+            HitRec nx;
+            if (!trace(cur.p, i, nx) || !(dot(nx.n, -i) > 0.0f))
+                break;
+            float cos_w = dot(cur.n, i);
+            f3 f = IMPORTANCE ? bsdf_eval(S.mats, cur.mat, cur.n, i, cur.v) : bsdf_eval(S.mats, cur.mat, cur.n, cur.v, i);
+            Pathlet &nxt = path[len];
+            nxt.p = nx.p;
+            nxt.n = nx.n;
+            nxt.v = -i;
+            nxt.transport = mul(cur.transport, (f * cos_w) / dens);
+            nxt.mat = nx.mat;
+            len++;
+        }
+        return len;
+    }
+
+    // bidirect_mis_path_tracer::sample (pathtracer.cpp:775-803): camera subpath from the cached
+    // first hit, emission sample (light by power, point by area, cosine direction;
+    // pathtracer.cpp:754-767, light.cpp:26-32), light subpath, then
+    // transport_all_connectible_subpaths (pathtracer.cpp:351-452): per total length, the
+    // average over its strategies with uniform weights; (0,0), every light_plen == 0 and every
+    // connection strategy counts (occluded or not), cam_plen == 0 does not.
+    X3D_DI f3 li_bdpt(HitRec const &first, bool valid, f3 ray_v, Pcg &g, uint32_t max_len, Pathlet *cam,
+                      Pathlet *lig) {
+        f3 rad = mk(0.0f, 0.0f, 0.0f);
+        if (!valid || max_len == 0 || S.num_lights == 0)
+            return rad;
+        cam[0].p = first.p;
+        cam[0].n = first.n;
+        cam[0].v = -ray_v;
+        cam[0].transport = mk(1.0f, 1.0f, 1.0f);
+        cam[0].mat = first.mat;
+        uint32_t cam_len = extend_walk<false>(cam, 1, max_len, g);
+
+        LightSample em = sample_light(S, g);
+        float e0 = g.draw();
+        float e1 = g.draw();
+        f3 w = cos_hemisphere(em.n, e0, e1);
+        float w_dens = dot(em.n, w) / kPi;
+        uint32_t light_len = 0;
+        HitRec v0;
+        if (trace(em.p, w, v0) && dot(v0.n, -w) > 0.0f) {
+            lig[0].p = v0.p;
+            lig[0].n = v0.n;
+            lig[0].v = -w;
+            lig[0].transport = mk(1.0f, 1.0f, 1.0f);
+            lig[0].mat = v0.mat;
+            light_len = extend_walk<true>(lig, 1, max_len, g);
+        }
+        float cos_e = dot(em.n, w);
+        f3 emitted = cos_e > 0.0f ? light_rad(S, em.light) * cos_e : mk(0.0f, 0.0f, 0.0f);
+        f3 light_emission = emitted / (w_dens * em.area_dens);
+
+        for (uint32_t plen = 1; plen <= cam_len + light_len + 1; plen++) {
+            int cp = static_cast<int>(plen - 1 < cam_len ? plen - 1 : cam_len);
+            uint32_t lp = plen - 1 - static_cast<uint32_t>(cp);
+            f3 psum = mk(0.0f, 0.0f, 0.0f);
+            float wsum = 0.0f;
+            for (; cp >= 0 && lp <= light_len; lp++, cp--) {
+                if (lp == 0 && cp == 0) {
+                    if (first.light >= 0)
+                        psum = psum + light_radiance(S, first.light, cam[0].v, cam[0].n);
+                    wsum = wsum + 1.0f;
+                } else if (lp == 0) {
+                    Pathlet const cj = cam[cp - 1];
+                    f3 tr = mk(0.0f, 0.0f, 0.0f), wi;
+                    float r2, cl;
+                    if (!direct(em, cj.p, cj.n, cj.mat, cj.v, tr, wi, r2, cl))
+                        tr = mk(0.0f, 0.0f, 0.0f);
+                    psum = psum + mul(tr, cj.transport);
+                    wsum = wsum + 1.0f;
+                } else if (cp > 0) {
+                    Pathlet const lj = lig[lp - 1];
+                    Pathlet const cj = cam[cp - 1];
+                    f3 join = cj.p - lj.p;
+                    float dist = norm(join);
+                    join = join / dist;
+                    float cos_wo = dot(lj.n, join);
+                    float cos_wi = dot(cj.n, -join);
+                    if (cos_wo > 0.0f && cos_wi > 0.0f && !occluded(lj.p, join, 1e-3f, dist - 1e-3f)) {
+                        f3 sub = mul(light_emission, lj.transport);
+                        float geo = cos_wi * cos_wo / (dist * dist);
+                        f3 fl = bsdf_eval(S.mats, lj.mat, lj.n, join, lj.v);
+                        f3 fc = bsdf_eval(S.mats, cj.mat, cj.n, cj.v, -join);
+                        f3 tr = mul(mul(sub, fl), fc) * geo;
+                        psum = psum + mul(tr, cj.transport);
+                    }
+                    wsum = wsum + 1.0f;
+                }
+            }
+            if (wsum > 0.0f)
+                rad = rad + psum / wsum;
+        }
+        return rad;
     }
 
     // direct_path_tracer (pathtracer.cpp:508-526).

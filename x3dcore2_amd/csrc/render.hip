@@ -312,6 +312,66 @@ __global__ __launch_bounds__(kWG) void k_render_mis(RenderParams P, uint32_t *ne
     }
 }
 
+constexpr uint64_t kBdptItems = 1u << 22; // (pixel, sample) items per generation: 64 MiB of sample radiance
+
+// Bidirectional tracer. Work items are the (pixel, sample) pairs of one generation of samples,
+// pixel-major within a sample so neighbouring lanes start from neighbouring first hits. A lane
+// whose sample is done takes the next unclaimed item from a global counter, so lanes stay busy
+// although subpath lengths and connection counts vary widely between samples. Each sample's
+// radiance is stored; k_sum_samples then adds every pixel's samples in sample order, as the
+// oracle does.
+__global__ __launch_bounds__(kWG) void k_render_bdpt(RenderParams P, uint32_t gen_begin, uint32_t gen_samples,
+                                                     float4 *samples, uint32_t *next_item) {
+    __shared__ uint2 lds[kLdsDepth * kWG];
+    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
+    Tracer<kWG, kLdsDepth> tr{P.S, st};
+    Pathlet cam[kMaxPathLen], lig[kMaxPathLen];
+    uint32_t npix = P.width * P.height;
+    uint32_t total = npix * gen_samples;
+    uint32_t grid_threads = gridDim.x * kWG;
+    for (uint32_t q = blockIdx.x * kWG + threadIdx.x; q < total; q = grid_threads + atomicAdd(next_item, 1u)) {
+        uint32_t pix = q % npix, s = gen_begin + q / npix;
+        f3 o, v;
+        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        bool valid;
+        HitRec first = load_first(P, pix, valid);
+        Pcg g = pcg_for(P.seed, pix, s);
+        f3 L = tr.li_bdpt(first, valid, v, g, P.max_path_len, cam, lig);
+        samples[q] = make_float4(L.x, L.y, L.z, 0.0f);
+    }
+    if (P.cnt) {
+        flush_counter(&P.cnt->closest, tr.nclosest);
+        flush_counter(&P.cnt->anyhit, tr.nany);
+        flush_counter(&P.cnt->pops, tr.pops);
+        flush_counter(&P.cnt->tests, tr.tests);
+        flush_counter(&P.cnt->spops, tr.spops);
+        flush_counter(&P.cnt->stests, tr.stests);
+    }
+}
+
+// Adds samples [gen_begin, gen_begin + gen_samples) of every pixel, in order, to the running sum
+// (P.acc, or zero for the first generation of a render starting at sample 0); the last
+// generation writes the scaled film.
+__global__ __launch_bounds__(256) void k_sum_samples(RenderParams P, uint32_t gen_begin, uint32_t gen_samples,
+                                                     const float4 *samples) {
+    uint32_t npix = P.width * P.height;
+    bool last = gen_begin + gen_samples >= P.s_end;
+    float scale = 1.0f / static_cast<float>(P.spp_total);
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < npix; pix += gridDim.x * blockDim.x) {
+        f3 acc = mk(0.0f, 0.0f, 0.0f);
+        if (gen_begin > 0) {
+            float4 q = P.acc[pix];
+            acc = mk(q.x, q.y, q.z);
+        }
+        for (uint32_t k = 0; k < gen_samples; k++)
+            acc = acc + xyz(samples[static_cast<size_t>(k) * npix + pix]);
+        if (last && P.finalize)
+            P.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
+        else
+            P.acc[pix] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------------------------
@@ -408,7 +468,7 @@ struct x3d_scene {
 
 struct x3d_renderer {
     x3d_render_options opt;
-    x3d::DevBuf first, acc, out, spill, counters;
+    x3d::DevBuf first, acc, out, spill, counters, samples;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
     x3d::WavefrontEngine *wf = nullptr;
@@ -511,6 +571,7 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     case X3D_PT_UNIDIRECT_LT1: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT_LT1>, npix); break;
     case X3D_PT_UNIDIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT>, npix); break;
     case X3D_PT_DIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_DIRECT>, npix); break;
+    case X3D_PT_BIDIRECT_MIS: grid_render = persistent_grid(k_render_bdpt, npix * static_cast<uint64_t>(spp)); break;
     default: grid_render = persistent_grid(k_render_plain<X3D_PT_NORMAL>, npix); break;
     }
     uint32_t grid = grid_first > grid_render ? grid_first : grid_render;
@@ -551,6 +612,25 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     case X3D_PT_UNIDIRECT_LT1: launch_plain<X3D_PT_UNIDIRECT_LT1>(grid_render, r->stream, P); break;
     case X3D_PT_UNIDIRECT: launch_plain<X3D_PT_UNIDIRECT>(grid_render, r->stream, P); break;
     case X3D_PT_DIRECT: launch_plain<X3D_PT_DIRECT>(grid_render, r->stream, P); break;
+    case X3D_PT_BIDIRECT_MIS: {
+        // generations of whole samples, at most queue_capacity (default kBdptItems) items each
+        uint64_t items = r->opt.queue_capacity ? r->opt.queue_capacity : kBdptItems;
+        uint32_t gen = static_cast<uint32_t>(items / npix);
+        gen = gen < 1 ? 1 : (gen > spp ? spp : gen);
+        X3D_HIP(r->samples.ensure(static_cast<size_t>(gen) * npix * sizeof(float4)));
+        float4 *samples = static_cast<float4 *>(r->samples.p);
+        uint32_t g_sum = device_info().cus * 4u;
+        for (uint32_t s0 = 0; s0 < spp; s0 += gen) {
+            uint32_t cnt = spp - s0 < gen ? spp - s0 : gen;
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream,
+                               reinterpret_cast<unsigned long long *>(next_pixel), 1u);
+            hipLaunchKernelGGL(k_render_bdpt, dim3(grid_render), dim3(kWG), 0, r->stream, P, s0, cnt, samples,
+                               next_pixel);
+            hipLaunchKernelGGL(k_sum_samples, dim3(g_sum), dim3(256), 0, r->stream, P, s0, cnt,
+                               static_cast<const float4 *>(samples));
+        }
+        break;
+    }
     default: launch_plain<X3D_PT_NORMAL>(grid_render, r->stream, P); break;
     }
     X3D_HIP(hipGetLastError());
@@ -980,10 +1060,14 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
     case X3D_PT_UNIDIRECT:
     case X3D_PT_UNIDIRECT_LT1:
     case X3D_PT_UNIDIRECT_MIS:
+    case X3D_PT_BIDIRECT_MIS:
         break;
     default:
         return fail(X3D_ERR_UNSUPPORTED, "tracer type " + std::to_string(opts->pt_type) + " is not implemented");
     }
+    if (opts->pt_type == X3D_PT_BIDIRECT_MIS && (opts->max_path_len < 1 || opts->max_path_len > x3d::kMaxPathLen))
+        return fail(X3D_ERR_INVALID_ARGUMENT,
+                    "max_path_len must be in [1, " + std::to_string(x3d::kMaxPathLen) + "] for BIDIRECT_MIS");
     if (opts->form != X3D_FORM_PLAIN && opts->form != X3D_FORM_WAVEFRONT)
         return fail(X3D_ERR_INVALID_ARGUMENT, "unknown render form " + std::to_string(opts->form));
     if (opts->form == X3D_FORM_WAVEFRONT && opts->pt_type != X3D_PT_UNIDIRECT_MIS)

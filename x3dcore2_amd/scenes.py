@@ -322,6 +322,18 @@ def furnace_sphere(radius=10.0, res=30, albedo=0.7, radiance=1.0) -> SceneDesc:
                      translate_camera(0.0, 0.0, 0.0, aspect=1.0), 16, 16, 64, 5)
 
 
+def furnace_box(half=2.5, albedo=0.7, radiance=1.0) -> SceneDesc:
+    """Closed cube (12 flat triangles, inward normals) that is both a Lambertian reflector and an
+    area light, with the camera inside. Shading normals equal the geometric ones, so walks never
+    end on a back-facing hit and the radiance carried by paths of at most n vertices is exactly
+    radiance * (1 - albedo^n) / (1 - albedo) per channel."""
+    m = box((0.5, 0.5, 0.5), (2 * half, 2 * half, 2 * half), 0.0, 0)
+    m.normals = -m.normals
+    mats = [Material(OREN_NAYAR, (albedo, albedo, albedo))]
+    return SceneDesc("furnace_box", mats, [m], [(0, (radiance, radiance, radiance))],
+                     translate_camera(0.5, 0.5, 2.0), 16, 16, 64, 5)
+
+
 def random_soup(num_tris: int, seed: int, extent=1.0) -> SceneDesc:
     """Random triangles in a cube (for hit-id parity on non-axis-aligned geometry)."""
     rng = np.random.default_rng(seed)

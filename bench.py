@@ -10,7 +10,7 @@ HBM. Both execution forms are timed on the same workload — the plain one-threa
 (so the RMSE against any reference is equal). `value` is the wavefront form's samples/s. Prints
 one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload spheres|box|indirect] [--spp N]
+    python bench.py [--steps K] [--warmup W] [--workload spheres|box|indirect|batch] [--spp N]
                     [--no-cpu-baseline] [--no-autotune | --wf-config WG,DEPTH,SORT,CAP]
                     [--probe-spp N] [--form both|plain|wavefront]
 """
@@ -123,6 +123,86 @@ def traffic_from_profile(name, kernel):
     return float(fields["hbm_bytes_per_launch"])
 
 
+def bench_secondary(args):
+    """Secondary workloads (one JSON line each; not the headline metric):
+    indirect: the bidirectional tracer on the indirectly lit room with four 20k-triangle GGX
+              spheres, 512 x 512, 32 spp, subpaths of up to 6 vertices;
+    batch:    64 seeded scenes (1k-2M triangles each, 128^2-1024^2, 4-64 spp, depth 6) in one
+              persistent launch, against rendering the same jobs one after another."""
+    import x3dcore2_amd as x3d
+    from x3dcore2_amd import scenes
+    from x3dcore2_amd.device import DeviceArray, synchronize
+
+    out = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes)"}
+    if args.workload == "indirect":
+        desc = scenes.indirect_scene()
+        scene = x3d.Scene(desc)
+        w, h, spp = desc.width, desc.height, args.spp or desc.spp
+        r = x3d.Renderer(x3d.PT_BIDIRECT_MIS, max_path_len=6, seed=1)
+        film = DeviceArray(w * h * 16)
+        for _ in range(args.warmup):
+            r.render_device(scene, desc.camera, w, h, spp, film.ptr)
+        synchronize()
+        t0 = time.perf_counter()
+        stats = [r.render_device(scene, desc.camera, w, h, spp, film.ptr) for _ in range(args.steps)]
+        synchronize()
+        dt = time.perf_counter() - t0
+        rays = sum(st.closest_queries + st.anyhit_queries for st in stats)
+        out.update(metric="bidirectional path tracing Msamples/s", unit="Msamples/s",
+                   value=w * h * spp * args.steps / dt / 1e6, ms_per_step=dt * 1e3 / args.steps,
+                   mrays_per_s=rays / dt / 1e6, rays_per_sample=rays / (w * h * spp * args.steps),
+                   config={"workload": f"indirect room + 4 GGX spheres ({desc.num_tris} tris)", "width": w,
+                           "height": h, "spp": spp, "max_path_len": 6,
+                           "tracer": "bidirectional, reference uniform strategy weights"})
+        print(json.dumps(out), flush=True)
+        return
+    descs = scenes.batch_scenes()
+    t0 = time.perf_counter()
+    built = [x3d.Scene(d) for d in descs]
+    log(f"batch: {len(descs)} scenes, {sum(d.num_tris for d in descs)} tris, built in {time.perf_counter() - t0:.1f} s")
+    films = [DeviceArray(d.width * d.height * 16) for d in descs]
+    items = [(built[k], d.camera, d.width, d.height, d.spp) for k, d in enumerate(descs)]
+    ptrs = [f.ptr for f in films]
+    samples = sum(d.width * d.height * d.spp for d in descs)
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=1)
+    # one after another: the plain single-job kernel per job
+    synchronize()
+    t0 = time.perf_counter()
+    for k, d in enumerate(descs):
+        r.render_device(built[k], d.camera, d.width, d.height, d.spp, ptrs[k])
+    synchronize()
+    seq = time.perf_counter() - t0
+    log(f"sequential single-job renders: {seq * 1e3:.1f} ms")
+    tune = {}
+    if not args.no_autotune:
+        # probe with an eighth of each job's samples: same scenes, tiles and ordering problem
+        rep = r.autotune_batch([(sc, cam, w, h, max(1, spp // 8)) for (sc, cam, w, h, spp) in items])
+        tune = {"workgroup_size": rep.workgroup_size, "stack_depth": rep.stack_depth,
+                "sort_key": ["default", "cost", "job order"][rep.sort_key], "candidates": rep.candidates,
+                "best_ms": rep.best_ms, "default_ms": rep.default_ms}
+        log(f"batch autotune: {tune}")
+    for _ in range(args.warmup):
+        r.render_batch_device(items, ptrs)
+    synchronize()
+    t0 = time.perf_counter()
+    res = [r.render_batch_device(items, ptrs) for _ in range(args.steps)]
+    synchronize()
+    dt = time.perf_counter() - t0
+    st, lat = res[-1]
+    lat = np.array(lat)
+    out.update(metric="batched rendering Msamples/s (64 scenes, one persistent launch)", unit="Msamples/s",
+               value=samples * args.steps / dt / 1e6, ms_per_step=dt * 1e3 / args.steps,
+               kernel_ms=st.kernel_ms, sequential_ms=seq * 1e3, speedup_vs_sequential=seq / (dt / args.steps),
+               mrays_per_s=(st.closest_queries + st.anyhit_queries) / (st.kernel_ms * 1e-3) / 1e6,
+               latency_ms={"p50": float(np.percentile(lat, 50)), "p90": float(np.percentile(lat, 90)),
+                           "max": float(lat.max())},
+               autotune=tune,
+               config={"workload": f"64 seeded scenes, {sum(d.num_tris for d in descs)} tris", "samples": samples,
+                       "max_depth": 6, "tiles": st.tiles, "tracer": "unidirectional MIS"})
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +233,9 @@ def main():
               "(sample-split sharding across GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload in ("indirect", "batch"):
+        bench_secondary(args)
+        return
     import x3dcore2_amd as x3d
     from x3dcore2_amd.device import DeviceArray
 

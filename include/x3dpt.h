@@ -254,6 +254,43 @@ int x3d_render_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera 
 /* PCG draws of stream (seed, pixel, sample) computed on the device (for RNG parity tests). */
 int x3d_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out);
 
+/* ---- batched rendering (no reference counterpart: pt_image_renderer::render takes one scene,
+ * renderer.h:57-59). Many jobs (scene, camera, film size, spp) share one persistent launch,
+ * captured as a hipGraph and replayed while the batch keeps its shape. The renderer must be a
+ * UNIDIRECT_MIS renderer; each job's film is bit-identical to x3d_render of that job with the
+ * same renderer options. Options used: max_depth, seed, workgroup_size (64/128/256),
+ * stack_depth (8/16), sort_key (NONE: hand tiles out in job order; otherwise most expensive
+ * job first). Jobs may share scenes. ---- */
+typedef struct x3d_batch_job {
+    const x3d_scene *scene;
+    x3d_camera camera;
+    uint32_t width, height, spp; /* width, height >= 2; spp >= 1 */
+    uint32_t reserved;
+    float *rgba;       /* film float[4*w*h]: host memory for x3d_render_batch, device memory for
+                          x3d_render_batch_device */
+    double latency_ms; /* out: device time from the launch's start to the job's completion */
+} x3d_batch_job;
+
+typedef struct x3d_batch_stats {
+    uint64_t num_samples;     /* sum of w*h*spp */
+    uint64_t closest_queries; /* first hits included */
+    uint64_t anyhit_queries;
+    uint64_t closest_nodes, closest_tests, anyhit_nodes, anyhit_tests;
+    double kernel_ms;         /* event-timed batch launch */
+    double total_ms;          /* wall time of the call */
+    uint32_t tiles;           /* work tiles of 1024 pixels */
+    uint32_t reserved;
+} x3d_batch_stats;
+
+int x3d_render_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs, x3d_batch_stats *stats);
+int x3d_render_batch_device(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs,
+                            x3d_batch_stats *stats);
+
+/* Autotuner for a batch: times every workgroup size x LDS stack depth x sort key on the given
+ * jobs (their films are written), keeps the fastest in the renderer's options. */
+int x3d_renderer_autotune_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs,
+                                x3d_autotune_report *report);
+
 #ifdef __cplusplus
 }
 #endif

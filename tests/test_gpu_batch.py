@@ -1,0 +1,70 @@
+"""Batched rendering through the C ABI: every job's film must be bit-identical to rendering that
+job alone with the same renderer (and so to the oracle), whatever the workgroup size, stack depth
+and tile order, on first capture and on graph replay."""
+
+import numpy as np
+import pytest
+
+from x3dcore2_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def batch(x3d):
+    descs = scenes.batch_scenes(count=9, seed=77, min_tris=200, max_tris=40000, resolutions=(32, 48, 64),
+                                spps=(1, 2, 5))
+    descs.append(scenes.box_scene())
+    descs[-1].width = descs[-1].height = 40
+    descs[-1].spp = 3
+    print("[gpu] batch scenes:", [(d.num_tris, d.width, d.spp) for d in descs], flush=True)
+    built = [x3d.Scene(d) for d in descs]
+    items = [(built[k], d.camera, d.width, d.height, d.spp) for k, d in enumerate(descs)]
+    items.append((built[0], descs[0].camera, 33, 17, 2))  # a scene shared by two jobs, ragged size
+    return descs, built, items
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(workgroup_size=64, stack_depth=8, sort_key=2),
+                                dict(workgroup_size=128, sort_key=1)])
+def test_batch_films_identical_to_single_renders(x3d, batch, kw):
+    descs, built, items = batch
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=9, **kw)
+    singles = [r.render(s, cam, w, h, spp)[0] for (s, cam, w, h, spp) in items]
+    for rep in range(2):  # capture, then replay
+        print(f"[gpu] batch {kw} pass {rep}", flush=True)
+        films, st, lat = r.render_batch(items)
+        for k, (f, ref) in enumerate(zip(films, singles)):
+            assert np.array_equal(_bits(f), _bits(ref)), (k, np.abs(f - ref).max())
+        assert st.num_samples == sum(w * h * spp for (_, _, w, h, spp) in items)
+        assert st.closest_queries >= sum(w * h for (_, _, w, h, _) in items)
+        assert all(0 < x <= st.kernel_ms * 1.01 + 0.05 for x in lat), (lat, st.kernel_ms)
+
+
+def test_batch_matches_oracle(x3d, oracle_mod, batch):
+    descs, built, items = batch
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=3)
+    films, _, _ = r.render_batch(items[:3] + items[-2:])
+    for k, (f, d) in enumerate(zip(films, descs[:3] + descs[-1:])):
+        ref = oracle_mod.Oracle(d).render(oracle_mod.TR_UNIDIRECT_MIS, d.width, d.height, d.spp, seed=3, max_depth=6)
+        assert np.array_equal(_bits(f), _bits(ref)), k
+
+
+def test_batch_autotune_and_errors(x3d, batch):
+    descs, built, items = batch
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=9)
+    ref, _, _ = r.render_batch(items)
+    rep = r.autotune_batch(items)
+    assert rep.candidates == 12 and rep.best_ms > 0 and rep.workgroup_size in (64, 128, 256)
+    films, _, _ = r.render_batch(items)
+    for a, b in zip(films, ref):
+        assert np.array_equal(_bits(a), _bits(b))
+    with pytest.raises(x3d.X3DError):
+        x3d.Renderer(x3d.PT_BIDIRECT_MIS).render_batch(items[:1])
+    with pytest.raises(x3d.X3DError):
+        r.render_batch([(built[0], descs[0].camera, 1, 8, 1)])
+    films, st, lat = r.render_batch([])
+    assert films == [] and st.num_samples == 0

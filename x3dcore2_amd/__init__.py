@@ -85,6 +85,18 @@ class AutotuneReport(C.Structure):
 
 _lib: Optional[C.CDLL] = None
 
+class BatchJob(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", CameraDesc), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("spp", C.c_uint32), ("reserved", C.c_uint32), ("rgba", C.c_void_p), ("latency_ms", C.c_double)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("num_samples", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
+                ("closest_nodes", C.c_uint64), ("closest_tests", C.c_uint64), ("anyhit_nodes", C.c_uint64),
+                ("anyhit_tests", C.c_uint64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
+                ("tiles", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 # Every exported symbol declared in include/x3dpt.h.
 EXPORTS = [
     "x3d_last_error", "x3d_version", "x3d_scene_create", "x3d_scene_destroy", "x3d_scene_add_material",
@@ -92,7 +104,7 @@ EXPORTS = [
     "x3d_scene_dump_bvh", "x3d_intersect", "x3d_occluded", "x3d_intersect_device", "x3d_occluded_device",
     "x3d_bsdf_eval", "x3d_bsdf_pdf", "x3d_bsdf_sample", "x3d_camera_rays", "x3d_render_options_default",
     "x3d_renderer_create", "x3d_renderer_destroy", "x3d_render", "x3d_render_device", "x3d_pcg_draws",
-    "x3d_renderer_autotune",
+    "x3d_renderer_autotune", "x3d_render_batch", "x3d_render_batch_device", "x3d_renderer_autotune_batch",
 ]
 
 
@@ -296,6 +308,38 @@ class Renderer:
         _check(lib().x3d_render_device(self.h, scene.h, C.byref(camera_desc(camera)), width, height, spp,
                                        C.c_void_p(d_rgba), C.byref(st)))
         return st
+
+    @staticmethod
+    def _jobs(items, films):
+        arr = (BatchJob * len(items))()
+        for k, (scene, camera, w, h, spp) in enumerate(items):
+            j = arr[k]
+            j.scene, j.camera, j.width, j.height, j.spp = scene.h.value, camera_desc(camera), w, h, spp
+            j.rgba = films[k] if isinstance(films[k], int) else films[k].ctypes.data
+        return arr
+
+    def render_batch(self, items):
+        """items: [(Scene, camera, width, height, spp)]. Returns (films [h*w, 4] float32 per job,
+        BatchStats, per-job latency in ms)."""
+        films = [np.zeros((h * w, 4), np.float32) for (_, _, w, h, _) in items]
+        arr = self._jobs(items, films)
+        st = BatchStats()
+        _check(lib().x3d_render_batch(self.h, len(items), arr, C.byref(st)))
+        return films, st, [arr[k].latency_ms for k in range(len(items))]
+
+    def render_batch_device(self, items, d_films):
+        """Same with device films (device pointers as ints)."""
+        arr = self._jobs(items, d_films)
+        st = BatchStats()
+        _check(lib().x3d_render_batch_device(self.h, len(items), arr, C.byref(st)))
+        return st, [arr[k].latency_ms for k in range(len(items))]
+
+    def autotune_batch(self, items) -> AutotuneReport:
+        films = [np.zeros((h * w, 4), np.float32) for (_, _, w, h, _) in items]
+        arr = self._jobs(items, films)
+        rep = AutotuneReport()
+        _check(lib().x3d_renderer_autotune_batch(self.h, len(items), arr, C.byref(rep)))
+        return rep
 
     def autotune(self, scene: Scene, camera, width, height, probe_spp) -> AutotuneReport:
         rep = AutotuneReport()

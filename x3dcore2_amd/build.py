@@ -9,6 +9,7 @@ and packed-FP32 instructions disabled on the device.
 
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -41,12 +42,16 @@ def build_library(verbose: bool = True) -> str:
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     gxx = shutil.which("g++") or "g++"
     scene_o = os.path.join(BUILD, "scene.o")
-    _run([gxx, *GXX_FLAGS, "-c", os.path.join(CSRC, "scene.cpp"), "-o", scene_o])
+    cmds = [[gxx, *GXX_FLAGS, "-c", os.path.join(CSRC, "scene.cpp"), "-o", scene_o]]
     objs = [scene_o]
-    for unit in ("render", "wavefront"):
+    for unit in ("render", "wavefront", "batch"):
         o = os.path.join(BUILD, unit + ".o")
-        _run([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, unit + ".hip"), "-o", o])
+        cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, unit + ".hip"), "-o", o])
         objs.append(o)
+    # independent translation units compile concurrently
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(cmds)) as pool:
+        for f in [pool.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
           f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")])

@@ -19,6 +19,7 @@
 #include "../../include/x3dpt.h"
 #include "render_common.h"
 #include "scene.h"
+#include "batch.h"
 #include "wavefront.h"
 
 namespace x3d {
@@ -436,28 +437,6 @@ template <class K> uint32_t persistent_grid(K kernel, uint64_t items) {
     return static_cast<uint32_t>(g < 1 ? 1 : g);
 }
 
-// Growable device buffer.
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() {
-        if (p)
-            (void)hipFree(p);
-    }
-    hipError_t ensure(size_t n) {
-        if (n <= bytes)
-            return hipSuccess;
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess)
-            bytes = n;
-        return e;
-    }
-};
-
 uint32_t spill_entries(DevScene const &S) { return spill_entries_for(S, kLdsDepth); }
 
 } // namespace x3d
@@ -472,9 +451,13 @@ struct x3d_renderer {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
     x3d::WavefrontEngine *wf = nullptr;
+    x3d::BatchEngine *batch = nullptr;
+    std::vector<x3d::DevBuf> batch_films; // device films of x3d_render_batch
     ~x3d_renderer() {
         if (wf)
             x3d::wf_destroy(wf);
+        if (batch)
+            x3d::batch_destroy(batch);
         if (evm)
             (void)hipEventDestroy(evm);
         if (ev0)
@@ -681,6 +664,78 @@ int check_render_args(x3d_renderer *r, const x3d_scene *scene, const x3d_camera 
         return fail(X3D_ERR_INVALID_ARGUMENT, "film too large");
     if (spp == 0)
         return fail(X3D_ERR_INVALID_ARGUMENT, "spp must be > 0");
+    return X3D_OK;
+}
+
+int check_batch(x3d_renderer *r, uint32_t n, x3d_batch_job const *jobs) {
+    if (!r || (n && !jobs))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    if (r->opt.pt_type != X3D_PT_UNIDIRECT_MIS || r->opt.form != X3D_FORM_PLAIN)
+        return fail(X3D_ERR_STATE, "batched rendering needs a plain-form UNIDIRECT_MIS renderer");
+    for (uint32_t i = 0; i < n; i++) {
+        int rc = check_render_args(r, jobs[i].scene, &jobs[i].camera, jobs[i].width, jobs[i].height, jobs[i].spp);
+        if (rc)
+            return fail(rc, "job " + std::to_string(i) + ": " + g_last_error);
+        if (!jobs[i].rgba)
+            return fail(X3D_ERR_INVALID_ARGUMENT, "job " + std::to_string(i) + ": rgba is null");
+    }
+    return X3D_OK;
+}
+
+// Runs the batch with films already in device memory (films[i] for job i).
+int batch_impl(x3d_renderer *r, uint32_t n, x3d_batch_job *jobs, float *const *films, x3d_batch_stats *stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (n == 0) {
+        if (stats)
+            *stats = x3d_batch_stats{};
+        return X3D_OK;
+    }
+    if (!r->batch)
+        r->batch = x3d::batch_create();
+    X3D_HIP(r->counters.ensure(2 * sizeof(Counters) + 64));
+    std::vector<x3d::BatchJob> bj(n);
+    uint64_t samples = 0, tiles = 0;
+    x3d::BatchTuning tune;
+    for (uint32_t i = 0; i < n; i++) {
+        x3d::BatchJob &J = bj[i];
+        J = x3d::BatchJob{};
+        J.S = jobs[i].scene->s.dev;
+        J.cam = cam_params(jobs[i].camera);
+        J.out = reinterpret_cast<float4 *>(films[i]);
+        J.width = jobs[i].width;
+        J.height = jobs[i].height;
+        J.spp = jobs[i].spp;
+        uint64_t npix = static_cast<uint64_t>(J.width) * J.height;
+        samples += npix * J.spp;
+        tiles += (npix + tune.tile_px - 1) / tune.tile_px;
+    }
+    if (r->opt.workgroup_size)
+        tune.wg = static_cast<int>(r->opt.workgroup_size);
+    if (r->opt.stack_depth)
+        tune.stack_depth = static_cast<int>(r->opt.stack_depth);
+    tune.sort_by_cost = r->opt.sort_key != X3D_SORT_NONE;
+    Counters *cnt = static_cast<Counters *>(r->counters.p);
+    x3d::BatchResult res;
+    std::string e = x3d::batch_render(r->batch, bj, r->opt.max_depth, r->opt.seed, tune, cnt, r->stream, &res);
+    if (!e.empty())
+        return fail(X3D_ERR_DEVICE, e);
+    for (uint32_t i = 0; i < n; i++)
+        jobs[i].latency_ms = res.job_ms.empty() ? 0.0 : res.job_ms[i];
+    if (stats) {
+        Counters c{};
+        X3D_HIP(hipMemcpy(&c, cnt, sizeof c, hipMemcpyDeviceToHost));
+        *stats = x3d_batch_stats{};
+        stats->num_samples = samples;
+        stats->closest_queries = c.closest;
+        stats->anyhit_queries = c.anyhit;
+        stats->closest_nodes = c.pops;
+        stats->closest_tests = c.tests;
+        stats->anyhit_nodes = c.spops;
+        stats->anyhit_tests = c.stests;
+        stats->kernel_ms = res.kernel_ms;
+        stats->tiles = static_cast<uint32_t>(tiles);
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
     return X3D_OK;
 }
 
@@ -1208,6 +1263,93 @@ int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_cam
     r->opt.stack_depth = rep.stack_depth;
     r->opt.sort_key = rep.sort_key;
     r->opt.queue_capacity = rep.queue_capacity;
+    if (report)
+        *report = rep;
+    return X3D_OK;
+}
+
+int x3d_render_batch_device(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs, x3d_batch_stats *stats) {
+    int rc = check_batch(r, num_jobs, jobs);
+    if (rc)
+        return rc;
+    std::vector<float *> films(num_jobs);
+    for (uint32_t i = 0; i < num_jobs; i++)
+        films[i] = jobs[i].rgba;
+    return batch_impl(r, num_jobs, jobs, films.data(), stats);
+}
+
+int x3d_render_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs, x3d_batch_stats *stats) {
+    int rc = check_batch(r, num_jobs, jobs);
+    if (rc)
+        return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    if (r->batch_films.size() < num_jobs)
+        r->batch_films = std::vector<x3d::DevBuf>(num_jobs);
+    std::vector<float *> films(num_jobs);
+    for (uint32_t i = 0; i < num_jobs; i++) {
+        X3D_HIP(r->batch_films[i].ensure(static_cast<size_t>(jobs[i].width) * jobs[i].height * sizeof(float4)));
+        films[i] = static_cast<float *>(r->batch_films[i].p);
+    }
+    rc = batch_impl(r, num_jobs, jobs, films.data(), stats);
+    if (rc)
+        return rc;
+    for (uint32_t i = 0; i < num_jobs; i++)
+        X3D_HIP(hipMemcpy(jobs[i].rgba, films[i], static_cast<size_t>(jobs[i].width) * jobs[i].height * sizeof(float4),
+                          hipMemcpyDeviceToHost));
+    if (stats)
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return X3D_OK;
+}
+
+int x3d_renderer_autotune_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs, x3d_autotune_report *report) {
+    int rc = check_batch(r, num_jobs, jobs);
+    if (rc)
+        return rc;
+    x3d_render_options saved = r->opt;
+    x3d_autotune_report rep{};
+    rep.best_ms = 1e300;
+    const uint32_t wgs[] = {64, 128, 256};
+    const uint32_t depths[] = {8, 16};
+    const int32_t sorts[] = {X3D_SORT_MATERIAL, X3D_SORT_NONE};
+    auto time_config = [&](uint32_t wg, uint32_t d, int32_t so, double *ms) -> int {
+        r->opt.workgroup_size = wg;
+        r->opt.stack_depth = d;
+        r->opt.sort_key = so;
+        double best = 1e300;
+        for (int k = 0; k < 3; k++) { // first call captures the graph; best of the other two
+            x3d_batch_stats st;
+            int e = x3d_render_batch(r, num_jobs, jobs, &st);
+            if (e)
+                return e;
+            if (k > 0 && st.kernel_ms < best)
+                best = st.kernel_ms;
+        }
+        *ms = best;
+        return X3D_OK;
+    };
+    rc = time_config(0, 0, X3D_SORT_DEFAULT, &rep.default_ms);
+    for (uint32_t wg : wgs)
+        for (uint32_t d : depths)
+            for (int32_t so : sorts) {
+                double ms = 0;
+                if (!rc)
+                    rc = time_config(wg, d, so, &ms);
+                if (rc)
+                    break;
+                rep.candidates++;
+                if (ms < rep.best_ms) {
+                    rep.best_ms = ms;
+                    rep.workgroup_size = wg;
+                    rep.stack_depth = d;
+                    rep.sort_key = so;
+                }
+            }
+    r->opt = saved;
+    if (rc)
+        return rc;
+    r->opt.workgroup_size = rep.workgroup_size;
+    r->opt.stack_depth = rep.stack_depth;
+    r->opt.sort_key = rep.sort_key;
     if (report)
         *report = rep;
     return X3D_OK;

@@ -64,6 +64,31 @@ inline uint32_t spill_entries_for(DevScene const &S, int lds_depth) {
     return S.max_stack > static_cast<uint32_t>(lds_depth) ? S.max_stack - lds_depth + 1 : 1;
 }
 
+// Growable device buffer (host side; contents are not preserved when it grows).
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(DevBuf const &) = delete;
+    DevBuf &operator=(DevBuf const &) = delete;
+    ~DevBuf() {
+        if (p)
+            (void)hipFree(p);
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= bytes)
+            return hipSuccess;
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess)
+            bytes = n;
+        return e;
+    }
+};
+
 struct RenderParams {
     DevScene S;
     CamParams cam;

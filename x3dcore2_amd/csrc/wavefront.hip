@@ -489,29 +489,8 @@ __global__ void k_wf_count(WfCtl *ctl, Counters *cnt) {
 // ---------------------------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------------------------
-struct WfBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~WfBuf() {
-        if (p)
-            (void)hipFree(p);
-    }
-    hipError_t ensure(size_t n) {
-        if (n <= bytes)
-            return hipSuccess;
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess)
-            bytes = n;
-        return e;
-    }
-};
-
 struct WavefrontEngine {
-    WfBuf xp, xn, wo, beta, L, wi, rng, lbuf, act0, act1, sq, eq, fin, eh, ctl, spill;
+    DevBuf xp, xn, wo, beta, L, wi, rng, lbuf, act0, act1, sq, eq, fin, eh, ctl, spill;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     WfArgs key{};

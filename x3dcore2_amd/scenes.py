@@ -12,6 +12,7 @@ does in its own frame (corelib/src/camera.cpp:38-48).
 from __future__ import annotations
 
 import dataclasses
+import functools
 import math
 from typing import List, Optional, Tuple
 
@@ -285,6 +286,54 @@ def indirect_scene(width=512, height=512, spp=32, max_path_len=6, level=5) -> Sc
     desc = SceneDesc("indirect", mats, meshes, lights, translate_camera(0.5, 0.5, 2.0), width, height, spp,
                      max_path_len)
     return desc
+
+
+@functools.lru_cache(maxsize=None)
+def _unit_icosphere(level: int) -> Tuple[np.ndarray, np.ndarray]:
+    v, f = icosphere(level)
+    return v.astype(np.float32), f.astype(np.uint32)
+
+
+def batch_scenes(count=64, seed=0x5EED0003, min_tris=1_000, max_tris=2_000_000, max_depth=6,
+                 resolutions=(128, 256, 512, 1024), spps=(4, 16, 64)) -> List[SceneDesc]:
+    """Seeded batch of scenes: the room and ceiling emitter plus instanced icospheres (unit
+    icospheres of subdivision level 1-7, scaled and placed, flattened to world space) and boxes,
+    total triangle count log-uniform in [min_tris, max_tris]; each object Lambertian or GGX
+    (roughness U[0.05, 0.8]); square resolution and spp drawn from the given sets."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        target = float(math.exp(rng.uniform(math.log(min_tris), math.log(max_tris))))
+        mats: List[Material] = []
+        meshes: List[Mesh] = []
+        lights = []
+        _room(mats, meshes)
+        _ceiling_emitter(mats, meshes, lights)
+        tris = 12
+        while tris + 12 <= target:
+            left = target - tris
+            alb = tuple(float(a) for a in rng.uniform(0.1, 0.9, size=3))
+            if rng.random() < 0.5:
+                mats.append(Material(OREN_NAYAR, alb))
+            else:
+                mats.append(Material(COOK_TORR, alb, roughness=float(rng.uniform(0.05, 0.8)), ior=GLOSSY_IOR))
+            mat = len(mats) - 1
+            r = float(math.exp(rng.uniform(math.log(0.02), math.log(0.15))))
+            c = rng.uniform(r, 1.0 - r, size=3)
+            top = int(min(7, math.floor(math.log(max(left, 20) / 20.0, 4))))
+            if top < 1 or rng.random() < 0.25:
+                meshes.append(box(tuple(c), (2 * r, 2 * r, 2 * r), float(rng.uniform(0, math.pi)), mat))
+                tris += 12
+                continue
+            level = int(rng.integers(max(1, top - 1), top + 1))
+            v, f = _unit_icosphere(level)
+            meshes.append(Mesh((v * np.float32(r) + c.astype(np.float32)).astype(np.float32), v.copy(), f, mat))
+            tris += f.shape[0]
+        res = int(rng.choice(resolutions))
+        spp = int(rng.choice(spps))
+        out.append(SceneDesc(f"batch{k}", mats, meshes, lights, translate_camera(0.5, 0.5, 2.0), res, res, spp,
+                             max_depth))
+    return out
 
 
 def furnace_sphere(radius=10.0, res=30, albedo=0.7, radiance=1.0) -> SceneDesc:

@@ -78,8 +78,18 @@ int x3d_scene_add_area_light(x3d_scene *scene, int32_t mesh_id, const float radi
 
 /* Builds the SAH BVH on the host with the reference's layout (bvh_path_space_layout::commit,
  * pathspace.cpp:292-328), the light power CDF (lightsources.cpp:45-52), and uploads the
- * scene to device memory. The scene is immutable afterwards. */
+ * scene to device memory. Geometry and lights are immutable afterwards. */
 int x3d_scene_commit(x3d_scene *scene);
+
+/* Material parameters of a committed scene (no reference counterpart; for inverse rendering):
+ * materials [first, first+count) get albedo[3*k..] (OREN_NAYAR, COOK_TORR) and roughness[k]
+ * (COOK_TORR, > 0) when those arrays are non-NULL. update_sampling != 0 also moves the density
+ * the GGX lobe is sampled with to the new roughness; with 0 the sampling density stays, so that
+ * renders are smooth in the evaluated parameters for fixed seeds (the derivative that
+ * x3d_render_backward returns). Mixtures have no parameters of their own (their entries are
+ * skipped). The device table is updated synchronously before the call returns. */
+int x3d_scene_set_material_params(x3d_scene *scene, uint32_t first, uint32_t count, const float *albedo,
+                                  const float *roughness, int32_t update_sampling);
 
 typedef struct x3d_scene_info {
     uint32_t num_meshes, num_tris, num_nodes, num_interior, num_lights, num_materials;
@@ -285,6 +295,32 @@ typedef struct x3d_batch_stats {
 int x3d_render_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs, x3d_batch_stats *stats);
 int x3d_render_batch_device(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs,
                             x3d_batch_stats *stats);
+
+/* ---- differentiation (no reference counterpart: the reference has no differentiable
+ * renderer). Gradient of a film loss with respect to material parameters for the
+ * UNIDIRECT_MIS tracer (plain form, max_depth <= 8, at most 64 materials). ---- */
+
+/* Forward pass of the pair: the film of x3d_render (firefly filter off). */
+int x3d_render_forward(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                       uint32_t height, uint32_t spp, float *rgba, x3d_render_stats *stats);
+
+/* Backward pass: given dl_dfilm = dLoss/dfilm (float[4*w*h], rgb used, same layout as the
+ * film), writes grad = dLoss/dparams, float[4*num_materials]: for material m, grad[4m..4m+2] is
+ * the derivative with respect to its albedo (OREN_NAYAR, COOK_TORR) and grad[4m+3] with respect
+ * to its roughness (COOK_TORR); mixtures get zeros (their operands carry the gradient). Each
+ * path is re-traced from its PCG stream (path replay; nothing is kept from the forward pass).
+ * The sampled directions, sampling densities and MIS weights are held fixed, so the result is
+ * the exact derivative of the film for fixed seeds when the GGX sampling density is held
+ * (x3d_scene_set_material_params with update_sampling = 0), and an unbiased estimate of the
+ * gradient of the expected film in any case. Visibility does not depend on the parameters (no
+ * boundary terms). The sums are reproducible run to run. */
+int x3d_render_backward(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                        uint32_t height, uint32_t spp, const float *dl_dfilm, float *grad,
+                        x3d_render_stats *stats);
+/* Same with device pointers (d_dl_dfilm: float[4*w*h], d_grad: float[4*num_materials]). */
+int x3d_render_backward_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera,
+                               uint32_t width, uint32_t height, uint32_t spp, const float *d_dl_dfilm,
+                               float *d_grad, x3d_render_stats *stats);
 
 /* Autotuner for a batch: times every workgroup size x LDS stack depth x sort key on the given
  * jobs (their films are written), keeps the fastest in the renderer's options. */

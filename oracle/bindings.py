@@ -108,10 +108,33 @@ class Oracle(_Backend):
         lib.orc_render.argtypes = [C.c_void_p, _f32p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int,
                                    C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int,
                                    C.c_uint32, C.c_void_p, C.c_void_p]
+        lib.orc_set_material_params.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                                C.c_int]
+        lib.orc_render_backward.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, _f32p, C.c_int, C.c_uint32, _f64p]
         self.h = C.c_void_p(lib.orc_scene_new())
         self.desc = desc
         if desc is not None:
             self._load_desc(desc)
+
+    def set_material_params(self, first, albedo=None, roughness=None, update_sampling=True):
+        """albedo: [count, 3] or None; roughness: [count] or None."""
+        a = None if albedo is None else np.ascontiguousarray(albedo, np.float32).reshape(-1, 3)
+        r = None if roughness is None else np.ascontiguousarray(roughness, np.float32).reshape(-1)
+        count = a.shape[0] if a is not None else r.shape[0]
+        self.lib.orc_set_material_params(self.h, first, count, a.ctypes.data if a is not None else None,
+                                         r.ctypes.data if r is not None else None, int(update_sampling))
+
+    def render_backward(self, width, height, spp, dl_dfilm, seed=1, max_depth=5, prec=1, threads=None,
+                        camera=None):
+        """Gradient of a film loss (dl_dfilm: [h*w, 4]) w.r.t. (albedo rgb, roughness) of every
+        material: [num_materials, 4] float64."""
+        cam = (camera or self.desc.camera).packed()
+        nm = len(self.desc.materials)
+        grad = np.zeros((nm, 4), np.float64)
+        self.lib.orc_render_backward(self.h, cam, max_depth, seed, width, height, spp, _f32(dl_dfilm), prec,
+                                     threads or os.cpu_count() or 1, grad)
+        return grad
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -541,10 +541,43 @@ template <class T> T alpha2_of(Material const &m) {
     // float in the FP32 build: m_alpha2(2 * roughness * roughness) (material.cpp:379)
     return T(2) * T(m.roughness) * T(m.roughness);
 }
+// alpha^2 of the GGX sampling density (and its pdf): the evaluated roughness unless the
+// evaluated parameters were changed with sampling left alone (orc_set_material_params).
+template <class T> T sample_alpha2_of(Material const &m) {
+    return T(2) * T(m.sample_roughness) * T(m.sample_roughness);
+}
 
 template <class T> V3<T> clampv(T x) { return V3<T>(x); }
 
 } // namespace
+
+// oren_nayar::eval factor k (material.cpp:134-161): the lobe is (albedo / pi) * k; false when
+// either cosine is cut off.
+template <class T> bool oren_nayar_k(Material const &m, V3<T> n, V3<T> o, V3<T> i, T *k_out) {
+    T sg = T(m.sigma);
+    T sigma2 = sg * sg;
+    T A = T(1) - T(0.5f) * sigma2 / (sigma2 + T(0.33f));
+    T B = T(0.45f) * sigma2 / (sigma2 + T(0.09f));
+    T ci = inner(i, n), co = inner(o, n);
+    T cos_thei = ci < T(0) ? T(0) : (ci > T(1) ? T(1) : ci);
+    T cos_theo = co < T(0) ? T(0) : (co > T(1) ? T(1) : co);
+    if (near0(cos_thei) || near0(cos_theo))
+        return false;
+    T ca, cb;
+    if (cos_thei < cos_theo) {
+        ca = cos_thei;
+        cb = cos_theo;
+    } else {
+        ca = cos_theo;
+        cb = cos_thei;
+    }
+    T sin_a = std::sqrt(T(1.0f) - ca * ca);
+    T sin_b = std::sqrt(T(1.0f) - cb * cb);
+    T cos_io = ca * cb + sin_a * sin_b;
+    T tan_b = sin_b / cos_theo;
+    *k_out = A + B * smax(T(0.0f), cos_io) * sin_a * tan_b;
+    return true;
+}
 
 template <class T> V3<T> bsdf_eval(Scene const &s, int mat, V3<T> n, V3<T> o, V3<T> i) {
     if (mat < 0) // mat_fail_safe::eval (material.cpp:64-68)
@@ -552,28 +585,9 @@ template <class T> V3<T> bsdf_eval(Scene const &s, int mat, V3<T> n, V3<T> o, V3
     Material const &m = s.mats[mat];
     V3<T> alb = cvt<T>(m.albedo);
     if (m.kind == MAT_OREN_NAYAR) { // material.cpp:134-161
-        T sg = T(m.sigma);
-        T sigma2 = sg * sg;
-        T A = T(1) - T(0.5f) * sigma2 / (sigma2 + T(0.33f));
-        T B = T(0.45f) * sigma2 / (sigma2 + T(0.09f));
-        T ci = inner(i, n), co = inner(o, n);
-        T cos_thei = ci < T(0) ? T(0) : (ci > T(1) ? T(1) : ci);
-        T cos_theo = co < T(0) ? T(0) : (co > T(1) ? T(1) : co);
-        if (near0(cos_thei) || near0(cos_theo))
+        T k;
+        if (!oren_nayar_k<T>(m, n, o, i, &k))
             return V3<T>();
-        T ca, cb;
-        if (cos_thei < cos_theo) {
-            ca = cos_thei;
-            cb = cos_theo;
-        } else {
-            ca = cos_theo;
-            cb = cos_thei;
-        }
-        T sin_a = std::sqrt(T(1.0f) - ca * ca);
-        T sin_b = std::sqrt(T(1.0f) - cb * cb);
-        T cos_io = ca * cb + sin_a * sin_b;
-        T tan_b = sin_b / cos_theo;
-        T k = A + B * smax(T(0.0f), cos_io) * sin_a * tan_b;
         return ((T(1.0f) / pi_v<T>()) * alb) * k;
     }
     if (m.kind == MAT_COOK_TORR) { // material.cpp:171-198
@@ -618,7 +632,7 @@ template <class T> T bsdf_pdf(Scene const &s, int mat, V3<T> n, V3<T> o, V3<T> i
         T ct = inner(n, h);
         if (!(ct > T(0)))
             return T(0);
-        return ggx_d(alpha2_of<T>(m), n, h) * ct / (T(4.0f) * hdo);
+        return ggx_d(sample_alpha2_of<T>(m), n, h) * ct / (T(4.0f) * hdo);
     }
     T r = T(m.ratio);
     return r * bsdf_pdf<T>(s, m.sub0, n, o, i) + (T(1) - r) * bsdf_pdf<T>(s, m.sub1, n, o, i);
@@ -645,7 +659,7 @@ V3<T> bsdf_sample(Scene const &s, int mat, V3<T> n, V3<T> o, R &g, T *dens, bool
     }
     Material const &m = s.mats[mat];
     if (m.kind == MAT_COOK_TORR) { // material.cpp:224-252
-        T ms = alpha2_of<T>(m);
+        T ms = sample_alpha2_of<T>(m);
         V3<T> u, v;
         basis(n, &u, &v);
         T e0 = T(g.draw());
@@ -820,11 +834,26 @@ V3<T> direct_from(Ctx<T> &c, LightSample<T> const &ls, Hit<T> const &x, V3<T> wo
     return mod(illum, f) / ls.area_dens;
 }
 
+// One vertex of a recorded MIS path (for the adjoint): shading normal, outgoing direction,
+// sampled BSDF direction, light-sample direction, the NEE contribution made at the vertex and
+// the emission reached through the BSDF sample (both already weighted by throughput and MIS).
+template <class T> struct AdjVertT {
+    V3<T> n, wo, wi, wl, c_nee, c_em;
+    int mat = -1;
+    bool nee = false, cont = false;
+};
+constexpr uint32_t kAdjMaxDepth = 8;
+
 // Config mode: unidirectional path tracing with next-event estimation and BSDF sampling
 // combined by the power heuristic (Veach 1997), fixed maximum of opt.max_depth bounces.
-template <class T> V3<T> li_mis(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg &g) {
+// With rec, every bounce's vertex is recorded (rec[0 .. *nrec)).
+template <class T>
+V3<T> li_mis(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg &g, AdjVertT<T> *rec = nullptr,
+             uint32_t *nrec = nullptr) {
     Scene const &s = c.s;
     V3<T> L, beta(T(1));
+    if (nrec)
+        *nrec = 0;
     if (!first.valid())
         return L;
     Hit<T> x = first;
@@ -834,6 +863,14 @@ template <class T> V3<T> li_mis(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg
         L = L + light_radiance<T>(s.lights[lx], wo, x.n);
     for (uint32_t k = 0; k < c.opt.max_depth; k++) {
         int mat = mat_of<T>(s, x.geo);
+        AdjVertT<T> *r = rec ? rec + k : nullptr;
+        if (r) {
+            *r = AdjVertT<T>();
+            r->n = x.n;
+            r->wo = wo;
+            r->mat = mat;
+            *nrec = k + 1;
+        }
         if (!s.lights.empty()) {
             LightSample<T> ls = sample_light<T>(s, g);
             V3<T> wi;
@@ -844,7 +881,13 @@ template <class T> V3<T> li_mis(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg
                 T pl = ls.area_dens * r2 / cl;
                 T pb = bsdf_pdf<T>(s, mat, x.n, wo, wi);
                 T w = pl * pl / (pl * pl + pb * pb);
-                L = L + mod(beta, d * w);
+                V3<T> add = mod(beta, d * w);
+                L = L + add;
+                if (r) {
+                    r->nee = true;
+                    r->wl = wi;
+                    r->c_nee = add;
+                }
             }
         }
         T dens;
@@ -857,18 +900,115 @@ template <class T> V3<T> li_mis(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg
         T cw = inner(x.n, wi);
         V3<T> f = bsdf_eval<T>(s, mat, x.n, wo, wi);
         beta = mod(beta, f) * cw / dens;
+        if (r) {
+            r->cont = true;
+            r->wi = wi;
+        }
         int ln = s.meshes[nx.geo].light;
         if (ln >= 0) {
             T cl = inner(nx.n, -wi);
             T pl = (light_pick_prob<T>(s, ln) / T(s.meshes[s.lights[ln].mesh].area)) *
                    (nx.t * nx.t) / cl;
             T w = dens * dens / (dens * dens + pl * pl);
-            L = L + mod(beta, light_radiance<T>(s.lights[ln], -wi, nx.n) * w);
+            V3<T> add = mod(beta, light_radiance<T>(s.lights[ln], -wi, nx.n) * w);
+            L = L + add;
+            if (r)
+                r->c_em = add;
         }
         x = nx;
         wo = -wi;
     }
     return L;
+}
+
+// Scalar factor s of a non-mixture lobe (its value is s * albedo) and ds/droughness.
+// Oren-Nayar: s = k / pi with the cut-offs of material.cpp:134-161, no roughness parameter.
+// cook_torr: s = F D G / (4 cos_i cos_o) (material.cpp:171-198) with alpha^2 = u = 2 r^2:
+//   dD/du = D (t - u) / (u (u + t)), t = tan^2 of the half vector's angle;
+//   dG/du = -G^2 (dLambda_i + dLambda_o), dLambda/du = tan^2 / (4 sqrt(1 + u tan^2));
+//   ds/dr = s (dD/du / D + dG/du / G) * 4 r.
+template <class T> void lobe_scalar(Material const &m, V3<T> n, V3<T> o, V3<T> i, T *sv, T *ds_dr) {
+    *sv = T(0);
+    *ds_dr = T(0);
+    if (m.kind == MAT_OREN_NAYAR) {
+        T k;
+        if (oren_nayar_k<T>(m, n, o, i, &k))
+            *sv = (T(1.0f) / pi_v<T>()) * k;
+        return;
+    }
+    T cos_o = smax(T(0.0f), inner(n, o));
+    T cos_i = smax(T(0.0f), inner(n, i));
+    if (near0(cos_i) || near0(cos_o))
+        return;
+    V3<T> h = i + o;
+    if (near0(h))
+        return;
+    h = normalize(h);
+    T u = alpha2_of<T>(m);
+    T f = fresnel(m, i, h);
+    T d = ggx_d(u, n, h);
+    T g = ggx_g(u, i, o, n);
+    *sv = f * d * g / (T(4.0f) * cos_i * cos_o);
+    T ct = inner(n, h);
+    T t = (T(1) - ct * ct) / (ct * ct);
+    auto dlam = [&](V3<T> w) {
+        T c = inner(n, w);
+        T t2 = (T(1) - c * c) / (c * c);
+        return t2 / (T(4) * std::sqrt(T(1) + u * t2));
+    };
+    T dlog = (t - u) / (u * (u + t)) - g * (dlam(i) + dlam(o));
+    *ds_dr = *sv * dlog * T(4) * T(m.roughness);
+}
+
+// Adds Q-weighted derivatives of log f (per channel) of material `mat` at (n, o, i) to grad
+// (4 per material: d albedo rgb, d roughness). f_c = sum over lobes w_l s_l a_{l,c}, so
+// d f_c / d a_{l,c} = w_l s_l and d f_c / d r_l = w_l a_{l,c} ds_l/dr.
+template <class T> void add_bsdf_grad(Scene const &s, int mat, V3<T> n, V3<T> o, V3<T> i, V3<T> Q, double *grad) {
+    if (mat < 0)
+        return; // fail-safe material: no parameters
+    V3<T> f = bsdf_eval<T>(s, mat, n, o, i);
+    T fc[3] = {f.x, f.y, f.z}, qc[3] = {Q.x, Q.y, Q.z};
+    auto lobe = [&](int lm, T wgt) {
+        Material const &L = s.mats[lm];
+        T sv, ds;
+        lobe_scalar<T>(L, n, o, i, &sv, &ds);
+        T ac[3] = {T(L.albedo.x), T(L.albedo.y), T(L.albedo.z)};
+        for (int c = 0; c < 3; c++) {
+            if (fc[c] == T(0))
+                continue;
+            grad[4 * lm + c] += static_cast<double>(qc[c] * wgt * sv / fc[c]);
+            if (L.kind == MAT_COOK_TORR)
+                grad[4 * lm + 3] += static_cast<double>(qc[c] * wgt * ac[c] * ds / fc[c]);
+        }
+    };
+    Material const &m = s.mats[mat];
+    if (m.kind == MAT_MIXTURE) {
+        lobe(m.sub0, T(m.ratio));
+        lobe(m.sub1, T(1) - T(m.ratio));
+    } else {
+        lobe(mat, T(1));
+    }
+}
+
+// Path-replay adjoint of one sample (Vicini et al. 2021, specialised to BSDF parameters with
+// the sampled directions, sampling densities and MIS weights held fixed): each contribution's
+// derivative is the contribution times the d log f of every BSDF evaluation it contains. The
+// path is re-traced from its PCG stream; `after` sums the contributions made after the BSDF
+// sample of vertex k (which all contain f at that vertex). gw = dLoss/dpixel / spp.
+template <class T> void mis_adjoint(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg &g, V3<T> gw, double *grad) {
+    AdjVertT<T> rec[kAdjMaxDepth];
+    uint32_t n = 0;
+    li_mis<T>(c, first, ray_v, g, rec, &n);
+    V3<T> tail;
+    for (int k = static_cast<int>(n) - 1; k >= 0; k--) {
+        AdjVertT<T> const &r = rec[k];
+        V3<T> after = r.c_em + tail;
+        if (r.cont)
+            add_bsdf_grad<T>(c.s, r.mat, r.n, r.wo, r.wi, mod(gw, after), grad);
+        if (r.nee)
+            add_bsdf_grad<T>(c.s, r.mat, r.n, r.wo, r.wl, mod(gw, r.c_nee), grad);
+        tail = after + r.c_nee;
+    }
 }
 
 // Reference mode: unidirect_lt1_path_tracer (pathtracer.cpp:587-653), NEE without MIS,
@@ -1167,11 +1307,31 @@ int orc_add_cook_torr(void *h, float const *albedo, float roughness, float ior_r
     m.kind = MAT_COOK_TORR;
     m.albedo = ld3(albedo);
     m.roughness = roughness;
+    m.sample_roughness = roughness;
     m.ior_re = ior_re;
     m.ior_im = ior_im;
     s->mats.push_back(m);
     return static_cast<int>(s->mats.size()) - 1;
 }
+// x3d_scene_set_material_params: albedo (Oren-Nayar, cook_torr) and roughness (cook_torr) of
+// materials [first, first + count); update_sampling also moves the GGX sampling roughness.
+void orc_set_material_params(void *h, uint32_t first, uint32_t count, float const *albedo, float const *roughness,
+                             int update_sampling) {
+    Scene *s = static_cast<Scene *>(h);
+    for (uint32_t k = 0; k < count; k++) {
+        Material &m = s->mats[first + k];
+        if (m.kind == MAT_MIXTURE)
+            continue;
+        if (albedo)
+            m.albedo = ld3(albedo + 3 * k);
+        if (roughness && m.kind == MAT_COOK_TORR) {
+            m.roughness = roughness[k];
+            if (update_sampling)
+                m.sample_roughness = roughness[k];
+        }
+    }
+}
+
 int orc_add_mixture(void *h, int m0, int m1, float ratio) {
     Scene *s = static_cast<Scene *>(h);
     Material m;
@@ -1423,6 +1583,72 @@ void orc_render(void *h, float const *cam, int tracer, uint32_t max_depth, uint3
         for (auto &ts : tstats)
             for (int i = 0; i < 6; i++)
                 stats[i] += ts[i];
+}
+
+// Gradient of a film loss with respect to material parameters for the config-mode MIS tracer
+// (x3d_render_backward): dl = dLoss/dfilm, float[4*w*h] (rgb used); grad = double[4*nmat]
+// (d albedo rgb, d roughness per material), overwritten. prec 0 runs the paths in FP32, 1 in
+// FP64; the sums are double either way and do not depend on the thread count.
+void orc_render_backward(void *h, float const *cam, uint32_t max_depth, uint64_t seed, uint32_t w, uint32_t hh,
+                         uint32_t spp, float const *dl, int prec, uint32_t threads, double *grad) {
+    Scene *s = static_cast<Scene *>(h);
+    Camera c{cam[0], cam[1], cam[2], {}};
+    std::memcpy(c.m, cam + 3, sizeof(c.m));
+    RenderOpts opt;
+    opt.tracer = TR_UNIDIRECT_MIS;
+    opt.max_depth = max_depth;
+    opt.seed = seed;
+    uint32_t nm = static_cast<uint32_t>(s->mats.size());
+    uint32_t total = w * hh;
+    if (threads == 0)
+        threads = 1;
+    // pixel rows are split into fixed per-thread ranges so the double sums are reproducible
+    std::vector<std::vector<double>> part(threads, std::vector<double>(4 * nm, 0.0));
+    auto work = [&](uint32_t tid) {
+        double *g = part[tid].data();
+        uint32_t begin = static_cast<uint32_t>(static_cast<uint64_t>(total) * tid / threads);
+        uint32_t end = static_cast<uint32_t>(static_cast<uint64_t>(total) * (tid + 1) / threads);
+        for (uint32_t pix = begin; pix < end; pix++) {
+            V3f o, v;
+            camera_ray(c, pix % w, pix / w, w, hh, &o, &v);
+            float inv = 1.0f / static_cast<float>(spp);
+            if (prec == 0) {
+                Ctx<float> cx{*s, opt, nullptr};
+                Hit<float> first = intersect<float>(*s, o, v, TRAV_CULLED, nullptr);
+                if (inner(first.n, -v) <= 0.0f)
+                    first = Hit<float>();
+                V3f gw(dl[4 * pix] * inv, dl[4 * pix + 1] * inv, dl[4 * pix + 2] * inv);
+                for (uint32_t k = 0; k < spp; k++) {
+                    Pcg pg = pcg_for(seed, pix, k);
+                    mis_adjoint<float>(cx, first, v, pg, gw, g);
+                }
+            } else {
+                Ctx<double> cx{*s, opt, nullptr};
+                V3<double> od = cvt<double>(o), vd = cvt<double>(v);
+                Hit<double> first = intersect<double>(*s, od, vd, TRAV_CULLED, nullptr);
+                if (inner(first.n, -vd) <= 0.0)
+                    first = Hit<double>();
+                double sd = static_cast<double>(spp);
+                V3<double> gw(static_cast<double>(dl[4 * pix]) / sd, static_cast<double>(dl[4 * pix + 1]) / sd,
+                              static_cast<double>(dl[4 * pix + 2]) / sd);
+                for (uint32_t k = 0; k < spp; k++) {
+                    Pcg pg = pcg_for(seed, pix, k);
+                    mis_adjoint<double>(cx, first, vd, pg, gw, g);
+                }
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < threads; t++)
+        pool.emplace_back(work, t);
+    for (auto &t : pool)
+        t.join();
+    for (uint32_t i = 0; i < 4 * nm; i++) {
+        double acc = 0.0;
+        for (uint32_t t = 0; t < threads; t++)
+            acc += part[t][i];
+        grad[i] = acc;
+    }
 }
 
 } // extern "C"

@@ -155,6 +155,7 @@ struct Material {
     V3f albedo{0.8f, 0.8f, 0.8f};
     float sigma = 0.0f;     // Oren-Nayar roughness (0 = Lambertian).
     float roughness = 0.0f; // cook_torr roughness r, alpha^2 = 2 r^2 (material.cpp:379).
+    float sample_roughness = 0.0f; // roughness of the GGX sampling density (= roughness when added)
     float ior_re = 1.0f, ior_im = 0.0f;
     int sub0 = -1, sub1 = -1; // mixture operands
     float ratio = 0.5f;

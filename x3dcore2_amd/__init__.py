@@ -105,6 +105,7 @@ EXPORTS = [
     "x3d_bsdf_eval", "x3d_bsdf_pdf", "x3d_bsdf_sample", "x3d_camera_rays", "x3d_render_options_default",
     "x3d_renderer_create", "x3d_renderer_destroy", "x3d_render", "x3d_render_device", "x3d_pcg_draws",
     "x3d_renderer_autotune", "x3d_render_batch", "x3d_render_batch_device", "x3d_renderer_autotune_batch",
+    "x3d_scene_set_material_params", "x3d_render_forward", "x3d_render_backward", "x3d_render_backward_device",
 ]
 
 
@@ -234,6 +235,19 @@ class Scene:
         _check(lib().x3d_scene_get_info(self.h, C.byref(i)))
         return i
 
+    @property
+    def num_materials(self) -> int:
+        """Materials added by the caller (the committed table also holds the fail-safe one)."""
+        return self.info().num_materials - 1
+
+    def set_material_params(self, first, albedo=None, roughness=None, update_sampling=True):
+        """albedo: [count, 3] or None; roughness: [count] or None (x3d_scene_set_material_params)."""
+        a = None if albedo is None else np.ascontiguousarray(albedo, np.float32).reshape(-1, 3)
+        r = None if roughness is None else np.ascontiguousarray(roughness, np.float32).reshape(-1)
+        count = a.shape[0] if a is not None else r.shape[0]
+        _check(lib().x3d_scene_set_material_params(self.h, first, count, _ptr(a) if a is not None else None,
+                                                   _ptr(r) if r is not None else None, int(update_sampling)))
+
     def bvh(self):
         i = self.info()
         bounds = np.zeros((i.num_nodes, 6), np.float32)
@@ -340,6 +354,23 @@ class Renderer:
         rep = AutotuneReport()
         _check(lib().x3d_renderer_autotune_batch(self.h, len(items), arr, C.byref(rep)))
         return rep
+
+    def forward(self, scene: Scene, camera, width, height, spp):
+        rgba = np.zeros((height * width, 4), np.float32)
+        st = RenderStats()
+        _check(lib().x3d_render_forward(self.h, scene.h, C.byref(camera_desc(camera)), width, height, spp, _ptr(rgba),
+                                        C.byref(st)))
+        return rgba, st
+
+    def backward(self, scene: Scene, camera, width, height, spp, dl_dfilm):
+        """dl_dfilm: [h*w, 4] (rgb used). Returns ([num_materials, 4] float32 gradient:
+        albedo rgb, roughness), stats."""
+        dl = np.ascontiguousarray(dl_dfilm, np.float32).reshape(height * width, 4)
+        grad = np.zeros((scene.num_materials, 4), np.float32)
+        st = RenderStats()
+        _check(lib().x3d_render_backward(self.h, scene.h, C.byref(camera_desc(camera)), width, height, spp,
+                                         _ptr(dl), _ptr(grad), C.byref(st)))
+        return grad, st
 
     def autotune(self, scene: Scene, camera, width, height, probe_spp) -> AutotuneReport:
         rep = AutotuneReport()

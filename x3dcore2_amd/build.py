@@ -44,7 +44,7 @@ def build_library(verbose: bool = True) -> str:
     scene_o = os.path.join(BUILD, "scene.o")
     cmds = [[gxx, *GXX_FLAGS, "-c", os.path.join(CSRC, "scene.cpp"), "-o", scene_o]]
     objs = [scene_o]
-    for unit in ("render", "wavefront", "batch"):
+    for unit in ("render", "wavefront", "batch", "adjoint"):
         o = os.path.join(BUILD, unit + ".o")
         cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, unit + ".hip"), "-o", o])
         objs.append(o)

@@ -1,0 +1,211 @@
+// Path-replay adjoint of the fixed-depth MIS tracer with respect to material parameters.
+//
+// Each (pixel, sample) path is traced again from its PCG stream (nothing from the forward pass
+// is stored), recording per vertex the directions and the NEE / emission contributions it made.
+// With the sampled directions, the sampling densities and the MIS weights held fixed (the
+// sampling density of a GGX lobe is a parameter of its own, see x3d_scene_set_material_params),
+// the derivative of a contribution is the contribution times the sum of d log f over the BSDF
+// evaluations it contains (Vicini, Speierer, Jakob 2021, "Path Replay Backpropagation",
+// specialised to BSDF parameters). Walking the recorded vertices backwards, `after` holds the
+// contributions made after the BSDF sample of the current vertex, which all contain its f.
+//
+// Gradients go to per-wave slots in LDS (one float per material parameter), then per block in
+// wave order, then over blocks in block order: with the static (pixel, sample) assignment the
+// result does not depend on timing.
+
+#include "adjoint.h"
+
+#include <algorithm>
+
+namespace x3d {
+
+namespace {
+
+// Scalar factor s of a non-mixture lobe (the lobe is s * albedo) and ds/droughness.
+// cook_torr: s = F D G / (4 cos_i cos_o) with u = alpha^2 = 2 r^2:
+//   dD/du = D (t - u) / (u (u + t)), t = tan^2 of the half vector's angle to n;
+//   dG/du = -G^2 (dLambda_i + dLambda_o), dLambda/du = tan^2 / (4 sqrt(1 + u tan^2));
+//   ds/dr = s (dD/du / D + dG/du / G) 4 r.
+X3D_DI void lobe_scalar(DevMaterial const &m, f3 n, f3 o, f3 i, float *sv, float *ds_dr) {
+    *sv = 0.0f;
+    *ds_dr = 0.0f;
+    if (m.kind == kMatOrenNayar) {
+        float k;
+        if (oren_nayar_k(m, n, o, i, &k))
+            *sv = (1.0f / kPi) * k;
+        return;
+    }
+    if (m.kind != kMatCookTorr)
+        return;
+    float cos_o = smax(0.0f, dot(n, o));
+    float cos_i = smax(0.0f, dot(n, i));
+    if (near0(cos_i) || near0(cos_o))
+        return;
+    f3 h = i + o;
+    if (near0(h))
+        return;
+    h = normalize(h);
+    float u = m.alpha2;
+    float f = fresnel(m, i, h);
+    float d = ggx_d(u, n, h);
+    float g = ggx_g(u, i, o, n);
+    *sv = f * d * g / (4.0f * cos_i * cos_o);
+    float ct = dot(n, h);
+    float t = (1.0f - ct * ct) / (ct * ct);
+    auto dlam = [&](f3 w) {
+        float c = dot(n, w);
+        float t2 = (1.0f - c * c) / (c * c);
+        return t2 / (4.0f * __builtin_sqrtf(1.0f + u * t2));
+    };
+    float dlog = (t - u) / (u * (u + t)) - g * (dlam(i) + dlam(o));
+    *ds_dr = *sv * dlog * 4.0f * m.roughness;
+}
+
+// Adds Q-weighted d log f_c (per channel) of material `mat` at (n, o, i) to the slots
+// (4 per material: d albedo rgb, d roughness): f_c = sum over lobes w_l s_l a_{l,c}.
+X3D_DI void add_bsdf_grad(DevScene const &S, int32_t mat, f3 n, f3 o, f3 i, f3 Q, float *slots) {
+    if (mat < 0 || static_cast<uint32_t>(mat) >= S.fail_safe_mat)
+        return; // the fail-safe material has no parameters
+    f3 f = bsdf_eval(S.mats, mat, n, o, i);
+    const float fc[3] = {f.x, f.y, f.z}, qc[3] = {Q.x, Q.y, Q.z};
+    auto lobe = [&](int32_t lm, float wgt) {
+        DevMaterial const &L = S.mats[lm];
+        if (static_cast<uint32_t>(lm) >= kAdjMaxMats)
+            return;
+        float sv, ds;
+        lobe_scalar(L, n, o, i, &sv, &ds);
+        float gr = 0.0f;
+        for (int c = 0; c < 3; c++) {
+            if (fc[c] == 0.0f)
+                continue;
+            atomicAdd(&slots[4 * lm + c], qc[c] * wgt * sv / fc[c]);
+            gr += qc[c] * wgt * L.albedo[c] * ds / fc[c];
+        }
+        if (L.kind == kMatCookTorr)
+            atomicAdd(&slots[4 * lm + 3], gr);
+    };
+    DevMaterial const &m = S.mats[mat];
+    if (m.kind == kMatMixture) {
+        lobe(m.sub0, m.ratio);
+        lobe(m.sub1, 1.0f - m.ratio);
+    } else {
+        lobe(mat, 1.0f);
+    }
+}
+
+constexpr int kAdjWaves = kWG / 64;
+
+__global__ __launch_bounds__(kWG) void k_adjoint(RenderParams P, const float4 *dl, uint32_t num_mats,
+                                                 float *partial) {
+    __shared__ uint2 lds[kLdsDepth * kWG];
+    __shared__ float slots[kAdjWaves][4 * kAdjMaxMats];
+    for (uint32_t i = threadIdx.x; i < kAdjWaves * 4 * kAdjMaxMats; i += kWG)
+        (&slots[0][0])[i] = 0.0f;
+    __syncthreads();
+    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
+    Tracer<kWG, kLdsDepth> tr{P.S, st};
+    float *my = slots[threadIdx.x >> 6];
+    uint32_t npix = P.width * P.height;
+    uint64_t total = static_cast<uint64_t>(npix) * P.spp_total;
+    float inv = 1.0f / static_cast<float>(P.spp_total);
+    AdjVert rec[kAdjMaxDepth];
+    for (uint64_t q = blockIdx.x * static_cast<uint64_t>(kWG) + threadIdx.x; q < total;
+         q += static_cast<uint64_t>(gridDim.x) * kWG) {
+        uint32_t pix = static_cast<uint32_t>(q % npix), s = static_cast<uint32_t>(q / npix);
+        f3 o, v;
+        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        bool valid;
+        HitRec first = load_first(P, pix, valid);
+        float4 d = dl[pix];
+        f3 gw = mk(d.x * inv, d.y * inv, d.z * inv);
+        PathState ps;
+        tr.mis_begin(ps, first, valid, v, pcg_for(P.seed, pix, s), P.max_depth);
+        uint32_t nrec = 0;
+        while (ps.alive) {
+            nrec = ps.k + 1;
+            tr.mis_bounce<true>(ps, P.max_depth, rec);
+        }
+        f3 tail = mk(0.0f, 0.0f, 0.0f);
+        for (int k = static_cast<int>(nrec) - 1; k >= 0; k--) {
+            AdjVert const &r = rec[k];
+            f3 after = r.c_em + tail;
+            if (r.flags & kAdjCont)
+                add_bsdf_grad(P.S, r.mat, r.n, r.wo, r.wi, mul(gw, after), my);
+            if (r.flags & kAdjNee)
+                add_bsdf_grad(P.S, r.mat, r.n, r.wo, r.wl, mul(gw, r.c_nee), my);
+            tail = after + r.c_nee;
+        }
+    }
+    if (P.cnt) {
+        flush_counter(&P.cnt->closest, tr.nclosest);
+        flush_counter(&P.cnt->anyhit, tr.nany);
+        flush_counter(&P.cnt->pops, tr.pops);
+        flush_counter(&P.cnt->tests, tr.tests);
+        flush_counter(&P.cnt->spops, tr.spops);
+        flush_counter(&P.cnt->stests, tr.stests);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 4 * num_mats; i += kWG) {
+        float acc = 0.0f;
+        for (int w = 0; w < kAdjWaves; w++)
+            acc += slots[w][i];
+        partial[static_cast<size_t>(blockIdx.x) * 4 * num_mats + i] = acc;
+    }
+}
+
+__global__ void k_adjoint_reduce(const float *partial, uint32_t blocks, uint32_t n, float *grad) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double acc = 0.0;
+    for (uint32_t b = 0; b < blocks; b++)
+        acc += static_cast<double>(partial[static_cast<size_t>(b) * n + i]);
+    grad[i] = static_cast<float>(acc);
+}
+
+} // namespace
+
+struct AdjointEngine {
+    DevBuf partial, spill;
+};
+
+AdjointEngine *adjoint_create() { return new AdjointEngine(); }
+void adjoint_destroy(AdjointEngine *e) { delete e; }
+
+std::string adjoint_render(AdjointEngine *e, RenderParams const &P, const float4 *d_dl, uint32_t num_mats,
+                           float *d_grad, hipStream_t stream) {
+    if (num_mats > kAdjMaxMats)
+        return "the adjoint supports at most " + std::to_string(kAdjMaxMats) + " materials";
+    if (P.max_depth > kAdjMaxDepth)
+        return "the adjoint supports max_depth <= " + std::to_string(kAdjMaxDepth);
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess)
+        return "hipGetDevice failed";
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_adjoint, kWG, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    uint64_t items = static_cast<uint64_t>(P.width) * P.height * P.spp_total;
+    uint64_t want = (items + kWG - 1) / kWG;
+    uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>(want, static_cast<uint64_t>(cus) * per_cu));
+    if (grid < 1)
+        grid = 1;
+    uint32_t n = 4 * (num_mats ? num_mats : 1);
+    if (hipError_t err = e->partial.ensure(sizeof(float) * static_cast<size_t>(grid) * n); err != hipSuccess)
+        return std::string("hipMalloc: ") + hipGetErrorString(err);
+    RenderParams Q = P;
+    Q.spill_per_thread = spill_entries_for(P.S, kLdsDepth);
+    if (hipError_t err = e->spill.ensure(sizeof(uint2) * static_cast<size_t>(grid) * kWG * Q.spill_per_thread);
+        err != hipSuccess)
+        return std::string("hipMalloc: ") + hipGetErrorString(err);
+    Q.spill = static_cast<uint2 *>(e->spill.p);
+    hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(kWG), 0, stream, Q, d_dl, num_mats,
+                       static_cast<float *>(e->partial.p));
+    hipLaunchKernelGGL(k_adjoint_reduce, dim3((n + 255) / 256), dim3(256), 0, stream,
+                       static_cast<const float *>(e->partial.p), grid, 4 * num_mats, d_grad);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess)
+        return std::string("adjoint launch: ") + hipGetErrorString(err);
+    return "";
+}
+
+} // namespace x3d

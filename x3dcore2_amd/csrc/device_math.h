@@ -220,30 +220,39 @@ X3D_DI float ggx_g(float a2, f3 i, f3 o, f3 n) {
     return 1.0f / (1.0f + ggx_lambda(a2, n, i) + ggx_lambda(a2, n, o));
 }
 
+// oren_nayar::eval factor k (material.cpp:134-161): the lobe is (albedo / pi) * k; false when
+// either cosine is cut off.
+X3D_DI bool oren_nayar_k(DevMaterial const &m, f3 n, f3 o, f3 i, float *k) {
+    float ci = dot(i, n), co = dot(o, n);
+    float cos_thei = ci < 0.0f ? 0.0f : (ci > 1.0f ? 1.0f : ci);
+    float cos_theo = co < 0.0f ? 0.0f : (co > 1.0f ? 1.0f : co);
+    if (near0(cos_thei) || near0(cos_theo))
+        return false;
+    float ca, cb;
+    if (cos_thei < cos_theo) {
+        ca = cos_thei;
+        cb = cos_theo;
+    } else {
+        ca = cos_theo;
+        cb = cos_thei;
+    }
+    float sin_a = __builtin_sqrtf(1.0f - ca * ca);
+    float sin_b = __builtin_sqrtf(1.0f - cb * cb);
+    float cos_io = ca * cb + sin_a * sin_b;
+    float tan_b = sin_b / cos_theo;
+    *k = m.on_a + m.on_b * smax(0.0f, cos_io) * sin_a * tan_b;
+    return true;
+}
+
 // Evaluation of one non-mixture lobe.
 X3D_DI f3 lobe_eval(DevMaterial const &m, f3 n, f3 o, f3 i) {
     f3 alb = mk(m.albedo[0], m.albedo[1], m.albedo[2]);
     if (m.kind == kMatFailSafe)
         return (1.0f / kPi) * alb;
     if (m.kind == kMatOrenNayar) {
-        float ci = dot(i, n), co = dot(o, n);
-        float cos_thei = ci < 0.0f ? 0.0f : (ci > 1.0f ? 1.0f : ci);
-        float cos_theo = co < 0.0f ? 0.0f : (co > 1.0f ? 1.0f : co);
-        if (near0(cos_thei) || near0(cos_theo))
+        float k;
+        if (!oren_nayar_k(m, n, o, i, &k))
             return mk(0.0f, 0.0f, 0.0f);
-        float ca, cb;
-        if (cos_thei < cos_theo) {
-            ca = cos_thei;
-            cb = cos_theo;
-        } else {
-            ca = cos_theo;
-            cb = cos_thei;
-        }
-        float sin_a = __builtin_sqrtf(1.0f - ca * ca);
-        float sin_b = __builtin_sqrtf(1.0f - cb * cb);
-        float cos_io = ca * cb + sin_a * sin_b;
-        float tan_b = sin_b / cos_theo;
-        float k = m.on_a + m.on_b * smax(0.0f, cos_io) * sin_a * tan_b;
         return ((1.0f / kPi) * alb) * k;
     }
     // cook_torr
@@ -279,7 +288,7 @@ X3D_DI float lobe_pdf(DevMaterial const &m, f3 n, f3 o, f3 i) {
     float ct = dot(n, h);
     if (!(ct > 0.0f))
         return 0.0f;
-    return ggx_d(m.alpha2, n, h) * ct / (4.0f * hdo);
+    return ggx_d(m.sample_alpha2, n, h) * ct / (4.0f * hdo);
 }
 
 X3D_DI f3 lobe_sample(DevMaterial const &m, f3 n, f3 o, Pcg &g, float *dens) {
@@ -290,7 +299,7 @@ X3D_DI f3 lobe_sample(DevMaterial const &m, f3 n, f3 o, Pcg &g, float *dens) {
         *dens = dot(i, n) / kPi;
         return i;
     }
-    float ms = m.alpha2;
+    float ms = m.sample_alpha2;
     f3 u, v;
     basis(n, &u, &v);
     float e0 = g.draw();

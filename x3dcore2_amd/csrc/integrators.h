@@ -76,6 +76,17 @@ struct Pathlet {
     int32_t mat;
 };
 
+// One vertex of a recorded MIS path (for the adjoint): shading normal, outgoing direction,
+// sampled BSDF direction, light-sample direction, the NEE contribution made at the vertex and
+// the emission reached through the BSDF sample (both weighted by throughput and MIS weight).
+struct AdjVert {
+    f3 n, wo, wi, wl, c_nee, c_em;
+    int32_t mat;
+    uint32_t flags; // kAdjNee: NEE contributed; kAdjCont: the BSDF sample continued the path
+};
+constexpr uint32_t kAdjNee = 1u, kAdjCont = 2u;
+constexpr uint32_t kAdjMaxDepth = 8; // deepest path the adjoint records
+
 // State of one MIS path between bounces (see Tracer::mis_begin / mis_bounce).
 struct PathState {
     HitRec x;
@@ -150,8 +161,19 @@ template <int WG, int LDSD> struct Tracer {
             ps.L = ps.L + light_radiance(S, first.light, ps.wo, first.n);
     }
 
-    X3D_DI void mis_bounce(PathState &ps, uint32_t max_depth) {
+    // With REC, the vertex is recorded in rec[ps.k] (the adjoint's replay).
+    template <bool REC = false> X3D_DI void mis_bounce(PathState &ps, uint32_t max_depth, AdjVert *rec = nullptr) {
         HitRec const &x = ps.x;
+        AdjVert *r = nullptr;
+        if constexpr (REC) {
+            r = rec + ps.k;
+            r->n = x.n;
+            r->wo = ps.wo;
+            r->mat = x.mat;
+            r->flags = 0;
+            r->c_nee = mk(0.0f, 0.0f, 0.0f);
+            r->c_em = mk(0.0f, 0.0f, 0.0f);
+        }
         if (S.num_lights > 0) {
             LightSample ls = sample_light(S, ps.g);
             f3 d, wi;
@@ -160,7 +182,13 @@ template <int WG, int LDSD> struct Tracer {
                 float pl = ls.area_dens * r2 / cl;
                 float pb = bsdf_pdf(S.mats, x.mat, x.n, ps.wo, wi);
                 float w = pl * pl / (pl * pl + pb * pb);
-                ps.L = ps.L + mul(ps.beta, d * w);
+                f3 add = mul(ps.beta, d * w);
+                ps.L = ps.L + add;
+                if constexpr (REC) {
+                    r->flags |= kAdjNee;
+                    r->wl = wi;
+                    r->c_nee = add;
+                }
             }
         }
         float dens;
@@ -173,12 +201,19 @@ template <int WG, int LDSD> struct Tracer {
         float cw = dot(x.n, wi);
         f3 f = bsdf_eval(S.mats, x.mat, x.n, ps.wo, wi);
         ps.beta = mul(ps.beta, f) * cw / dens;
+        if constexpr (REC) {
+            r->flags |= kAdjCont;
+            r->wi = wi;
+        }
         if (nx.light >= 0) {
             float cl = dot(nx.n, -wi);
             DevLight const &Ln = S.lights[nx.light];
             float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
             float w = dens * dens / (dens * dens + pl * pl);
-            ps.L = ps.L + mul(ps.beta, light_radiance(S, nx.light, -wi, nx.n) * w);
+            f3 add = mul(ps.beta, light_radiance(S, nx.light, -wi, nx.n) * w);
+            ps.L = ps.L + add;
+            if constexpr (REC)
+                r->c_em = add;
         }
         ps.x = nx;
         ps.wo = -wi;

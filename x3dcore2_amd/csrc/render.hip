@@ -19,6 +19,7 @@
 #include "../../include/x3dpt.h"
 #include "render_common.h"
 #include "scene.h"
+#include "adjoint.h"
 #include "batch.h"
 #include "wavefront.h"
 
@@ -453,7 +454,11 @@ struct x3d_renderer {
     x3d::WavefrontEngine *wf = nullptr;
     x3d::BatchEngine *batch = nullptr;
     std::vector<x3d::DevBuf> batch_films; // device films of x3d_render_batch
+    x3d::AdjointEngine *adj = nullptr;
+    x3d::DevBuf adj_dl, adj_grad;          // host-pointer variant of x3d_render_backward
     ~x3d_renderer() {
+        if (adj)
+            x3d::adjoint_destroy(adj);
         if (wf)
             x3d::wf_destroy(wf);
         if (batch)
@@ -524,8 +529,11 @@ template <int T> void launch_plain(uint32_t grid, hipStream_t s, RenderParams co
     hipLaunchKernelGGL(k_render_plain<T>, dim3(grid), dim3(kWG), 0, s, P);
 }
 
+// Renders into d_out; with d_dl (dLoss/dfilm) it runs the adjoint of the MIS tracer instead and
+// writes the material gradient to d_grad.
 int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
-                uint32_t height, uint32_t spp, float *d_out, x3d_render_stats *stats) {
+                uint32_t height, uint32_t spp, float *d_out, x3d_render_stats *stats,
+                const float4 *d_dl = nullptr, float *d_grad = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     RenderParams P{};
     P.S = scene->s.dev;
@@ -572,7 +580,15 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     P.s_begin = 0;
     P.s_end = spp;
     P.finalize = 1;
-    if (r->wf) {
+    if (d_dl) {
+        if (!r->adj)
+            r->adj = x3d::adjoint_create();
+        uint32_t user_mats = static_cast<uint32_t>(scene->s.materials.size()) - 1; // fail-safe appended last
+        std::string e = x3d::adjoint_render(r->adj, P, d_dl, user_mats, d_grad, r->stream);
+        if (!e.empty())
+            return fail(e.find("hip") != std::string::npos ? X3D_ERR_DEVICE : X3D_ERR_UNSUPPORTED, e);
+        tracer = -1; // launched
+    } else if (r->wf) {
         x3d::WavefrontTuning tune;
         if (r->opt.workgroup_size)
             tune.trace_wg = static_cast<int>(r->opt.workgroup_size);
@@ -780,6 +796,7 @@ int x3d_scene_add_material(x3d_scene *scene, const x3d_material_desc *d, int32_t
         if (!(d->roughness > 0.0f))
             return fail(X3D_ERR_INVALID_ARGUMENT, "cook_torr roughness must be > 0");
         m.alpha2 = 2 * d->roughness * d->roughness; // material.cpp:379
+        m.sample_alpha2 = m.alpha2;
         m.roughness = d->roughness;
         m.ior_re = d->ior_re;
         m.ior_im = d->ior_im;
@@ -864,6 +881,41 @@ int x3d_scene_commit(x3d_scene *scene) {
     std::string e = scene->s.commit();
     if (!e.empty())
         return fail(e.find("hip") != std::string::npos ? X3D_ERR_DEVICE : X3D_ERR_INVALID_ARGUMENT, e);
+    return X3D_OK;
+}
+
+int x3d_scene_set_material_params(x3d_scene *scene, uint32_t first, uint32_t count, const float *albedo,
+                                  const float *roughness, int32_t update_sampling) {
+    int rc = check_scene(scene);
+    if (rc)
+        return rc;
+    Scene &s = scene->s;
+    uint32_t user = static_cast<uint32_t>(s.materials.size()) - 1; // fail-safe appended last
+    if (first > user || count > user - first)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "material range out of bounds");
+    for (uint32_t k = 0; k < count; k++) {
+        DevMaterial const &m = s.materials[first + k];
+        if (albedo && !finite3(albedo + 3 * k))
+            return fail(X3D_ERR_INVALID_ARGUMENT, "albedo must be finite");
+        if (roughness && m.kind == kMatCookTorr && !(roughness[k] > 0.0f && std::isfinite(roughness[k])))
+            return fail(X3D_ERR_INVALID_ARGUMENT, "cook_torr roughness must be > 0");
+    }
+    for (uint32_t k = 0; k < count; k++) {
+        DevMaterial &m = s.materials[first + k];
+        if (m.kind == kMatMixture)
+            continue;
+        if (albedo)
+            for (int c = 0; c < 3; c++)
+                m.albedo[c] = albedo[3 * k + c];
+        if (roughness && m.kind == kMatCookTorr) {
+            m.roughness = roughness[k];
+            m.alpha2 = 2 * roughness[k] * roughness[k]; // material.cpp:379
+            if (update_sampling)
+                m.sample_alpha2 = m.alpha2;
+        }
+    }
+    X3D_HIP(hipMemcpy(s.buf.mats, s.materials.data(), s.materials.size() * sizeof(DevMaterial),
+                      hipMemcpyHostToDevice));
     return X3D_OK;
 }
 
@@ -1352,6 +1404,61 @@ int x3d_renderer_autotune_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_jo
     r->opt.sort_key = rep.sort_key;
     if (report)
         *report = rep;
+    return X3D_OK;
+}
+
+int x3d_render_forward(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                       uint32_t height, uint32_t spp, float *rgba, x3d_render_stats *stats) {
+    return x3d_render(r, scene, camera, width, height, spp, 0, rgba, stats);
+}
+
+namespace {
+int check_backward(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                   uint32_t height, uint32_t spp, const void *dl, const void *grad) {
+    int rc = check_render_args(r, scene, camera, width, height, spp);
+    if (rc)
+        return rc;
+    if (!dl || !grad)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "dl_dfilm or grad is null");
+    if (r->opt.pt_type != X3D_PT_UNIDIRECT_MIS || r->opt.form != X3D_FORM_PLAIN)
+        return fail(X3D_ERR_STATE, "the adjoint needs a plain-form UNIDIRECT_MIS renderer");
+    if (r->opt.max_depth > x3d::kAdjMaxDepth)
+        return fail(X3D_ERR_UNSUPPORTED, "the adjoint supports max_depth <= " + std::to_string(x3d::kAdjMaxDepth));
+    if (scene->s.materials.size() - 1 > x3d::kAdjMaxMats)
+        return fail(X3D_ERR_UNSUPPORTED, "the adjoint supports at most " + std::to_string(x3d::kAdjMaxMats) +
+                                             " materials");
+    return X3D_OK;
+}
+} // namespace
+
+int x3d_render_backward_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                               uint32_t height, uint32_t spp, const float *d_dl_dfilm, float *d_grad,
+                               x3d_render_stats *stats) {
+    int rc = check_backward(r, scene, camera, width, height, spp, d_dl_dfilm, d_grad);
+    if (rc)
+        return rc;
+    return render_impl(r, scene, camera, width, height, spp, nullptr, stats,
+                       reinterpret_cast<const float4 *>(d_dl_dfilm), d_grad);
+}
+
+int x3d_render_backward(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                        uint32_t height, uint32_t spp, const float *dl_dfilm, float *grad, x3d_render_stats *stats) {
+    int rc = check_backward(r, scene, camera, width, height, spp, dl_dfilm, grad);
+    if (rc)
+        return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    size_t npix = static_cast<size_t>(width) * height;
+    size_t ng = 4 * (scene->s.materials.size() - 1);
+    X3D_HIP(r->adj_dl.ensure(npix * sizeof(float4)));
+    X3D_HIP(r->adj_grad.ensure(std::max<size_t>(ng, 4) * sizeof(float)));
+    X3D_HIP(hipMemcpy(r->adj_dl.p, dl_dfilm, npix * sizeof(float4), hipMemcpyHostToDevice));
+    rc = render_impl(r, scene, camera, width, height, spp, nullptr, stats, static_cast<const float4 *>(r->adj_dl.p),
+                     static_cast<float *>(r->adj_grad.p));
+    if (rc)
+        return rc;
+    X3D_HIP(hipMemcpy(grad, r->adj_grad.p, ng * sizeof(float), hipMemcpyDeviceToHost));
+    if (stats)
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return X3D_OK;
 }
 

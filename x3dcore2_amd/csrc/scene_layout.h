@@ -32,7 +32,9 @@ struct DevMaterial { // 64 B
     float ratio;
     int32_t sub0, sub1;
     float roughness, sigma;
-    int32_t pad[2];
+    float sample_alpha2; // alpha^2 the GGX lobe is sampled with (and its pdf); alpha2 unless the
+                         // evaluated parameters were changed with update_sampling off
+    int32_t pad;
 };
 static_assert(sizeof(DevMaterial) == 64, "DevMaterial layout");
 

@@ -288,6 +288,72 @@ def indirect_scene(width=512, height=512, spp=32, max_path_len=6, level=5) -> Sc
     return desc
 
 
+ADJOINT_GROUPS = 16
+
+
+def adjoint_truth(seed=0x5EED0004) -> Tuple[np.ndarray, np.ndarray]:
+    """Seeded ground-truth parameters of the inverse-rendering workload: albedo [16, 3] in
+    [0.15, 0.85] and GGX roughness [16] in [0.2, 0.7]."""
+    rng = np.random.default_rng(seed)
+    return (rng.uniform(0.15, 0.85, size=(ADJOINT_GROUPS, 3)).astype(np.float32),
+            rng.uniform(0.2, 0.7, size=ADJOINT_GROUPS).astype(np.float32))
+
+
+def adjoint_scene(albedo=None, roughness=None, width=256, height=256, spp=32, max_depth=4) -> SceneDesc:
+    """The box room cut into 16 parametric surfaces: floor (front and back halves), ceiling,
+    back, left and right walls, and the five visible faces of each block (the blocks' bottoms
+    share the front floor's parameters). Surface m is a 50/50 mixture of a Lambertian and a GGX
+    lobe (reference mat_mixture of oren_nayar and cook_torr) that share albedo[m]; the GGX lobe
+    has roughness[m]. Material ids: 3m Lambertian, 3m+1 GGX, 3m+2 the mixture; the emitter's
+    black Lambertian is 48."""
+    albedo = np.full((ADJOINT_GROUPS, 3), 0.5, np.float32) if albedo is None else np.asarray(albedo, np.float32)
+    roughness = np.full(ADJOINT_GROUPS, 0.4, np.float32) if roughness is None else np.asarray(roughness, np.float32)
+    mats: List[Material] = []
+    for m in range(ADJOINT_GROUPS):
+        a = tuple(float(x) for x in albedo[m])
+        mats.append(Material(OREN_NAYAR, a))
+        mats.append(Material(COOK_TORR, a, roughness=float(roughness[m]), ior=GLOSSY_IOR))
+        mats.append(Material(MIXTURE, sub0=3 * m, sub1=3 * m + 1, ratio=0.5))
+    mix = lambda m: 3 * m + 2
+    meshes = [
+        quad((0, 0, 0.5), (1, 0, 0.5), (1, 0, 1), (0, 0, 1), (0, 1, 0), mix(0)),  # floor, front half
+        quad((0, 0, 0), (1, 0, 0), (1, 0, 0.5), (0, 0, 0.5), (0, 1, 0), mix(1)),  # floor, back half
+        quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0), (0, -1, 0), mix(2)),  # ceiling
+        quad((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0), (0, 0, 1), mix(3)),  # back
+        quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 0, 0), mix(4)),  # left
+        quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (-1, 0, 0), mix(5)),  # right
+    ]
+    # box() face order: +x, -x, +y, -y, +z, -z (4 vertices, 2 triangles each)
+    for base, (center, size, ang) in ((6, ((0.33, 0.15, 0.62), (0.3, 0.3, 0.3), -17.0)),
+                                      (11, ((0.66, 0.3, 0.35), (0.3, 0.6, 0.3), 17.0))):
+        b = box(center, size, math.radians(ang), 0)
+        groups = [base, base + 1, base + 2, 0, base + 3, base + 4]
+        for f, g in enumerate(groups):
+            meshes.append(Mesh(b.verts[4 * f:4 * f + 4].copy(), b.normals[4 * f:4 * f + 4].copy(),
+                               np.array([[0, 1, 2], [0, 2, 3]], np.uint32), mix(g)))
+    lights = []
+    _ceiling_emitter(mats, meshes, lights)
+    return SceneDesc("adjoint", mats, meshes, lights, translate_camera(0.5, 0.5, 2.0), width, height, spp,
+                     max_depth)
+
+
+def tie_gradient(grad: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """Per-material gradient [num_materials, 4] of an adjoint_scene -> gradient with respect to
+    the 16 shared albedos [16, 3] and roughnesses [16]."""
+    g = np.asarray(grad, np.float64)
+    ga = np.stack([g[3 * m, :3] + g[3 * m + 1, :3] for m in range(ADJOINT_GROUPS)])
+    gr = np.array([g[3 * m + 1, 3] for m in range(ADJOINT_GROUPS)])
+    return ga, gr
+
+
+def set_adjoint_params(scene, albedo, roughness, update_sampling=False):
+    """Writes the 16 shared parameters into an adjoint_scene's material table (a device Scene
+    or an oracle, both with set_material_params(first, albedo, roughness, update_sampling))."""
+    a = np.repeat(np.asarray(albedo, np.float32), 3, axis=0).reshape(-1, 3)  # ON, GGX, mixture
+    r = np.repeat(np.asarray(roughness, np.float32), 3)
+    scene.set_material_params(0, a, r, update_sampling)
+
+
 @functools.lru_cache(maxsize=None)
 def _unit_icosphere(level: int) -> Tuple[np.ndarray, np.ndarray]:
     v, f = icosphere(level)

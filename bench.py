@@ -10,7 +10,7 @@ HBM. Both execution forms are timed on the same workload — the plain one-threa
 (so the RMSE against any reference is equal). `value` is the wavefront form's samples/s. Prints
 one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--workload spheres|box|indirect|batch] [--spp N]
+    python bench.py [--steps K] [--warmup W] [--workload spheres|box|indirect|batch|adjoint] [--spp N]
                     [--no-cpu-baseline] [--no-autotune | --wf-config WG,DEPTH,SORT,CAP]
                     [--probe-spp N] [--form both|plain|wavefront]
 """
@@ -128,7 +128,9 @@ def bench_secondary(args):
     indirect: the bidirectional tracer on the indirectly lit room with four 20k-triangle GGX
               spheres, 512 x 512, 32 spp, subpaths of up to 6 vertices;
     batch:    64 seeded scenes (1k-2M triangles each, 128^2-1024^2, 4-64 spp, depth 6) in one
-              persistent launch, against rendering the same jobs one after another."""
+              persistent launch, against rendering the same jobs one after another;
+    adjoint:  gradient evaluations (forward render, L2 loss on the host, path-replay backward)
+              of the 16-surface inverse-rendering room, 256 x 256, 32 spp, depth 4."""
     import x3dcore2_amd as x3d
     from x3dcore2_amd import scenes
     from x3dcore2_amd.device import DeviceArray, synchronize
@@ -155,6 +157,37 @@ def bench_secondary(args):
                    config={"workload": f"indirect room + 4 GGX spheres ({desc.num_tris} tris)", "width": w,
                            "height": h, "spp": spp, "max_path_len": 6,
                            "tracer": "bidirectional, reference uniform strategy weights"})
+        print(json.dumps(out), flush=True)
+        return
+    if args.workload == "adjoint":
+        ta, tr = scenes.adjoint_truth()
+        desc = scenes.adjoint_scene()
+        scene = x3d.Scene(desc)
+        w, h, spp, depth = desc.width, desc.height, args.spp or desc.spp, desc.max_depth
+        r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=depth, seed=1)
+        scenes.set_adjoint_params(scene, ta, tr)
+        target, _ = r.forward(scene, desc.camera, w, h, spp)
+        scenes.set_adjoint_params(scene, np.full((16, 3), 0.5), np.full(16, 0.4))
+
+        def step():
+            img, sf = r.forward(scene, desc.camera, w, h, spp)
+            dl = np.zeros_like(img)
+            dl[:, :3] = 2 * (img[:, :3] - target[:, :3])
+            g, sb = r.backward(scene, desc.camera, w, h, spp, dl)
+            return sf, sb
+
+        for _ in range(args.warmup):
+            step()
+        t0 = time.perf_counter()
+        st = [step() for _ in range(args.steps)]
+        dt = time.perf_counter() - t0
+        fwd = float(np.mean([a.kernel_ms for a, _ in st]))
+        bwd = float(np.mean([b.kernel_ms for _, b in st]))
+        out.update(metric="grad evals/s (forward + path-replay backward, host loss)", unit="grad evals/s",
+                   value=args.steps / dt, ms_per_step=dt * 1e3 / args.steps, forward_kernel_ms=fwd,
+                   backward_kernel_ms=bwd, backward_msamples_per_s=w * h * spp / (bwd * 1e-3) / 1e6,
+                   config={"workload": "box room, 16 surfaces x (albedo rgb, GGX roughness)", "width": w,
+                           "height": h, "spp": spp, "max_depth": depth, "parameters": 64})
         print(json.dumps(out), flush=True)
         return
     descs = scenes.batch_scenes()
@@ -233,7 +266,7 @@ def main():
               "(sample-split sharding across GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload in ("indirect", "batch"):
+    if args.workload in ("indirect", "batch", "adjoint"):
         bench_secondary(args)
         return
     import x3dcore2_amd as x3d

@@ -70,3 +70,12 @@ def test_render_option_errors_without_gpu(x3d):
         with pytest.raises(x3d.X3DError) as e:
             x3d.Renderer(**kw)
         assert e.value.args, kw
+
+
+def test_every_export_has_a_ctypes_signature(x3d):
+    """Pointer arguments must not go through ctypes' default int conversion (64-bit addresses
+    would be truncated)."""
+    lib = x3d.lib()
+    for name in x3d.EXPORTS:
+        if name not in ("x3d_last_error", "x3d_version"):
+            assert getattr(lib, name).argtypes is not None, name

@@ -149,6 +149,17 @@ def lib() -> C.CDLL:
     L.x3d_render_device.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp, C.POINTER(RenderStats)]
     L.x3d_pcg_draws.argtypes = [u64, u64, u32, u32, vp]
     L.x3d_renderer_autotune.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, C.POINTER(AutotuneReport)]
+    L.x3d_render_batch.argtypes = [vp, u32, C.POINTER(BatchJob), C.POINTER(BatchStats)]
+    L.x3d_render_batch_device.argtypes = [vp, u32, C.POINTER(BatchJob), C.POINTER(BatchStats)]
+    L.x3d_renderer_autotune_batch.argtypes = [vp, u32, C.POINTER(BatchJob), C.POINTER(AutotuneReport)]
+    L.x3d_scene_set_material_params.argtypes = [vp, u32, u32, vp, vp, i32]
+    L.x3d_render_forward.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp, C.POINTER(RenderStats)]
+    L.x3d_render_backward.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp, vp, C.POINTER(RenderStats)]
+    L.x3d_render_backward_device.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp, vp,
+                                             C.POINTER(RenderStats)]
+    missing = [n for n in EXPORTS if n not in ("x3d_last_error", "x3d_version") and getattr(L, n).argtypes is None]
+    if missing:  # a pointer passed without argtypes would be truncated to a C int
+        raise ImportError(f"no ctypes signature for {missing}")
     _lib = L
     return L
 

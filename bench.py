@@ -193,7 +193,9 @@ def bench_secondary(args):
     descs = scenes.batch_scenes()
     t0 = time.perf_counter()
     built = [x3d.Scene(d) for d in descs]
-    log(f"batch: {len(descs)} scenes, {sum(d.num_tris for d in descs)} tris, built in {time.perf_counter() - t0:.1f} s")
+    scene_bytes = sum(b.info().device_bytes for b in built)
+    log(f"batch: {len(descs)} scenes, {sum(d.num_tris for d in descs)} tris, built in {time.perf_counter() - t0:.1f} s, "
+        f"{scene_bytes / 2**20:.0f} MiB in HBM")
     films = [DeviceArray(d.width * d.height * 16) for d in descs]
     items = [(built[k], d.camera, d.width, d.height, d.spp) for k, d in enumerate(descs)]
     ptrs = [f.ptr for f in films]
@@ -232,7 +234,8 @@ def bench_secondary(args):
                            "max": float(lat.max())},
                autotune=tune,
                config={"workload": f"64 seeded scenes, {sum(d.num_tris for d in descs)} tris", "samples": samples,
-                       "max_depth": 6, "tiles": st.tiles, "tracer": "unidirectional MIS"})
+                       "max_depth": 6, "tiles": st.tiles, "tracer": "unidirectional MIS",
+                       "scene_bytes": scene_bytes})
     print(json.dumps(out), flush=True)
 
 

@@ -234,3 +234,18 @@ def test_bdpt_generations_and_edge_cases(x3d, oracle_mod):
     dark.lights = []
     img, _ = x3d.Renderer(x3d.PT_BIDIRECT_MIS, seed=4).render(x3d.Scene(dark), dark.camera, 16, 16, 2)
     assert not img[:, :3].any()
+
+
+def test_plain_mis_both_work_shapes_bit_exact(x3d, oracle_mod):
+    """The plain MIS tracer runs one pixel per lane on large films and (pixel, sample) items on
+    films smaller than the chip's lane count; both give the oracle's film."""
+    desc = QUERY_SCENES["spheres_small"]()
+    gs = x3d.Scene(desc)
+    orc = oracle_mod.Oracle(desc)
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=21)
+    for (w, h, spp) in ((1280, 1024, 2), (200, 150, 6)):
+        print(f"[gpu] plain MIS {w}x{h}x{spp}", flush=True)
+        img, _ = r.render(gs, desc.camera, w, h, spp)
+        pix = np.random.default_rng(w).choice(w * h, size=3000, replace=False).astype(np.uint32)
+        ref = orc.render(x3d.PT_UNIDIRECT_MIS, w, h, spp, seed=21, max_depth=6, pixels=pix)
+        assert np.array_equal(_bits(img[pix]), _bits(ref))

@@ -54,40 +54,30 @@ constexpr float kCullSlack = 1.0f / 4096.0f;
 // Slab test of aabb::intersect (tensor.h:815-840) on one box.
 X3D_DI bool slab(float mnx, float mxx, float mny, float mxy, float mnz, float mxz, f3 o, f3 vinv,
                  float tmin, float tmax, float *t0) {
+    // The reference returns as soon as tmin > tmax after an axis. Here all three axes are always
+    // evaluated and `out` records whether such an exit would have happened, so the result is
+    // the same (including NaN entry distances, which std::max / std::min propagate or drop in
+    // argument order) while the loop body has no branches and all node loads issue up front.
+    bool out = false;
     float k0 = (mnx - o.x) * vinv.x, k1 = (mxx - o.x) * vinv.x;
-    if (k0 > k1) {
-        tmin = smax(k1, tmin);
-        tmax = smin(k0, tmax);
-    } else {
-        tmin = smax(k0, tmin);
-        tmax = smin(k1, tmax);
-    }
-    if (tmin > tmax)
-        return false;
+    bool sw = k0 > k1;
+    tmin = smax(sw ? k1 : k0, tmin);
+    tmax = smin(sw ? k0 : k1, tmax);
+    out = out || tmin > tmax;
     k0 = (mny - o.y) * vinv.y;
     k1 = (mxy - o.y) * vinv.y;
-    if (k0 > k1) {
-        tmin = smax(k1, tmin);
-        tmax = smin(k0, tmax);
-    } else {
-        tmin = smax(k0, tmin);
-        tmax = smin(k1, tmax);
-    }
-    if (tmin > tmax)
-        return false;
+    sw = k0 > k1;
+    tmin = smax(sw ? k1 : k0, tmin);
+    tmax = smin(sw ? k0 : k1, tmax);
+    out = out || tmin > tmax;
     k0 = (mnz - o.z) * vinv.z;
     k1 = (mxz - o.z) * vinv.z;
-    if (k0 > k1) {
-        tmin = smax(k1, tmin);
-        tmax = smin(k0, tmax);
-    } else {
-        tmin = smax(k0, tmin);
-        tmax = smin(k1, tmax);
-    }
-    if (tmin > tmax)
-        return false;
+    sw = k0 > k1;
+    tmin = smax(sw ? k1 : k0, tmin);
+    tmax = smin(sw ? k0 : k1, tmax);
+    out = out || tmin > tmax;
     *t0 = tmin;
-    return true;
+    return !out;
 }
 
 // ray::intersect (tensor.h:688-725), operation for operation.

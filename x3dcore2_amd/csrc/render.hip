@@ -314,7 +314,7 @@ __global__ __launch_bounds__(kWG) void k_render_mis(RenderParams P, uint32_t *ne
     }
 }
 
-constexpr uint64_t kBdptItems = 1u << 22; // (pixel, sample) items per generation: 64 MiB of sample radiance
+constexpr uint64_t kGenItems = 1u << 22; // (pixel, sample) items per generation: 64 MiB of sample radiance
 
 // Bidirectional tracer. Work items are the (pixel, sample) pairs of one generation of samples,
 // pixel-major within a sample so neighbouring lanes start from neighbouring first hits. A lane
@@ -340,6 +340,53 @@ __global__ __launch_bounds__(kWG) void k_render_bdpt(RenderParams P, uint32_t ge
         Pcg g = pcg_for(P.seed, pix, s);
         f3 L = tr.li_bdpt(first, valid, v, g, P.max_path_len, cam, lig);
         samples[q] = make_float4(L.x, L.y, L.z, 0.0f);
+    }
+    if (P.cnt) {
+        flush_counter(&P.cnt->closest, tr.nclosest);
+        flush_counter(&P.cnt->anyhit, tr.nany);
+        flush_counter(&P.cnt->pops, tr.pops);
+        flush_counter(&P.cnt->tests, tr.tests);
+        flush_counter(&P.cnt->spops, tr.spops);
+        flush_counter(&P.cnt->stests, tr.stests);
+    }
+}
+
+// Plain MIS tracer for films with fewer pixels than the chip holds lanes (one pixel per lane
+// would leave most lanes idle): work items are the (pixel, sample) pairs of a generation, as in
+// k_render_bdpt; a lane whose path ends stores its radiance and starts the next unclaimed item
+// at once (per-bounce regeneration, as in k_render_mis). k_sum_samples then adds every pixel's
+// samples in sample order, so the film equals k_render_mis's bit for bit.
+__global__ __launch_bounds__(kWG) void k_render_mis_items(RenderParams P, uint32_t gen_begin, uint32_t gen_samples,
+                                                          float4 *samples, uint32_t *next_item) {
+    __shared__ uint2 lds[kLdsDepth * kWG];
+    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
+    Tracer<kWG, kLdsDepth> tr{P.S, st};
+    uint32_t npix = P.width * P.height;
+    uint32_t total = npix * gen_samples;
+    uint32_t grid_threads = gridDim.x * kWG;
+    uint32_t q = blockIdx.x * kWG + threadIdx.x;
+    PathState ps;
+    auto begin = [&](uint32_t item) {
+        uint32_t pix = item % npix, s = gen_begin + item / npix;
+        f3 o, v;
+        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        bool valid;
+        HitRec first = load_first(P, pix, valid);
+        tr.mis_begin(ps, first, valid, v, pcg_for(P.seed, pix, s), P.max_depth);
+    };
+    if (q < total) {
+        begin(q);
+        for (;;) {
+            if (ps.alive)
+                tr.mis_bounce(ps, P.max_depth);
+            if (!ps.alive) {
+                samples[q] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+                q = grid_threads + atomicAdd(next_item, 1u);
+                if (q >= total)
+                    break;
+                begin(q);
+            }
+        }
     }
     if (P.cnt) {
         flush_counter(&P.cnt->closest, tr.nclosest);
@@ -557,8 +604,15 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     int tracer = r->opt.pt_type;
     uint32_t grid_first = persistent_grid(k_first_hit, npix);
     uint32_t grid_render = 1;
+    // Films with fewer pixels than 4x the lanes the plain MIS kernel keeps resident run the
+    // MIS tracer over (pixel, sample) items instead (same film).
+    bool mis_items = tracer == X3D_PT_UNIDIRECT_MIS && !r->wf && !d_dl &&
+                     npix < 4ull * kWG * persistent_grid(k_render_mis, ~0ull >> 8);
     switch (tracer) {
-    case X3D_PT_UNIDIRECT_MIS: grid_render = persistent_grid(k_render_mis, npix); break;
+    case X3D_PT_UNIDIRECT_MIS:
+        grid_render = mis_items ? persistent_grid(k_render_mis_items, npix * static_cast<uint64_t>(spp))
+                                : persistent_grid(k_render_mis, npix);
+        break;
     case X3D_PT_UNIDIRECT_LT1: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT_LT1>, npix); break;
     case X3D_PT_UNIDIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT>, npix); break;
     case X3D_PT_DIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_DIRECT>, npix); break;
@@ -603,17 +657,11 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
             return fail(X3D_ERR_DEVICE, e);
         tracer = -1; // launched
     }
-    switch (tracer) {
-    case -1: break;
-    case X3D_PT_UNIDIRECT_MIS:
-        hipLaunchKernelGGL(k_render_mis, dim3(grid_render), dim3(kWG), 0, r->stream, P, next_pixel);
-        break;
-    case X3D_PT_UNIDIRECT_LT1: launch_plain<X3D_PT_UNIDIRECT_LT1>(grid_render, r->stream, P); break;
-    case X3D_PT_UNIDIRECT: launch_plain<X3D_PT_UNIDIRECT>(grid_render, r->stream, P); break;
-    case X3D_PT_DIRECT: launch_plain<X3D_PT_DIRECT>(grid_render, r->stream, P); break;
-    case X3D_PT_BIDIRECT_MIS: {
-        // generations of whole samples, at most queue_capacity (default kBdptItems) items each
-        uint64_t items = r->opt.queue_capacity ? r->opt.queue_capacity : kBdptItems;
+    int rc = X3D_OK;
+    // Generations of whole samples, at most queue_capacity (default kGenItems) (pixel, sample)
+    // items each, rendered by `kernel` into the sample buffer and summed per pixel in order.
+    auto run_generations = [&](void (*kernel)(RenderParams, uint32_t, uint32_t, float4 *, uint32_t *)) -> int {
+        uint64_t items = r->opt.queue_capacity ? r->opt.queue_capacity : kGenItems;
         uint32_t gen = static_cast<uint32_t>(items / npix);
         gen = gen < 1 ? 1 : (gen > spp ? spp : gen);
         X3D_HIP(r->samples.ensure(static_cast<size_t>(gen) * npix * sizeof(float4)));
@@ -623,13 +671,31 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
             uint32_t cnt = spp - s0 < gen ? spp - s0 : gen;
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream,
                                reinterpret_cast<unsigned long long *>(next_pixel), 1u);
-            hipLaunchKernelGGL(k_render_bdpt, dim3(grid_render), dim3(kWG), 0, r->stream, P, s0, cnt, samples,
-                               next_pixel);
+            hipLaunchKernelGGL(kernel, dim3(grid_render), dim3(kWG), 0, r->stream, P, s0, cnt, samples, next_pixel);
             hipLaunchKernelGGL(k_sum_samples, dim3(g_sum), dim3(256), 0, r->stream, P, s0, cnt,
                                static_cast<const float4 *>(samples));
         }
+        return X3D_OK;
+    };
+    switch (tracer) {
+    case -1: break;
+    case X3D_PT_UNIDIRECT_MIS:
+        if (mis_items) {
+            rc = run_generations(k_render_mis_items);
+            if (rc)
+                return rc;
+        } else {
+            hipLaunchKernelGGL(k_render_mis, dim3(grid_render), dim3(kWG), 0, r->stream, P, next_pixel);
+        }
         break;
-    }
+    case X3D_PT_UNIDIRECT_LT1: launch_plain<X3D_PT_UNIDIRECT_LT1>(grid_render, r->stream, P); break;
+    case X3D_PT_UNIDIRECT: launch_plain<X3D_PT_UNIDIRECT>(grid_render, r->stream, P); break;
+    case X3D_PT_DIRECT: launch_plain<X3D_PT_DIRECT>(grid_render, r->stream, P); break;
+    case X3D_PT_BIDIRECT_MIS:
+        rc = run_generations(k_render_bdpt);
+        if (rc)
+            return rc;
+        break;
     default: launch_plain<X3D_PT_NORMAL>(grid_render, r->stream, P); break;
     }
     X3D_HIP(hipGetLastError());

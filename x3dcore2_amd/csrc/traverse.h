@@ -46,7 +46,14 @@ template <int WG, int LDS_DEPTH> struct TravStack {
     }
     X3D_DI uint2 pop() {
         sp--;
-        return sp < LDS_DEPTH ? lds[sp * WG] : spill[sp - LDS_DEPTH];
+        if (sp < LDS_DEPTH)
+            return lds[sp * WG];
+        // A distinct (non-temporal) load keeps the compiler from merging the two reads into one
+        // generic-address load through a selected pointer, which would turn every pop into a
+        // flat load that waits on both the LDS and the vector memory counters.
+        unsigned long long e = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(spill) +
+                                                          (sp - LDS_DEPTH));
+        return make_uint2(static_cast<uint32_t>(e), static_cast<uint32_t>(e >> 32));
     }
     // Next stacked node whose entry distance does not exceed `cut`, or kDone.
     X3D_DI uint32_t pop_next(float cut) {

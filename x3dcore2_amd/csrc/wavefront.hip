@@ -9,8 +9,9 @@
 //   shade      per extension result: throughput update, MIS-weighted emission, next vertex;
 //              surviving paths are appended to per-material-kind active lists (material sort)
 // The traversal kernels keep every lane busy: a lane whose ray finishes writes its result and
-// takes the next ray (one atomic per wave for all lanes that need one) without waiting for the
-// rest of its wave. All paths of a generation (a fixed number of samples for every pixel) start
+// takes the next ray from its wave's private batch (one atomic per 128 rays on the shared
+// queue) without waiting for the rest of its wave. Queue counters live on separate cache lines
+// and appends are aggregated per wave. All paths of a generation (a fixed number of samples for every pixel) start
 // together, so they all sit at the same depth; one generation's launches are captured once as
 // a hipGraph and replayed. Each path performs exactly the operations of the plain kernel in the
 // same order (same PCG stream, same float expressions), and each pixel's samples are summed in
@@ -25,9 +26,15 @@ namespace x3d {
 
 constexpr int kKinds = 3; // diffuse (Oren-Nayar / fail-safe), GGX, mixture
 
+// Every queue counter sits alone in its own 256-byte block: counters on one cache line would
+// serialize each other's atomics at a single L2 channel.
+struct alignas(256) WfCounter {
+    uint32_t v;
+};
+
 struct WfCtl {
-    uint32_t act_count[2][4];
-    uint32_t sq_count, sq_head, eq_count, eq_head, fin_count;
+    WfCounter act_count[2][4];
+    WfCounter sq_count, sq_head, eq_count, eq_head, fin_count;
     uint32_t gen;
     uint32_t pad[2];
 };
@@ -83,23 +90,45 @@ X3D_DI uint32_t wave_append(uint32_t *counter, bool pred) {
 X3D_DI void push_active(WfArgs const &A, int parity, int kind, uint32_t path, bool live) {
     for (int k = 0; k < (A.sort_by_material ? kKinds : 1); k++) {
         bool mine = live && (!A.sort_by_material || kind == k);
-        uint32_t j = wave_append(&A.ctl->act_count[parity][k], mine);
+        uint32_t j = wave_append(&A.ctl->act_count[parity][k].v, mine);
         if (mine)
             A.act[parity][static_cast<size_t>(k) * A.capacity + j] = path;
     }
 }
 
-// Lanes of the calling (possibly partial) wave take consecutive items with one atomic.
-X3D_DI uint32_t wave_fetch(uint32_t *head) {
+// Work fetch of the persistent traversal kernels: the wave takes kFetchBatch items from
+// the global queue at a time into a private cursor [slot.x, slot.y) in LDS and hands them to its
+// lanes as they free up, so the shared counter sees one atomic per kFetchBatch rays instead of
+// one per refill. Only the lanes that need an item call it (they are the active lanes).
+// The cursor is written by one lane and read by the others later: relaxed atomic loads and
+// stores keep the compiler from answering those reads from registers.
+constexpr uint32_t kFetchBatch = 128;
+X3D_DI uint32_t cursor_load(uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT); }
+X3D_DI void cursor_store(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+X3D_DI uint32_t wave_fetch_batched(uint32_t *head, uint32_t *slot) {
     unsigned long long mask = __ballot(1);
     uint32_t lane = __lane_id();
     uint32_t leader = __ffsll(static_cast<long long>(mask)) - 1;
     uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t cnt = __popcll(mask);
+    uint32_t next = cursor_load(slot), end = cursor_load(slot + 1);
+    uint32_t rem = end - next;
+    if (rem >= cnt) {
+        if (lane == leader)
+            cursor_store(slot, next + cnt);
+        return next + rank;
+    }
     uint32_t base = 0;
     if (lane == leader)
-        base = atomicAdd(head, static_cast<uint32_t>(__popcll(mask)));
+        base = atomicAdd(head, kFetchBatch);
     base = __shfl(base, leader, 64);
-    return base + rank;
+    if (lane == leader) {
+        cursor_store(slot, base + (cnt - rem));
+        cursor_store(slot + 1, base + kFetchBatch);
+    }
+    return rank < rem ? next + rank : base + (rank - rem);
 }
 
 __global__ void k_wf_begin_render(WfCtl *ctl) {
@@ -119,18 +148,18 @@ __global__ void k_wf_reset(WfCtl *ctl, int parity) {
     uint32_t t = threadIdx.x;
     if (t < 4) {
         if (parity < 0) {
-            ctl->act_count[0][t] = 0;
-            ctl->act_count[1][t] = 0;
+            ctl->act_count[0][t].v = 0;
+            ctl->act_count[1][t].v = 0;
         } else {
-            ctl->act_count[parity ^ 1][t] = 0;
+            ctl->act_count[parity ^ 1][t].v = 0;
         }
     }
     if (t == 0) {
-        ctl->sq_count = 0;
-        ctl->sq_head = 0;
-        ctl->eq_count = 0;
-        ctl->eq_head = 0;
-        ctl->fin_count = 0;
+        ctl->sq_count.v = 0;
+        ctl->sq_head.v = 0;
+        ctl->eq_count.v = 0;
+        ctl->eq_head.v = 0;
+        ctl->fin_count.v = 0;
     }
 }
 
@@ -174,8 +203,8 @@ __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
 }
 
 __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
-    uint32_t c0 = A.ctl->act_count[parity][0], c1 = A.ctl->act_count[parity][1],
-             c2 = A.ctl->act_count[parity][2];
+    uint32_t c0 = A.ctl->act_count[parity][0].v, c1 = A.ctl->act_count[parity][1].v,
+             c2 = A.ctl->act_count[parity][2].v;
     uint32_t total = c0 + c1 + c2;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         uint32_t kind = t < c0 ? 0u : (t < c0 + c1 ? 1u : 2u);
@@ -227,16 +256,16 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
         bool ok = dens > 0.0f;
         if (ok)
             A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
-        uint32_t q = wave_append(&A.ctl->sq_count, has_shadow);
+        uint32_t q = wave_append(&A.ctl->sq_count.v, has_shadow);
         if (has_shadow) {
             A.sq[3 * q] = make_float4(x.p.x, x.p.y, x.p.z, sdist - 1e-3f);
             A.sq[3 * q + 1] = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(path));
             A.sq[3 * q + 2] = make_float4(sc.x, sc.y, sc.z, 0.0f);
         }
-        q = wave_append(&A.ctl->eq_count, ok);
+        q = wave_append(&A.ctl->eq_count.v, ok);
         if (ok)
             A.eq[q] = path;
-        q = wave_append(&A.ctl->fin_count, !ok);
+        q = wave_append(&A.ctl->fin_count.v, !ok);
         if (!ok)
             A.fin[q] = path;
     }
@@ -245,12 +274,18 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
 template <int WG, int LDSD>
 __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     __shared__ uint2 lds[LDSD * WG];
+    __shared__ uint32_t cursor[2 * (WG / 64)];
     TravStack<WG, LDSD> st = make_stack<WG, LDSD>(lds, A.spill, A.spill_per_thread);
     DevScene const &S = A.S;
     const float4 *tris = static_cast<const float4 *>(S.tris);
-    const uint32_t n = A.ctl->sq_count;
+    const uint32_t n = A.ctl->sq_count.v;
     uint32_t pops = 0, tests = 0;
-    uint32_t i = wave_fetch(&A.ctl->sq_head);
+    uint32_t *slot = cursor + 2 * (threadIdx.x >> 6);
+    if (__lane_id() == 0) {
+        cursor_store(slot, 0u);
+        cursor_store(slot + 1, 0u);
+    }
+    uint32_t i = wave_fetch_batched(&A.ctl->sq_head.v, slot);
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
     float tmax = 0.0f;
     uint32_t ref = kDone, leaf = kDone;
@@ -312,7 +347,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
                 f3 nl = xyz(L) + xyz(c);
                 A.L[path] = make_float4(nl.x, nl.y, nl.z, L.w);
             }
-            i = wave_fetch(&A.ctl->sq_head);
+            i = wave_fetch_batched(&A.ctl->sq_head.v, slot);
             if (i < n)
                 init(i);
         }
@@ -326,12 +361,18 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
 template <int WG, int LDSD>
 __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     __shared__ uint2 lds[LDSD * WG];
+    __shared__ uint32_t cursor[2 * (WG / 64)];
     TravStack<WG, LDSD> st = make_stack<WG, LDSD>(lds, A.spill, A.spill_per_thread);
     DevScene const &S = A.S;
     const float4 *tris = static_cast<const float4 *>(S.tris);
-    const uint32_t n = A.ctl->eq_count;
+    const uint32_t n = A.ctl->eq_count.v;
     uint32_t pops = 0, tests = 0;
-    uint32_t i = wave_fetch(&A.ctl->eq_head);
+    uint32_t *slot = cursor + 2 * (threadIdx.x >> 6);
+    if (__lane_id() == 0) {
+        cursor_store(slot, 0u);
+        cursor_store(slot + 1, 0u);
+    }
+    uint32_t i = wave_fetch_batched(&A.ctl->eq_head.v, slot);
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
     ClosestHit h{-1, __builtin_inff(), 0.0f, 0.0f};
     uint32_t best_leaf = 0, ref = kDone, leaf = kDone;
@@ -388,7 +429,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
         }
         if (ref == kDone) {
             A.eh[i] = h;
-            i = wave_fetch(&A.ctl->eq_head);
+            i = wave_fetch_batched(&A.ctl->eq_head.v, slot);
             if (i < n)
                 init(i);
         }
@@ -400,7 +441,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
 }
 
 __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t bounce) {
-    uint32_t ne = A.ctl->eq_count, nf = A.ctl->fin_count;
+    uint32_t ne = A.ctl->eq_count.v, nf = A.ctl->fin_count.v;
     uint32_t total = ne + nf;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
         bool live = false;
@@ -481,8 +522,8 @@ __global__ __launch_bounds__(256) void k_wf_accumulate(WfArgs A) {
 // Ray-query counts (one closest-hit per extension ray, one any-hit per shadow ray).
 __global__ void k_wf_count(WfCtl *ctl, Counters *cnt) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && cnt) {
-        atomicAdd(&cnt->closest, static_cast<unsigned long long>(ctl->eq_count));
-        atomicAdd(&cnt->anyhit, static_cast<unsigned long long>(ctl->sq_count));
+        atomicAdd(&cnt->closest, static_cast<unsigned long long>(ctl->eq_count.v));
+        atomicAdd(&cnt->anyhit, static_cast<unsigned long long>(ctl->sq_count.v));
     }
 }
 
@@ -634,20 +675,20 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     A.cnt = P.cnt;
     A.sort_by_material = tune.sort_by_material;
 
-    int dev = 0, cus = 0;
-    WF_HIP(hipGetDevice(&dev));
-    WF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    uint32_t g_item = static_cast<uint32_t>(cus) * 8u; // grid-strided per-path kernels
+    // Grid-strided per-path kernels get exactly the blocks that are resident at once: a larger
+    // grid would run its surplus waves as a second, late round.
+    uint32_t g_gen = grid_for(k_wf_generate, 256), g_nee = grid_for(k_wf_nee_bsdf, 256),
+             g_shade = grid_for(k_wf_shade, 256), g_acc = grid_for(k_wf_accumulate, 256);
     uint32_t generations = (P.s_end - P.s_begin + gen_samples - 1) / gen_samples;
 
     bool profile = tune.profile != 0;
     auto record_generation = [&](hipStream_t s) {
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, -1);
-        hipLaunchKernelGGL(k_wf_generate, dim3(g_item), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_wf_generate, dim3(g_gen), dim3(256), 0, s, A);
         for (uint32_t k = 0; k < P.max_depth; k++) {
             int par = static_cast<int>(k & 1u);
             hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, par);
-            hipLaunchKernelGGL(k_wf_nee_bsdf, dim3(g_item), dim3(256), 0, s, A, par);
+            hipLaunchKernelGGL(k_wf_nee_bsdf, dim3(g_nee), dim3(256), 0, s, A, par);
             if (profile)
                 (void)hipEventRecord(e->ev[0], s);
             hipLaunchKernelGGL(tk.shadow, dim3(g_shadow), dim3(wg), 0, s, A);
@@ -665,9 +706,9 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
                 e->trace_launches++;
             }
             hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, s, A.ctl, A.cnt);
-            hipLaunchKernelGGL(k_wf_shade, dim3(g_item), dim3(256), 0, s, A, par, k);
+            hipLaunchKernelGGL(k_wf_shade, dim3(g_shade), dim3(256), 0, s, A, par, k);
         }
-        hipLaunchKernelGGL(k_wf_accumulate, dim3(g_item), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_wf_accumulate, dim3(g_acc), dim3(256), 0, s, A);
         hipLaunchKernelGGL(k_wf_next_gen, dim3(1), dim3(64), 0, s, A.ctl);
     };
 

@@ -69,31 +69,64 @@ X3D_DI int kind_of(DevScene const &S, int32_t mat) {
     return k == kMatCookTorr ? 1 : (k == kMatMixture ? 2 : 0);
 }
 
-// Lanes of the calling wave with pred set take consecutive slots of the queue behind `counter`,
-// with one atomic per wave. Every active lane must call it (pred may be false).
-X3D_DI uint32_t wave_append(uint32_t *counter, bool pred) {
-    unsigned long long mask = __ballot(pred);
-    if (mask == 0)
-        return 0;
+// Block-aggregated appends to NQ queues at once: one LDS atomic per wave and queue, then one
+// global atomic per block and queue (a queue counter takes one atomic per 256 items instead of
+// one per 64). Every thread of the block must call it the same number of times; `scratch` is
+// 2 * NQ words of LDS, zero before the first call. idx[q] is valid where pred[q] is set.
+template <int NQ>
+X3D_DI void block_append(uint32_t *const counters[NQ], const bool pred[NQ], uint32_t idx[NQ], uint32_t *scratch) {
     uint32_t lane = __lane_id();
-    uint32_t leader = __ffsll(static_cast<long long>(mask)) - 1;
-    uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t base = 0;
-    if (lane == leader)
-        base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, leader, 64);
-    return base + rank;
+    unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t woff[NQ], rank[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        unsigned long long mask = __ballot(pred[q]);
+        rank[q] = __popcll(mask & below);
+        uint32_t cnt = __popcll(mask);
+        uint32_t o = 0;
+        if (lane == 0 && cnt)
+            o = atomicAdd(&scratch[q], cnt);
+        woff[q] = __shfl(o, 0, 64);
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        uint32_t tot = scratch[threadIdx.x];
+        scratch[NQ + threadIdx.x] = tot ? atomicAdd(counters[threadIdx.x], tot) : 0u;
+        scratch[threadIdx.x] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        idx[q] = scratch[NQ + q] + woff[q] + rank[q];
 }
 
 // Appends `path` to the active list of its material kind (all kinds in list 0 without the
-// material sort). Every active lane must call it; `live` says whether this lane appends.
-X3D_DI void push_active(WfArgs const &A, int parity, int kind, uint32_t path, bool live) {
-    for (int k = 0; k < (A.sort_by_material ? kKinds : 1); k++) {
-        bool mine = live && (!A.sort_by_material || kind == k);
-        uint32_t j = wave_append(&A.ctl->act_count[parity][k].v, mine);
-        if (mine)
-            A.act[parity][static_cast<size_t>(k) * A.capacity + j] = path;
+// material sort). Block-wide: see block_append.
+X3D_DI void push_active(WfArgs const &A, int parity, int kind, uint32_t path, bool live, uint32_t *scratch) {
+    if (A.sort_by_material) {
+        uint32_t *const c[kKinds] = {&A.ctl->act_count[parity][0].v, &A.ctl->act_count[parity][1].v,
+                                     &A.ctl->act_count[parity][2].v};
+        const bool p[kKinds] = {live && kind == 0, live && kind == 1, live && kind == 2};
+        uint32_t idx[kKinds];
+        block_append<kKinds>(c, p, idx, scratch);
+        if (live)
+            A.act[parity][static_cast<size_t>(kind) * A.capacity + (kind == 0 ? idx[0] : (kind == 1 ? idx[1] : idx[2]))] =
+                path;
+    } else {
+        uint32_t *const c[1] = {&A.ctl->act_count[parity][0].v};
+        const bool p[1] = {live};
+        uint32_t idx[1];
+        block_append<1>(c, p, idx, scratch);
+        if (live)
+            A.act[parity][idx[0]] = path;
     }
+}
+
+// Zeroes the block's append scratch (call once at kernel start, before any block_append).
+X3D_DI void clear_scratch(uint32_t *scratch, int words) {
+    if (threadIdx.x < static_cast<uint32_t>(words))
+        scratch[threadIdx.x] = 0;
+    __syncthreads();
 }
 
 // Work fetch of the persistent traversal kernels: the wave takes kFetchBatch items from
@@ -170,104 +203,125 @@ X3D_DI uint32_t gen_count(WfArgs const &A, uint32_t gen) {
 }
 
 __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
+    __shared__ uint32_t scratch[2 * kKinds];
+    clear_scratch(scratch, 2 * kKinds);
     uint32_t gen = A.ctl->gen;
     uint32_t n = gen_count(A, gen) * A.npix;
-    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) {
-        uint32_t s_local = id / A.npix, pix = id - s_local * A.npix;
-        uint32_t s = A.s_begin + gen * A.gen_samples + s_local;
-        f3 o, v;
-        camera_ray(A.cam, pix % A.width, pix / A.width, A.width, A.height, o, v);
-        RenderParams RP;
-        RP.first = A.first;
-        bool valid;
-        HitRec x = load_first(RP, pix, valid);
-        // Tracer::mis_begin
-        Pcg g = pcg_for(A.seed, pix, s);
-        f3 L = mk(0.0f, 0.0f, 0.0f);
-        f3 wo = -v;
-        if (valid && x.light >= 0)
-            L = L + light_radiance(A.S, x.light, wo, x.n);
-        bool live = valid && A.max_depth > 0;
-        if (!live) {
-            A.lbuf[id] = make_float4(L.x, L.y, L.z, 0.0f);
-        } else {
-            A.xp[id] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
-            A.xn[id] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
-            A.wo[id] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
-            A.beta[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
-            A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
-            A.rng[id] = g.state;
+    // block-uniform trip count (block_append synchronizes the block)
+    for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
+        uint32_t id = b + threadIdx.x;
+        bool live = false;
+        int kind = 0;
+        if (id < n) {
+            uint32_t s_local = id / A.npix, pix = id - s_local * A.npix;
+            uint32_t s = A.s_begin + gen * A.gen_samples + s_local;
+            f3 o, v;
+            camera_ray(A.cam, pix % A.width, pix / A.width, A.width, A.height, o, v);
+            RenderParams RP;
+            RP.first = A.first;
+            bool valid;
+            HitRec x = load_first(RP, pix, valid);
+            // Tracer::mis_begin
+            Pcg g = pcg_for(A.seed, pix, s);
+            f3 L = mk(0.0f, 0.0f, 0.0f);
+            f3 wo = -v;
+            if (valid && x.light >= 0)
+                L = L + light_radiance(A.S, x.light, wo, x.n);
+            live = valid && A.max_depth > 0;
+            if (!live) {
+                A.lbuf[id] = make_float4(L.x, L.y, L.z, 0.0f);
+            } else {
+                A.xp[id] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
+                A.xn[id] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
+                A.wo[id] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
+                A.beta[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+                A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
+                A.rng[id] = g.state;
+                kind = kind_of(A.S, x.mat);
+            }
         }
-        push_active(A, 0, live ? kind_of(A.S, x.mat) : 0, id, live);
+        push_active(A, 0, kind, id, live, scratch);
     }
 }
 
 __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
+    __shared__ uint32_t scratch[6];
+    clear_scratch(scratch, 6);
     uint32_t c0 = A.ctl->act_count[parity][0].v, c1 = A.ctl->act_count[parity][1].v,
              c2 = A.ctl->act_count[parity][2].v;
     uint32_t total = c0 + c1 + c2;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        uint32_t kind = t < c0 ? 0u : (t < c0 + c1 ? 1u : 2u);
-        uint32_t j = t - (kind == 0 ? 0u : (kind == 1 ? c0 : c0 + c1));
-        uint32_t path = A.act[parity][static_cast<size_t>(kind) * A.capacity + j];
-        float4 xp = A.xp[path], xn = A.xn[path], wo4 = A.wo[path], b4 = A.beta[path];
-        HitRec x;
-        x.p = xyz(xp);
-        x.n = xyz(xn);
-        x.mat = static_cast<int32_t>(__float_as_uint(xp.w));
-        x.light = static_cast<int32_t>(__float_as_uint(xn.w));
-        f3 wo = xyz(wo4), beta = xyz(b4);
-        uint32_t pix = __float_as_uint(wo4.w);
-        Pcg g;
-        g.state = A.rng[path];
-        g.inc = (static_cast<uint64_t>(pix) << 1u) | 1u;
-        // next-event estimation (Tracer::direct + the MIS weight of Tracer::mis_bounce)
-        bool has_shadow = false;
-        f3 sc = mk(0.0f, 0.0f, 0.0f), sdir = mk(0.0f, 0.0f, 0.0f);
+    uint32_t *const queues[3] = {&A.ctl->sq_count.v, &A.ctl->eq_count.v, &A.ctl->fin_count.v};
+    // block-uniform trip count (block_append synchronizes the block)
+    for (uint32_t b = blockIdx.x * blockDim.x; b < total; b += gridDim.x * blockDim.x) {
+        uint32_t t = b + threadIdx.x;
+        bool in = t < total;
+        bool has_shadow = false, ok = false;
+        uint32_t path = 0;
+        f3 xp3 = mk(0.0f, 0.0f, 0.0f), sc = mk(0.0f, 0.0f, 0.0f), sdir = mk(0.0f, 0.0f, 0.0f);
         float sdist = 0.0f;
-        if (A.S.num_lights > 0) {
-            LightSample ls = sample_light(A.S, g);
-            f3 l = x.p - ls.p;
-            float r2 = dot(l, l);
-            f3 in = l / __builtin_sqrtf(r2);
-            float co = dot(ls.n, in);
-            float ci = dot(x.n, -in);
-            if (co > 0 && ci > 0) {
-                f3 illum = ((light_rad(A.S, ls.light) * ci) * co) / r2;
-                if (!near0(illum)) {
-                    float dist = norm(l);
-                    f3 wl = -l / dist;
-                    f3 f = bsdf_eval(A.S.mats, x.mat, x.n, wo, wl);
-                    f3 d = mul(illum, f) / ls.area_dens;
-                    float cl = dot(ls.n, -wl);
-                    float pl = ls.area_dens * r2 / cl;
-                    float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
-                    float w = pl * pl / (pl * pl + pb * pb);
-                    sc = mul(beta, d * w);
-                    sdist = dist;
-                    sdir = wl;
-                    has_shadow = true;
+        if (in) {
+            uint32_t kind = t < c0 ? 0u : (t < c0 + c1 ? 1u : 2u);
+            uint32_t j = t - (kind == 0 ? 0u : (kind == 1 ? c0 : c0 + c1));
+            path = A.act[parity][static_cast<size_t>(kind) * A.capacity + j];
+            float4 xp = A.xp[path], xn = A.xn[path], wo4 = A.wo[path], b4 = A.beta[path];
+            HitRec x;
+            x.p = xyz(xp);
+            x.n = xyz(xn);
+            x.mat = static_cast<int32_t>(__float_as_uint(xp.w));
+            x.light = static_cast<int32_t>(__float_as_uint(xn.w));
+            xp3 = x.p;
+            f3 wo = xyz(wo4), beta = xyz(b4);
+            uint32_t pix = __float_as_uint(wo4.w);
+            Pcg g;
+            g.state = A.rng[path];
+            g.inc = (static_cast<uint64_t>(pix) << 1u) | 1u;
+            // next-event estimation (Tracer::direct + the MIS weight of Tracer::mis_bounce)
+            if (A.S.num_lights > 0) {
+                LightSample ls = sample_light(A.S, g);
+                f3 l = x.p - ls.p;
+                float r2 = dot(l, l);
+                f3 inv = l / __builtin_sqrtf(r2);
+                float co = dot(ls.n, inv);
+                float ci = dot(x.n, -inv);
+                if (co > 0 && ci > 0) {
+                    f3 illum = ((light_rad(A.S, ls.light) * ci) * co) / r2;
+                    if (!near0(illum)) {
+                        float dist = norm(l);
+                        f3 wl = -l / dist;
+                        f3 f = bsdf_eval(A.S.mats, x.mat, x.n, wo, wl);
+                        f3 d = mul(illum, f) / ls.area_dens;
+                        float cl = dot(ls.n, -wl);
+                        float pl = ls.area_dens * r2 / cl;
+                        float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
+                        float w = pl * pl / (pl * pl + pb * pb);
+                        sc = mul(beta, d * w);
+                        sdist = dist;
+                        sdir = wl;
+                        has_shadow = true;
+                    }
                 }
             }
+            float dens;
+            f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, false);
+            A.rng[path] = g.state;
+            ok = dens > 0.0f;
+            if (ok)
+                A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
         }
-        float dens;
-        f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, false);
-        A.rng[path] = g.state;
-        bool ok = dens > 0.0f;
-        if (ok)
-            A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
-        uint32_t q = wave_append(&A.ctl->sq_count.v, has_shadow);
-        if (has_shadow) {
-            A.sq[3 * q] = make_float4(x.p.x, x.p.y, x.p.z, sdist - 1e-3f);
+        // one call site with all lanes active: block_append synchronizes the block
+        const bool want[3] = {in && has_shadow, in && ok, in && !ok};
+        uint32_t idx[3];
+        block_append<3>(queues, want, idx, scratch);
+        if (want[0]) {
+            uint32_t q = idx[0];
+            A.sq[3 * q] = make_float4(xp3.x, xp3.y, xp3.z, sdist - 1e-3f);
             A.sq[3 * q + 1] = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(path));
             A.sq[3 * q + 2] = make_float4(sc.x, sc.y, sc.z, 0.0f);
         }
-        q = wave_append(&A.ctl->eq_count.v, ok);
-        if (ok)
-            A.eq[q] = path;
-        q = wave_append(&A.ctl->fin_count.v, !ok);
-        if (!ok)
-            A.fin[q] = path;
+        if (want[1])
+            A.eq[idx[1]] = path;
+        if (want[2])
+            A.fin[idx[2]] = path;
     }
 }
 
@@ -441,13 +495,19 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
 }
 
 __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t bounce) {
+    __shared__ uint32_t scratch[2 * kKinds];
+    clear_scratch(scratch, 2 * kKinds);
     uint32_t ne = A.ctl->eq_count.v, nf = A.ctl->fin_count.v;
     uint32_t total = ne + nf;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    // block-uniform trip count (block_append synchronizes the block)
+    for (uint32_t b = blockIdx.x * blockDim.x; b < total; b += gridDim.x * blockDim.x) {
+        uint32_t t = b + threadIdx.x;
         bool live = false;
         int kind = 0;
-        uint32_t path;
-        if (t >= ne) {
+        uint32_t path = 0;
+        if (t >= total) {
+            // no item: only takes part in the block-wide append below
+        } else if (t >= ne) {
             path = A.fin[t - ne];
             A.lbuf[path] = A.L[path];
         } else {
@@ -495,7 +555,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
                 }
             }
         }
-        push_active(A, parity ^ 1, kind, path, live);
+        push_active(A, parity ^ 1, kind, path, live, scratch);
     }
 }
 

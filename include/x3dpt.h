@@ -288,7 +288,7 @@ typedef struct x3d_batch_stats {
     uint64_t closest_nodes, closest_tests, anyhit_nodes, anyhit_tests;
     double kernel_ms;         /* event-timed batch launch */
     double total_ms;          /* wall time of the call */
-    uint32_t tiles;           /* work tiles of 1024 pixels */
+    uint32_t tiles;           /* work tiles of 64 pixels */
     uint32_t reserved;
 } x3d_batch_stats;
 

@@ -25,9 +25,10 @@ struct BatchTuning {
     int wg = 256;          // workgroup size (64, 128, 256)
     int stack_depth = 16;  // LDS stack entries per lane (8, 16)
     int sort_by_cost = 1;  // hand out the tiles of the most expensive jobs first
-    uint32_t tile_px = 1024; // pixels per tile (multiple of 64)
     int use_graph = 1;     // replay the launch as a hipGraph when the batch is unchanged
 };
+
+constexpr uint32_t kBatchTilePx = 64; // pixels per work tile (one per lane of a wave)
 
 struct BatchEngine;
 

@@ -1,14 +1,16 @@
 // Batched rendering: many independent jobs (scene, camera, film size, spp) share one persistent
 // launch of the fixed-depth MIS tracer.
 //
-// Work is cut into tiles of consecutive pixels of one job. Each wave takes a whole tile from a
-// global counter, so the scene of a tile is uniform across the wave (its descriptor is read into
-// scalar registers once per tile); inside the tile a lane whose pixel is finished takes the next
-// unclaimed pixel of the tile (ballot + popcount, no atomics), and a lane whose sample is
-// finished starts the pixel's next sample, exactly as in the single-job plain kernel. Tiles are
-// handed out most expensive job first, so big jobs do not start last and form the tail. Each
-// pixel performs the operations of the single-job render in the same order (first hit, then its
-// samples in order), so every film is bit-identical to rendering its job alone.
+// Work is cut into tiles of 64 consecutive pixels of one job. Each wave takes a whole tile from a
+// global counter, so the scene of a tile is uniform across the wave (its descriptor is read with
+// the tile index made wave-uniform). The wave first traces the tile's 64 camera rays (one per
+// lane) and keeps the first hits in LDS; then its lanes work through the tile's (pixel, sample)
+// items in chunks of up to kChunk samples per pixel, a lane whose path ends taking the next item
+// at once (per-bounce regeneration, no waiting for the slowest pixel); each item's radiance goes
+// to the wave's slice of a sample buffer, and lane l then adds pixel l's samples of the chunk in
+// sample order. Tiles are handed out most expensive job first, so big jobs do not form the tail.
+// Every pixel performs the single-job render's operations in the same order (first hit, samples
+// summed in order), so every film is bit-identical to rendering its job alone.
 
 #include "batch.h"
 
@@ -17,6 +19,9 @@
 #include <cstring>
 
 namespace x3d {
+
+constexpr uint32_t kTilePx = kBatchTilePx;
+constexpr uint32_t kChunk = 64;  // samples per pixel per pass over a tile (sample buffer depth)
 
 struct BatchCtl {
     uint32_t next_tile;
@@ -27,10 +32,11 @@ struct BatchCtl {
 struct BatchArgs {
     const BatchJob *jobs;
     const uint32_t *tile_job; // job of each tile, in hand-out order
-    uint32_t num_jobs, num_tiles, tile_px, max_depth;
+    uint32_t num_jobs, num_tiles, max_depth;
     uint64_t seed;
     uint2 *spill;
     uint32_t spill_per_thread;
+    float4 *samples; // kChunk x kTilePx float4 per resident wave
     Counters *cnt;
     BatchCtl *ctl;
     uint32_t *tiles_left;            // per job
@@ -51,11 +57,26 @@ __global__ void k_batch_init(BatchArgs A) {
     }
 }
 
+// LDS cells written by one lane and read by others later: relaxed atomics keep the compiler from
+// answering the reads from registers.
+X3D_DI uint32_t lds_load(uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT); }
+X3D_DI void lds_store(uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT); }
+
 template <int WG, int LDSD> __global__ __launch_bounds__(WG) void k_render_batch(BatchArgs A) {
+    constexpr int kWaves = WG / 64;
     __shared__ uint2 lds[LDSD * WG];
+    // per wave: first hits of the tile's pixels (p.xyz, t | n.xyz, mat | light, geo, valid) and
+    // the item cursor
+    constexpr uint32_t kHitWords = 11;
+    __shared__ uint32_t hits[kWaves][kTilePx * kHitWords];
+    __shared__ uint32_t cursor[kWaves];
     TravStack<WG, LDSD> st = make_stack<WG, LDSD>(lds, A.spill, A.spill_per_thread);
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t *hit_lds = hits[wave];
+    uint32_t *cur = &cursor[wave];
+    float4 *buf = A.samples + static_cast<size_t>(blockIdx.x * kWaves + wave) * kChunk * kTilePx;
     if (lane == 0)
         atomicMin(&A.ctl->start_clock, static_cast<unsigned long long>(wall_clock64()));
     uint32_t nclosest = 0, nany = 0, pops = 0, tests = 0, spops = 0, stests = 0;
@@ -71,49 +92,89 @@ template <int WG, int LDSD> __global__ __launch_bounds__(WG) void k_render_batch
         const DevScene S = J.S;
         Tracer<WG, LDSD> tr{S, st};
         const uint32_t w = J.width, h = J.height, spp = J.spp;
-        const uint32_t p_end = min((t - J.first_tile + 1) * A.tile_px, w * h);
+        const uint32_t p0 = (t - J.first_tile) * kTilePx;
+        const uint32_t npx = min(kTilePx, w * h - p0);
         const float scale = 1.0f / static_cast<float>(spp);
-        uint32_t next = (t - J.first_tile) * A.tile_px; // next unclaimed pixel (wave-uniform)
-        bool active = false;
-        uint32_t pix = 0, s = 0;
-        f3 v = mk(0.0f, 0.0f, 0.0f), acc = mk(0.0f, 0.0f, 0.0f);
-        HitRec first{};
-        bool valid = false;
-        PathState ps;
-        ps.alive = false;
-        for (;;) {
-            const uint64_t need = __ballot(!active);
-            if (!active) {
-                uint32_t cand = next + static_cast<uint32_t>(__popcll(need & below));
-                if (cand < p_end) {
-                    pix = cand;
-                    active = true;
-                    f3 o;
-                    camera_ray(J.cam, pix % w, pix / w, w, h, o, v);
-                    // compute_first_hit drops hits seen from behind (pathtracer.cpp:463-464)
-                    valid = tr.trace(o, v, first) && !(dot(first.n, -v) <= 0.0f);
-                    acc = mk(0.0f, 0.0f, 0.0f);
-                    s = 0;
-                    tr.mis_begin(ps, first, valid, v, pcg_for(A.seed, pix, s), A.max_depth);
-                }
-            }
-            next += static_cast<uint32_t>(__popcll(need));
-            if (__ballot(active) == 0)
-                break;
-            if (active) {
-                if (ps.alive)
-                    tr.mis_bounce(ps, A.max_depth);
-                if (!ps.alive) {
-                    acc = acc + ps.L;
-                    if (++s == spp) {
-                        J.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
-                        active = false;
-                    } else {
+        // first hits, one pixel per lane; compute_first_hit drops hits seen from behind
+        // (pathtracer.cpp:463-464)
+        if (lane < npx) {
+            uint32_t pix = p0 + lane;
+            f3 o, v;
+            camera_ray(J.cam, pix % w, pix / w, w, h, o, v);
+            HitRec fh{};
+            bool valid = tr.trace(o, v, fh) && !(dot(fh.n, -v) <= 0.0f);
+            uint32_t *c = hit_lds + kHitWords * lane;
+            lds_store(c + 0, __float_as_uint(fh.p.x));
+            lds_store(c + 1, __float_as_uint(fh.p.y));
+            lds_store(c + 2, __float_as_uint(fh.p.z));
+            lds_store(c + 3, __float_as_uint(fh.t));
+            lds_store(c + 4, __float_as_uint(fh.n.x));
+            lds_store(c + 5, __float_as_uint(fh.n.y));
+            lds_store(c + 6, __float_as_uint(fh.n.z));
+            lds_store(c + 7, static_cast<uint32_t>(fh.mat));
+            lds_store(c + 8, static_cast<uint32_t>(fh.light));
+            lds_store(c + 9, static_cast<uint32_t>(fh.geo));
+            lds_store(c + 10, valid ? 1u : 0u);
+        }
+        f3 acc = mk(0.0f, 0.0f, 0.0f); // lane l: running sum of pixel p0 + l
+        for (uint32_t s0 = 0; s0 < spp; s0 += kChunk) {
+            const uint32_t cs = min(kChunk, spp - s0);
+            const uint32_t items = npx * cs; // item q: pixel slot q % npx, sample s0 + q / npx
+            if (lane == 0)
+                lds_store(cur, 0u);
+            bool active = false;
+            uint32_t q = 0, slot = 0;
+            PathState ps;
+            ps.alive = false;
+            for (;;) {
+                // lanes without an item take the next ones of the tile (rank order)
+                const uint64_t need = __ballot(!active);
+                const uint32_t base = lds_load(cur);
+                if (!active) {
+                    uint32_t cand = base + static_cast<uint32_t>(__popcll(need & below));
+                    if (cand < items) {
+                        q = cand;
+                        slot = q % npx;
+                        uint32_t s = s0 + q / npx, pix = p0 + slot;
+                        f3 o, v;
+                        camera_ray(J.cam, pix % w, pix / w, w, h, o, v);
+                        uint32_t *c = hit_lds + kHitWords * slot;
+                        HitRec first;
+                        first.p = mk(__uint_as_float(lds_load(c + 0)), __uint_as_float(lds_load(c + 1)),
+                                     __uint_as_float(lds_load(c + 2)));
+                        first.t = __uint_as_float(lds_load(c + 3));
+                        first.n = mk(__uint_as_float(lds_load(c + 4)), __uint_as_float(lds_load(c + 5)),
+                                     __uint_as_float(lds_load(c + 6)));
+                        first.mat = static_cast<int32_t>(lds_load(c + 7));
+                        first.light = static_cast<int32_t>(lds_load(c + 8));
+                        first.geo = static_cast<int32_t>(lds_load(c + 9));
+                        bool valid = lds_load(c + 10) != 0u;
+                        active = true;
                         tr.mis_begin(ps, first, valid, v, pcg_for(A.seed, pix, s), A.max_depth);
                     }
                 }
+                if (lane == 0)
+                    lds_store(cur, base + static_cast<uint32_t>(__popcll(need)));
+                if (__ballot(active) == 0)
+                    break;
+                if (active) {
+                    if (ps.alive)
+                        tr.mis_bounce(ps, A.max_depth);
+                    if (!ps.alive) {
+                        buf[(q / npx) * kTilePx + slot] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+                        active = false;
+                    }
+                }
             }
+            // the chunk's samples, written by any lane, are read back by the lane of their pixel
+            __threadfence();
+            if (lane < npx)
+                for (uint32_t k = 0; k < cs; k++)
+                    acc = acc + xyz(buf[k * kTilePx + lane]);
+            __threadfence();
         }
+        if (lane < npx)
+            J.out[p0 + lane] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
         nclosest += tr.nclosest;
         nany += tr.nany;
         pops += tr.pops;
@@ -135,7 +196,7 @@ template <int WG, int LDSD> __global__ __launch_bounds__(WG) void k_render_batch
 // Host side.
 // ---------------------------------------------------------------------------------------------
 struct BatchEngine {
-    DevBuf jobs, tile_job, ctl, tiles_left, done_clock, spill;
+    DevBuf jobs, tile_job, ctl, tiles_left, done_clock, spill, samples;
     std::vector<BatchJob> h_jobs; // host copies stay alive while the uploads are in flight
     std::vector<uint32_t> h_tile_job;
     hipGraph_t graph = nullptr;
@@ -188,7 +249,7 @@ std::string batch_render(BatchEngine *e, std::vector<BatchJob> jobs, uint32_t ma
     const uint32_t n = static_cast<uint32_t>(jobs.size());
     if (n == 0)
         return "";
-    const uint32_t tile_px = tune.tile_px;
+    const uint32_t tile_px = kTilePx;
     // hand-out order: most expensive first (samples x log2 of the triangle count)
     std::vector<uint32_t> order(n);
     for (uint32_t i = 0; i < n; i++)
@@ -234,6 +295,7 @@ std::string batch_render(BatchEngine *e, std::vector<BatchJob> jobs, uint32_t ma
     BT_HIP(e->tiles_left.ensure(sizeof(uint32_t) * n));
     BT_HIP(e->done_clock.ensure(sizeof(unsigned long long) * n));
     BT_HIP(e->spill.ensure(sizeof(uint2) * static_cast<size_t>(grid) * pk.wg * spill_per_thread));
+    BT_HIP(e->samples.ensure(sizeof(float4) * static_cast<size_t>(grid) * (pk.wg / 64) * kChunk * kTilePx));
     e->h_jobs = jobs;
     BT_HIP(hipMemcpyAsync(e->jobs.p, e->h_jobs.data(), sizeof(BatchJob) * n, hipMemcpyHostToDevice, stream));
     BT_HIP(hipMemcpyAsync(e->tile_job.p, e->h_tile_job.data(), sizeof(uint32_t) * tiles, hipMemcpyHostToDevice,
@@ -244,11 +306,11 @@ std::string batch_render(BatchEngine *e, std::vector<BatchJob> jobs, uint32_t ma
     A.tile_job = static_cast<const uint32_t *>(e->tile_job.p);
     A.num_jobs = n;
     A.num_tiles = static_cast<uint32_t>(tiles);
-    A.tile_px = tile_px;
     A.max_depth = max_depth;
     A.seed = seed;
     A.spill = static_cast<uint2 *>(e->spill.p);
     A.spill_per_thread = spill_per_thread;
+    A.samples = static_cast<float4 *>(e->samples.p);
     A.cnt = cnt;
     A.ctl = static_cast<BatchCtl *>(e->ctl.p);
     A.tiles_left = static_cast<uint32_t *>(e->tiles_left.p);

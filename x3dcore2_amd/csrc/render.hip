@@ -789,7 +789,7 @@ int batch_impl(x3d_renderer *r, uint32_t n, x3d_batch_job *jobs, float *const *f
         J.spp = jobs[i].spp;
         uint64_t npix = static_cast<uint64_t>(J.width) * J.height;
         samples += npix * J.spp;
-        tiles += (npix + tune.tile_px - 1) / tune.tile_px;
+        tiles += (npix + kBatchTilePx - 1) / kBatchTilePx;
     }
     if (r->opt.workgroup_size)
         tune.wg = static_cast<int>(r->opt.workgroup_size);

@@ -189,16 +189,19 @@ typedef struct x3d_render_options {
     uint32_t max_depth;    /* UNIDIRECT_MIS: bounces after the first hit */
     uint32_t max_path_len; /* BIDIRECT_MIS: vertices per subpath (reference: 8) */
     uint64_t seed;         /* PCG seed; sample s of pixel p uses stream (seed, p, s) */
-    uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default */
-    uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default (4M), max 64M;
+    uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default (64) */
+    uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default (16M), max 64M;
                               BIDIRECT_MIS: (pixel, sample) items per generation (default 4M) */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
-    uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16 */
+    uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 8 */
     int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */
-    uint32_t reserved[3];
+    uint32_t batch_split;    /* batches: jobs of at least this many samples (w*h*spp) run in the
+                              wavefront form after the shared launch of the others; 0 = default
+                              (4M), 0xFFFFFFFF = every job in the shared launch */
+    uint32_t reserved[2];
 } x3d_render_options;
 
-/* Wavefront sort keys: group active paths by material kind (default) or keep append order. */
+/* Wavefront sort keys: keep append order (default) or group active paths by material kind. */
 #define X3D_SORT_DEFAULT 0
 #define X3D_SORT_MATERIAL 1
 #define X3D_SORT_NONE 2
@@ -265,12 +268,14 @@ int x3d_render_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera 
 int x3d_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out);
 
 /* ---- batched rendering (no reference counterpart: pt_image_renderer::render takes one scene,
- * renderer.h:57-59). Many jobs (scene, camera, film size, spp) share one persistent launch,
- * captured as a hipGraph and replayed while the batch keeps its shape. The renderer must be a
- * UNIDIRECT_MIS renderer; each job's film is bit-identical to x3d_render of that job with the
- * same renderer options. Options used: max_depth, seed, workgroup_size (64/128/256),
- * stack_depth (8/16), sort_key (NONE: hand tiles out in job order; otherwise most expensive
- * job first). Jobs may share scenes. ---- */
+ * renderer.h:57-59). Jobs (scene, camera, film size, spp) with fewer than batch_split samples
+ * share one persistent launch, captured as a hipGraph and replayed while the batch keeps its
+ * shape; larger jobs then run one after another in the wavefront form (fewest samples first).
+ * The renderer must be a plain-form UNIDIRECT_MIS renderer; each job's film is bit-identical
+ * to x3d_render of that job. Options used: max_depth, seed, batch_split; for the shared launch
+ * workgroup_size (64/128/256), stack_depth (8/16), sort_key (NONE: hand tiles out in job order;
+ * otherwise most expensive job first); for the large jobs queue_capacity. Jobs may share
+ * scenes. ---- */
 typedef struct x3d_batch_job {
     const x3d_scene *scene;
     x3d_camera camera;
@@ -278,7 +283,7 @@ typedef struct x3d_batch_job {
     uint32_t reserved;
     float *rgba;       /* film float[4*w*h]: host memory for x3d_render_batch, device memory for
                           x3d_render_batch_device */
-    double latency_ms; /* out: device time from the launch's start to the job's completion */
+    double latency_ms; /* out: device time from the batch's start to the job's completion */
 } x3d_batch_job;
 
 typedef struct x3d_batch_stats {
@@ -286,10 +291,10 @@ typedef struct x3d_batch_stats {
     uint64_t closest_queries; /* first hits included */
     uint64_t anyhit_queries;
     uint64_t closest_nodes, closest_tests, anyhit_nodes, anyhit_tests;
-    double kernel_ms;         /* event-timed batch launch */
+    double kernel_ms;         /* device time of the whole batch (HIP events) */
     double total_ms;          /* wall time of the call */
-    uint32_t tiles;           /* work tiles of 64 pixels */
-    uint32_t reserved;
+    uint32_t tiles;           /* work tiles of 64 pixels in the shared launch */
+    uint32_t alone_jobs;      /* jobs at or above batch_split, rendered on their own */
 } x3d_batch_stats;
 
 int x3d_render_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs, x3d_batch_stats *stats);

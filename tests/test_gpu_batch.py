@@ -1,6 +1,7 @@
 """Batched rendering through the C ABI: every job's film must be bit-identical to rendering that
-job alone with the same renderer (and so to the oracle), whatever the workgroup size, stack depth
-and tile order, on first capture and on graph replay."""
+job alone with the same renderer (and so to the oracle), whatever the workgroup size, stack depth,
+tile order and split between the shared launch and jobs rendered on their own in the wavefront
+form, on first capture and on graph replay."""
 
 import numpy as np
 import pytest
@@ -29,7 +30,8 @@ def batch(x3d):
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(workgroup_size=64, stack_depth=8, sort_key=2),
-                                dict(workgroup_size=128, sort_key=1)])
+                                dict(workgroup_size=128, sort_key=1),
+                                dict(batch_split=3000), dict(batch_split=1, queue_capacity=5000)])
 def test_batch_films_identical_to_single_renders(x3d, batch, kw):
     descs, built, items = batch
     r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=9, **kw)
@@ -41,6 +43,8 @@ def test_batch_films_identical_to_single_renders(x3d, batch, kw):
             assert np.array_equal(_bits(f), _bits(ref)), (k, np.abs(f - ref).max())
         assert st.num_samples == sum(w * h * spp for (_, _, w, h, spp) in items)
         assert st.closest_queries >= sum(w * h for (_, _, w, h, _) in items)
+        split = kw.get("batch_split", 1 << 22)
+        assert st.alone_jobs == sum(w * h * spp >= split for (_, _, w, h, spp) in items)
         assert all(0 < x <= st.kernel_ms * 1.01 + 0.05 for x in lat), (lat, st.kernel_ms)
 
 

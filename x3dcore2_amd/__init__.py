@@ -65,7 +65,7 @@ class RenderOptions(C.Structure):
     _fields_ = [("pt_type", C.c_int32), ("form", C.c_int32), ("max_depth", C.c_uint32),
                 ("max_path_len", C.c_uint32), ("seed", C.c_uint64), ("workgroup_size", C.c_uint32),
                 ("queue_capacity", C.c_uint32), ("sort_key", C.c_int32), ("stack_depth", C.c_uint32),
-                ("profile", C.c_int32), ("reserved", C.c_uint32 * 3)]
+                ("profile", C.c_int32), ("batch_split", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class RenderStats(C.Structure):
@@ -94,7 +94,7 @@ class BatchStats(C.Structure):
     _fields_ = [("num_samples", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
                 ("closest_nodes", C.c_uint64), ("closest_tests", C.c_uint64), ("anyhit_nodes", C.c_uint64),
                 ("anyhit_tests", C.c_uint64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
-                ("tiles", C.c_uint32), ("reserved", C.c_uint32)]
+                ("tiles", C.c_uint32), ("alone_jobs", C.c_uint32)]
 
 
 # Every exported symbol declared in include/x3dpt.h.
@@ -304,13 +304,13 @@ class Renderer:
     """pt_image_renderer equivalent (x3d_renderer)."""
 
     def __init__(self, pt_type=PT_UNIDIRECT_MIS, form=FORM_PLAIN, max_depth=5, max_path_len=8, seed=1,
-                 workgroup_size=0, queue_capacity=0, sort_key=0, stack_depth=0, profile=0):
+                 workgroup_size=0, queue_capacity=0, sort_key=0, stack_depth=0, profile=0, batch_split=0):
         L = lib()
         o = RenderOptions()
         _check(L.x3d_render_options_default(pt_type, C.byref(o)))
         o.form, o.max_depth, o.max_path_len, o.seed = form, max_depth, max_path_len, seed
         o.workgroup_size, o.queue_capacity, o.sort_key = workgroup_size, queue_capacity, sort_key
-        o.stack_depth, o.profile = stack_depth, profile
+        o.stack_depth, o.profile, o.batch_split = stack_depth, profile, batch_split
         h = C.c_void_p()
         _check(L.x3d_renderer_create(C.byref(o), C.byref(h)))
         self.h = h

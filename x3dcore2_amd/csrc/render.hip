@@ -500,6 +500,8 @@ struct x3d_renderer {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
     x3d::WavefrontEngine *wf = nullptr;
     x3d::BatchEngine *batch = nullptr;
+    x3d::WavefrontEngine *batch_wf = nullptr; // jobs of a batch that run on their own
+    hipEvent_t bev0 = nullptr, bev1 = nullptr;
     std::vector<x3d::DevBuf> batch_films; // device films of x3d_render_batch
     x3d::AdjointEngine *adj = nullptr;
     x3d::DevBuf adj_dl, adj_grad;          // host-pointer variant of x3d_render_backward
@@ -510,6 +512,11 @@ struct x3d_renderer {
             x3d::wf_destroy(wf);
         if (batch)
             x3d::batch_destroy(batch);
+        if (batch_wf)
+            x3d::wf_destroy(batch_wf);
+        for (hipEvent_t e : {bev0, bev1})
+            if (e)
+                (void)hipEventDestroy(e);
         if (evm)
             (void)hipEventDestroy(evm);
         if (ev0)
@@ -578,8 +585,26 @@ template <int T> void launch_plain(uint32_t grid, hipStream_t s, RenderParams co
 
 // Renders into d_out; with d_dl (dLoss/dfilm) it runs the adjoint of the MIS tracer instead and
 // writes the material gradient to d_grad.
+// Wavefront tuning from the renderer's options (0 / DEFAULT fields keep the defaults).
+x3d::WavefrontTuning wf_tuning(x3d_render_options const &o) {
+    x3d::WavefrontTuning tune;
+    if (o.workgroup_size)
+        tune.trace_wg = static_cast<int>(o.workgroup_size);
+    if (o.queue_capacity)
+        tune.paths_in_flight = o.queue_capacity;
+    if (o.stack_depth)
+        tune.stack_depth = static_cast<int>(o.stack_depth);
+    if (o.sort_key != X3D_SORT_DEFAULT)
+        tune.sort_by_material = o.sort_key == X3D_SORT_MATERIAL;
+    tune.profile = o.profile;
+    return tune;
+}
+
+// One render. `wf` (with `wf_tune`) selects the wavefront form of the MIS tracer, null the plain
+// kernels; d_dl / d_grad select the adjoint pass.
 int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
                 uint32_t height, uint32_t spp, float *d_out, x3d_render_stats *stats,
+                x3d::WavefrontEngine *wf = nullptr, x3d::WavefrontTuning const *wf_tune = nullptr,
                 const float4 *d_dl = nullptr, float *d_grad = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     RenderParams P{};
@@ -606,7 +631,7 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     uint32_t grid_render = 1;
     // Films with fewer pixels than 4x the lanes the plain MIS kernel keeps resident run the
     // MIS tracer over (pixel, sample) items instead (same film).
-    bool mis_items = tracer == X3D_PT_UNIDIRECT_MIS && !r->wf && !d_dl &&
+    bool mis_items = tracer == X3D_PT_UNIDIRECT_MIS && !wf && !d_dl &&
                      npix < 4ull * kWG * persistent_grid(k_render_mis, ~0ull >> 8);
     switch (tracer) {
     case X3D_PT_UNIDIRECT_MIS:
@@ -642,17 +667,8 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         if (!e.empty())
             return fail(e.find("hip") != std::string::npos ? X3D_ERR_DEVICE : X3D_ERR_UNSUPPORTED, e);
         tracer = -1; // launched
-    } else if (r->wf) {
-        x3d::WavefrontTuning tune;
-        if (r->opt.workgroup_size)
-            tune.trace_wg = static_cast<int>(r->opt.workgroup_size);
-        if (r->opt.queue_capacity)
-            tune.paths_in_flight = r->opt.queue_capacity;
-        if (r->opt.stack_depth)
-            tune.stack_depth = static_cast<int>(r->opt.stack_depth);
-        tune.sort_by_material = r->opt.sort_key != X3D_SORT_NONE;
-        tune.profile = r->opt.profile;
-        std::string e = x3d::wf_render(r->wf, P, tune, r->stream);
+    } else if (wf) {
+        std::string e = x3d::wf_render(wf, P, *wf_tune, r->stream);
         if (!e.empty())
             return fail(X3D_ERR_DEVICE, e);
         tracer = -1; // launched
@@ -727,8 +743,8 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         stats->extend_ms = stats->shadow_ms = 0.0;
         stats->trace_launches = 0;
-        if (r->wf && r->opt.profile)
-            x3d::wf_last_timing(r->wf, &stats->shadow_ms, &stats->extend_ms, &stats->trace_launches);
+        if (wf && wf_tune->profile)
+            x3d::wf_last_timing(wf, &stats->shadow_ms, &stats->extend_ms, &stats->trace_launches);
     }
     return X3D_OK;
 }
@@ -764,7 +780,12 @@ int check_batch(x3d_renderer *r, uint32_t n, x3d_batch_job const *jobs) {
     return X3D_OK;
 }
 
-// Runs the batch with films already in device memory (films[i] for job i).
+constexpr uint64_t kBatchSplitDefault = 1ull << 22;
+
+// Runs the batch with films already in device memory (films[i] for job i): the jobs below the
+// split share one persistent launch (small films cannot fill the GPU on their own); each job at
+// or above it then runs alone in the wavefront form, fewest samples first. Either way a job's
+// film is bit-identical to rendering it alone.
 int batch_impl(x3d_renderer *r, uint32_t n, x3d_batch_job *jobs, float *const *films, x3d_batch_stats *stats) {
     auto t0 = std::chrono::steady_clock::now();
     if (n == 0) {
@@ -774,48 +795,96 @@ int batch_impl(x3d_renderer *r, uint32_t n, x3d_batch_job *jobs, float *const *f
     }
     if (!r->batch)
         r->batch = x3d::batch_create();
+    if (!r->bev0) {
+        X3D_HIP(hipEventCreate(&r->bev0));
+        X3D_HIP(hipEventCreate(&r->bev1));
+    }
     X3D_HIP(r->counters.ensure(2 * sizeof(Counters) + 64));
-    std::vector<x3d::BatchJob> bj(n);
+    const uint64_t split = r->opt.batch_split ? r->opt.batch_split : kBatchSplitDefault;
+    std::vector<x3d::BatchJob> bj;
+    std::vector<uint32_t> shared, alone;
     uint64_t samples = 0, tiles = 0;
-    x3d::BatchTuning tune;
     for (uint32_t i = 0; i < n; i++) {
-        x3d::BatchJob &J = bj[i];
-        J = x3d::BatchJob{};
+        uint64_t npix = static_cast<uint64_t>(jobs[i].width) * jobs[i].height;
+        uint64_t js = npix * jobs[i].spp;
+        samples += js;
+        if (js >= split) {
+            alone.push_back(i);
+            continue;
+        }
+        shared.push_back(i);
+        x3d::BatchJob J{};
         J.S = jobs[i].scene->s.dev;
         J.cam = cam_params(jobs[i].camera);
         J.out = reinterpret_cast<float4 *>(films[i]);
         J.width = jobs[i].width;
         J.height = jobs[i].height;
         J.spp = jobs[i].spp;
-        uint64_t npix = static_cast<uint64_t>(J.width) * J.height;
-        samples += npix * J.spp;
+        bj.push_back(J);
         tiles += (npix + kBatchTilePx - 1) / kBatchTilePx;
     }
+    x3d::BatchTuning tune;
     if (r->opt.workgroup_size)
         tune.wg = static_cast<int>(r->opt.workgroup_size);
     if (r->opt.stack_depth)
         tune.stack_depth = static_cast<int>(r->opt.stack_depth);
     tune.sort_by_cost = r->opt.sort_key != X3D_SORT_NONE;
     Counters *cnt = static_cast<Counters *>(r->counters.p);
-    x3d::BatchResult res;
-    std::string e = x3d::batch_render(r->batch, bj, r->opt.max_depth, r->opt.seed, tune, cnt, r->stream, &res);
-    if (!e.empty())
-        return fail(X3D_ERR_DEVICE, e);
-    for (uint32_t i = 0; i < n; i++)
-        jobs[i].latency_ms = res.job_ms.empty() ? 0.0 : res.job_ms[i];
+    Counters total{};
+    X3D_HIP(hipEventRecord(r->bev0, r->stream));
+    if (!shared.empty()) {
+        x3d::BatchResult res;
+        std::string e = x3d::batch_render(r->batch, bj, r->opt.max_depth, r->opt.seed, tune, cnt, r->stream, &res);
+        if (!e.empty())
+            return fail(X3D_ERR_DEVICE, e);
+        for (size_t k = 0; k < shared.size(); k++)
+            jobs[shared[k]].latency_ms = res.job_ms.empty() ? 0.0 : res.job_ms[k];
+        X3D_HIP(hipMemcpy(&total, cnt, sizeof total, hipMemcpyDeviceToHost));
+    }
+    if (!alone.empty()) {
+        // the shared launch's knobs (workgroup size, stack depth, sort key) do not apply here
+        x3d::WavefrontTuning wt;
+        if (r->opt.queue_capacity)
+            wt.paths_in_flight = r->opt.queue_capacity;
+        if (!r->batch_wf)
+            r->batch_wf = x3d::wf_create();
+        std::stable_sort(alone.begin(), alone.end(), [&](uint32_t a, uint32_t b) {
+            return static_cast<uint64_t>(jobs[a].width) * jobs[a].height * jobs[a].spp <
+                   static_cast<uint64_t>(jobs[b].width) * jobs[b].height * jobs[b].spp;
+        });
+        for (uint32_t i : alone) {
+            x3d_render_stats st{};
+            int rc = render_impl(r, jobs[i].scene, &jobs[i].camera, jobs[i].width, jobs[i].height, jobs[i].spp,
+                                 films[i], &st, r->batch_wf, &wt);
+            if (rc)
+                return rc;
+            float ms = 0;
+            X3D_HIP(hipEventElapsedTime(&ms, r->bev0, r->ev1)); // ev1: the end of that render
+            jobs[i].latency_ms = ms;
+            total.closest += st.closest_queries;
+            total.anyhit += st.anyhit_queries;
+            total.pops += st.closest_nodes;
+            total.tests += st.closest_tests;
+            total.spops += st.anyhit_nodes;
+            total.stests += st.anyhit_tests;
+        }
+    }
+    X3D_HIP(hipEventRecord(r->bev1, r->stream));
+    X3D_HIP(hipEventSynchronize(r->bev1));
     if (stats) {
-        Counters c{};
-        X3D_HIP(hipMemcpy(&c, cnt, sizeof c, hipMemcpyDeviceToHost));
+        float ms = 0;
+        X3D_HIP(hipEventElapsedTime(&ms, r->bev0, r->bev1));
         *stats = x3d_batch_stats{};
         stats->num_samples = samples;
-        stats->closest_queries = c.closest;
-        stats->anyhit_queries = c.anyhit;
-        stats->closest_nodes = c.pops;
-        stats->closest_tests = c.tests;
-        stats->anyhit_nodes = c.spops;
-        stats->anyhit_tests = c.stests;
-        stats->kernel_ms = res.kernel_ms;
+        stats->closest_queries = total.closest;
+        stats->anyhit_queries = total.anyhit;
+        stats->closest_nodes = total.pops;
+        stats->closest_tests = total.tests;
+        stats->anyhit_nodes = total.spops;
+        stats->anyhit_tests = total.stests;
+        stats->kernel_ms = ms;
         stats->tiles = static_cast<uint32_t>(tiles);
+        stats->alone_jobs = static_cast<uint32_t>(alone.size());
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return X3D_OK;
@@ -1284,7 +1353,8 @@ int x3d_render_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera 
         return rc;
     if (!d_rgba)
         return fail(X3D_ERR_INVALID_ARGUMENT, "d_rgba is null");
-    return render_impl(r, scene, camera, width, height, spp, d_rgba, stats);
+    x3d::WavefrontTuning tune = wf_tuning(r->opt);
+    return render_impl(r, scene, camera, width, height, spp, d_rgba, stats, r->wf, &tune);
 }
 
 int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width, uint32_t height,
@@ -1299,7 +1369,8 @@ int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera
     uint64_t npix = static_cast<uint64_t>(width) * height;
     X3D_HIP(r->out.ensure(npix * sizeof(float4)));
     auto t0 = std::chrono::steady_clock::now();
-    rc = render_impl(r, scene, camera, width, height, spp, static_cast<float *>(r->out.p), stats);
+    x3d::WavefrontTuning tune = wf_tuning(r->opt);
+    rc = render_impl(r, scene, camera, width, height, spp, static_cast<float *>(r->out.p), stats, r->wf, &tune);
     if (rc)
         return rc;
     X3D_HIP(hipMemcpy(rgba, r->out.p, npix * sizeof(float4), hipMemcpyDeviceToHost));
@@ -1327,7 +1398,9 @@ int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_cam
         x3d_render_stats st;
         double best = 1e300;
         for (int k = 0; k < 3; k++) { // first call captures the graph; best of the other two
-            int e = render_impl(r, scene, camera, width, height, probe_spp, static_cast<float *>(r->out.p), &st);
+            x3d::WavefrontTuning tune = wf_tuning(r->opt);
+            int e = render_impl(r, scene, camera, width, height, probe_spp, static_cast<float *>(r->out.p), &st,
+                                r->wf, &tune);
             if (e)
                 return e;
             if (k > 0 && st.trace_ms < best)
@@ -1503,7 +1576,7 @@ int x3d_render_backward_device(x3d_renderer *r, const x3d_scene *scene, const x3
     int rc = check_backward(r, scene, camera, width, height, spp, d_dl_dfilm, d_grad);
     if (rc)
         return rc;
-    return render_impl(r, scene, camera, width, height, spp, nullptr, stats,
+    return render_impl(r, scene, camera, width, height, spp, nullptr, stats, nullptr, nullptr,
                        reinterpret_cast<const float4 *>(d_dl_dfilm), d_grad);
 }
 
@@ -1518,8 +1591,8 @@ int x3d_render_backward(x3d_renderer *r, const x3d_scene *scene, const x3d_camer
     X3D_HIP(r->adj_dl.ensure(npix * sizeof(float4)));
     X3D_HIP(r->adj_grad.ensure(std::max<size_t>(ng, 4) * sizeof(float)));
     X3D_HIP(hipMemcpy(r->adj_dl.p, dl_dfilm, npix * sizeof(float4), hipMemcpyHostToDevice));
-    rc = render_impl(r, scene, camera, width, height, spp, nullptr, stats, static_cast<const float4 *>(r->adj_dl.p),
-                     static_cast<float *>(r->adj_grad.p));
+    rc = render_impl(r, scene, camera, width, height, spp, nullptr, stats, nullptr, nullptr,
+                     static_cast<const float4 *>(r->adj_dl.p), static_cast<float *>(r->adj_grad.p));
     if (rc)
         return rc;
     X3D_HIP(hipMemcpy(grad, r->adj_grad.p, ng * sizeof(float), hipMemcpyDeviceToHost));

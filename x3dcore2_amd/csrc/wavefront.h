@@ -11,14 +11,14 @@ namespace x3d {
 
 struct WavefrontEngine;
 
-constexpr uint32_t kDefaultQueueCapacity = 1u << 22;
+constexpr uint32_t kDefaultQueueCapacity = 1u << 24; // 16M paths: ~2.7 GB of path state and queues
 constexpr uint32_t kMaxQueueCapacity = 1u << 26; // 64M paths: ~10 GB of path state and queues
 
 struct WavefrontTuning {
     uint32_t paths_in_flight = kDefaultQueueCapacity; // queue capacity (paths of one generation)
-    int trace_wg = 256;                  // workgroup size of the traversal kernels (64, 128, 256)
-    int stack_depth = 16;                // LDS stack entries per lane of the traversal kernels (8, 16)
-    int sort_by_material = 1;            // group active paths by material kind before shading
+    int trace_wg = 64;                   // workgroup size of the traversal kernels (64, 128, 256)
+    int stack_depth = 8;                 // LDS stack entries per lane of the traversal kernels (8, 16)
+    int sort_by_material = 0;            // group active paths by material kind before shading
     int use_graph = 1;                   // replay one generation's launches as a hipGraph
     int profile = 0;                     // time the traversal kernels with events (no graph)
 };

@@ -373,10 +373,22 @@ def main():
         nodes = (sp.closest_nodes - sp.first_hit_nodes) / launches
         tests = (sp.closest_tests - sp.first_hit_tests) / launches
         launch_ms = sp.extend_ms / launches
-        kernel = f"k_wf_extend<{wg or 256},{sd or 16}>"
+        kernel = f"k_wf_extend<{wg or 64},{sd or 8}>"
         traffic = traffic_from_profile(f"traffic_{desc.name}_wavefront.md", kernel)
         out["wavefront_kernel_ms"] = {"extend": sp.extend_ms, "shadow": sp.shadow_ms, "launches": sp.trace_launches,
                                       "all_sample_kernels": sp.trace_ms}
+        # ablation: the same configuration with both ray queues sorted by key before traversal
+        ra = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=depth, seed=1, workgroup_size=wg,
+                          queue_capacity=tuned.get("queue_capacity", 0), sort_key=sort, stack_depth=sd,
+                          ray_order=x3d.RAY_ORDER_SORTED)
+        ea, _ = timed_renders(x3d, ra, scene, desc, R["film"], 1, 1, "wavefront, sorted rays")
+        spa = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=depth, seed=1, workgroup_size=wg,
+                           queue_capacity=tuned.get("queue_capacity", 0), sort_key=sort, stack_depth=sd,
+                           ray_order=x3d.RAY_ORDER_SORTED, profile=1).render_device(scene, desc.camera, w, h, spp,
+                                                                                    R["film"].ptr)
+        out["ray_order_ablation"] = {"append_order_ms_per_step": out["ms_per_step"], "sorted_ms_per_step": ea * 1e3,
+                                     "sorted_extend_ms": spa.extend_ms, "sorted_shadow_ms": spa.shadow_ms}
+        del ra
     else:
         nodes = (s0.closest_nodes - s0.first_hit_nodes) + s0.anyhit_nodes
         tests = (s0.closest_tests - s0.first_hit_tests) + s0.anyhit_tests

@@ -198,8 +198,17 @@ typedef struct x3d_render_options {
     uint32_t batch_split;    /* batches: jobs of at least this many samples (w*h*spp) run in the
                               wavefront form after the shared launch of the others; 0 = default
                               (4M), 0xFFFFFFFF = every job in the shared launch */
-    uint32_t reserved[2];
+    int32_t ray_order;       /* WAVEFRONT: X3D_RAY_ORDER_* */
+    uint32_t reserved;
 } x3d_render_options;
+
+/* Wavefront ray order before each traversal kernel: the order the rays were queued in (default),
+ * or sorted by direction octant and origin cell (a counting sort per bounce; the traversal
+ * kernels fetch rays dynamically, which already keeps their lanes busy, so the sort costs more
+ * than it saves on the measured workloads). The film does not depend on it. */
+#define X3D_RAY_ORDER_DEFAULT 0
+#define X3D_RAY_ORDER_SORTED 1
+#define X3D_RAY_ORDER_APPEND 2
 
 /* Wavefront sort keys: keep append order (default) or group active paths by material kind. */
 #define X3D_SORT_DEFAULT 0
@@ -238,7 +247,8 @@ typedef struct x3d_render_stats {
  * sort key {material, none} at the default capacity; stage 2 tries queue capacities
  * {2M, 8M, 16M} paths for the stage-1 winner, skipping any above width*height*probe_spp (a
  * probe that small cannot tell them apart). Each candidate is timed as the best of two renders
- * after one warm-up render. The film does not depend on the configuration. */
+ * after one warm-up render; the default configuration is kept unless the winner is at least 2%
+ * faster. The film does not depend on the configuration. */
 typedef struct x3d_autotune_report {
     uint32_t workgroup_size, stack_depth, queue_capacity;
     int32_t sort_key;

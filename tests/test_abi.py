@@ -65,6 +65,8 @@ def test_render_option_errors_without_gpu(x3d):
         dict(pt_type=x3d.PT_UNIDIRECT_MIS, workgroup_size=96),
         dict(pt_type=x3d.PT_UNIDIRECT_MIS, stack_depth=12),
         dict(pt_type=x3d.PT_UNIDIRECT_MIS, queue_capacity=(1 << 26) + 1),
+        dict(pt_type=x3d.PT_UNIDIRECT_MIS, ray_order=3),
+        dict(pt_type=x3d.PT_UNIDIRECT_MIS, sort_key=-1),
     ]
     for kw in bad:
         with pytest.raises(x3d.X3DError) as e:

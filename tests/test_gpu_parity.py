@@ -75,6 +75,43 @@ def test_any_hit_matches(scene_pair, oracle_mod):
     assert np.array_equal(g, ref)
 
 
+def _plane_rays(gs, n, seed):
+    """Rays parallel to a coordinate plane whose origin lies exactly on a BVH box plane of the
+    axis they do not move along: the slab test meets 0 * inf = NaN there, where the reference's
+    std::max / std::min keep or drop the NaN by argument order."""
+    rng = np.random.default_rng(seed)
+    bounds, _, _ = gs.bvh()
+    lo, hi = bounds[:, :3].min(0), bounds[:, 3:].max(0)
+    node = rng.integers(0, bounds.shape[0], size=n)
+    axis = rng.integers(0, 3, size=n)
+    side = rng.integers(0, 2, size=n)
+    o = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+    o[np.arange(n), axis] = bounds[node, axis + 3 * side]
+    d = rng.normal(size=(n, 3))
+    d[np.arange(n), axis] = 0.0
+    one = rng.random(n) < 0.5  # half of them move along a single axis
+    keep = (axis + 1 + rng.integers(0, 2, size=n)) % 3
+    d[one] = 0.0
+    d[one, keep[one]] = rng.choice([-1.0, 1.0], size=one.sum())
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d.astype(np.float32)], axis=1).astype(np.float32)
+
+
+@pytest.mark.parametrize("traversal", [0, 1])
+def test_axis_parallel_rays_on_box_planes_bit_exact(scene_pair, oracle_mod, traversal):
+    desc, gs, orc = scene_pair
+    rays = _plane_rays(gs, 20000, 4)
+    g = gs.intersect(rays, traversal)
+    o = orc.intersect(rays, oracle_mod.TRAV_CULLED)
+    assert (o["prim"] >= 0).mean() > 0.1
+    assert np.array_equal(g["prim"], o["prim"])
+    hit = o["prim"] >= 0
+    assert np.array_equal(_bits(g["t"][hit]), _bits(o["t"][hit]))
+    tmin = np.full(rays.shape[0], 1e-4, np.float32)
+    tmax = np.where(hit, o["t"] * 1.001, 2.0).astype(np.float32)
+    assert np.array_equal(gs.occluded(rays, tmin, tmax), orc.occluded(rays, tmin, tmax))
+
+
 def _bsdf_inputs(n, seed):
     rng = np.random.default_rng(seed)
     nrm = rng.normal(size=(n, 3))
@@ -175,9 +212,10 @@ def test_render_deterministic_and_shape_independent(x3d):
 
 
 WAVEFRONT_CASES = [
-    dict(),                                      # defaults: one generation, material sort, graph
+    dict(),                                      # defaults: one generation, graph
     dict(workgroup_size=64, queue_capacity=3000),   # several generations, partial last one
-    dict(workgroup_size=128, sort_key=2),           # no material sort
+    dict(workgroup_size=128, sort_key=1),           # material sort
+    dict(workgroup_size=256, stack_depth=16, ray_order=1, queue_capacity=1000),  # rays sorted by key
 ]
 
 

@@ -23,6 +23,7 @@ PT_NORMAL, PT_POSITION, PT_DIRECT, PT_UNIDIRECT, PT_UNIDIRECT_LT1, PT_BIDIRECT_L
 PT_UNIDIRECT_MIS = 16
 FORM_PLAIN, FORM_WAVEFRONT = 0, 1
 SORT_DEFAULT, SORT_MATERIAL, SORT_NONE = 0, 1, 2
+RAY_ORDER_DEFAULT, RAY_ORDER_SORTED, RAY_ORDER_APPEND = 0, 1, 2
 TRAVERSAL_CULLED, TRAVERSAL_REFERENCE = 0, 1
 
 
@@ -65,7 +66,8 @@ class RenderOptions(C.Structure):
     _fields_ = [("pt_type", C.c_int32), ("form", C.c_int32), ("max_depth", C.c_uint32),
                 ("max_path_len", C.c_uint32), ("seed", C.c_uint64), ("workgroup_size", C.c_uint32),
                 ("queue_capacity", C.c_uint32), ("sort_key", C.c_int32), ("stack_depth", C.c_uint32),
-                ("profile", C.c_int32), ("batch_split", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("profile", C.c_int32), ("batch_split", C.c_uint32), ("ray_order", C.c_int32),
+                ("reserved", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -304,13 +306,14 @@ class Renderer:
     """pt_image_renderer equivalent (x3d_renderer)."""
 
     def __init__(self, pt_type=PT_UNIDIRECT_MIS, form=FORM_PLAIN, max_depth=5, max_path_len=8, seed=1,
-                 workgroup_size=0, queue_capacity=0, sort_key=0, stack_depth=0, profile=0, batch_split=0):
+                 workgroup_size=0, queue_capacity=0, sort_key=0, stack_depth=0, profile=0, batch_split=0,
+                 ray_order=0):
         L = lib()
         o = RenderOptions()
         _check(L.x3d_render_options_default(pt_type, C.byref(o)))
         o.form, o.max_depth, o.max_path_len, o.seed = form, max_depth, max_path_len, seed
         o.workgroup_size, o.queue_capacity, o.sort_key = workgroup_size, queue_capacity, sort_key
-        o.stack_depth, o.profile, o.batch_split = stack_depth, profile, batch_split
+        o.stack_depth, o.profile, o.batch_split, o.ray_order = stack_depth, profile, batch_split, ray_order
         h = C.c_void_p()
         _check(L.x3d_renderer_create(C.byref(o), C.byref(h)))
         self.h = h

@@ -51,6 +51,25 @@ constexpr float kTMax = 1000.0f; // pathspace.cpp:336
 // Relative slack of the distance cull (see traverse.h).
 constexpr float kCullSlack = 1.0f / 4096.0f;
 
+// Fast form of slab(): when none of the six plane distances is NaN, the per-axis updates reduce
+// to one maximum of the entry distances and one minimum of the exit distances (min / max
+// instructions, no compare-and-select pairs), and the reference's early exit after any axis
+// equals one final comparison, because the entry distance only grows and the exit distance only
+// shrinks from axis to axis. Values equal those of slab() (a zero's sign may differ, which no
+// comparison sees). *nan reports a NaN distance (0 * inf: a ray parallel to an axis whose origin
+// lies on a box plane); the caller then takes slab() itself.
+X3D_DI bool slab_fast(float mnx, float mxx, float mny, float mxy, float mnz, float mxz, f3 o, f3 vinv,
+                      float tmin, float tmax, float *t0, bool *nan) {
+    float ax = (mnx - o.x) * vinv.x, bx = (mxx - o.x) * vinv.x;
+    float ay = (mny - o.y) * vinv.y, by = (mxy - o.y) * vinv.y;
+    float az = (mnz - o.z) * vinv.z, bz = (mxz - o.z) * vinv.z;
+    *nan = __builtin_isunordered(ax, bx) || __builtin_isunordered(ay, by) || __builtin_isunordered(az, bz);
+    float lo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+    float hi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
+    *t0 = lo;
+    return !(lo > hi);
+}
+
 // Slab test of aabb::intersect (tensor.h:815-840) on one box.
 X3D_DI bool slab(float mnx, float mxx, float mny, float mxy, float mnz, float mxz, f3 o, f3 vinv,
                  float tmin, float tmax, float *t0) {

@@ -596,6 +596,8 @@ x3d::WavefrontTuning wf_tuning(x3d_render_options const &o) {
         tune.stack_depth = static_cast<int>(o.stack_depth);
     if (o.sort_key != X3D_SORT_DEFAULT)
         tune.sort_by_material = o.sort_key == X3D_SORT_MATERIAL;
+    if (o.ray_order != X3D_RAY_ORDER_DEFAULT)
+        tune.sort_rays = o.ray_order == X3D_RAY_ORDER_SORTED;
     tune.profile = o.profile;
     return tune;
 }
@@ -1319,6 +1321,8 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
         return fail(X3D_ERR_INVALID_ARGUMENT, "workgroup_size must be 0, 64, 128 or 256");
     if (opts->sort_key < X3D_SORT_DEFAULT || opts->sort_key > X3D_SORT_NONE)
         return fail(X3D_ERR_INVALID_ARGUMENT, "unknown sort key");
+    if (opts->ray_order < X3D_RAY_ORDER_DEFAULT || opts->ray_order > X3D_RAY_ORDER_APPEND)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "unknown ray order");
     if (opts->stack_depth != 0 && opts->stack_depth != 8 && opts->stack_depth != 16)
         return fail(X3D_ERR_INVALID_ARGUMENT, "stack_depth must be 0, 8 or 16");
     if (opts->queue_capacity > x3d::kMaxQueueCapacity)
@@ -1448,6 +1452,16 @@ int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_cam
     if (rc) {
         r->opt = saved;
         return rc;
+    }
+    // Timing noise between near-equal candidates must not move the configuration: the defaults
+    // stay unless the winner is at least 2% faster.
+    if (rep.best_ms > 0.98 * rep.default_ms) {
+        x3d::WavefrontTuning def;
+        rep.workgroup_size = static_cast<uint32_t>(def.trace_wg);
+        rep.stack_depth = static_cast<uint32_t>(def.stack_depth);
+        rep.sort_key = def.sort_by_material ? X3D_SORT_MATERIAL : X3D_SORT_NONE;
+        rep.queue_capacity = def.paths_in_flight;
+        rep.best_ms = rep.default_ms;
     }
     r->opt = saved;
     r->opt.workgroup_size = rep.workgroup_size;

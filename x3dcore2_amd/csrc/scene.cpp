@@ -421,6 +421,11 @@ std::string Scene::upload() {
     dev.max_stack = max_leaf_depth + 1;
     dev.num_prims = static_cast<uint32_t>(prims.size());
     dev.num_interior = num_interior;
+    for (int a = 0; a < 3; a++) {
+        float lo = flat.empty() ? 0.0f : flat[0].mn[a], hi = flat.empty() ? 0.0f : flat[0].mx[a];
+        dev.box_lo[a] = lo;
+        dev.box_inv[a] = hi > lo ? 1.0f / (hi - lo) : 0.0f;
+    }
     return "";
 }
 

@@ -74,6 +74,8 @@ struct DevScene {
     uint32_t max_stack;     // deepest root-to-leaf path (stack entries needed)
     uint32_t num_prims;
     uint32_t num_interior;
+    float box_lo[3];  // scene bounds (root box): lower corner
+    float box_inv[3]; // 1 / extent per axis (0 for a flat axis)
 };
 
 } // namespace x3d

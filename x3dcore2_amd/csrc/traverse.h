@@ -89,8 +89,13 @@ X3D_DI uint32_t visit_interior(DevScene const &S, uint32_t ref, f3 o, f3 vinv, f
     float4 f0, f1, f2, f3n;
     load_node(S, ref, f0, f1, f2, f3n);
     float tl = 0.0f, tr = 0.0f;
-    bool hl = slab(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, tmin, tmax, &tl);
-    bool hr = slab(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, tmin, tmax, &tr);
+    bool nl, nr;
+    bool hl = slab_fast(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, tmin, tmax, &tl, &nl);
+    bool hr = slab_fast(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, tmin, tmax, &tr, &nr);
+    if (nl)
+        hl = slab(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, tmin, tmax, &tl);
+    if (nr)
+        hr = slab(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, tmin, tmax, &tr);
     uint32_t rl = __float_as_uint(f3n.x), rr = __float_as_uint(f3n.y);
     if (!REF_ORDER) {
         hl = hl && !(tl > cut);

@@ -19,6 +19,7 @@ struct WavefrontTuning {
     int trace_wg = 64;                   // workgroup size of the traversal kernels (64, 128, 256)
     int stack_depth = 8;                 // LDS stack entries per lane of the traversal kernels (8, 16)
     int sort_by_material = 0;            // group active paths by material kind before shading
+    int sort_rays = 0;                   // reorder both ray queues by origin cell and direction
     int use_graph = 1;                   // replay one generation's launches as a hipGraph
     int profile = 0;                     // time the traversal kernels with events (no graph)
 };

@@ -62,7 +62,35 @@ struct WfArgs {
     uint32_t spill_per_thread;
     Counters *cnt;
     int sort_by_material;
+    // Ray reordering: the traversal kernels (and shade) read sq_in / eq_in, which are either the
+    // append-order queues or their copies sorted by ray key (queue 0 shadow, 1 extension).
+    int sort_rays;
+    const float4 *sq_in;
+    const uint32_t *eq_in;
+    float4 *sq_sorted;
+    uint32_t *eq_sorted;
+    uint32_t *ray_key[2], *ray_rank[2]; // per queue entry
+    uint32_t *bin_mat[2];               // [kSortBlocks][kSortBins] counts, then offsets
+    uint32_t *bin_tot[2];               // [kSortBins]
 };
+
+// Ray keys: direction octant (3 bits) above the Morton code of the origin's cell in an 8^3 grid
+// over the scene box (9 bits). Rays with equal keys start close together and head the same
+// general way, so a wave of them visits largely the same nodes.
+constexpr uint32_t kSortBins = 4096;
+constexpr uint32_t kSortBlocks = 128; // blocks of the count / scatter passes (rows of bin_mat)
+constexpr uint32_t kSortWG = 1024;
+
+X3D_DI uint32_t spread3(uint32_t v) { return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4); }
+
+X3D_DI uint32_t ray_key(DevScene const &S, f3 o, f3 d) {
+    // fmaxf first: a NaN coordinate lands in cell 0
+    uint32_t cx = static_cast<uint32_t>(fminf(fmaxf((o.x - S.box_lo[0]) * S.box_inv[0] * 8.0f, 0.0f), 7.0f));
+    uint32_t cy = static_cast<uint32_t>(fminf(fmaxf((o.y - S.box_lo[1]) * S.box_inv[1] * 8.0f, 0.0f), 7.0f));
+    uint32_t cz = static_cast<uint32_t>(fminf(fmaxf((o.z - S.box_lo[2]) * S.box_inv[2] * 8.0f, 0.0f), 7.0f));
+    uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    return (oct << 9) | spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+}
 
 X3D_DI int kind_of(DevScene const &S, int32_t mat) {
     int32_t k = S.mats[mat].kind;
@@ -258,6 +286,7 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
         bool has_shadow = false, ok = false;
         uint32_t path = 0;
         f3 xp3 = mk(0.0f, 0.0f, 0.0f), sc = mk(0.0f, 0.0f, 0.0f), sdir = mk(0.0f, 0.0f, 0.0f);
+        f3 edir = mk(0.0f, 0.0f, 0.0f);
         float sdist = 0.0f;
         if (in) {
             uint32_t kind = t < c0 ? 0u : (t < c0 + c1 ? 1u : 2u);
@@ -307,6 +336,7 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
             ok = dens > 0.0f;
             if (ok)
                 A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
+            edir = wi;
         }
         // one call site with all lanes active: block_append synchronizes the block
         const bool want[3] = {in && has_shadow, in && ok, in && !ok};
@@ -317,11 +347,117 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
             A.sq[3 * q] = make_float4(xp3.x, xp3.y, xp3.z, sdist - 1e-3f);
             A.sq[3 * q + 1] = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(path));
             A.sq[3 * q + 2] = make_float4(sc.x, sc.y, sc.z, 0.0f);
+            if (A.sort_rays)
+                A.ray_key[0][q] = ray_key(A.S, xp3, sdir);
         }
-        if (want[1])
+        if (want[1]) {
             A.eq[idx[1]] = path;
+            if (A.sort_rays)
+                A.ray_key[1][idx[1]] = ray_key(A.S, xp3, edir);
+        }
         if (want[2])
             A.fin[idx[2]] = path;
+    }
+}
+
+// Counting sort of both ray queues by key (blockIdx.y = queue), in four passes: per-block key
+// counts over contiguous chunks (with each entry's rank among equal keys of its block), column
+// totals, per-(block, key) output offsets, scatter. No global atomics.
+X3D_DI uint32_t queue_count(WfArgs const &A, int q) { return q == 0 ? A.ctl->sq_count.v : A.ctl->eq_count.v; }
+
+__global__ __launch_bounds__(kSortWG) void k_wf_sort_count(WfArgs A) {
+    __shared__ uint32_t hist[kSortBins];
+    const int q = blockIdx.y;
+    for (uint32_t b = threadIdx.x; b < kSortBins; b += kSortWG)
+        hist[b] = 0u;
+    __syncthreads();
+    const uint32_t n = queue_count(A, q);
+    const uint32_t chunk = (n + kSortBlocks - 1) / kSortBlocks;
+    const uint32_t begin = blockIdx.x * chunk, end = min(begin + chunk, n);
+    const uint32_t *key = A.ray_key[q];
+    uint32_t *rank = A.ray_rank[q];
+    for (uint32_t i = begin + threadIdx.x; i < end; i += kSortWG)
+        rank[i] = atomicAdd(&hist[key[i]], 1u);
+    __syncthreads();
+    uint32_t *row = A.bin_mat[q] + static_cast<size_t>(blockIdx.x) * kSortBins;
+    for (uint32_t b = threadIdx.x; b < kSortBins; b += kSortWG)
+        row[b] = hist[b];
+}
+
+// One thread per key: total over the count rows.
+__global__ __launch_bounds__(256) void k_wf_sort_totals(WfArgs A) {
+    const int q = blockIdx.y;
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t *m = A.bin_mat[q];
+    uint32_t t = 0;
+    for (uint32_t r = 0; r < kSortBlocks; r++)
+        t += m[static_cast<size_t>(r) * kSortBins + b];
+    A.bin_tot[q][b] = t;
+}
+
+// One thread per key: exclusive scan of the totals (every block scans all of them in LDS), then
+// the running offsets of that key over the count rows replace the counts.
+__global__ __launch_bounds__(256) void k_wf_sort_offsets(WfArgs A) {
+    constexpr uint32_t kPer = kSortBins / 256;
+    __shared__ uint32_t part[256];
+    const int q = blockIdx.y;
+    const uint32_t *tot = A.bin_tot[q];
+    uint32_t own[kPer], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; k++) {
+        own[k] = tot[threadIdx.x * kPer + k];
+        sum += own[k];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the 256 partial sums
+    for (uint32_t d = 1; d < 256; d <<= 1) {
+        uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    // exclusive prefix of key b = blockIdx.x * 256 + threadIdx.x
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t owner = b / kPer;
+    uint32_t base = owner ? part[owner - 1] : 0u;
+    for (uint32_t k = owner * kPer; k < b; k++)
+        base += tot[k];
+    // rows in groups of 16: the loads of a group issue together before its stores
+    uint32_t *m = A.bin_mat[q];
+    for (uint32_t r0 = 0; r0 < kSortBlocks; r0 += 16) {
+        uint32_t c[16];
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++)
+            c[k] = m[static_cast<size_t>(r0 + k) * kSortBins + b];
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) {
+            m[static_cast<size_t>(r0 + k) * kSortBins + b] = base;
+            base += c[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSortWG) void k_wf_sort_scatter(WfArgs A) {
+    __shared__ uint32_t off[kSortBins];
+    const int q = blockIdx.y;
+    const uint32_t *row = A.bin_mat[q] + static_cast<size_t>(blockIdx.x) * kSortBins;
+    for (uint32_t b = threadIdx.x; b < kSortBins; b += kSortWG)
+        off[b] = row[b];
+    __syncthreads();
+    const uint32_t n = queue_count(A, q);
+    const uint32_t chunk = (n + kSortBlocks - 1) / kSortBlocks;
+    const uint32_t begin = blockIdx.x * chunk, end = min(begin + chunk, n);
+    const uint32_t *key = A.ray_key[q], *rank = A.ray_rank[q];
+    for (uint32_t i = begin + threadIdx.x; i < end; i += kSortWG) {
+        uint32_t dst = off[key[i]] + rank[i];
+        if (q == 0) {
+            A.sq_sorted[3 * dst] = A.sq[3 * i];
+            A.sq_sorted[3 * dst + 1] = A.sq[3 * i + 1];
+            A.sq_sorted[3 * dst + 2] = A.sq[3 * i + 2];
+        } else {
+            A.eq_sorted[dst] = A.eq[i];
+        }
     }
 }
 
@@ -345,7 +481,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     uint32_t ref = kDone, leaf = kDone;
     bool blocked = false;
     auto init = [&](uint32_t k) {
-        float4 a = A.sq[3 * k], b = A.sq[3 * k + 1];
+        float4 a = A.sq_in[3 * k], b = A.sq_in[3 * k + 1];
         o = xyz(a);
         tmax = a.w;
         v = xyz(b);
@@ -395,8 +531,8 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
         }
         if (ref == kDone) {
             if (!blocked) {
-                uint32_t path = __float_as_uint(A.sq[3 * i + 1].w);
-                float4 c = A.sq[3 * i + 2];
+                uint32_t path = __float_as_uint(A.sq_in[3 * i + 1].w);
+                float4 c = A.sq_in[3 * i + 2];
                 float4 L = A.L[path];
                 f3 nl = xyz(L) + xyz(c);
                 A.L[path] = make_float4(nl.x, nl.y, nl.z, L.w);
@@ -431,7 +567,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     ClosestHit h{-1, __builtin_inff(), 0.0f, 0.0f};
     uint32_t best_leaf = 0, ref = kDone, leaf = kDone;
     auto init = [&](uint32_t k) {
-        uint32_t path = A.eq[k];
+        uint32_t path = A.eq_in[k];
         o = xyz(A.xp[path]);
         v = xyz(A.wi[path]);
         vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
@@ -511,7 +647,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
             path = A.fin[t - ne];
             A.lbuf[path] = A.L[path];
         } else {
-            path = A.eq[t];
+            path = A.eq_in[t];
             ClosestHit h = A.eh[t];
             float4 L4 = A.L[path];
             f3 L = xyz(L4);
@@ -592,6 +728,7 @@ __global__ void k_wf_count(WfCtl *ctl, Counters *cnt) {
 // ---------------------------------------------------------------------------------------------
 struct WavefrontEngine {
     DevBuf xp, xn, wo, beta, L, wi, rng, lbuf, act0, act1, sq, eq, fin, eh, ctl, spill;
+    DevBuf sq_sorted, eq_sorted, ray_key, ray_rank, bin_mat, bin_tot; // ray reordering
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     WfArgs key{};
@@ -653,7 +790,7 @@ TraceKernels pick_trace(int wg, int ldsd) {
 bool same_config(WfArgs const &a, WfArgs const &b, WavefrontTuning const &ta, WavefrontTuning const &tb) {
     return std::memcmp(&a, &b, sizeof(WfArgs)) == 0 && ta.paths_in_flight == tb.paths_in_flight &&
            ta.trace_wg == tb.trace_wg && ta.stack_depth == tb.stack_depth &&
-           ta.sort_by_material == tb.sort_by_material;
+           ta.sort_by_material == tb.sort_by_material && ta.sort_rays == tb.sort_rays;
 }
 
 } // namespace
@@ -690,6 +827,14 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     WF_HIP(e->fin.ensure(sizeof(uint32_t) * cap));
     WF_HIP(e->eh.ensure(sizeof(ClosestHit) * cap));
     WF_HIP(e->ctl.ensure(sizeof(WfCtl)));
+    if (tune.sort_rays) {
+        WF_HIP(e->sq_sorted.ensure(3 * f4));
+        WF_HIP(e->eq_sorted.ensure(sizeof(uint32_t) * cap));
+        WF_HIP(e->ray_key.ensure(2 * sizeof(uint32_t) * static_cast<size_t>(cap)));
+        WF_HIP(e->ray_rank.ensure(2 * sizeof(uint32_t) * static_cast<size_t>(cap)));
+        WF_HIP(e->bin_mat.ensure(2 * sizeof(uint32_t) * static_cast<size_t>(kSortBlocks) * kSortBins));
+        WF_HIP(e->bin_tot.ensure(2 * sizeof(uint32_t) * kSortBins));
+    }
 
     TraceKernels tk = pick_trace(tune.trace_wg, tune.stack_depth);
     int wg = tk.wg;
@@ -734,6 +879,21 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     A.spill_per_thread = spill_per_thread;
     A.cnt = P.cnt;
     A.sort_by_material = tune.sort_by_material;
+    A.sort_rays = tune.sort_rays;
+    A.sq_in = A.sq;
+    A.eq_in = A.eq;
+    if (tune.sort_rays) {
+        A.sq_sorted = static_cast<float4 *>(e->sq_sorted.p);
+        A.eq_sorted = static_cast<uint32_t *>(e->eq_sorted.p);
+        A.sq_in = A.sq_sorted;
+        A.eq_in = A.eq_sorted;
+        for (int q = 0; q < 2; q++) {
+            A.ray_key[q] = static_cast<uint32_t *>(e->ray_key.p) + static_cast<size_t>(q) * cap;
+            A.ray_rank[q] = static_cast<uint32_t *>(e->ray_rank.p) + static_cast<size_t>(q) * cap;
+            A.bin_mat[q] = static_cast<uint32_t *>(e->bin_mat.p) + static_cast<size_t>(q) * kSortBlocks * kSortBins;
+            A.bin_tot[q] = static_cast<uint32_t *>(e->bin_tot.p) + static_cast<size_t>(q) * kSortBins;
+        }
+    }
 
     // Grid-strided per-path kernels get exactly the blocks that are resident at once: a larger
     // grid would run its surplus waves as a second, late round.
@@ -749,6 +909,12 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
             int par = static_cast<int>(k & 1u);
             hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, par);
             hipLaunchKernelGGL(k_wf_nee_bsdf, dim3(g_nee), dim3(256), 0, s, A, par);
+            if (A.sort_rays) {
+                hipLaunchKernelGGL(k_wf_sort_count, dim3(kSortBlocks, 2), dim3(kSortWG), 0, s, A);
+                hipLaunchKernelGGL(k_wf_sort_totals, dim3(kSortBins / 256, 2), dim3(256), 0, s, A);
+                hipLaunchKernelGGL(k_wf_sort_offsets, dim3(kSortBins / 256, 2), dim3(256), 0, s, A);
+                hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSortBlocks, 2), dim3(kSortWG), 0, s, A);
+            }
             if (profile)
                 (void)hipEventRecord(e->ev[0], s);
             hipLaunchKernelGGL(tk.shadow, dim3(g_shadow), dim3(wg), 0, s, A);

@@ -20,6 +20,11 @@ struct WavefrontTuning {
     int stack_depth = 8;                 // LDS stack entries per lane of the traversal kernels (8, 16)
     int sort_by_material = 0;            // group active paths by material kind before shading
     int sort_rays = 0;                   // reorder both ray queues by origin cell and direction
+    int refill_below = 8;                // traversal kernels leave the descent phase (to test
+                                         // leaves and refill finished lanes) once fewer than this
+                                         // many lanes of the wave still look for a leaf
+    int leaf_below = 8;                  // ... and the leaf phase once fewer than this many
+                                         // lanes still hold a leaf
     int use_graph = 1;                   // replay one generation's launches as a hipGraph
     int profile = 0;                     // time the traversal kernels with events (no graph)
 };

@@ -62,6 +62,8 @@ struct WfArgs {
     uint32_t spill_per_thread;
     Counters *cnt;
     int sort_by_material;
+    uint32_t refill_below; // traversal: leave the descent when fewer lanes still seek a leaf
+    uint32_t leaf_below;   // traversal: leave the leaf tests when fewer lanes still hold a leaf
     // Ray reordering: the traversal kernels (and shade) read sq_in / eq_in, which are either the
     // append-order queues or their copies sorted by ray key (queue 0 shadow, 1 extension).
     int sort_rays;
@@ -505,7 +507,8 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
                 leaf = ref;
                 ref = st.pop_next(__builtin_inff());
             }
-            if (__all(leaf != kDone))
+            // descend until fewer than refill_below lanes still look for their first leaf
+            if (__popcll(__ballot(leaf == kDone)) < A.refill_below)
                 break;
         }
         while (leaf != kDone) {
@@ -528,8 +531,11 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
             } else {
                 leaf = kDone;
             }
+            // test leaves until fewer than leaf_below lanes still hold one (the rest keep theirs)
+            if (__popcll(__ballot(leaf != kDone)) < A.leaf_below)
+                break;
         }
-        if (ref == kDone) {
+        if (ref == kDone && leaf == kDone) {
             if (!blocked) {
                 uint32_t path = __float_as_uint(A.sq_in[3 * i + 1].w);
                 float4 c = A.sq_in[3 * i + 2];
@@ -592,7 +598,8 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
                 leaf = ref;
                 ref = st.pop_next(cut);
             }
-            if (__all(leaf != kDone))
+            // descend until fewer than refill_below lanes still look for their first leaf
+            if (__popcll(__ballot(leaf == kDone)) < A.refill_below)
                 break;
         }
         while (leaf != kDone) {
@@ -616,8 +623,11 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
             } else {
                 leaf = kDone;
             }
+            // test leaves until fewer than leaf_below lanes still hold one (the rest keep theirs)
+            if (__popcll(__ballot(leaf != kDone)) < A.leaf_below)
+                break;
         }
-        if (ref == kDone) {
+        if (ref == kDone && leaf == kDone) {
             A.eh[i] = h;
             i = wave_fetch_batched(&A.ctl->eq_head.v, slot);
             if (i < n)
@@ -790,7 +800,8 @@ TraceKernels pick_trace(int wg, int ldsd) {
 bool same_config(WfArgs const &a, WfArgs const &b, WavefrontTuning const &ta, WavefrontTuning const &tb) {
     return std::memcmp(&a, &b, sizeof(WfArgs)) == 0 && ta.paths_in_flight == tb.paths_in_flight &&
            ta.trace_wg == tb.trace_wg && ta.stack_depth == tb.stack_depth &&
-           ta.sort_by_material == tb.sort_by_material && ta.sort_rays == tb.sort_rays;
+           ta.sort_by_material == tb.sort_by_material && ta.sort_rays == tb.sort_rays &&
+           ta.refill_below == tb.refill_below && ta.leaf_below == tb.leaf_below;
 }
 
 } // namespace
@@ -880,6 +891,8 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     A.cnt = P.cnt;
     A.sort_by_material = tune.sort_by_material;
     A.sort_rays = tune.sort_rays;
+    A.refill_below = static_cast<uint32_t>(tune.refill_below);
+    A.leaf_below = static_cast<uint32_t>(tune.leaf_below);
     A.sq_in = A.sq;
     A.eq_in = A.eq;
     if (tune.sort_rays) {

@@ -56,14 +56,20 @@ constexpr float kCullSlack = 1.0f / 4096.0f;
 // instructions, no compare-and-select pairs), and the reference's early exit after any axis
 // equals one final comparison, because the entry distance only grows and the exit distance only
 // shrinks from axis to axis. Values equal those of slab() (a zero's sign may differ, which no
-// comparison sees). *nan reports a NaN distance (0 * inf: a ray parallel to an axis whose origin
-// lies on a box plane); the caller then takes slab() itself.
+// comparison sees). A distance (b - o) * vinv of finite floats is never NaN (overflow gives
+// +-inf); with |b|, |o| <= 1e30 the difference is finite, so NaN needs a non-finite vinv: a zero
+// (or denormal) direction component, i.e. a ray parallel to an axis, whose origin lies on a box
+// plane (0 * inf). Rays for which that cannot be ruled out take slab() itself.
+X3D_DI bool slab_may_nan(f3 o, f3 vinv, bool far_coords) {
+    return far_coords || !(__builtin_isfinite(vinv.x) && __builtin_isfinite(vinv.y) && __builtin_isfinite(vinv.z)) ||
+           !(fabsf(o.x) <= 1e30f && fabsf(o.y) <= 1e30f && fabsf(o.z) <= 1e30f);
+}
+
 X3D_DI bool slab_fast(float mnx, float mxx, float mny, float mxy, float mnz, float mxz, f3 o, f3 vinv,
-                      float tmin, float tmax, float *t0, bool *nan) {
+                      float tmin, float tmax, float *t0) {
     float ax = (mnx - o.x) * vinv.x, bx = (mxx - o.x) * vinv.x;
     float ay = (mny - o.y) * vinv.y, by = (mxy - o.y) * vinv.y;
     float az = (mnz - o.z) * vinv.z, bz = (mxz - o.z) * vinv.z;
-    *nan = __builtin_isunordered(ax, bx) || __builtin_isunordered(ay, by) || __builtin_isunordered(az, bz);
     float lo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
     float hi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tmax));
     *t0 = lo;

@@ -421,10 +421,14 @@ std::string Scene::upload() {
     dev.max_stack = max_leaf_depth + 1;
     dev.num_prims = static_cast<uint32_t>(prims.size());
     dev.num_interior = num_interior;
+    dev.far_coords = 0;
     for (int a = 0; a < 3; a++) {
         float lo = flat.empty() ? 0.0f : flat[0].mn[a], hi = flat.empty() ? 0.0f : flat[0].mx[a];
         dev.box_lo[a] = lo;
         dev.box_inv[a] = hi > lo ? 1.0f / (hi - lo) : 0.0f;
+        // every node box lies inside the root box
+        if (!(std::fabs(lo) <= 1e30f && std::fabs(hi) <= 1e30f))
+            dev.far_coords = 1;
     }
     return "";
 }

@@ -76,6 +76,7 @@ struct DevScene {
     uint32_t num_interior;
     float box_lo[3];  // scene bounds (root box): lower corner
     float box_inv[3]; // 1 / extent per axis (0 for a flat axis)
+    uint32_t far_coords; // some box coordinate exceeds 1e30 in magnitude (slab_may_nan)
 };
 
 } // namespace x3d

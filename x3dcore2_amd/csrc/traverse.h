@@ -85,17 +85,18 @@ X3D_DI void load_node(DevScene const &S, uint32_t idx, float4 &f0, float4 &f1, f
 // (pathspace.cpp:371-381 with its LIFO stack).
 template <bool REF_ORDER, int WG, int LDS_DEPTH>
 X3D_DI uint32_t visit_interior(DevScene const &S, uint32_t ref, f3 o, f3 vinv, float tmin, float tmax, float cut,
-                               TravStack<WG, LDS_DEPTH> &st) {
+                               TravStack<WG, LDS_DEPTH> &st, bool may_nan) {
     float4 f0, f1, f2, f3n;
     load_node(S, ref, f0, f1, f2, f3n);
     float tl = 0.0f, tr = 0.0f;
-    bool nl, nr;
-    bool hl = slab_fast(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, tmin, tmax, &tl, &nl);
-    bool hr = slab_fast(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, tmin, tmax, &tr, &nr);
-    if (nl)
+    bool hl, hr;
+    if (!may_nan) {
+        hl = slab_fast(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, tmin, tmax, &tl);
+        hr = slab_fast(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, tmin, tmax, &tr);
+    } else {
         hl = slab(f0.x, f0.y, f0.z, f0.w, f2.x, f2.y, o, vinv, tmin, tmax, &tl);
-    if (nr)
         hr = slab(f1.x, f1.y, f1.z, f1.w, f2.z, f2.w, o, vinv, tmin, tmax, &tr);
+    }
     uint32_t rl = __float_as_uint(f3n.x), rr = __float_as_uint(f3n.y);
     if (!REF_ORDER) {
         hl = hl && !(tl > cut);
@@ -124,6 +125,7 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
     if (S.root_ref == kEmptyRef)
         return h;
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
+    const bool may_nan = slab_may_nan(o, vinv, S.far_coords != 0);
     const float4 *tris = static_cast<const float4 *>(S.tris);
     uint32_t best_leaf = 0;
     uint32_t ref = S.root_ref, leaf = kDone;
@@ -137,7 +139,7 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
         while (is_interior(ref)) {
             pops++;
             float cut = REF_ORDER ? __builtin_inff() : h.t + h.t * kCullSlack;
-            ref = visit_interior<REF_ORDER>(S, ref, o, vinv, kTMin, kTMax, cut, st);
+            ref = visit_interior<REF_ORDER>(S, ref, o, vinv, kTMin, kTMax, cut, st, may_nan);
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
                 ref = st.pop_next(cut);
@@ -182,6 +184,7 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
     if (S.root_ref == kEmptyRef)
         return false;
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
+    const bool may_nan = slab_may_nan(o, vinv, S.far_coords != 0);
     const float4 *tris = static_cast<const float4 *>(S.tris);
     uint32_t ref = S.root_ref, leaf = kDone;
     if (!is_interior(ref)) {
@@ -192,7 +195,7 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
     for (;;) {
         while (is_interior(ref)) {
             pops++;
-            ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st);
+            ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, may_nan);
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
                 ref = st.pop_next(__builtin_inff());

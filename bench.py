@@ -107,18 +107,20 @@ def timed_renders(x3d, r, scene, desc, film, steps, warmup, name):
     return time.perf_counter() - t0, stats
 
 
-def traffic_from_profile(name, kernel):
+def traffic_from_profile(name, kernel, config=None):
     """HBM bytes per launch recorded in profiles/<name> (copy-calibrated FETCH_SIZE + WRITE_SIZE),
-    used only when the summary was taken on the same kernel instance."""
+    used only when the summary was taken on the same kernel instance and launch configuration."""
     tpath = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(tpath):
         return None
     fields = {}
     for line in open(tpath):
-        if ":" in line and line.split(":", 1)[0] in ("kernel", "hbm_bytes_per_launch"):
+        if ":" in line and line.split(":", 1)[0] in ("kernel", "config", "hbm_bytes_per_launch"):
             k, v = line.split(":", 1)
             fields[k] = v.strip()
     if fields.get("kernel") != kernel or "hbm_bytes_per_launch" not in fields:
+        return None
+    if config is not None and fields.get("config") != config:
         return None
     return float(fields["hbm_bytes_per_launch"])
 
@@ -259,6 +261,7 @@ def main():
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the sorted-ray-queue ablation renders")
     ap.add_argument("--wf-config", default="", metavar="WG,DEPTH,SORT,CAP",
                     help="fixed wavefront configuration instead of autotuning (SORT: material|none)")
     ap.add_argument("--probe-spp", type=int, default=16,
@@ -374,9 +377,18 @@ def main():
         tests = (sp.closest_tests - sp.first_hit_tests) / launches
         launch_ms = sp.extend_ms / launches
         kernel = f"k_wf_extend<{wg or 64},{sd or 8}>"
-        traffic = traffic_from_profile(f"traffic_{desc.name}_wavefront.md", kernel)
+        launch_cfg = (f"{wg or 64},{sd or 8},{'material' if sort == 1 else 'none'},"
+                      f"{tuned.get('queue_capacity', 0) or 1 << 26}")
+        traffic = traffic_from_profile(f"traffic_{desc.name}_wavefront.md", kernel, launch_cfg)
         out["wavefront_kernel_ms"] = {"extend": sp.extend_ms, "shadow": sp.shadow_ms, "launches": sp.trace_launches,
                                       "all_sample_kernels": sp.trace_ms}
+    else:
+        nodes = (s0.closest_nodes - s0.first_hit_nodes) + s0.anyhit_nodes
+        tests = (s0.closest_tests - s0.first_hit_tests) + s0.anyhit_tests
+        launch_ms = float(np.mean([s.trace_ms for s in R["stats"]]))
+        kernel = "k_render_mis"
+        traffic = traffic_from_profile(f"traffic_{desc.name}_plain.md", kernel)
+    if best == "wavefront" and not args.no_ablation:
         # ablation: the same configuration with both ray queues sorted by key before traversal
         ra = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=depth, seed=1, workgroup_size=wg,
                           queue_capacity=tuned.get("queue_capacity", 0), sort_key=sort, stack_depth=sd,
@@ -389,12 +401,6 @@ def main():
         out["ray_order_ablation"] = {"append_order_ms_per_step": out["ms_per_step"], "sorted_ms_per_step": ea * 1e3,
                                      "sorted_extend_ms": spa.extend_ms, "sorted_shadow_ms": spa.shadow_ms}
         del ra
-    else:
-        nodes = (s0.closest_nodes - s0.first_hit_nodes) + s0.anyhit_nodes
-        tests = (s0.closest_tests - s0.first_hit_tests) + s0.anyhit_tests
-        launch_ms = float(np.mean([s.trace_ms for s in R["stats"]]))
-        kernel = "k_render_mis"
-        traffic = traffic_from_profile(f"traffic_{desc.name}_plain.md", kernel)
     alg_bytes = BYTES_PER_NODE * nodes + BYTES_PER_TEST * tests
     alg_flops = FLOPS_PER_NODE * nodes + FLOPS_PER_TEST * tests
     achieved_gbs = alg_bytes / (launch_ms * 1e-3) / 1e9
@@ -407,6 +413,8 @@ def main():
         "unit": "GB/s",
         "frac": achieved_gbs / HBM_PEAK_GBS,
         "traffic": traffic,
+        # real HBM bytes (copy-calibrated PMC) per launch over the launch time, against the peak
+        "hbm_traffic_frac": (traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
         "alg_bytes_per_launch": alg_bytes,
         "launch_ms": launch_ms,
         "valu_tflops": achieved_tflops,

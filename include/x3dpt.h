@@ -190,8 +190,9 @@ typedef struct x3d_render_options {
     uint32_t max_path_len; /* BIDIRECT_MIS: vertices per subpath (reference: 8) */
     uint64_t seed;         /* PCG seed; sample s of pixel p uses stream (seed, p, s) */
     uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default (64) */
-    uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default (16M), max 64M;
-                              BIDIRECT_MIS: (pixel, sample) items per generation (default 4M) */
+    uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default = max = 64M
+                              (216 B per path; a render allocates for min(capacity, w*h*spp));
+                              BIDIRECT_MIS: (pixel, sample) items per generation (default 16M) */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
     uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 8 */
     int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */
@@ -245,8 +246,8 @@ typedef struct x3d_render_stats {
  * workload with candidate wavefront configurations, keeps the fastest in the renderer's options
  * and reports it. Stage 1 tries traversal workgroup size {64,128,256} x LDS stack depth {8,16} x
  * sort key {material, none} at the default capacity; stage 2 tries queue capacities
- * {2M, 8M, 16M} paths for the stage-1 winner, skipping any above width*height*probe_spp (a
- * probe that small cannot tell them apart). Each candidate is timed as the best of two renders
+ * {2M, 4M, 8M, 16M, 32M} paths for the stage-1 winner, skipping any above width*height*probe_spp
+ * (a probe that small cannot tell them apart). Each candidate is timed as the best of two renders
  * after one warm-up render; the default configuration is kept unless the winner is at least 2%
  * faster. The film does not depend on the configuration. */
 typedef struct x3d_autotune_report {

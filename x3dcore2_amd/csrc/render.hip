@@ -314,7 +314,7 @@ __global__ __launch_bounds__(kWG) void k_render_mis(RenderParams P, uint32_t *ne
     }
 }
 
-constexpr uint64_t kGenItems = 1u << 22; // (pixel, sample) items per generation: 64 MiB of sample radiance
+constexpr uint64_t kGenItems = 1u << 24; // (pixel, sample) items per generation: 256 MiB of sample radiance
 
 // Bidirectional tracer. Work items are the (pixel, sample) pairs of one generation of samples,
 // pixel-major within a sample so neighbouring lanes start from neighbouring first hits. A lane
@@ -1434,7 +1434,7 @@ int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_cam
     const uint32_t wgs[] = {64, 128, 256};
     const uint32_t depths[] = {8, 16};
     const int32_t sorts[] = {X3D_SORT_MATERIAL, X3D_SORT_NONE};
-    const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
+    const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
     const uint64_t probe_paths = npix * probe_spp;
     x3d_autotune_report rep{};
     rep.best_ms = 1e300;

@@ -11,8 +11,8 @@ namespace x3d {
 
 struct WavefrontEngine;
 
-constexpr uint32_t kDefaultQueueCapacity = 1u << 24; // 16M paths: ~2.7 GB of path state and queues
-constexpr uint32_t kMaxQueueCapacity = 1u << 26; // 64M paths: ~10 GB of path state and queues
+constexpr uint32_t kDefaultQueueCapacity = 1u << 26; // 64M paths: ~13.5 GiB of path state and queues
+constexpr uint32_t kMaxQueueCapacity = 1u << 26;
 
 struct WavefrontTuning {
     uint32_t paths_in_flight = kDefaultQueueCapacity; // queue capacity (paths of one generation)

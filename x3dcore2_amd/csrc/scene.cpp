@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <cstring>
 
 namespace x3d {
@@ -225,7 +226,7 @@ inline float bits_f(uint32_t u) {
 } // namespace
 
 Scene::~Scene() {
-    void *ptrs[] = {buf.nodes, buf.tris, buf.tri_attr, buf.mats, buf.lights, buf.light_cum_power,
+    void *ptrs[] = {buf.nodes, buf.nodes4, buf.tris, buf.tri_attr, buf.mats, buf.lights, buf.light_cum_power,
                     buf.light_tris, buf.light_tri_cum};
     for (void *q : ptrs)
         if (q)
@@ -327,6 +328,101 @@ std::string Scene::upload() {
         q[2] = {l.mn[2], l.mx[2], r.mn[2], r.mx[2]};
         q[3] = {bits_f(ref_of(static_cast<uint32_t>(n + 1))), bits_f(ref_of(flat[n].next_child)), 0.0f, 0.0f};
     }
+    // 4-wide traversal tree over the same leaves: each 4-wide node takes a binary interior node
+    // and repeatedly replaces its interior child of largest surface area by that child's two
+    // children, up to four children. The child boxes are boxes of the binary tree, so a ray
+    // reaches exactly the leaves it reaches there (an inner box contains its descendants' boxes,
+    // and the slab test is monotone in the box for rays that meet no NaN).
+    // Record (8 x float4, 128 B): min x, max x, min y, max y, min z, max z of the four children,
+    // then the four child refs (interior: 4-wide node index; kDone4: no child).
+    std::vector<F4> nodes4;
+    uint32_t max_push4 = 0;
+    std::function<uint32_t(uint32_t, uint32_t)> collapse = [&](uint32_t n, uint32_t pushes) -> uint32_t {
+        FlatNode const &f = flat[n];
+        if (f.num_prims > 0) {
+            max_push4 = std::max(max_push4, pushes);
+            return leaf_ref(f.prim_start, f.num_prims);
+        }
+        uint32_t c[4] = {n + 1, f.next_child, 0, 0};
+        uint32_t k = 2;
+        while (k < 4) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (uint32_t i = 0; i < k; i++) {
+                FlatNode const &q = flat[c[i]];
+                if (q.num_prims > 0)
+                    continue;
+                float dx = q.mx[0] - q.mn[0], dy = q.mx[1] - q.mn[1], dz = q.mx[2] - q.mn[2];
+                float a = dx * dy + dy * dz + dz * dx;
+                if (a > best_area) {
+                    best_area = a;
+                    best = static_cast<int>(i);
+                }
+            }
+            if (best < 0)
+                break;
+            uint32_t m = c[best];
+            c[best] = m + 1;
+            c[k++] = flat[m].next_child;
+        }
+        uint32_t idx = static_cast<uint32_t>(nodes4.size() / 8);
+        nodes4.resize(nodes4.size() + 8);
+        uint32_t refs[4] = {kDone4, kDone4, kDone4, kDone4};
+        for (uint32_t i = 0; i < k; i++)
+            refs[i] = collapse(c[i], pushes + k - 1);
+        F4 *q = &nodes4[8 * static_cast<size_t>(idx)];
+        float v[6][4];
+        for (uint32_t i = 0; i < 4; i++)
+            for (int a = 0; a < 3; a++) {
+                v[2 * a][i] = i < k ? flat[c[i]].mn[a] : 0.0f;
+                v[2 * a + 1][i] = i < k ? flat[c[i]].mx[a] : 0.0f;
+            }
+        for (int r = 0; r < 6; r++)
+            q[r] = {v[r][0], v[r][1], v[r][2], v[r][3]};
+        q[6] = {bits_f(refs[0]), bits_f(refs[1]), bits_f(refs[2]), bits_f(refs[3])};
+        q[7] = {0.0f, 0.0f, 0.0f, 0.0f};
+        return idx;
+    };
+    uint32_t root4 = flat.empty() ? kEmptyRef : collapse(0, 0);
+    num_nodes4 = static_cast<uint32_t>(nodes4.size() / 8);
+    max_stack4 = max_push4 + 1;
+    {
+        // Consistency of the 4-wide tree before anything traverses it: refs in range, every
+        // binary leaf reached exactly once, every node reached from the root.
+        std::vector<uint8_t> seen_leaf(prims.size(), 0), seen_node(num_nodes4, 0);
+        size_t leaves = 0, binary_leaves = 0;
+        for (auto const &f : flat)
+            binary_leaves += f.num_prims > 0 ? 1 : 0;
+        std::vector<uint32_t> todo;
+        if (root4 != kEmptyRef)
+            todo.push_back(root4);
+        while (!todo.empty()) {
+            uint32_t r = todo.back();
+            todo.pop_back();
+            if (r & kLeafBit) {
+                uint32_t start = r & 0x0FFFFFFFu, cnt = ((r >> 28) & 3u) + 1u;
+                if (start + cnt > prims.size() || seen_leaf[start])
+                    return "internal error: 4-wide tree leaf out of range or repeated";
+                seen_leaf[start] = 1;
+                leaves++;
+                continue;
+            }
+            if (r >= num_nodes4 || seen_node[r])
+                return "internal error: 4-wide tree node out of range or repeated";
+            seen_node[r] = 1;
+            F4 const &q = nodes4[8 * static_cast<size_t>(r) + 6];
+            const float fr[4] = {q.x, q.y, q.z, q.w};
+            for (float f : fr) {
+                uint32_t c;
+                std::memcpy(&c, &f, 4);
+                if (c != kDone4)
+                    todo.push_back(c);
+            }
+        }
+        if (leaves != binary_leaves || std::count(seen_node.begin(), seen_node.end(), 1) != num_nodes4)
+            return "internal error: 4-wide tree does not cover the binary tree";
+    }
+
     std::vector<F4> tris(3 * prims.size()), attr(3 * prims.size());
     for (size_t i = 0; i < prims.size(); i++) {
         HostMesh const &m = meshes[prims[i].mesh];
@@ -397,6 +493,7 @@ std::string Scene::upload() {
     std::string e;
     buf.bytes = 0;
     if (!(e = device_copy(&buf.nodes, nodes, &buf.bytes)).empty() ||
+        !(e = device_copy(&buf.nodes4, nodes4, &buf.bytes)).empty() ||
         !(e = device_copy(&buf.tris, tris, &buf.bytes)).empty() ||
         !(e = device_copy(&buf.tri_attr, attr, &buf.bytes)).empty() ||
         !(e = device_copy(&buf.mats, materials, &buf.bytes)).empty() ||
@@ -407,6 +504,8 @@ std::string Scene::upload() {
         return e;
 
     dev.nodes = buf.nodes;
+    dev.nodes4 = buf.nodes4;
+    dev.root4 = root4;
     dev.tris = buf.tris;
     dev.tri_attr = buf.tri_attr;
     dev.mats = static_cast<const DevMaterial *>(buf.mats);
@@ -418,7 +517,7 @@ std::string Scene::upload() {
     dev.num_lights = static_cast<uint32_t>(lights.size());
     dev.num_mats = static_cast<uint32_t>(materials.size());
     dev.total_power = total_power;
-    dev.max_stack = max_leaf_depth + 1;
+    dev.max_stack = std::max(max_leaf_depth + 1, max_stack4);
     dev.num_prims = static_cast<uint32_t>(prims.size());
     dev.num_interior = num_interior;
     dev.far_coords = 0;

@@ -36,8 +36,8 @@ struct PrimRef {
 };
 
 struct DeviceBuffers {
-    void *nodes = nullptr, *tris = nullptr, *tri_attr = nullptr, *mats = nullptr, *lights = nullptr,
-         *light_cum_power = nullptr, *light_tris = nullptr, *light_tri_cum = nullptr;
+    void *nodes = nullptr, *nodes4 = nullptr, *tris = nullptr, *tri_attr = nullptr, *mats = nullptr,
+         *lights = nullptr, *light_cum_power = nullptr, *light_tris = nullptr, *light_tri_cum = nullptr;
     uint64_t bytes = 0;
 };
 
@@ -55,6 +55,7 @@ class Scene {
     std::vector<PrimRef> prims;
     uint32_t max_leaf_depth = 0;
     uint32_t num_interior = 0;
+    uint32_t num_nodes4 = 0, max_stack4 = 0; // 4-wide traversal tree (see upload())
     double build_seconds = 0, upload_seconds = 0;
     DevScene dev{};
     DeviceBuffers buf;

@@ -117,6 +117,55 @@ X3D_DI uint32_t visit_interior(DevScene const &S, uint32_t ref, f3 o, f3 vinv, f
     return st.pop_next(REF_ORDER ? __builtin_inff() : cut);
 }
 
+// Visits 4-wide node `ref` (a ray that cannot meet NaN in the slab test): tests the four child
+// boxes, drops children entering beyond `cut`, pushes the hit children but the nearest, farthest
+// first, and returns the nearest (or the next stacked node, or kDone). Children with equal entry
+// distances may come in either order; the order never changes a result.
+template <int WG, int LDS_DEPTH>
+X3D_DI uint32_t visit_interior4(DevScene const &S, uint32_t ref, f3 o, f3 vinv, float tmin, float tmax, float cut,
+                                TravStack<WG, LDS_DEPTH> &st) {
+    const float4 *n = static_cast<const float4 *>(S.nodes4) + 8u * ref;
+    const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5], rf = n[6];
+    float t[4];
+    uint32_t r[4];
+    auto child = [&](int i, float a, float b, float c, float d, float e, float f, float refbits) {
+        float tt;
+        bool h = slab_fast(a, b, c, d, e, f, o, vinv, tmin, tmax, &tt);
+        uint32_t rr = __float_as_uint(refbits);
+        h = h && rr != kDone4 && !(tt > cut);
+        t[i] = h ? tt : __builtin_inff();
+        r[i] = h ? rr : kDone;
+    };
+    child(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, rf.x);
+    child(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, rf.y);
+    child(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, rf.z);
+    child(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, rf.w);
+    // sorting network, ascending entry distance (misses carry +inf and sort last)
+    auto cswap = [&](int a, int b) {
+        bool sw = t[b] < t[a];
+        float ta = sw ? t[b] : t[a], tb = sw ? t[a] : t[b];
+        uint32_t ra = sw ? r[b] : r[a], rb = sw ? r[a] : r[b];
+        t[a] = ta;
+        t[b] = tb;
+        r[a] = ra;
+        r[b] = rb;
+    };
+    cswap(0, 1);
+    cswap(2, 3);
+    cswap(0, 2);
+    cswap(1, 3);
+    cswap(1, 2);
+    if (r[3] != kDone)
+        st.push(r[3], t[3]);
+    if (r[2] != kDone)
+        st.push(r[2], t[2]);
+    if (r[1] != kDone)
+        st.push(r[1], t[1]);
+    if (r[0] != kDone)
+        return r[0];
+    return st.pop_next(cut);
+}
+
 // REF_ORDER: the reference's traversal order without culling.
 template <int WG, int LDS_DEPTH, bool REF_ORDER>
 X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_DEPTH> &st,
@@ -125,10 +174,11 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
     if (S.root_ref == kEmptyRef)
         return h;
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
-    const bool may_nan = slab_may_nan(o, vinv, S.far_coords != 0);
+    // the 4-wide tree, unless the ray may meet NaN or the reference order is asked for
+    const bool wide = !REF_ORDER && !slab_may_nan(o, vinv, S.far_coords != 0);
     const float4 *tris = static_cast<const float4 *>(S.tris);
     uint32_t best_leaf = 0;
-    uint32_t ref = S.root_ref, leaf = kDone;
+    uint32_t ref = wide ? S.root4 : S.root_ref, leaf = kDone;
     if (!is_interior(ref)) {
         leaf = ref;
         ref = kDone;
@@ -137,9 +187,14 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
     for (;;) {
         // Descend; park the first leaf met and keep going until all active lanes hold one.
         while (is_interior(ref)) {
-            pops++;
             float cut = REF_ORDER ? __builtin_inff() : h.t + h.t * kCullSlack;
-            ref = visit_interior<REF_ORDER>(S, ref, o, vinv, kTMin, kTMax, cut, st, may_nan);
+            if (wide) {
+                pops += 2; // a 4-wide node is counted as two binary nodes (same bytes and tests)
+                ref = visit_interior4(S, ref, o, vinv, kTMin, kTMax, cut, st);
+            } else {
+                pops++;
+                ref = visit_interior<REF_ORDER>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);
+            }
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
                 ref = st.pop_next(cut);
@@ -184,9 +239,9 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
     if (S.root_ref == kEmptyRef)
         return false;
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
-    const bool may_nan = slab_may_nan(o, vinv, S.far_coords != 0);
+    const bool wide = !slab_may_nan(o, vinv, S.far_coords != 0);
     const float4 *tris = static_cast<const float4 *>(S.tris);
-    uint32_t ref = S.root_ref, leaf = kDone;
+    uint32_t ref = wide ? S.root4 : S.root_ref, leaf = kDone;
     if (!is_interior(ref)) {
         leaf = ref;
         ref = kDone;
@@ -194,8 +249,13 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
     st.sp = 0;
     for (;;) {
         while (is_interior(ref)) {
-            pops++;
-            ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, may_nan);
+            if (wide) {
+                pops += 2;
+                ref = visit_interior4(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st);
+            } else {
+                pops++;
+                ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, true);
+            }
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
                 ref = st.pop_next(__builtin_inff());

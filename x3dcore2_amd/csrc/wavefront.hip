@@ -481,17 +481,17 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
     float tmax = 0.0f;
     uint32_t ref = kDone, leaf = kDone;
-    bool blocked = false, may_nan = false;
+    bool blocked = false, wide = false;
     auto init = [&](uint32_t k) {
         float4 a = A.sq_in[3 * k], b = A.sq_in[3 * k + 1];
         o = xyz(a);
         tmax = a.w;
         v = xyz(b);
         vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
-        may_nan = slab_may_nan(o, vinv, S.far_coords != 0);
+        wide = !slab_may_nan(o, vinv, S.far_coords != 0);
         blocked = false;
         st.sp = 0;
-        ref = S.root_ref;
+        ref = wide ? S.root4 : S.root_ref;
         leaf = kDone;
         if (ref != kEmptyRef && !is_interior(ref)) {
             leaf = ref;
@@ -502,8 +502,13 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
         init(i);
     while (i < n) {
         while (is_interior(ref)) {
-            pops++;
-            ref = visit_interior<false>(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st, may_nan);
+            if (wide) {
+                pops += 2; // a 4-wide node is counted as two binary nodes (same bytes and tests)
+                ref = visit_interior4(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st);
+            } else {
+                pops++;
+                ref = visit_interior<false>(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st, true);
+            }
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
                 ref = st.pop_next(__builtin_inff());
@@ -573,17 +578,17 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
     ClosestHit h{-1, __builtin_inff(), 0.0f, 0.0f};
     uint32_t best_leaf = 0, ref = kDone, leaf = kDone;
-    bool may_nan = false;
+    bool wide = false;
     auto init = [&](uint32_t k) {
         uint32_t path = A.eq_in[k];
         o = xyz(A.xp[path]);
         v = xyz(A.wi[path]);
         vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
-        may_nan = slab_may_nan(o, vinv, S.far_coords != 0);
+        wide = !slab_may_nan(o, vinv, S.far_coords != 0);
         h = ClosestHit{-1, __builtin_inff(), 0.0f, 0.0f};
         best_leaf = 0;
         st.sp = 0;
-        ref = S.root_ref;
+        ref = wide ? S.root4 : S.root_ref;
         leaf = kDone;
         if (ref != kEmptyRef && !is_interior(ref)) {
             leaf = ref;
@@ -594,9 +599,14 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
         init(i);
     while (i < n) {
         while (is_interior(ref)) {
-            pops++;
             float cut = h.t + h.t * kCullSlack;
-            ref = visit_interior<false>(S, ref, o, vinv, kTMin, kTMax, cut, st, may_nan);
+            if (wide) {
+                pops += 2;
+                ref = visit_interior4(S, ref, o, vinv, kTMin, kTMax, cut, st);
+            } else {
+                pops++;
+                ref = visit_interior<false>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);
+            }
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
                 ref = st.pop_next(cut);

@@ -328,13 +328,14 @@ std::string Scene::upload() {
         q[2] = {l.mn[2], l.mx[2], r.mn[2], r.mx[2]};
         q[3] = {bits_f(ref_of(static_cast<uint32_t>(n + 1))), bits_f(ref_of(flat[n].next_child)), 0.0f, 0.0f};
     }
-    // 4-wide traversal tree over the same leaves: each 4-wide node takes a binary interior node
-    // and repeatedly replaces its interior child of largest surface area by that child's two
-    // children, up to four children. The child boxes are boxes of the binary tree, so a ray
+    // Wide traversal tree over the same leaves: each wide node takes a binary interior node and
+    // repeatedly replaces its interior child of largest surface area by that child's two
+    // children, up to kTreeWidth children. The child boxes are boxes of the binary tree, so a ray
     // reaches exactly the leaves it reaches there (an inner box contains its descendants' boxes,
     // and the slab test is monotone in the box for rays that meet no NaN).
-    // Record (8 x float4, 128 B): min x, max x, min y, max y, min z, max z of the four children,
-    // then the four child refs (interior: 4-wide node index; kDone4: no child).
+    // Record (kWideRows x float4): for each plane (min x, max x, min y, max y, min z, max z) the
+    // children in groups of four, then the child refs in groups of four (interior: wide node
+    // index; kDone4: no child), padding.
     std::vector<F4> nodes4;
     uint32_t max_push4 = 0;
     std::function<uint32_t(uint32_t, uint32_t)> collapse = [&](uint32_t n, uint32_t pushes) -> uint32_t {
@@ -343,9 +344,9 @@ std::string Scene::upload() {
             max_push4 = std::max(max_push4, pushes);
             return leaf_ref(f.prim_start, f.num_prims);
         }
-        uint32_t c[4] = {n + 1, f.next_child, 0, 0};
+        uint32_t c[kTreeWidth] = {n + 1, f.next_child};
         uint32_t k = 2;
-        while (k < 4) {
+        while (k < kTreeWidth) {
             int best = -1;
             float best_area = -1.0f;
             for (uint32_t i = 0; i < k; i++) {
@@ -365,29 +366,35 @@ std::string Scene::upload() {
             c[best] = m + 1;
             c[k++] = flat[m].next_child;
         }
-        uint32_t idx = static_cast<uint32_t>(nodes4.size() / 8);
-        nodes4.resize(nodes4.size() + 8);
-        uint32_t refs[4] = {kDone4, kDone4, kDone4, kDone4};
+        uint32_t idx = static_cast<uint32_t>(nodes4.size() / kWideRows);
+        nodes4.resize(nodes4.size() + kWideRows);
+        uint32_t refs[kTreeWidth];
+        for (uint32_t i = 0; i < kTreeWidth; i++)
+            refs[i] = kDone4;
         for (uint32_t i = 0; i < k; i++)
             refs[i] = collapse(c[i], pushes + k - 1);
-        F4 *q = &nodes4[8 * static_cast<size_t>(idx)];
-        float v[6][4];
-        for (uint32_t i = 0; i < 4; i++)
+        F4 *q = &nodes4[kWideRows * static_cast<size_t>(idx)];
+        for (uint32_t g = 0; g < kWideGroups; g++) {
             for (int a = 0; a < 3; a++) {
-                v[2 * a][i] = i < k ? flat[c[i]].mn[a] : 0.0f;
-                v[2 * a + 1][i] = i < k ? flat[c[i]].mx[a] : 0.0f;
+                float lo[4], hi[4];
+                for (uint32_t j = 0; j < 4; j++) {
+                    uint32_t i = 4 * g + j;
+                    lo[j] = i < k ? flat[c[i]].mn[a] : 0.0f;
+                    hi[j] = i < k ? flat[c[i]].mx[a] : 0.0f;
+                }
+                q[(2 * a) * kWideGroups + g] = {lo[0], lo[1], lo[2], lo[3]};
+                q[(2 * a + 1) * kWideGroups + g] = {hi[0], hi[1], hi[2], hi[3]};
             }
-        for (int r = 0; r < 6; r++)
-            q[r] = {v[r][0], v[r][1], v[r][2], v[r][3]};
-        q[6] = {bits_f(refs[0]), bits_f(refs[1]), bits_f(refs[2]), bits_f(refs[3])};
-        q[7] = {0.0f, 0.0f, 0.0f, 0.0f};
+            q[6 * kWideGroups + g] = {bits_f(refs[4 * g]), bits_f(refs[4 * g + 1]), bits_f(refs[4 * g + 2]),
+                                      bits_f(refs[4 * g + 3])};
+        }
         return idx;
     };
     uint32_t root4 = flat.empty() ? kEmptyRef : collapse(0, 0);
-    num_nodes4 = static_cast<uint32_t>(nodes4.size() / 8);
+    num_nodes4 = static_cast<uint32_t>(nodes4.size() / kWideRows);
     max_stack4 = max_push4 + 1;
     {
-        // Consistency of the 4-wide tree before anything traverses it: refs in range, every
+        // Consistency of the wide tree before anything traverses it: refs in range, every
         // binary leaf reached exactly once, every node reached from the root.
         std::vector<uint8_t> seen_leaf(prims.size(), 0), seen_node(num_nodes4, 0);
         size_t leaves = 0, binary_leaves = 0;
@@ -402,25 +409,27 @@ std::string Scene::upload() {
             if (r & kLeafBit) {
                 uint32_t start = r & 0x0FFFFFFFu, cnt = ((r >> 28) & 3u) + 1u;
                 if (start + cnt > prims.size() || seen_leaf[start])
-                    return "internal error: 4-wide tree leaf out of range or repeated";
+                    return "internal error: wide tree leaf out of range or repeated";
                 seen_leaf[start] = 1;
                 leaves++;
                 continue;
             }
             if (r >= num_nodes4 || seen_node[r])
-                return "internal error: 4-wide tree node out of range or repeated";
+                return "internal error: wide tree node out of range or repeated";
             seen_node[r] = 1;
-            F4 const &q = nodes4[8 * static_cast<size_t>(r) + 6];
-            const float fr[4] = {q.x, q.y, q.z, q.w};
-            for (float f : fr) {
-                uint32_t c;
-                std::memcpy(&c, &f, 4);
-                if (c != kDone4)
-                    todo.push_back(c);
+            for (uint32_t g = 0; g < kWideGroups; g++) {
+                F4 const &q = nodes4[kWideRows * static_cast<size_t>(r) + 6 * kWideGroups + g];
+                const float fr[4] = {q.x, q.y, q.z, q.w};
+                for (float f : fr) {
+                    uint32_t c;
+                    std::memcpy(&c, &f, 4);
+                    if (c != kDone4)
+                        todo.push_back(c);
+                }
             }
         }
         if (leaves != binary_leaves || std::count(seen_node.begin(), seen_node.end(), 1) != num_nodes4)
-            return "internal error: 4-wide tree does not cover the binary tree";
+            return "internal error: wide tree does not cover the binary tree";
     }
 
     std::vector<F4> tris(3 * prims.size()), attr(3 * prims.size());

@@ -7,9 +7,9 @@
 //             f2 = (c0.min.z, c0.max.z, c1.min.z, c1.max.z)
 //             f3 = (ref c0, ref c1, -, -) as uint32 bits; c0 = left child (flattened n + 1),
 //             c1 = right child (flattened next_child) of the reference layout.
-//   nodes4    8 x float4 per 4-wide node (128 B): the same tree with every other level folded
-//             in: (min x)[4], (max x)[4], (min y)[4], (max y)[4], (min z)[4], (max z)[4] of the
-//             four children, their refs (kDone4 = empty slot), padding.
+//   nodes4    kWideRows x float4 per wide node (128 B for 4 children): the same tree with
+//             levels folded in: (min x)[W], (max x)[W], (min y)[W], (max y)[W], (min z)[W],
+//             (max z)[W] of the children, their refs (kDone4 = empty slot), padding.
 //   tris      3 x float4 per primitive, in BVH primitive order: v0 (w = mesh id bits),
 //             v1 (w = triangle index bits), v2 (w = 0).
 //   tri_attr  3 x float4 per primitive: n0 (w = material id bits), n1 (w = light id bits or
@@ -60,11 +60,18 @@ inline constexpr uint32_t leaf_ref(uint32_t start, uint32_t count) {
     return kLeafBit | ((count - 1u) << 28) | start;
 }
 
-constexpr uint32_t kDone4 = 0xFFFFFFFFu; // empty child slot of a 4-wide node
+constexpr uint32_t kDone4 = 0xFFFFFFFFu; // empty child slot of a wide node
+// Children per node of the wide traversal tree, and float4 rows per node: for each of the six
+// planes (min x, max x, min y, max y, min z, max z) W/4 rows of four children, then W/4 rows of
+// child refs, padded to a power of two (128 B for W = 4).
+constexpr uint32_t kTreeWidth = 4;
+constexpr uint32_t kWideGroups = kTreeWidth / 4;
+constexpr uint32_t kWideRows = kTreeWidth == 4 ? 8 : 16;
+static_assert(kTreeWidth == 4 || kTreeWidth == 8, "wide tree of 4 or 8 children");
 
 struct DevScene {
     const void *nodes;      // float4[4 * num_interior]
-    const void *nodes4;     // float4[8 * 4-wide nodes]: the 4-wide traversal tree
+    const void *nodes4;     // float4[kWideRows * wide nodes]: the wide traversal tree
     const void *tris;       // float4[3 * num_prims]
     const void *tri_attr;   // float4[3 * num_prims]
     const DevMaterial *mats;
@@ -73,7 +80,7 @@ struct DevScene {
     const void *light_tris;       // float4[6 * total emitter triangles]
     const float *light_tri_cum;   // cumulative area per emitter triangle (per light)
     uint32_t root_ref;
-    uint32_t root4;         // root of the 4-wide tree (a leaf ref when the root is a leaf)
+    uint32_t root4;         // root of the wide tree (a leaf ref when the root is a leaf)
     uint32_t num_lights;
     uint32_t num_mats;
     uint32_t fail_safe_mat; // index of the fail-safe Lambertian 0.8

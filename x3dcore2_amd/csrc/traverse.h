@@ -117,50 +117,69 @@ X3D_DI uint32_t visit_interior(DevScene const &S, uint32_t ref, f3 o, f3 vinv, f
     return st.pop_next(REF_ORDER ? __builtin_inff() : cut);
 }
 
-// Visits 4-wide node `ref` (a ray that cannot meet NaN in the slab test): tests the four child
-// boxes, drops children entering beyond `cut`, pushes the hit children but the nearest, farthest
+// Compare-exchange of two (entry distance, ref) slots, ascending distance.
+X3D_DI void cswap(float *t, uint32_t *r, int a, int b) {
+    bool sw = t[b] < t[a];
+    float ta = sw ? t[b] : t[a], tb = sw ? t[a] : t[b];
+    uint32_t ra = sw ? r[b] : r[a], rb = sw ? r[a] : r[b];
+    t[a] = ta;
+    t[b] = tb;
+    r[a] = ra;
+    r[b] = rb;
+}
+
+// Visits wide node `ref` (a ray that cannot meet NaN in the slab test): tests the child boxes,
+// drops children entering beyond `cut`, pushes the hit children but the nearest, farthest
 // first, and returns the nearest (or the next stacked node, or kDone). Children with equal entry
 // distances may come in either order; the order never changes a result.
 template <int WG, int LDS_DEPTH>
-X3D_DI uint32_t visit_interior4(DevScene const &S, uint32_t ref, f3 o, f3 vinv, float tmin, float tmax, float cut,
-                                TravStack<WG, LDS_DEPTH> &st) {
-    const float4 *n = static_cast<const float4 *>(S.nodes4) + 8u * ref;
-    const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5], rf = n[6];
-    float t[4];
-    uint32_t r[4];
-    auto child = [&](int i, float a, float b, float c, float d, float e, float f, float refbits) {
-        float tt;
-        bool h = slab_fast(a, b, c, d, e, f, o, vinv, tmin, tmax, &tt);
-        uint32_t rr = __float_as_uint(refbits);
-        h = h && rr != kDone4 && !(tt > cut);
-        t[i] = h ? tt : __builtin_inff();
-        r[i] = h ? rr : kDone;
-    };
-    child(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, rf.x);
-    child(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, rf.y);
-    child(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, rf.z);
-    child(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, rf.w);
-    // sorting network, ascending entry distance (misses carry +inf and sort last)
-    auto cswap = [&](int a, int b) {
-        bool sw = t[b] < t[a];
-        float ta = sw ? t[b] : t[a], tb = sw ? t[a] : t[b];
-        uint32_t ra = sw ? r[b] : r[a], rb = sw ? r[a] : r[b];
-        t[a] = ta;
-        t[b] = tb;
-        r[a] = ra;
-        r[b] = rb;
-    };
-    cswap(0, 1);
-    cswap(2, 3);
-    cswap(0, 2);
-    cswap(1, 3);
-    cswap(1, 2);
-    if (r[3] != kDone)
-        st.push(r[3], t[3]);
-    if (r[2] != kDone)
-        st.push(r[2], t[2]);
-    if (r[1] != kDone)
-        st.push(r[1], t[1]);
+X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, float tmin, float tmax, float cut,
+                           TravStack<WG, LDS_DEPTH> &st) {
+    const float4 *n = static_cast<const float4 *>(S.nodes4) + kWideRows * ref;
+    float4 row[7 * kWideGroups];
+#pragma unroll
+    for (uint32_t i = 0; i < 7 * kWideGroups; i++)
+        row[i] = n[i];
+    float t[kTreeWidth];
+    uint32_t r[kTreeWidth];
+#pragma unroll
+    for (uint32_t g = 0; g < kWideGroups; g++) {
+        const float4 *p = row + g;
+        const float lx[4] = {p[0].x, p[0].y, p[0].z, p[0].w};
+        const float hx[4] = {p[kWideGroups].x, p[kWideGroups].y, p[kWideGroups].z, p[kWideGroups].w};
+        const float ly[4] = {p[2 * kWideGroups].x, p[2 * kWideGroups].y, p[2 * kWideGroups].z, p[2 * kWideGroups].w};
+        const float hy[4] = {p[3 * kWideGroups].x, p[3 * kWideGroups].y, p[3 * kWideGroups].z, p[3 * kWideGroups].w};
+        const float lz[4] = {p[4 * kWideGroups].x, p[4 * kWideGroups].y, p[4 * kWideGroups].z, p[4 * kWideGroups].w};
+        const float hz[4] = {p[5 * kWideGroups].x, p[5 * kWideGroups].y, p[5 * kWideGroups].z, p[5 * kWideGroups].w};
+        const float rf[4] = {p[6 * kWideGroups].x, p[6 * kWideGroups].y, p[6 * kWideGroups].z, p[6 * kWideGroups].w};
+#pragma unroll
+        for (uint32_t j = 0; j < 4; j++) {
+            float tt;
+            bool h = slab_fast(lx[j], hx[j], ly[j], hy[j], lz[j], hz[j], o, vinv, tmin, tmax, &tt);
+            uint32_t rr = __float_as_uint(rf[j]);
+            h = h && rr != kDone4 && !(tt > cut);
+            t[4 * g + j] = h ? tt : __builtin_inff(); // misses sort last
+            r[4 * g + j] = h ? rr : kDone;
+        }
+    }
+    // sorting network, ascending entry distance
+    if (kTreeWidth == 4) {
+        cswap(t, r, 0, 1);
+        cswap(t, r, 2, 3);
+        cswap(t, r, 0, 2);
+        cswap(t, r, 1, 3);
+        cswap(t, r, 1, 2);
+    } else {
+        constexpr int kNet[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 1}, {2, 3},
+                                     {4, 5}, {6, 7}, {2, 4}, {3, 5}, {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+        for (int c = 0; c < 19; c++)
+            cswap(t, r, kNet[c][0], kNet[c][1]);
+    }
+#pragma unroll
+    for (int i = kTreeWidth - 1; i > 0; i--)
+        if (r[i] != kDone)
+            st.push(r[i], t[i]);
     if (r[0] != kDone)
         return r[0];
     return st.pop_next(cut);
@@ -174,7 +193,7 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
     if (S.root_ref == kEmptyRef)
         return h;
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
-    // the 4-wide tree, unless the ray may meet NaN or the reference order is asked for
+    // the wide tree, unless the ray may meet NaN or the reference order is asked for
     const bool wide = !REF_ORDER && !slab_may_nan(o, vinv, S.far_coords != 0);
     const float4 *tris = static_cast<const float4 *>(S.tris);
     uint32_t best_leaf = 0;
@@ -189,8 +208,8 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
         while (is_interior(ref)) {
             float cut = REF_ORDER ? __builtin_inff() : h.t + h.t * kCullSlack;
             if (wide) {
-                pops += 2; // a 4-wide node is counted as two binary nodes (same bytes and tests)
-                ref = visit_interior4(S, ref, o, vinv, kTMin, kTMax, cut, st);
+                pops += kTreeWidth / 2; // a wide node counts as W/2 binary nodes (same bytes and tests)
+                ref = visit_wide(S, ref, o, vinv, kTMin, kTMax, cut, st);
             } else {
                 pops++;
                 ref = visit_interior<REF_ORDER>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);
@@ -250,8 +269,8 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
     for (;;) {
         while (is_interior(ref)) {
             if (wide) {
-                pops += 2;
-                ref = visit_interior4(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st);
+                pops += kTreeWidth / 2;
+                ref = visit_wide(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, true);

@@ -503,8 +503,8 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     while (i < n) {
         while (is_interior(ref)) {
             if (wide) {
-                pops += 2; // a 4-wide node is counted as two binary nodes (same bytes and tests)
-                ref = visit_interior4(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st);
+                pops += kTreeWidth / 2; // a wide node counts as W/2 binary nodes (same bytes and tests)
+                ref = visit_wide(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st, true);
@@ -601,8 +601,8 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
         while (is_interior(ref)) {
             float cut = h.t + h.t * kCullSlack;
             if (wide) {
-                pops += 2;
-                ref = visit_interior4(S, ref, o, vinv, kTMin, kTMax, cut, st);
+                pops += kTreeWidth / 2;
+                ref = visit_wide(S, ref, o, vinv, kTMin, kTMax, cut, st);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);

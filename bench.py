@@ -376,8 +376,8 @@ def main():
         nodes = (sp.closest_nodes - sp.first_hit_nodes) / launches
         tests = (sp.closest_tests - sp.first_hit_tests) / launches
         launch_ms = sp.extend_ms / launches
-        kernel = f"k_wf_extend<{wg or 64},{sd or 8}>"
-        launch_cfg = (f"{wg or 64},{sd or 8},{'material' if sort == 1 else 'none'},"
+        kernel = f"k_wf_extend<{wg or 64},{sd or 16}>"
+        launch_cfg = (f"{wg or 64},{sd or 16},{'material' if sort == 1 else 'none'},"
                       f"{tuned.get('queue_capacity', 0) or 1 << 26}")
         traffic = traffic_from_profile(f"traffic_{desc.name}_wavefront.md", kernel, launch_cfg)
         out["wavefront_kernel_ms"] = {"extend": sp.extend_ms, "shadow": sp.shadow_ms, "launches": sp.trace_launches,

@@ -194,7 +194,7 @@ typedef struct x3d_render_options {
                               (216 B per path; a render allocates for min(capacity, w*h*spp));
                               BIDIRECT_MIS: (pixel, sample) items per generation (default 16M) */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
-    uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 8 */
+    uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16 */
     int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */
     uint32_t batch_split;    /* batches: jobs of at least this many samples (w*h*spp) run in the
                               wavefront form after the shared launch of the others; 0 = default

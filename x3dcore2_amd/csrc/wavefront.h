@@ -17,7 +17,7 @@ constexpr uint32_t kMaxQueueCapacity = 1u << 26;
 struct WavefrontTuning {
     uint32_t paths_in_flight = kDefaultQueueCapacity; // queue capacity (paths of one generation)
     int trace_wg = 64;                   // workgroup size of the traversal kernels (64, 128, 256)
-    int stack_depth = 8;                 // LDS stack entries per lane of the traversal kernels (8, 16)
+    int stack_depth = 16;                // LDS stack entries per lane of the traversal kernels (8, 16)
     int sort_by_material = 0;            // group active paths by material kind before shading
     int sort_rays = 0;                   // reorder both ray queues by origin cell and direction
     int refill_below = 8;                // traversal kernels leave the descent phase (to test

@@ -61,7 +61,10 @@ constexpr float kCullSlack = 1.0f / 4096.0f;
 // (or denormal) direction component, i.e. a ray parallel to an axis, whose origin lies on a box
 // plane (0 * inf). Rays for which that cannot be ruled out take slab() itself.
 X3D_DI bool slab_may_nan(f3 o, f3 vinv, bool far_coords) {
+    // (a zero vinv, from an infinite direction component, is excluded as well: it would turn
+    // the infinite planes of an empty wide-node slot into NaN)
     return far_coords || !(__builtin_isfinite(vinv.x) && __builtin_isfinite(vinv.y) && __builtin_isfinite(vinv.z)) ||
+           vinv.x == 0.0f || vinv.y == 0.0f || vinv.z == 0.0f ||
            !(fabsf(o.x) <= 1e30f && fabsf(o.y) <= 1e30f && fabsf(o.z) <= 1e30f);
 }
 

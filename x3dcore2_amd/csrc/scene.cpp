@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <functional>
 #include <cstring>
 
@@ -379,8 +380,9 @@ std::string Scene::upload() {
                 float lo[4], hi[4];
                 for (uint32_t j = 0; j < 4; j++) {
                     uint32_t i = 4 * g + j;
-                    lo[j] = i < k ? flat[c[i]].mn[a] : 0.0f;
-                    hi[j] = i < k ? flat[c[i]].mx[a] : 0.0f;
+                    // an empty slot gets the box [+inf, +inf]: every wide-tree ray misses it
+                    lo[j] = i < k ? flat[c[i]].mn[a] : std::numeric_limits<float>::infinity();
+                    hi[j] = i < k ? flat[c[i]].mx[a] : std::numeric_limits<float>::infinity();
                 }
                 q[(2 * a) * kWideGroups + g] = {lo[0], lo[1], lo[2], lo[3]};
                 q[(2 * a + 1) * kWideGroups + g] = {hi[0], hi[1], hi[2], hi[3]};
@@ -393,6 +395,48 @@ std::string Scene::upload() {
     uint32_t root4 = flat.empty() ? kEmptyRef : collapse(0, 0);
     num_nodes4 = static_cast<uint32_t>(nodes4.size() / kWideRows);
     max_stack4 = max_push4 + 1;
+    if (kTopNodes > 0 && root4 != kEmptyRef && !(root4 & kLeafBit)) {
+        // The first kTopNodes nodes in breadth-first order (the top of the tree, which traversal
+        // kernels may keep in LDS) take indices 0.., the rest keep their depth-first order.
+        std::vector<uint32_t> order, renum(num_nodes4, kDone4);
+        order.push_back(root4);
+        renum[root4] = 0;
+        for (size_t h = 0; h < order.size() && order.size() < kTopNodes; h++)
+            for (uint32_t g = 0; g < kWideGroups; g++) {
+                F4 const &q = nodes4[kWideRows * static_cast<size_t>(order[h]) + 6 * kWideGroups + g];
+                const float fr[4] = {q.x, q.y, q.z, q.w};
+                for (float f : fr) {
+                    uint32_t c;
+                    std::memcpy(&c, &f, 4);
+                    if (c != kDone4 && !(c & kLeafBit) && order.size() < kTopNodes && renum[c] == kDone4) {
+                        renum[c] = static_cast<uint32_t>(order.size());
+                        order.push_back(c);
+                    }
+                }
+            }
+        uint32_t next = static_cast<uint32_t>(order.size());
+        for (uint32_t o = 0; o < num_nodes4; o++)
+            if (renum[o] == kDone4)
+                renum[o] = next++;
+        std::vector<F4> moved(nodes4.size());
+        for (uint32_t o = 0; o < num_nodes4; o++) {
+            F4 *dst = &moved[kWideRows * static_cast<size_t>(renum[o])];
+            std::copy(&nodes4[kWideRows * static_cast<size_t>(o)], &nodes4[kWideRows * static_cast<size_t>(o)] + kWideRows,
+                      dst);
+            for (uint32_t g = 0; g < kWideGroups; g++) {
+                F4 &q = dst[6 * kWideGroups + g];
+                float *fr[4] = {&q.x, &q.y, &q.z, &q.w};
+                for (float *f : fr) {
+                    uint32_t c;
+                    std::memcpy(&c, f, 4);
+                    if (c != kDone4 && !(c & kLeafBit))
+                        *f = bits_f(renum[c]);
+                }
+            }
+        }
+        nodes4.swap(moved);
+        root4 = renum[root4];
+    }
     {
         // Consistency of the wide tree before anything traverses it: refs in range, every
         // binary leaf reached exactly once, every node reached from the root.
@@ -515,6 +559,7 @@ std::string Scene::upload() {
     dev.nodes = buf.nodes;
     dev.nodes4 = buf.nodes4;
     dev.root4 = root4;
+    dev.num_nodes4 = num_nodes4;
     dev.tris = buf.tris;
     dev.tri_attr = buf.tri_attr;
     dev.mats = static_cast<const DevMaterial *>(buf.mats);

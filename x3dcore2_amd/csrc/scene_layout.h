@@ -9,7 +9,8 @@
 //             c1 = right child (flattened next_child) of the reference layout.
 //   nodes4    kWideRows x float4 per wide node (128 B for 4 children): the same tree with
 //             levels folded in: (min x)[W], (max x)[W], (min y)[W], (max y)[W], (min z)[W],
-//             (max z)[W] of the children, their refs (kDone4 = empty slot), padding.
+//             (max z)[W] of the children, their refs (kDone4 = empty slot, whose box is
+//             [+inf, +inf] on every axis), padding.
 //   tris      3 x float4 per primitive, in BVH primitive order: v0 (w = mesh id bits),
 //             v1 (w = triangle index bits), v2 (w = 0).
 //   tri_attr  3 x float4 per primitive: n0 (w = material id bits), n1 (w = light id bits or
@@ -68,6 +69,12 @@ constexpr uint32_t kTreeWidth = 4;
 constexpr uint32_t kWideGroups = kTreeWidth / 4;
 constexpr uint32_t kWideRows = kTreeWidth == 4 ? 8 : 16;
 static_assert(kTreeWidth == 4 || kTreeWidth == 8, "wide tree of 4 or 8 children");
+// Wide nodes numbered first in breadth-first order (the top of the tree), which the wavefront
+// traversal kernels copy into LDS per workgroup; 0 = none. Staging measured slower on the
+// spheres workload (16 nodes: extend +12%, 32 nodes: +24%): the extra LDS per workgroup costs
+// resident waves, the top nodes hit in L1 / L2 anyway, and lanes at different depths execute
+// both load paths. The mechanism stays for scenes where it may pay.
+constexpr uint32_t kTopNodes = 0;
 
 struct DevScene {
     const void *nodes;      // float4[4 * num_interior]
@@ -81,6 +88,7 @@ struct DevScene {
     const float *light_tri_cum;   // cumulative area per emitter triangle (per light)
     uint32_t root_ref;
     uint32_t root4;         // root of the wide tree (a leaf ref when the root is a leaf)
+    uint32_t num_nodes4;    // wide nodes
     uint32_t num_lights;
     uint32_t num_mats;
     uint32_t fail_safe_mat; // index of the fail-safe Lambertian 0.8

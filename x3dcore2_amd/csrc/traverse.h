@@ -131,15 +131,24 @@ X3D_DI void cswap(float *t, uint32_t *r, int a, int b) {
 // Visits wide node `ref` (a ray that cannot meet NaN in the slab test): tests the child boxes,
 // drops children entering beyond `cut`, pushes the hit children but the nearest, farthest
 // first, and returns the nearest (or the next stacked node, or kDone). Children with equal entry
-// distances may come in either order; the order never changes a result.
-template <int WG, int LDS_DEPTH>
+// distances may come in either order; the order never changes a result. Unordered (any-hit
+// queries, which stop at the first hit anywhere on the segment): the first hit child is
+// returned and the others pushed as they come, without the sort.
+// `top` (LDS) holds the first kTopNodes nodes when a kernel staged them, else null.
+template <bool ORDERED = true, int WG, int LDS_DEPTH>
 X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, float tmin, float tmax, float cut,
-                           TravStack<WG, LDS_DEPTH> &st) {
-    const float4 *n = static_cast<const float4 *>(S.nodes4) + kWideRows * ref;
+                           TravStack<WG, LDS_DEPTH> &st, const float4 *top = nullptr) {
     float4 row[7 * kWideGroups];
+    if (kTopNodes > 0 && top && ref < kTopNodes) {
 #pragma unroll
-    for (uint32_t i = 0; i < 7 * kWideGroups; i++)
-        row[i] = n[i];
+        for (uint32_t i = 0; i < 7 * kWideGroups; i++)
+            row[i] = top[kWideRows * ref + i];
+    } else {
+        const float4 *n = static_cast<const float4 *>(S.nodes4) + kWideRows * ref;
+#pragma unroll
+        for (uint32_t i = 0; i < 7 * kWideGroups; i++)
+            row[i] = n[i];
+    }
     float t[kTreeWidth];
     uint32_t r[kTreeWidth];
 #pragma unroll
@@ -156,11 +165,24 @@ X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, float
         for (uint32_t j = 0; j < 4; j++) {
             float tt;
             bool h = slab_fast(lx[j], hx[j], ly[j], hy[j], lz[j], hz[j], o, vinv, tmin, tmax, &tt);
+            // an empty slot's box [+inf, +inf] always misses (the ray's vinv is finite, nonzero)
             uint32_t rr = __float_as_uint(rf[j]);
-            h = h && rr != kDone4 && !(tt > cut);
+            h = h && !(tt > cut);
             t[4 * g + j] = h ? tt : __builtin_inff(); // misses sort last
             r[4 * g + j] = h ? rr : kDone;
         }
+    }
+    if (!ORDERED) {
+        uint32_t next = kDone;
+#pragma unroll
+        for (uint32_t i = 0; i < kTreeWidth; i++)
+            if (r[i] != kDone) {
+                if (next == kDone)
+                    next = r[i];
+                else
+                    st.push(r[i], t[i]);
+            }
+        return next != kDone ? next : st.pop_next(cut);
     }
     // sorting network, ascending entry distance
     if (kTreeWidth == 4) {
@@ -270,7 +292,7 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
         while (is_interior(ref)) {
             if (wide) {
                 pops += kTreeWidth / 2;
-                ref = visit_wide(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st);
+                ref = visit_wide<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, true);

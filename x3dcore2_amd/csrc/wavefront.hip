@@ -467,8 +467,15 @@ template <int WG, int LDSD>
 __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     __shared__ uint2 lds[LDSD * WG];
     __shared__ uint32_t cursor[2 * (WG / 64)];
+    __shared__ float4 top[kTopNodes > 0 ? kTopNodes * kWideRows : 1];
     TravStack<WG, LDSD> st = make_stack<WG, LDSD>(lds, A.spill, A.spill_per_thread);
     DevScene const &S = A.S;
+    if (kTopNodes > 0) {
+        const uint32_t nt = min(kTopNodes, S.num_nodes4) * kWideRows;
+        for (uint32_t q = threadIdx.x; q < nt; q += WG)
+            top[q] = static_cast<const float4 *>(S.nodes4)[q];
+        __syncthreads();
+    }
     const float4 *tris = static_cast<const float4 *>(S.tris);
     const uint32_t n = A.ctl->sq_count.v;
     uint32_t pops = 0, tests = 0;
@@ -504,7 +511,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
         while (is_interior(ref)) {
             if (wide) {
                 pops += kTreeWidth / 2; // a wide node counts as W/2 binary nodes (same bytes and tests)
-                ref = visit_wide(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st);
+                ref = visit_wide<false>(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st, top);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st, true);
@@ -564,8 +571,15 @@ template <int WG, int LDSD>
 __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     __shared__ uint2 lds[LDSD * WG];
     __shared__ uint32_t cursor[2 * (WG / 64)];
+    __shared__ float4 top[kTopNodes > 0 ? kTopNodes * kWideRows : 1];
     TravStack<WG, LDSD> st = make_stack<WG, LDSD>(lds, A.spill, A.spill_per_thread);
     DevScene const &S = A.S;
+    if (kTopNodes > 0) {
+        const uint32_t nt = min(kTopNodes, S.num_nodes4) * kWideRows;
+        for (uint32_t q = threadIdx.x; q < nt; q += WG)
+            top[q] = static_cast<const float4 *>(S.nodes4)[q];
+        __syncthreads();
+    }
     const float4 *tris = static_cast<const float4 *>(S.tris);
     const uint32_t n = A.ctl->eq_count.v;
     uint32_t pops = 0, tests = 0;
@@ -602,7 +616,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
             float cut = h.t + h.t * kCullSlack;
             if (wide) {
                 pops += kTreeWidth / 2;
-                ref = visit_wide(S, ref, o, vinv, kTMin, kTMax, cut, st);
+                ref = visit_wide(S, ref, o, vinv, kTMin, kTMax, cut, st, top);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);

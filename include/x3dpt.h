@@ -71,6 +71,29 @@ int x3d_scene_add_mesh(x3d_scene *scene, const float *verts, const float *normal
                        uint32_t num_verts, const uint32_t *tris, uint32_t num_tris,
                        int32_t material_id, int32_t *out_mesh_id);
 
+/* ---- Wavefront OBJ meshes (replaces e8util::wavefront_obj::load_geometry,
+ * resource.cpp:137-255). Lines "v x y z", "vn x y z", "vt u v" and triangle faces
+ * "f a/b/c a/b/c a/b/c" (1-based position / texcoord / normal indices; texcoord and normal may
+ * be empty) are read with the reference's pattern rules; other lines are skipped. Vertices keep
+ * their file order and take the normal and texcoord of the last face corner naming them. A face
+ * with more than three corners, a corner without exactly three '/'-separated fields, a missing
+ * or out-of-range index, a field that is not a number, or a file without vertices returns
+ * X3D_ERR_INVALID_ARGUMENT with the line in x3d_last_error(). ---- */
+typedef struct x3d_obj_mesh {
+    uint32_t num_verts, num_tris;
+    float *verts;    /* float[3 * num_verts] */
+    float *normals;  /* float[3 * num_verts]; zero where no face corner gives one */
+    float *uvs;      /* float[2 * num_verts]; zero where no face corner gives one */
+    uint32_t *tris;  /* uint32[3 * num_tris], 0-based */
+} x3d_obj_mesh;
+
+int x3d_obj_load(const char *path, x3d_obj_mesh *mesh); /* arrays owned by the library */
+int x3d_obj_free(x3d_obj_mesh *mesh);
+
+/* x3d_obj_load + x3d_scene_add_mesh: the reference's load_geometry() + path_space::load() with
+ * the identity transform, as in tst_renderer.cpp:58-102. */
+int x3d_scene_add_obj(x3d_scene *scene, const char *path, int32_t material_id, int32_t *out_mesh_id);
+
 /* Makes mesh `mesh_id` an area light of the given radiance (replaces area_light,
  * light.cpp:17-85, loaded into basic_light_sources, lightsources.cpp:11-36). */
 int x3d_scene_add_area_light(x3d_scene *scene, int32_t mesh_id, const float radiance[3],

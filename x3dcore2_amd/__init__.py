@@ -57,6 +57,11 @@ HIT_DTYPE = np.dtype([("geo", np.int32), ("tri", np.int32), ("prim", np.int32), 
 assert HIT_DTYPE.itemsize == C.sizeof(Hit)
 
 
+class ObjMesh(C.Structure):
+    _fields_ = [("num_verts", C.c_uint32), ("num_tris", C.c_uint32), ("verts", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)), ("tris", C.POINTER(C.c_uint32))]
+
+
 class CameraDesc(C.Structure):
     _fields_ = [("sensor_size", C.c_float), ("focal_length", C.c_float), ("aspect", C.c_float),
                 ("transform", C.c_float * 16)]
@@ -108,6 +113,7 @@ EXPORTS = [
     "x3d_renderer_create", "x3d_renderer_destroy", "x3d_render", "x3d_render_device", "x3d_pcg_draws",
     "x3d_renderer_autotune", "x3d_render_batch", "x3d_render_batch_device", "x3d_renderer_autotune_batch",
     "x3d_scene_set_material_params", "x3d_render_forward", "x3d_render_backward", "x3d_render_backward_device",
+    "x3d_obj_load", "x3d_obj_free", "x3d_scene_add_obj",
 ]
 
 
@@ -132,6 +138,9 @@ def lib() -> C.CDLL:
     L.x3d_scene_destroy.argtypes = [vp]
     L.x3d_scene_add_material.argtypes = [vp, C.POINTER(MaterialDesc), C.POINTER(i32)]
     L.x3d_scene_add_mesh.argtypes = [vp, vp, vp, vp, u32, vp, u32, i32, C.POINTER(i32)]
+    L.x3d_obj_load.argtypes = [C.c_char_p, C.POINTER(ObjMesh)]
+    L.x3d_obj_free.argtypes = [C.POINTER(ObjMesh)]
+    L.x3d_scene_add_obj.argtypes = [vp, C.c_char_p, i32, C.POINTER(i32)]
     L.x3d_scene_add_area_light.argtypes = [vp, i32, fp, C.POINTER(i32)]
     L.x3d_scene_commit.argtypes = [vp]
     L.x3d_scene_get_info.argtypes = [vp, C.POINTER(SceneInfo)]
@@ -225,6 +234,12 @@ class Scene:
         out = C.c_int32()
         _check(lib().x3d_scene_add_mesh(self.h, _ptr(v), _ptr(n), None, v.shape[0], _ptr(t), t.shape[0],
                                         int(material), C.byref(out)))
+        return out.value
+
+    def add_obj(self, path, material=-1) -> int:
+        """x3d_scene_add_obj: a Wavefront OBJ mesh with the reference loader's rules."""
+        out = C.c_int32()
+        _check(lib().x3d_scene_add_obj(self.h, os.fsencode(path), int(material), C.byref(out)))
         return out.value
 
     def add_area_light(self, mesh_id, radiance) -> int:
@@ -391,6 +406,20 @@ class Renderer:
         _check(lib().x3d_renderer_autotune(self.h, scene.h, C.byref(camera_desc(camera)), width, height, probe_spp,
                                            C.byref(rep)))
         return rep
+
+
+def load_obj(path):
+    """x3d_obj_load: (verts [n,3], normals [n,3], uvs [n,2], tris [m,3]) of a Wavefront OBJ mesh."""
+    m = ObjMesh()
+    _check(lib().x3d_obj_load(os.fsencode(path), C.byref(m)))
+    try:
+        nv, nt = m.num_verts, m.num_tris
+        out = (np.ctypeslib.as_array(m.verts, (nv, 3)).copy(), np.ctypeslib.as_array(m.normals, (nv, 3)).copy(),
+               np.ctypeslib.as_array(m.uvs, (nv, 2)).copy(),
+               np.ctypeslib.as_array(m.tris, (max(nt, 1), 3))[:nt].copy())
+    finally:
+        lib().x3d_obj_free(C.byref(m))
+    return out
 
 
 def camera_rays(camera, width, height) -> np.ndarray:

@@ -41,9 +41,11 @@ def build_library(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     gxx = shutil.which("g++") or "g++"
-    scene_o = os.path.join(BUILD, "scene.o")
-    cmds = [[gxx, *GXX_FLAGS, "-c", os.path.join(CSRC, "scene.cpp"), "-o", scene_o]]
-    objs = [scene_o]
+    objs, cmds = [], []
+    for unit in ("scene", "objfile"):
+        o = os.path.join(BUILD, unit + ".o")
+        cmds.append([gxx, *GXX_FLAGS, "-c", os.path.join(CSRC, unit + ".cpp"), "-o", o])
+        objs.append(o)
     for unit in ("render", "wavefront", "batch", "adjoint"):
         o = os.path.join(BUILD, unit + ".o")
         cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, unit + ".hip"), "-o", o])

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "../../include/x3dpt.h"
+#include "objfile.h"
 #include "render_common.h"
 #include "scene.h"
 #include "adjoint.h"
@@ -987,6 +989,58 @@ int x3d_scene_add_mesh(x3d_scene *scene, const float *verts, const float *normal
     if (out_mesh_id)
         *out_mesh_id = static_cast<int32_t>(scene->s.meshes.size()) - 1;
     return X3D_OK;
+}
+
+int x3d_obj_load(const char *path, x3d_obj_mesh *mesh) {
+    if (!path || !mesh)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    *mesh = x3d_obj_mesh{};
+    x3d::ObjMesh m;
+    std::string e = x3d::load_obj(path, m);
+    if (!e.empty())
+        return fail(X3D_ERR_INVALID_ARGUMENT, std::string(path) + ": " + e);
+    const size_t nv = m.verts.size() / 3, nt = m.tris.size() / 3;
+    if (nv > 0xFFFFFFFFu || nt > 0xFFFFFFFFu)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "mesh too large");
+    mesh->num_verts = static_cast<uint32_t>(nv);
+    mesh->num_tris = static_cast<uint32_t>(nt);
+    mesh->verts = new (std::nothrow) float[3 * nv];
+    mesh->normals = new (std::nothrow) float[3 * nv];
+    mesh->uvs = new (std::nothrow) float[2 * nv];
+    mesh->tris = new (std::nothrow) uint32_t[3 * nt + 1];
+    if (!mesh->verts || !mesh->normals || !mesh->uvs || !mesh->tris) {
+        x3d_obj_free(mesh);
+        return fail(X3D_ERR_OUT_OF_MEMORY, "allocation failed");
+    }
+    std::copy(m.verts.begin(), m.verts.end(), mesh->verts);
+    std::copy(m.normals.begin(), m.normals.end(), mesh->normals);
+    std::copy(m.uvs.begin(), m.uvs.end(), mesh->uvs);
+    std::copy(m.tris.begin(), m.tris.end(), mesh->tris);
+    return X3D_OK;
+}
+
+int x3d_obj_free(x3d_obj_mesh *mesh) {
+    if (!mesh)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    delete[] mesh->verts;
+    delete[] mesh->normals;
+    delete[] mesh->uvs;
+    delete[] mesh->tris;
+    *mesh = x3d_obj_mesh{};
+    return X3D_OK;
+}
+
+int x3d_scene_add_obj(x3d_scene *scene, const char *path, int32_t material_id, int32_t *out_mesh_id) {
+    if (!scene || !path)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    x3d_obj_mesh m{};
+    int rc = x3d_obj_load(path, &m);
+    if (rc)
+        return rc;
+    rc = x3d_scene_add_mesh(scene, m.verts, m.normals, m.uvs, m.num_verts, m.tris, m.num_tris, material_id,
+                            out_mesh_id);
+    x3d_obj_free(&m);
+    return rc;
 }
 
 int x3d_scene_add_area_light(x3d_scene *scene, int32_t mesh_id, const float radiance[3], int32_t *out_light_id) {

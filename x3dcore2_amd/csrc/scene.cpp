@@ -206,7 +206,7 @@ struct Builder {
 };
 
 template <class T> std::string device_copy(void **dst, std::vector<T> const &v, uint64_t *bytes) {
-    size_t n = std::max<size_t>(v.size() * sizeof(T), 16);
+    size_t n = std::max<size_t>(v.size() * sizeof(T), 256); // never empty: kernels may read element 0
     if (hipMalloc(dst, n) != hipSuccess)
         return "hipMalloc failed (" + std::to_string(n) + " bytes)";
     if (!v.empty() && hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)

@@ -333,12 +333,8 @@ struct HitRec {
     int32_t geo, mat, light;
 };
 
-X3D_DI HitRec finish_hit(DevScene const &S, ClosestHit const &h) {
-    const float4 *tris = static_cast<const float4 *>(S.tris);
-    const float4 *attr = static_cast<const float4 *>(S.tri_attr);
-    uint32_t i = static_cast<uint32_t>(h.prim);
-    float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
-    float4 na = attr[3 * i], nb = attr[3 * i + 1], nc = attr[3 * i + 2];
+// Hit record from the hit triangle's positions (a, b, c) and shading attributes (na, nb, nc).
+X3D_DI HitRec finish_hit(ClosestHit const &h, float4 a, float4 b, float4 c, float4 na, float4 nb, float4 nc) {
     float b0 = 1.0f - h.b1 - h.b2;
     HitRec r;
     r.p = b0 * xyz(a) + h.b1 * xyz(b) + h.b2 * xyz(c);
@@ -348,6 +344,13 @@ X3D_DI HitRec finish_hit(DevScene const &S, ClosestHit const &h) {
     r.mat = static_cast<int32_t>(__float_as_uint(na.w));
     r.light = static_cast<int32_t>(__float_as_uint(nb.w));
     return r;
+}
+
+X3D_DI HitRec finish_hit(DevScene const &S, ClosestHit const &h) {
+    const float4 *tris = static_cast<const float4 *>(S.tris);
+    const float4 *attr = static_cast<const float4 *>(S.tri_attr);
+    uint32_t i = static_cast<uint32_t>(h.prim);
+    return finish_hit(h, tris[3 * i], tris[3 * i + 1], tris[3 * i + 2], attr[3 * i], attr[3 * i + 1], attr[3 * i + 2]);
 }
 
 } // namespace x3d

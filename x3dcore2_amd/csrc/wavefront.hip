@@ -686,22 +686,29 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
         } else {
             path = A.eq_in[t];
             ClosestHit h = A.eh[t];
-            float4 L4 = A.L[path];
+            // Every load of the item issues at once: the path's state and the hit triangle (a
+            // miss reads triangle 0, which every scene with a possible hit has), so one memory
+            // round trip instead of three.
+            const uint32_t pi = h.prim >= 0 ? static_cast<uint32_t>(h.prim) : 0u;
+            const float4 *tris = static_cast<const float4 *>(A.S.tris);
+            const float4 *attr = static_cast<const float4 *>(A.S.tri_attr);
+            float4 ta = tris[3 * pi], tb = tris[3 * pi + 1], tc = tris[3 * pi + 2];
+            float4 na = attr[3 * pi], nb = attr[3 * pi + 1], nc = attr[3 * pi + 2];
+            float4 L4 = A.L[path], wi4 = A.wi[path], xn = A.xn[path], wo4 = A.wo[path], b4 = A.beta[path];
+            float4 xp4 = A.xp[path];
             f3 L = xyz(L4);
-            float4 wi4 = A.wi[path];
             f3 wi = xyz(wi4);
             HitRec nx;
             bool hit = h.prim >= 0;
             if (hit) {
-                nx = finish_hit(A.S, h);
+                nx = finish_hit(h, ta, tb, tc, na, nb, nc);
                 hit = !(dot(nx.n, -wi) <= 0.0f);
             }
             if (!hit) {
                 A.lbuf[path] = L4;
             } else {
                 float dens = wi4.w;
-                float4 xn = A.xn[path], wo4 = A.wo[path], b4 = A.beta[path];
-                int32_t xmat = static_cast<int32_t>(__float_as_uint(A.xp[path].w));
+                int32_t xmat = static_cast<int32_t>(__float_as_uint(xp4.w));
                 f3 xnrm = xyz(xn), wo = xyz(wo4), beta = xyz(b4);
                 float cw = dot(xnrm, wi);
                 f3 f = bsdf_eval(A.S.mats, xmat, xnrm, wo, wi);

@@ -288,7 +288,11 @@ int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_cam
 
 /* Renders the film: rgba float[4*w*h], pixel (x, y) at index x + y*w, row 0 = top; rgb is the
  * mean radiance over spp samples, alpha = 1/spp (renderer.cpp:270-295 with one task).
- * firefly_filter must be 0 (the filter is not implemented). stats may be NULL. */
+ * firefly_filter != 0 applies sampling_task::run's filter (renderer.cpp:46-83) to every sample
+ * image before the per-pixel sum: an interior pixel whose squared norm exceeds each of its 8
+ * neighbours' by more than 1.5 is capped, per component, at their mean squared norm + 1.5. Every
+ * tracer then keeps whole sample images in device memory (generations of at most queue_capacity
+ * (pixel, sample) items: 16M by default, 64M in the wavefront form). stats may be NULL. */
 int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
                uint32_t height, uint32_t spp, int32_t firefly_filter, float *rgba,
                x3d_render_stats *stats);

@@ -112,6 +112,9 @@ class Oracle(_Backend):
                                                 C.c_int]
         lib.orc_render_backward.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
                                             C.c_uint32, _f32p, C.c_int, C.c_uint32, _f64p]
+        lib.orc_firefly_accumulate.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
+        lib.orc_render_firefly.argtypes = [C.c_void_p, _f32p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint64,
+                                           C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
         self.h = C.c_void_p(lib.orc_scene_new())
         self.desc = desc
         if desc is not None:
@@ -124,6 +127,23 @@ class Oracle(_Backend):
         count = a.shape[0] if a is not None else r.shape[0]
         self.lib.orc_set_material_params(self.h, first, count, a.ctypes.data if a is not None else None,
                                          r.ctypes.data if r is not None else None, int(update_sampling))
+
+    def firefly_accumulate(self, samples):
+        """samples: [spp, h, w, 3] float32 -> per-pixel in-order sums [h, w, 3] of the
+        firefly-filtered samples."""
+        x = np.ascontiguousarray(samples, np.float32)
+        spp, h, w, _ = x.shape
+        out = np.zeros((h, w, 3), np.float32)
+        self.lib.orc_firefly_accumulate(x, w, h, spp, out)
+        return out
+
+    def render_firefly(self, tracer, width, height, spp, seed=1, max_depth=5, max_path_len=8, threads=None):
+        """orc_render (FP32) with the firefly filter: rgba [h*w, 4] float32."""
+        cam = self.desc.camera.packed()
+        out = np.zeros((width * height, 4), np.float32)
+        self.lib.orc_render_firefly(self.h, cam, tracer, max_depth, max_path_len, seed, width, height, spp,
+                                    threads or os.cpu_count() or 1, out)
+        return out
 
     def render_backward(self, width, height, spp, dl_dfilm, seed=1, max_depth=5, prec=1, threads=None,
                         camera=None):

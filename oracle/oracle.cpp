@@ -1585,6 +1585,88 @@ void orc_render(void *h, float const *cam, int tracer, uint32_t max_depth, uint3
                 stats[i] += ts[i];
 }
 
+// Firefly filter of sampling_task::run (renderer.cpp:46-83) over sample images followed by the
+// per-pixel in-order sum: samples = float[spp][h][w][3]; sum = float[h][w][3]. For an interior
+// pixel, the sample is capped component by component at the mean of its 8 neighbours' squared
+// norms plus FireFlyMinDiff = 1.5 (renderer.h:118) when its own squared norm exceeds every
+// neighbour's by more than 1.5; edge pixels are added unfiltered.
+void orc_firefly_accumulate(float const *samples, uint32_t w, uint32_t h, uint32_t spp, float *sum) {
+    const float kMinDiff = 1.5f;
+    for (uint32_t y = 0; y < h; y++)
+        for (uint32_t x = 0; x < w; x++) {
+            V3f acc;
+            for (uint32_t k = 0; k < spp; k++) {
+                float const *img = samples + static_cast<size_t>(k) * w * h * 3;
+                auto at = [&](uint32_t i, uint32_t j) { return ld3(img + 3 * (i + j * w)); };
+                V3f c = at(x, y);
+                if (x > 0 && x < w - 1 && y > 0 && y < h - 1) {
+                    float r00 = inner(at(x - 1, y - 1), at(x - 1, y - 1)), r10 = inner(at(x, y - 1), at(x, y - 1));
+                    float r20 = inner(at(x + 1, y - 1), at(x + 1, y - 1)), r01 = inner(at(x - 1, y), at(x - 1, y));
+                    float r11 = inner(c, c), r21 = inner(at(x + 1, y), at(x + 1, y));
+                    float r02 = inner(at(x - 1, y + 1), at(x - 1, y + 1)), r12 = inner(at(x, y + 1), at(x, y + 1));
+                    float r22 = inner(at(x + 1, y + 1), at(x + 1, y + 1));
+                    bool firefly = (r11 - r00) > kMinDiff && (r11 - r10) > kMinDiff && (r11 - r20) > kMinDiff &&
+                                   (r11 - r01) > kMinDiff && (r11 - r21) > kMinDiff && (r11 - r02) > kMinDiff &&
+                                   (r11 - r12) > kMinDiff && (r11 - r22) > kMinDiff;
+                    if (firefly) {
+                        float cap = 1.0f / 8.0f * (r00 + r10 + r20 + r01 + r21 + r02 + r12 + r22) + kMinDiff;
+                        c = V3f(cap < c.x ? cap : c.x, cap < c.y ? cap : c.y, cap < c.z ? cap : c.z);
+                    }
+                }
+                acc = acc + c;
+            }
+            st3(sum + 3 * (x + y * w), acc);
+        }
+}
+
+// orc_render (FP32 paths, all pixels) with the firefly filter: every sample image is rendered,
+// then filtered and summed per pixel in sample order (orc_firefly_accumulate).
+void orc_render_firefly(void *h, float const *cam, int tracer, uint32_t max_depth, uint32_t max_path_len,
+                        uint64_t seed, uint32_t w, uint32_t hh, uint32_t spp, uint32_t threads, float *rgba) {
+    Scene *s = static_cast<Scene *>(h);
+    Camera c{cam[0], cam[1], cam[2], {}};
+    std::memcpy(c.m, cam + 3, sizeof(c.m));
+    RenderOpts opt;
+    opt.tracer = tracer;
+    opt.max_depth = max_depth;
+    opt.max_path_len = max_path_len;
+    opt.seed = seed;
+    opt.traversal = TRAV_CULLED;
+    const uint32_t total = w * hh;
+    std::vector<float> samples(static_cast<size_t>(spp) * total * 3);
+    if (threads == 0)
+        threads = 1;
+    std::atomic<uint32_t> next{0};
+    auto work = [&](uint32_t) {
+        for (;;) {
+            uint32_t pix = next.fetch_add(1);
+            if (pix >= total)
+                break;
+            V3f o, v;
+            camera_ray(c, pix % w, pix / w, w, hh, &o, &v);
+            Hit<float> first = intersect<float>(*s, o, v, TRAV_CULLED, nullptr);
+            if (inner(first.n, -v) <= 0.0f)
+                first = Hit<float>();
+            for (uint32_t k = 0; k < spp; k++)
+                st3(samples.data() + 3 * (static_cast<size_t>(k) * total + pix),
+                    trace_sample<float>(*s, opt, first, v, pix, k, nullptr));
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < threads; t++)
+        pool.emplace_back(work, t);
+    for (auto &t : pool)
+        t.join();
+    std::vector<float> sum(3 * static_cast<size_t>(total));
+    orc_firefly_accumulate(samples.data(), w, hh, spp, sum.data());
+    const float scale = 1.0f / spp;
+    for (uint32_t pix = 0; pix < total; pix++) {
+        V3f a = ld3(sum.data() + 3 * pix);
+        st3(rgba + 4 * pix, a * scale);
+        rgba[4 * pix + 3] = scale;
+    }
+}
+
 // Gradient of a film loss with respect to material parameters for the config-mode MIS tracer
 // (x3d_render_backward): dl = dLoss/dfilm, float[4*w*h] (rgb used); grad = double[4*nmat]
 // (d albedo rgb, d roughness per material), overwritten. prec 0 runs the paths in FP32, 1 in

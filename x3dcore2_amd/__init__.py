@@ -339,10 +339,13 @@ class Renderer:
             _lib.x3d_renderer_destroy(self.h)
             self.h = None
 
-    def render(self, scene: Scene, camera, width, height, spp):
+    def render(self, scene: Scene, camera, width, height, spp, firefly_filter=False):
+        """Film [h*w, 4] (rgb = mean radiance, alpha = 1/spp) and RenderStats. firefly_filter
+        caps isolated bright samples in each sample image before the per-pixel sum."""
         rgba = np.zeros((height * width, 4), np.float32)
         st = RenderStats()
-        _check(lib().x3d_render(self.h, scene.h, C.byref(camera_desc(camera)), width, height, spp, 0, _ptr(rgba),
+        _check(lib().x3d_render(self.h, scene.h, C.byref(camera_desc(camera)), width, height, spp,
+                                1 if firefly_filter else 0, _ptr(rgba),
                                 C.byref(st)))
         return rgba, st
 

@@ -400,6 +400,43 @@ __global__ __launch_bounds__(kWG) void k_render_mis_items(RenderParams P, uint32
     }
 }
 
+// Reference-mode tracers over (pixel, sample) items (for the firefly filter, which needs whole
+// sample images): item q = pixel q % npix, sample gen_begin + q / npix, radiance to samples[q].
+template <int TRACER>
+__global__ __launch_bounds__(kWG) void k_render_plain_items(RenderParams P, uint32_t gen_begin, uint32_t gen_samples,
+                                                           float4 *samples, uint32_t *next_item) {
+    __shared__ uint2 lds[kLdsDepth * kWG];
+    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
+    Tracer<kWG, kLdsDepth> tr{P.S, st};
+    const uint32_t npix = P.width * P.height, total = npix * gen_samples;
+    for (uint32_t q = blockIdx.x * kWG + threadIdx.x; q < total; q = gridDim.x * kWG + atomicAdd(next_item, 1u)) {
+        uint32_t pix = q % npix, s = gen_begin + q / npix;
+        f3 o, v;
+        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        bool valid;
+        HitRec first = load_first(P, pix, valid);
+        Pcg g = pcg_for(P.seed, pix, s);
+        f3 L;
+        if (TRACER == X3D_PT_UNIDIRECT_LT1)
+            L = tr.li_lt1(first, valid, v, g);
+        else if (TRACER == X3D_PT_UNIDIRECT)
+            L = tr.li_unidirect(first, valid, v, g);
+        else if (TRACER == X3D_PT_DIRECT)
+            L = tr.li_direct(first, valid, v, g);
+        else // X3D_PT_NORMAL (normal_tracer, pathtracer.cpp:492-506)
+            L = valid ? (first.n + mk(1.0f, 1.0f, 1.0f)) / 2.0f : mk(0.0f, 0.0f, 0.0f);
+        samples[q] = make_float4(L.x, L.y, L.z, 0.0f);
+    }
+    if (P.cnt) {
+        flush_counter(&P.cnt->closest, tr.nclosest);
+        flush_counter(&P.cnt->anyhit, tr.nany);
+        flush_counter(&P.cnt->pops, tr.pops);
+        flush_counter(&P.cnt->tests, tr.tests);
+        flush_counter(&P.cnt->spops, tr.spops);
+        flush_counter(&P.cnt->stests, tr.stests);
+    }
+}
+
 // Adds samples [gen_begin, gen_begin + gen_samples) of every pixel, in order, to the running sum
 // (P.acc, or zero for the first generation of a render starting at sample 0); the last
 // generation writes the scaled film.
@@ -414,8 +451,14 @@ __global__ __launch_bounds__(256) void k_sum_samples(RenderParams P, uint32_t ge
             float4 q = P.acc[pix];
             acc = mk(q.x, q.y, q.z);
         }
-        for (uint32_t k = 0; k < gen_samples; k++)
-            acc = acc + xyz(samples[static_cast<size_t>(k) * npix + pix]);
+        if (P.firefly) {
+            const uint32_t x = pix % P.width, y = pix / P.width;
+            for (uint32_t k = 0; k < gen_samples; k++)
+                acc = acc + xyz(firefly_filtered(samples + static_cast<size_t>(k) * npix, x, y, P.width, P.height));
+        } else {
+            for (uint32_t k = 0; k < gen_samples; k++)
+                acc = acc + xyz(samples[static_cast<size_t>(k) * npix + pix]);
+        }
         if (last && P.finalize)
             P.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
         else
@@ -609,7 +652,7 @@ x3d::WavefrontTuning wf_tuning(x3d_render_options const &o) {
 int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
                 uint32_t height, uint32_t spp, float *d_out, x3d_render_stats *stats,
                 x3d::WavefrontEngine *wf = nullptr, x3d::WavefrontTuning const *wf_tune = nullptr,
-                const float4 *d_dl = nullptr, float *d_grad = nullptr) {
+                const float4 *d_dl = nullptr, float *d_grad = nullptr, bool firefly = false) {
     auto t0 = std::chrono::steady_clock::now();
     RenderParams P{};
     P.S = scene->s.dev;
@@ -621,6 +664,7 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     P.max_depth = r->opt.max_depth;
     P.max_path_len = r->opt.max_path_len;
     P.spill_per_thread = spill_entries(P.S);
+    P.firefly = firefly ? 1 : 0;
     uint64_t npix = static_cast<uint64_t>(width) * height;
     X3D_HIP(r->first.ensure(npix * 3 * sizeof(float4)));
     X3D_HIP(r->acc.ensure(npix * sizeof(float4)));
@@ -635,18 +679,32 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     uint32_t grid_render = 1;
     // Films with fewer pixels than 4x the lanes the plain MIS kernel keeps resident run the
     // MIS tracer over (pixel, sample) items instead (same film).
+    // The firefly filter works on whole sample images: every tracer then runs over items.
     bool mis_items = tracer == X3D_PT_UNIDIRECT_MIS && !wf && !d_dl &&
-                     npix < 4ull * kWG * persistent_grid(k_render_mis, ~0ull >> 8);
+                     (firefly || npix < 4ull * kWG * persistent_grid(k_render_mis, ~0ull >> 8));
+    const bool plain_items = firefly && tracer != X3D_PT_UNIDIRECT_MIS && tracer != X3D_PT_BIDIRECT_MIS;
     switch (tracer) {
     case X3D_PT_UNIDIRECT_MIS:
         grid_render = mis_items ? persistent_grid(k_render_mis_items, npix * static_cast<uint64_t>(spp))
                                 : persistent_grid(k_render_mis, npix);
         break;
-    case X3D_PT_UNIDIRECT_LT1: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT_LT1>, npix); break;
-    case X3D_PT_UNIDIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_UNIDIRECT>, npix); break;
-    case X3D_PT_DIRECT: grid_render = persistent_grid(k_render_plain<X3D_PT_DIRECT>, npix); break;
+    case X3D_PT_UNIDIRECT_LT1:
+        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_UNIDIRECT_LT1>, npix * spp)
+                                  : persistent_grid(k_render_plain<X3D_PT_UNIDIRECT_LT1>, npix);
+        break;
+    case X3D_PT_UNIDIRECT:
+        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_UNIDIRECT>, npix * spp)
+                                  : persistent_grid(k_render_plain<X3D_PT_UNIDIRECT>, npix);
+        break;
+    case X3D_PT_DIRECT:
+        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_DIRECT>, npix * spp)
+                                  : persistent_grid(k_render_plain<X3D_PT_DIRECT>, npix);
+        break;
     case X3D_PT_BIDIRECT_MIS: grid_render = persistent_grid(k_render_bdpt, npix * static_cast<uint64_t>(spp)); break;
-    default: grid_render = persistent_grid(k_render_plain<X3D_PT_NORMAL>, npix); break;
+    default:
+        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_NORMAL>, npix * spp)
+                                  : persistent_grid(k_render_plain<X3D_PT_NORMAL>, npix);
+        break;
     }
     uint32_t grid = grid_first > grid_render ? grid_first : grid_render;
     X3D_HIP(r->spill.ensure(static_cast<size_t>(grid) * kWG * P.spill_per_thread * sizeof(uint2)));
@@ -708,15 +766,35 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
             hipLaunchKernelGGL(k_render_mis, dim3(grid_render), dim3(kWG), 0, r->stream, P, next_pixel);
         }
         break;
-    case X3D_PT_UNIDIRECT_LT1: launch_plain<X3D_PT_UNIDIRECT_LT1>(grid_render, r->stream, P); break;
-    case X3D_PT_UNIDIRECT: launch_plain<X3D_PT_UNIDIRECT>(grid_render, r->stream, P); break;
-    case X3D_PT_DIRECT: launch_plain<X3D_PT_DIRECT>(grid_render, r->stream, P); break;
+    case X3D_PT_UNIDIRECT_LT1:
+        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_UNIDIRECT_LT1>)
+                         : (launch_plain<X3D_PT_UNIDIRECT_LT1>(grid_render, r->stream, P), X3D_OK);
+        if (rc)
+            return rc;
+        break;
+    case X3D_PT_UNIDIRECT:
+        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_UNIDIRECT>)
+                         : (launch_plain<X3D_PT_UNIDIRECT>(grid_render, r->stream, P), X3D_OK);
+        if (rc)
+            return rc;
+        break;
+    case X3D_PT_DIRECT:
+        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_DIRECT>)
+                         : (launch_plain<X3D_PT_DIRECT>(grid_render, r->stream, P), X3D_OK);
+        if (rc)
+            return rc;
+        break;
     case X3D_PT_BIDIRECT_MIS:
         rc = run_generations(k_render_bdpt);
         if (rc)
             return rc;
         break;
-    default: launch_plain<X3D_PT_NORMAL>(grid_render, r->stream, P); break;
+    default:
+        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_NORMAL>)
+                         : (launch_plain<X3D_PT_NORMAL>(grid_render, r->stream, P), X3D_OK);
+        if (rc)
+            return rc;
+        break;
     }
     X3D_HIP(hipGetLastError());
     X3D_HIP(hipEventRecord(r->ev1, r->stream));
@@ -1420,15 +1498,15 @@ int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera
     int rc = check_render_args(r, scene, camera, width, height, spp);
     if (rc)
         return rc;
-    if (firefly_filter)
-        return fail(X3D_ERR_UNSUPPORTED, "firefly filter is not implemented");
+
     if (!rgba)
         return fail(X3D_ERR_INVALID_ARGUMENT, "rgba is null");
     uint64_t npix = static_cast<uint64_t>(width) * height;
     X3D_HIP(r->out.ensure(npix * sizeof(float4)));
     auto t0 = std::chrono::steady_clock::now();
     x3d::WavefrontTuning tune = wf_tuning(r->opt);
-    rc = render_impl(r, scene, camera, width, height, spp, static_cast<float *>(r->out.p), stats, r->wf, &tune);
+    rc = render_impl(r, scene, camera, width, height, spp, static_cast<float *>(r->out.p), stats, r->wf, &tune,
+                     nullptr, nullptr, firefly_filter != 0);
     if (rc)
         return rc;
     X3D_HIP(hipMemcpy(rgba, r->out.p, npix * sizeof(float4), hipMemcpyDeviceToHost));

@@ -103,7 +103,34 @@ struct RenderParams {
     uint32_t spill_per_thread;
     int32_t finalize;
     Counters *cnt;
+    int32_t firefly; // firefly filter on the sample images (sampling_task::run, renderer.cpp:46-83)
 };
+
+// Firefly filter of sampling_task::run (renderer.cpp:46-83) for pixel (x, y) of one sample image
+// `img` (rgb per pixel, row-major, width w, height h): an interior pixel whose squared norm
+// exceeds each of its 8 neighbours' by more than FireFlyMinDiff = 1.5 (renderer.h:118) is capped,
+// component by component, at the mean of the neighbours' squared norms plus 1.5; edge pixels and
+// all others are returned unchanged.
+X3D_DI float4 firefly_filtered(const float4 *img, uint32_t x, uint32_t y, uint32_t w, uint32_t h) {
+    const float4 c = img[x + y * w];
+    if (x == 0 || x + 1 >= w || y == 0 || y + 1 >= h)
+        return c;
+    auto r = [&](uint32_t i, uint32_t j) {
+        float4 q = img[i + j * w];
+        return dot(mk(q.x, q.y, q.z), mk(q.x, q.y, q.z));
+    };
+    constexpr float kMinDiff = 1.5f;
+    const float r00 = r(x - 1, y - 1), r10 = r(x, y - 1), r20 = r(x + 1, y - 1);
+    const float r01 = r(x - 1, y), r11 = r(x, y), r21 = r(x + 1, y);
+    const float r02 = r(x - 1, y + 1), r12 = r(x, y + 1), r22 = r(x + 1, y + 1);
+    const bool firefly = (r11 - r00) > kMinDiff && (r11 - r10) > kMinDiff && (r11 - r20) > kMinDiff &&
+                         (r11 - r01) > kMinDiff && (r11 - r21) > kMinDiff && (r11 - r02) > kMinDiff &&
+                         (r11 - r12) > kMinDiff && (r11 - r22) > kMinDiff;
+    if (!firefly)
+        return c;
+    const float cap = 1.0f / 8.0f * (r00 + r10 + r20 + r01 + r21 + r02 + r12 + r22) + kMinDiff;
+    return make_float4(cap < c.x ? cap : c.x, cap < c.y ? cap : c.y, cap < c.z ? cap : c.z, c.w);
+}
 
 X3D_DI HitRec load_first(RenderParams const &P, uint32_t pix, bool &valid) {
     float4 a = P.first[3 * pix], b = P.first[3 * pix + 1], c = P.first[3 * pix + 2];

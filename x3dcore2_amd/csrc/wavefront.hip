@@ -64,6 +64,7 @@ struct WfArgs {
     int sort_by_material;
     uint32_t refill_below; // traversal: leave the descent when fewer lanes still seek a leaf
     uint32_t leaf_below;   // traversal: leave the leaf tests when fewer lanes still hold a leaf
+    int32_t firefly;       // firefly filter on each generation's sample images
     // Ray reordering: the traversal kernels (and shade) read sq_in / eq_in, which are either the
     // append-order queues or their copies sorted by ray key (queue 0 shadow, 1 extension).
     int sort_rays;
@@ -750,8 +751,14 @@ __global__ __launch_bounds__(256) void k_wf_accumulate(WfArgs A) {
             float4 q = A.acc[pix];
             acc = mk(q.x, q.y, q.z);
         }
-        for (uint32_t s = 0; s < cnt; s++)
-            acc = acc + xyz(A.lbuf[static_cast<size_t>(s) * A.npix + pix]);
+        if (A.firefly) {
+            const uint32_t x = pix % A.width, y = pix / A.width;
+            for (uint32_t s = 0; s < cnt; s++)
+                acc = acc + xyz(firefly_filtered(A.lbuf + static_cast<size_t>(s) * A.npix, x, y, A.width, A.height));
+        } else {
+            for (uint32_t s = 0; s < cnt; s++)
+                acc = acc + xyz(A.lbuf[static_cast<size_t>(s) * A.npix + pix]);
+        }
         if (last && A.finalize)
             A.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
         else
@@ -927,6 +934,7 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     A.sort_rays = tune.sort_rays;
     A.refill_below = static_cast<uint32_t>(tune.refill_below);
     A.leaf_below = static_cast<uint32_t>(tune.leaf_below);
+    A.firefly = P.firefly;
     A.sq_in = A.sq;
     A.eq_in = A.eq;
     if (tune.sort_rays) {

@@ -370,6 +370,48 @@ int x3d_render_backward_device(x3d_renderer *r, const x3d_scene *scene, const x3
 int x3d_renderer_autotune_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs,
                                 x3d_autotune_report *report);
 
+/* ---- film compositors (replace clamp_compositor and aces_compositor, compositor.h:35-58,
+ * compositor.cpp:29-98): the linear RGBA film of x3d_render -> the 8-bit frame
+ * (e8::pixel = 4 x uint8 r, g, b, a per pixel, frame.h:12; pixel (x, y) at x + y*w as
+ * if_frame::operator(), frame.cpp:35-41). Per channel byte = (uint8)(pow(CLAMP(v, 0, 1), 1/2.2)
+ * * 255), alpha byte = (uint8)(a * 255) (if_compositor::pixel_of). CLAMP clamps all four
+ * components to [0, 1] first. ACES multiplies rgb by the exposure and applies
+ * x (2.51 x + 0.03) / (x (2.43 x + 0.59) + 0.14) per component; with auto exposure (the
+ * default, as aces_compositor's) the exposure is exp of the mean of
+ * log(0.299 r + 0.587 g + 0.114 b + 1e-3) over the pixels with any component farther than 1e-5
+ * from 0 (a film with none gives NaN, as the reference's 0/0). The transcendental functions
+ * are the library's own, correctly rounded except within about 1e-14 of a rounding boundary,
+ * and the log sum is a fixed-order double sum: results do not depend on the device or run.
+ * NaN channels give byte 0; values the reference would convert out of range (undefined there)
+ * saturate to 0 or 255. ---- */
+#define X3D_COMPOSITOR_CLAMP 0
+#define X3D_COMPOSITOR_ACES 1
+
+typedef struct x3d_compositor x3d_compositor;
+
+typedef struct x3d_compose_stats {
+    float exposure;      /* exposure applied (ACES; 1 for CLAMP) */
+    uint32_t counted;    /* pixels in the auto-exposure mean (0 without auto exposure) */
+    double exposure_ms;  /* device time of the auto-exposure pass (HIP events) */
+    double compose_ms;   /* device time of the tonemap / gamma / 8-bit pass */
+    double total_ms;     /* wall time of the call */
+} x3d_compose_stats;
+
+int x3d_compositor_create(int32_t kind, x3d_compositor **out);
+int x3d_compositor_destroy(x3d_compositor *c);
+/* aces_compositor::enable_auto_exposure / exposure(e) (compositor.cpp:64-66); ignored by
+ * CLAMP. Defaults: auto exposure on, exposure 0. */
+int x3d_compositor_enable_auto_exposure(x3d_compositor *c, int32_t on);
+int x3d_compositor_set_exposure(x3d_compositor *c, float exposure);
+/* if_compositor::commit (compositor.cpp:41-62): rgba float[4*w*h] -> pixels uint8[4*w*h].
+ * w*h must be 1 .. 2^31. stats may be NULL. */
+int x3d_compositor_commit(x3d_compositor *c, const float *rgba, uint32_t width, uint32_t height,
+                          uint8_t *pixels, x3d_compose_stats *stats);
+/* Same with device pointers (e.g. the film of x3d_render_device), on the compositor's own
+ * stream; returns when the frame is written. */
+int x3d_compositor_commit_device(x3d_compositor *c, const float *d_rgba, uint32_t width,
+                                 uint32_t height, uint8_t *d_pixels, x3d_compose_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

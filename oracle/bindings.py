@@ -320,3 +320,43 @@ class Reference(_Backend):
         taken = self.lib.ref_render(self.h, tracer, cam.sensor, cam.focal, cam.aspect, _f32(cam.transform), width,
                                     height, spp, threads, out)
         return out, taken
+
+
+_compose_lib = None
+
+
+def _clib():
+    global _compose_lib
+    if _compose_lib is None:
+        lib = C.CDLL(ORACLE_SO)
+        args = [_f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_float, _u8p, C.POINTER(C.c_float),
+                C.POINTER(C.c_uint32)]
+        lib.orc_compose.argtypes = args
+        lib.orc_compose_libm.argtypes = args
+        lib.orc_compose_math.argtypes = [C.c_int, _f32p, C.c_uint32, _f32p]
+        _compose_lib = lib
+    return _compose_lib
+
+
+COMPOSE_CLAMP, COMPOSE_ACES = 0, 1
+
+
+def compose(rgba, width, height, kind, auto_exposure=True, exposure=0.0, libm=False):
+    """Film rgba [h*w, 4] float32 -> (frame uint8 [h*w, 4], exposure applied, pixels counted).
+    libm=False: the library's arithmetic (own log / exp, fixed-order double sum); True: the
+    reference's (std::pow / log / exp in float, float sum in pixel order)."""
+    x = _f32(rgba).reshape(-1)
+    assert x.size == 4 * width * height
+    out = np.zeros((width * height, 4), np.uint8)
+    e, n = C.c_float(), C.c_uint32()
+    f = _clib().orc_compose_libm if libm else _clib().orc_compose
+    f(x, width, height, kind, 1 if auto_exposure else 0, exposure, out.reshape(-1), C.byref(e), C.byref(n))
+    return out, e.value, n.value
+
+
+def compose_math(fn, x):
+    """The compositor's own pow(x, 1/2.2f) (fn 0), log (1), exp (2) on float32 x."""
+    x = _f32(x).reshape(-1)
+    out = np.zeros_like(x)
+    _clib().orc_compose_math(fn, x, x.size, out)
+    return out

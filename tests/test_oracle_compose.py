@@ -1,0 +1,181 @@
+"""The oracle's film compositors (clamp_compositor, aces_compositor; compositor.cpp:29-98) on the
+CPU, pinned by what the mathematics fixes (the reference's compositor.cpp needs Qt through
+frame.h and cannot be built here):
+
+- the compositor's own pow(x, 1/2.2f), log and exp equal the float64 results rounded to float32
+  (correct rounding) on random and special arguments;
+- if_compositor::pixel_of's byte k starts exactly at x = (k/255)^2.2 (the exponent being the
+  float 1/2.2f), for every k;
+- clamp: values below 0 / above 1 give 0 / 255, alpha 1/spp gives floor(255/spp);
+- ACES: the published curve x (2.51 x + 0.03) / (x (2.43 x + 0.59) + 0.14) evaluated in
+  float64 gives the same bytes away from byte boundaries; 0 gives 0, the curve's limit 2.51/2.43
+  gives 255, exposure 0 (aces_compositor's default when auto exposure is off) gives black;
+- auto exposure: a uniform film gives L + 1e-3, a two-valued film the geometric mean, and the
+  pixels counted are those with any component farther than 1e-5 from 0 (vec4 != 0.0f,
+  tensor.h:15-17, 121-131); a film with none gives NaN, i.e. black;
+- the library's arithmetic against the reference's (float std::pow / log / exp, float sum in
+  pixel order): same exposure to 1e-5 and bytes within one step on rendered and random films.
+"""
+
+import numpy as np
+import pytest
+
+GAMMA = np.float32(1.0) / np.float32(2.2)
+
+
+@pytest.fixture(scope="module")
+def ob(oracle_mod):
+    return oracle_mod
+
+
+def _film(w, h, rgb, alpha=0.25):
+    f = np.zeros((w * h, 4), np.float32)
+    f[:, :3] = rgb
+    f[:, 3] = alpha
+    return f
+
+
+def test_own_math_correctly_rounded(ob):
+    rng = np.random.default_rng(5)
+    x = rng.integers(1, 0x3F800000, size=400_000, dtype=np.uint32).view(np.float32)
+    special = np.array([0.0, 1.0, np.float32(1e-45), np.float32(1.17549435e-38), np.nextafter(np.float32(1), 0)],
+                       np.float32)
+    x = np.concatenate([x, special])
+    truth = np.power(x.astype(np.float64), np.float64(GAMMA)).astype(np.float32)
+    assert np.array_equal(ob.compose_math(0, x), truth)
+    y = rng.integers(1, 0x7F7FFFFF, size=400_000, dtype=np.uint32).view(np.float32)
+    assert np.array_equal(ob.compose_math(1, y), np.log(y.astype(np.float64)).astype(np.float32))
+    t = rng.uniform(-103.0, 88.7, 400_000).astype(np.float32)
+    assert np.array_equal(ob.compose_math(2, t), np.exp(t.astype(np.float64)).astype(np.float32))
+    sp = ob.compose_math(1, np.array([0.0, -1.0, np.inf, np.nan], np.float32))
+    assert sp[0] == -np.inf and np.isnan(sp[1]) and sp[2] == np.inf and np.isnan(sp[3])
+    ex = ob.compose_math(2, np.array([-np.inf, np.inf, np.nan, 0.0, 1000.0, -1000.0], np.float32))
+    assert ex[0] == 0 and ex[1] == np.inf and np.isnan(ex[2]) and ex[3] == 1 and ex[4] == np.inf and ex[5] == 0
+
+
+def test_gamma_byte_thresholds(ob):
+    k = np.arange(1, 256, dtype=np.float64)
+    t = (k / 255.0) ** (1.0 / np.float64(GAMMA))
+    for scale, want in ((1 + 1e-6, k), (1 - 1e-6, k - 1)):
+        x = (t * scale).astype(np.float32)
+        film = np.zeros((x.size, 4), np.float32)
+        film[:, 0] = film[:, 1] = film[:, 2] = x
+        frame, _, _ = ob.compose(film, x.size, 1, ob.COMPOSE_CLAMP)
+        for c in range(3):
+            assert np.array_equal(frame[:, c], want.astype(np.uint8))
+
+
+def test_clamp_compositor(ob):
+    f = np.array([[-1.0, 0.0, 2.0, 0.25], [0.5, 1.0, 1e30, 1.0], [np.nan, 0.1, 0.2, 2.0], [0.3, -0.0, 1.0, -1.0]],
+                 np.float32)
+    frame, e, n = ob.compose(f, 4, 1, ob.COMPOSE_CLAMP)
+    assert e == 1.0 and n == 0
+    assert frame[0].tolist() == [0, 0, 255, 63]
+    assert frame[1].tolist() == [186, 255, 255, 255]          # 0.5^(1/2.2) * 255 = 186.08
+    assert frame[2, 0] == 0 and frame[2, 3] == 255               # NaN -> 0; alpha clamped to 1
+    assert frame[3, 3] == 0
+    for spp in (1, 3, 7, 64):
+        fr, _, _ = ob.compose(_film(2, 2, 0.5, np.float32(1.0) / np.float32(spp)), 2, 2, ob.COMPOSE_CLAMP)
+        assert (fr[:, 3] == 255 // spp).all() or (spp == 1 and (fr[:, 3] == 255).all())
+
+
+def _aces64(x):
+    return x * (2.51 * x + 0.03) / (x * (2.43 * x + 0.59) + 0.14)
+
+
+def test_aces_curve_fixed_exposure(ob):
+    rng = np.random.default_rng(8)
+    v = np.concatenate([rng.uniform(0, 4, 3000), rng.lognormal(-2, 2, 3000)]).astype(np.float32)
+    e = np.float32(0.8)
+    ldr = np.clip(_aces64(v.astype(np.float64) * np.float64(e)), 0, 1)
+    val = ldr ** np.float64(GAMMA) * 255.0
+    safe = (np.abs(val - np.round(val)) > 1e-3) | (ldr >= 1)   # away from byte boundaries
+    film = np.zeros((v.size, 4), np.float32)
+    film[:, 0] = v
+    film[:, 1] = v
+    film[:, 2] = v
+    film[:, 3] = 0.5
+    frame, ee, n = ob.compose(film, v.size, 1, ob.COMPOSE_ACES, auto_exposure=False, exposure=e)
+    assert ee == e and n == 0 and safe.mean() > 0.99
+    assert np.array_equal(frame[safe, 0], np.floor(val[safe]).astype(np.uint8))
+    assert (frame[:, 3] == 127).all()
+    # 0 -> 0; the curve's limit 2.51/2.43 > 1 -> 255; 1 -> floor((2.54/3.16)^(1/2.2) * 255) = 230;
+    # above ~1e19 the float products overflow and inf / inf = NaN gives 0, as in the reference
+    f = np.array([[0, 1e6, 1.0, 1.0], [1e30, 0, 0, 0]], np.float32)
+    fr, _, _ = ob.compose(f, 2, 1, ob.COMPOSE_ACES, auto_exposure=False, exposure=1.0)
+    assert fr[0].tolist() == [0, 255, 230, 255] and fr[1, 0] == 0
+    # exposure 0 (the default without auto exposure): black
+    fr, _, _ = ob.compose(film, v.size, 1, ob.COMPOSE_ACES, auto_exposure=False, exposure=0.0)
+    assert not fr[:, :3].any()
+
+
+def test_auto_exposure_closed_forms(ob):
+    f = _film(40, 30, (0.2, 0.4, 0.1))
+    lum = 0.299 * 0.2 + 0.587 * 0.4 + 0.114 * 0.1
+    _, e, n = ob.compose(f, 40, 30, ob.COMPOSE_ACES)
+    assert n == 1200 and abs(e / (lum + 1e-3) - 1) < 1e-6
+    # two values: geometric mean of (lum + 1e-3)
+    f = _film(64, 33, 0.05)
+    f[::2, :3] = 3.0
+    _, e, n = ob.compose(f, 64, 33, ob.COMPOSE_ACES)
+    want = np.sqrt((0.05 + 1e-3) * (3.0 + 1e-3))
+    assert n == 64 * 33 and abs(e / want - 1) < 1e-6
+    # the counting rule: all four components within 1e-5 of 0 -> not counted
+    f = _film(10, 10, 0.5, alpha=0.0)
+    f[:50] = 0.0
+    f[50:60, :3] = 0.0
+    f[50:60, 3] = 0.5                      # alpha only: counted, log(1e-3)
+    f[60:70] = [1e-5, 0.0, -1e-5, 0.0]     # |v| <= 1e-5: not counted
+    f[70:80] = [2e-5, 0.0, 0.0, 0.0]       # counted
+    _, e, n = ob.compose(f, 10, 10, ob.COMPOSE_ACES)
+    logs = np.concatenate([np.full(10, np.log(1e-3)), np.full(10, np.log(0.299 * 2e-5 + 1e-3)),
+                           np.full(20, np.log(0.5 + 1e-3))])
+    assert n == 40 and abs(e / np.exp(logs.mean()) - 1) < 1e-5
+    # nothing counted: NaN exposure (0 / 0), black frame
+    fr, e, n = ob.compose(np.zeros((16, 4), np.float32), 4, 4, ob.COMPOSE_ACES)
+    assert n == 0 and np.isnan(e) and not fr.any()
+
+
+def test_multi_chunk_exposure_sum(ob):
+    """600 x 500 pixels: 293 chunks of 1024, more than the 256 lanes of the final tree."""
+    rng = np.random.default_rng(3)
+    f = rng.lognormal(-1, 2, (600 * 500, 4)).astype(np.float32)
+    _, e, n = ob.compose(f, 600, 500, ob.COMPOSE_ACES)
+    lum = (f[:, 0] * np.float32(0.299) + f[:, 1] * np.float32(0.587) + f[:, 2] * np.float32(0.114)
+           + np.float32(1e-3))
+    logs = np.log(lum.astype(np.float64)).astype(np.float32).astype(np.float64)
+    assert n == f.shape[0]
+    assert abs(e / np.exp(np.float64(np.float32(logs.mean()))) - 1) < 2e-7
+
+
+def _compare_forms(ob, film, w, h):
+    for kind, auto in ((ob.COMPOSE_CLAMP, False), (ob.COMPOSE_ACES, True), (ob.COMPOSE_ACES, False)):
+        a, ea, na = ob.compose(film, w, h, kind, auto, 1.3)
+        b, eb, nb = ob.compose(film, w, h, kind, auto, 1.3, libm=True)
+        assert na == nb
+        if kind == ob.COMPOSE_ACES and auto:
+            # the exact mean of the float logs: the library's double sum meets it; the reference's
+            # float sum in pixel order carries its rounding error (about n u mean|log|)
+            lum = (film[:, 0] * np.float32(0.299) + film[:, 1] * np.float32(0.587) + film[:, 2] * np.float32(0.114)
+                   + np.float32(1e-3))
+            exact = np.exp(np.float64(np.float32(np.log(lum.astype(np.float64)).astype(np.float32)
+                                                 .astype(np.float64).mean())))
+            assert abs(ea / exact - 1) < 2e-7 and abs(eb / exact - 1) < 1e-4
+        else:
+            assert ea == eb
+        d = np.abs(a.astype(np.int32) - b.astype(np.int32))
+        assert d.max() <= 1 and (d > 0).mean() < 2e-3, (kind, auto, d.max(), (d > 0).mean())
+
+
+def test_library_arithmetic_vs_reference_arithmetic(ob):
+    rng = np.random.default_rng(11)
+    film = rng.lognormal(-1.5, 1.5, (256 * 200, 4)).astype(np.float32)
+    film[:, 3] = 0.125
+    _compare_forms(ob, film, 256, 200)
+
+
+def test_rendered_film_forms(ob):
+    from x3dcore2_amd import scenes
+    desc = scenes.box_scene()
+    film = ob.Oracle(desc).render(ob.TR_UNIDIRECT_MIS, 96, 80, 4, seed=2)
+    _compare_forms(ob, film, 96, 80)

@@ -24,6 +24,7 @@ PT_UNIDIRECT_MIS = 16
 FORM_PLAIN, FORM_WAVEFRONT = 0, 1
 SORT_DEFAULT, SORT_MATERIAL, SORT_NONE = 0, 1, 2
 RAY_ORDER_DEFAULT, RAY_ORDER_SORTED, RAY_ORDER_APPEND = 0, 1, 2
+COMPOSITOR_CLAMP, COMPOSITOR_ACES = 0, 1
 TRAVERSAL_CULLED, TRAVERSAL_REFERENCE = 0, 1
 
 
@@ -104,6 +105,11 @@ class BatchStats(C.Structure):
                 ("tiles", C.c_uint32), ("alone_jobs", C.c_uint32)]
 
 
+class ComposeStats(C.Structure):
+    _fields_ = [("exposure", C.c_float), ("counted", C.c_uint32), ("exposure_ms", C.c_double),
+                ("compose_ms", C.c_double), ("total_ms", C.c_double)]
+
+
 # Every exported symbol declared in include/x3dpt.h.
 EXPORTS = [
     "x3d_last_error", "x3d_version", "x3d_scene_create", "x3d_scene_destroy", "x3d_scene_add_material",
@@ -114,6 +120,8 @@ EXPORTS = [
     "x3d_renderer_autotune", "x3d_render_batch", "x3d_render_batch_device", "x3d_renderer_autotune_batch",
     "x3d_scene_set_material_params", "x3d_render_forward", "x3d_render_backward", "x3d_render_backward_device",
     "x3d_obj_load", "x3d_obj_free", "x3d_scene_add_obj",
+    "x3d_compositor_create", "x3d_compositor_destroy", "x3d_compositor_enable_auto_exposure",
+    "x3d_compositor_set_exposure", "x3d_compositor_commit", "x3d_compositor_commit_device",
 ]
 
 
@@ -168,6 +176,12 @@ def lib() -> C.CDLL:
     L.x3d_render_backward.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp, vp, C.POINTER(RenderStats)]
     L.x3d_render_backward_device.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp, vp,
                                              C.POINTER(RenderStats)]
+    L.x3d_compositor_create.argtypes = [i32, C.POINTER(vp)]
+    L.x3d_compositor_destroy.argtypes = [vp]
+    L.x3d_compositor_enable_auto_exposure.argtypes = [vp, i32]
+    L.x3d_compositor_set_exposure.argtypes = [vp, C.c_float]
+    L.x3d_compositor_commit.argtypes = [vp, vp, u32, u32, vp, C.POINTER(ComposeStats)]
+    L.x3d_compositor_commit_device.argtypes = [vp, vp, u32, u32, vp, C.POINTER(ComposeStats)]
     missing = [n for n in EXPORTS if n not in ("x3d_last_error", "x3d_version") and getattr(L, n).argtypes is None]
     if missing:  # a pointer passed without argtypes would be truncated to a C int
         raise ImportError(f"no ctypes signature for {missing}")
@@ -435,3 +449,41 @@ def pcg_draws(seed, pixel, sample, n) -> np.ndarray:
     out = np.zeros(n, np.uint32)
     _check(lib().x3d_pcg_draws(seed, pixel, sample, n, _ptr(out)))
     return out
+
+
+class Compositor:
+    """clamp_compositor / aces_compositor equivalent (x3d_compositor): linear film -> 8-bit
+    RGBA frame on the GPU. ACES defaults to auto exposure, as the reference's."""
+
+    def __init__(self, kind=COMPOSITOR_ACES):
+        h = C.c_void_p()
+        _check(lib().x3d_compositor_create(int(kind), C.byref(h)))
+        self.h = h
+        self.kind = kind
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.x3d_compositor_destroy(self.h)
+            self.h = None
+
+    def enable_auto_exposure(self, on=True):
+        _check(lib().x3d_compositor_enable_auto_exposure(self.h, 1 if on else 0))
+
+    def set_exposure(self, e: float):
+        _check(lib().x3d_compositor_set_exposure(self.h, float(e)))
+
+    def commit(self, film: np.ndarray, width: int, height: int):
+        """film [h*w, 4] float32 -> (frame uint8 [h*w, 4] r, g, b, a, ComposeStats)."""
+        f = _f32(film)
+        if f.size != 4 * width * height:
+            raise ValueError("film size does not match width x height")
+        frame = np.zeros((width * height, 4), np.uint8)
+        st = ComposeStats()
+        _check(lib().x3d_compositor_commit(self.h, _ptr(f), width, height, _ptr(frame), C.byref(st)))
+        return frame, st
+
+    def commit_device(self, d_film: int, width: int, height: int, d_frame: int):
+        st = ComposeStats()
+        _check(lib().x3d_compositor_commit_device(self.h, C.c_void_p(d_film), width, height, C.c_void_p(d_frame),
+                                                  C.byref(st)))
+        return st

@@ -46,7 +46,7 @@ def build_library(verbose: bool = True) -> str:
         o = os.path.join(BUILD, unit + ".o")
         cmds.append([gxx, *GXX_FLAGS, "-c", os.path.join(CSRC, unit + ".cpp"), "-o", o])
         objs.append(o)
-    for unit in ("render", "wavefront", "batch", "adjoint"):
+    for unit in ("render", "wavefront", "batch", "adjoint", "compose"):
         o = os.path.join(BUILD, unit + ".o")
         cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, unit + ".hip"), "-o", o])
         objs.append(o)

@@ -422,9 +422,36 @@ def main():
         "node_visits_per_launch": nodes,
         "tri_tests_per_launch": tests,
     }
+    out["compose"] = bench_compose(x3d, DeviceArray, R["film"], films[best], w, h, args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(desc)
     print(json.dumps(out), flush=True)
+
+
+def bench_compose(x3d, DeviceArray, film_d, film_host, w, h, no_cpu, calls=20):
+    """The step after the film: ACES with auto exposure, gamma 2.2 -> 8-bit RGBA, on the film in
+    HBM (two HBM-bound passes: the exposure's log-luminance sum reads 16 B per pixel; the
+    tonemap reads 16 B and writes 4 B per pixel). Device times from the compositor's HIP events."""
+    frame_d = DeviceArray(w * h * 4)
+    c = x3d.Compositor(x3d.COMPOSITOR_ACES)
+    c.commit_device(film_d.ptr, w, h, frame_d.ptr)  # warm-up
+    st = [c.commit_device(film_d.ptr, w, h, frame_d.ptr) for _ in range(calls)]
+    e_ms = float(np.mean([s.exposure_ms for s in st]))
+    c_ms = float(np.mean([s.compose_ms for s in st]))
+    npix = w * h
+    res = {"compositor": "aces, auto exposure", "film": f"{w}x{h}", "exposure": st[-1].exposure,
+           "exposure_pass_ms": e_ms, "compose_pass_ms": c_ms, "call_ms": float(np.mean([s.total_ms for s in st])),
+           "exposure_pass_gbs": 16 * npix / (e_ms * 1e-3) / 1e9, "compose_pass_gbs": 20 * npix / (c_ms * 1e-3) / 1e9}
+    res["frac"] = 36 * npix / ((e_ms + c_ms) * 1e-3) / 1e9 / HBM_PEAK_GBS
+    if not no_cpu:
+        # the reference's compositor arithmetic (float std::log / exp / pow), one core
+        from oracle import bindings as ob
+
+        t0 = time.perf_counter()
+        ob.compose(film_host, w, h, ob.COMPOSE_ACES, libm=True)
+        res["cpu_ms"] = (time.perf_counter() - t0) * 1e3
+    log(f"compose: {res}")
+    return res
 
 
 if __name__ == "__main__":

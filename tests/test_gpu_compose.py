@@ -79,6 +79,21 @@ def test_compose_special_values(x3d, oracle_mod, kind, auto, e):
     _check(oracle_mod, x3d, film, w, h, kind, auto, e)
 
 
+def test_compose_at_gamma_thresholds(x3d, oracle_mod):
+    """Every 8-bit gamma step: channel values within 64 ulps of (k/255)^2.2, k = 1 .. 255 (the
+    GPU reads the byte from a threshold table; the oracle evaluates pow per value)."""
+    g = np.float32(1.0) / np.float32(2.2)
+    k = np.arange(1, 256)
+    c = ((k / 255.0) ** (1.0 / np.float64(g))).astype(np.float32).view(np.uint32).astype(np.int64)
+    u = (c[:, None] + np.arange(-64, 65)[None, :]).reshape(-1)
+    x = np.minimum(u, 0x3F800000).astype(np.uint32).view(np.float32)
+    n = x.size + (-x.size) % 3
+    v = np.resize(x, n).reshape(-1, 3)
+    film = np.concatenate([v, np.full((v.shape[0], 1), 0.5, np.float32)], 1)
+    w = film.shape[0]
+    _check(oracle_mod, x3d, film, w, 1, "clamp", False, 1.0)
+
+
 def test_compose_nothing_counted(x3d, oracle_mod):
     film = np.zeros((64 * 20, 4), np.float32)
     frame, st = _check(oracle_mod, x3d, film, 64, 20, "aces", True, 0.0)
@@ -95,7 +110,7 @@ def test_compose_rendered_film_on_device(x3d, oracle_mod):
     r.render_device(gs, desc.camera, w, h, spp, film_d.ptr)
     film = film_d.to_host(np.float32, (w * h, 4))
     c = x3d.Compositor(x3d.COMPOSITOR_ACES)
-    for _ in range(2):  # the exposure pass's ticket counter is reset for the next call
+    for _ in range(2):  # the second call reuses the compositor's scratch buffers
         st = c.commit_device(film_d.ptr, w, h, frame_d.ptr)
         frame = frame_d.to_host(np.uint8, (w * h, 4))
         ref, re, rn = oracle_mod.compose(film, w, h, oracle_mod.COMPOSE_ACES)

@@ -65,6 +65,19 @@ def test_gamma_byte_thresholds(ob):
             assert np.array_equal(frame[:, c], want.astype(np.uint8))
 
 
+def test_gamma_byte_is_step_function(ob):
+    """trunc(pow(x, 1/2.2f) * 255) is non-decreasing with exactly one step to k near each
+    (k/255)^2.2 (checked over 2^14 floats on either side): the premise of the GPU's threshold
+    table, which it builds from the same pow by bisection."""
+    k = np.arange(1, 256)
+    centers = ((k / 255.0) ** (1.0 / np.float64(GAMMA))).astype(np.float32).view(np.uint32).astype(np.int64)
+    w = 1 << 14
+    for c, kk in zip(centers, k):
+        u = np.arange(c - w, min(c + w, 0x3F800000) + 1, dtype=np.int64).astype(np.uint32)
+        b = np.floor(ob.compose_math(0, u.view(np.float32)) * np.float32(255.0)).astype(np.int32)
+        assert (np.diff(b) >= 0).all() and b[0] == kk - 1 and b[-1] == kk
+
+
 def test_clamp_compositor(ob):
     f = np.array([[-1.0, 0.0, 2.0, 0.25], [0.5, 1.0, 1e30, 1.0], [np.nan, 0.1, 0.2, 2.0], [0.3, -0.0, 1.0, -1.0]],
                  np.float32)

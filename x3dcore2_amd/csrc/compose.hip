@@ -22,18 +22,27 @@
 // range or NaN to unsigned char (undefined behaviour) give 0 for NaN and negatives, 255 at or
 // above 255.
 //
-// Kernels (both HBM-bound; 16 B read per pixel per pass, 4 B written by the compose pass):
-//   k_exposure   one block per 1024-pixel chunk writes the chunk's (sum, count); the last
-//                block to finish (ticket counter) sums the chunk results in fixed order and
-//                writes the exposure, then clears the ticket.
-//   k_compose    one block per 1024 pixels; each lane four pixels 256 apart (coalesced loads and
-//                stores), tonemap + gamma + 8-bit.
+// The 8-bit gamma byte trunc(pow(x, 1/2.2f) * 255) is a non-decreasing step function of x on
+// [0, 1] (pow correctly rounded, the product and truncation monotone), so it is the number of its
+// 255 thresholds t_k (the smallest float x whose byte is at least k) at or below x. The thresholds
+// are found once per process on the host by bisection over float bit patterns with the same pow;
+// the kernel estimates the byte with the hardware log2 / exp2 (error far below one step) and
+// corrects it against the thresholds in LDS, giving exactly the byte of the exact pow.
+//
+// Kernels (HBM-bound; 16 B read per pixel per pass, 4 B written by the compose pass):
+//   k_exposure_chunks  one block per 1024-pixel chunk writes the chunk's (sum, count).
+//   k_exposure_final   one block sums the chunk results in fixed order, writes the exposure.
+//   k_compose          one block per 1024 pixels; each lane four pixels 256 apart (coalesced
+//                      loads and stores), tonemap + gamma + 8-bit.
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/x3dpt.h"
@@ -64,13 +73,24 @@ constexpr double kInvLn2 = 0x1.71547652b82fep+0;
 constexpr double kSqrt2 = 0x1.6a09e667f3bcdp+0;
 constexpr float kInvGamma = 1.0f / 2.2f;
 
-__device__ inline double bits_to_double(uint64_t b) { return __longlong_as_double(static_cast<long long>(b)); }
-__device__ inline uint64_t double_to_bits(double x) { return static_cast<uint64_t>(__double_as_longlong(x)); }
+#define X3D_HD __host__ __device__ inline
+
+X3D_HD double bits_to_double(uint64_t b) {
+    double x;
+    memcpy(&x, &b, sizeof x);
+    return x;
+}
+
+X3D_HD uint64_t double_to_bits(double x) {
+    uint64_t b;
+    memcpy(&b, &x, sizeof b);
+    return b;
+}
 
 // Natural log in double: x = m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
 // s = (m - 1) / (m + 1), |s| <= 0.1716; the series 2 s (1 + s^2/3 + ... + s^22/23) is cut where
 // the next term is below 2^-60 of the sum.
-__device__ inline double log_d(double x) {
+X3D_HD double log_d(double x) {
     if (!(x > 0.0))
         return x == 0.0 ? -__builtin_inf() : __builtin_nan("");
     if (x == __builtin_inf())
@@ -108,7 +128,7 @@ __device__ inline double log_d(double x) {
 
 // e^t in double: t = k ln 2 + r, |r| <= ln 2 / 2, e^r by its Taylor polynomial of degree 13
 // (next term below 5e-18), scaled by 2^k in two steps.
-__device__ inline double exp_d(double t) {
+X3D_HD double exp_d(double t) {
     if (t != t)
         return t;
     if (t > 710.0)
@@ -139,7 +159,7 @@ __device__ inline double exp_d(double t) {
 }
 
 // pow(x, 1/2.2f) for x in [0, 1] (or NaN), rounded to float.
-__device__ inline float gamma_encode(float x) {
+X3D_HD float gamma_encode(float x) {
     if (x != x)
         return x;
     if (x == 0.0f)
@@ -149,7 +169,7 @@ __device__ inline float gamma_encode(float x) {
     return static_cast<float>(exp_d(static_cast<double>(kInvGamma) * log_d(static_cast<double>(x))));
 }
 
-__device__ inline uint32_t to_byte(float v) {
+X3D_HD uint32_t to_byte(float v) {
     if (!(v > 0.0f))
         return 0u;
     if (v >= 255.0f)
@@ -157,15 +177,69 @@ __device__ inline uint32_t to_byte(float v) {
     return static_cast<uint32_t>(v);
 }
 
-__device__ inline float clamp01(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
+X3D_HD float clamp01(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
+
+// The gamma byte of if_compositor::pixel_of for one channel value (already CLAMPed or NaN), by
+// the thresholds: t[k] (k = 1 .. 255) is the smallest float whose byte is at least k, t[256] is
+// +inf. NaN and values below t[1] give 0.
+__device__ inline uint32_t gamma_byte(float x, const float *t) {
+    if (!(x >= t[1]))
+        return 0u;
+    const float est = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * kInvGamma) * 255.0f;
+    int b = static_cast<int>(est);
+    b = b < 1 ? 1 : (b > 255 ? 255 : b);
+    while (x >= t[b + 1])
+        b++;
+    while (x < t[b])
+        b--;
+    return static_cast<uint32_t>(b);
+}
 
 // if_compositor::pixel_of (compositor.cpp:29-39), packed r | g << 8 | b << 16 | a << 24.
-__device__ inline uint32_t pixel_of(float r, float g, float b, float a) {
-    const uint32_t R = to_byte(gamma_encode(clamp01(r)) * 255.0f);
-    const uint32_t G = to_byte(gamma_encode(clamp01(g)) * 255.0f);
-    const uint32_t B = to_byte(gamma_encode(clamp01(b)) * 255.0f);
+__device__ inline uint32_t pixel_of(float r, float g, float b, float a, const float *t) {
+    const uint32_t R = gamma_byte(clamp01(r), t);
+    const uint32_t G = gamma_byte(clamp01(g), t);
+    const uint32_t B = gamma_byte(clamp01(b), t);
     const uint32_t A = to_byte(a * 255.0f);
     return R | (G << 8) | (B << 16) | (A << 24);
+}
+
+// The thresholds, once per process: for k = 1 .. 255 the smallest float x in (0, 1] with
+// to_byte(gamma_encode(x) * 255) >= k, by bisection over the bit patterns of [0, 1] (ordered as
+// integers). Each threshold is checked against the bytes of its 64 neighbours on either side.
+std::array<float, 257> const &gamma_thresholds() {
+    static std::array<float, 257> t;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        auto byte_of = [](uint32_t u) {
+            float x;
+            memcpy(&x, &u, sizeof x);
+            return to_byte(gamma_encode(x) * 255.0f);
+        };
+        const uint32_t one = 0x3f800000u;
+        t[0] = 0.0f;
+        for (uint32_t k = 1; k <= 255; k++) {
+            uint32_t lo = 0, hi = one; // byte(lo) < k <= byte(hi)
+            while (hi - lo > 1) {
+                uint32_t mid = lo + (hi - lo) / 2;
+                if (byte_of(mid) >= k)
+                    hi = mid;
+                else
+                    lo = mid;
+            }
+            for (uint32_t d = 1; d <= 64; d++) {
+                if ((hi + d <= one && byte_of(hi + d) < k) || (hi >= d && byte_of(hi - d) >= k))
+                    t[k] = NAN; // not a step function near the threshold: rejected at upload
+            }
+            if (t[k] == t[k]) {
+                float x;
+                memcpy(&x, &hi, sizeof x);
+                t[k] = x;
+            }
+        }
+        t[256] = INFINITY;
+    });
+    return t;
 }
 
 // aces_compositor::aces_tonemap per component (compositor.cpp:68-79).
@@ -193,10 +267,8 @@ __device__ inline float log_luminance(float4 c) {
 }
 
 struct ExposureCtl {
-    uint32_t ticket;  // blocks finished; cleared by the last one
     uint32_t counted; // pixels in the mean
     float exposure;
-    uint32_t pad;
 };
 
 // Pairwise tree over the block's 256 (sum, count) pairs; result in s_sum[0], s_cnt[0].
@@ -214,65 +286,74 @@ __device__ inline void block_tree(double *s_sum, uint32_t *s_cnt, double acc, ui
     }
 }
 
-__global__ __launch_bounds__(kComposeWG) void k_exposure(const float4 *film, uint32_t npix, uint32_t nchunks,
-                                                        double *chunk_sum, uint32_t *chunk_cnt, ExposureCtl *ctl) {
+__global__ __launch_bounds__(kComposeWG) void k_exposure_chunks(const float4 *film, uint32_t npix, double *chunk_sum,
+                                                               uint32_t *chunk_cnt) {
     __shared__ double s_sum[kComposeWG];
     __shared__ uint32_t s_cnt[kComposeWG];
-    __shared__ uint32_t s_last;
     const uint32_t t = threadIdx.x, c = blockIdx.x;
+    float4 q[kChunkPixels / kComposeWG];
+    for (uint32_t j = 0; j < kChunkPixels / kComposeWG; j++) {
+        const uint32_t p = c * kChunkPixels + j * kComposeWG + t;
+        q[j] = p < npix ? film[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     double acc = 0.0;
     uint32_t n = 0;
     for (uint32_t j = 0; j < kChunkPixels / kComposeWG; j++) {
-        const uint32_t p = c * kChunkPixels + j * kComposeWG + t;
-        if (p < npix) {
-            const float4 q = film[p];
-            if (counted(q)) {
-                acc = acc + static_cast<double>(log_luminance(q));
-                n++;
-            }
+        if (counted(q[j])) { // padding beyond npix is all zero: not counted
+            acc = acc + static_cast<double>(log_luminance(q[j]));
+            n++;
         }
     }
     block_tree(s_sum, s_cnt, acc, n);
     if (t == 0) {
         chunk_sum[c] = s_sum[0];
         chunk_cnt[c] = s_cnt[0];
-        __threadfence();
-        s_last = atomicAdd(&ctl->ticket, 1u) == nchunks - 1 ? 1u : 0u;
     }
-    __syncthreads();
-    if (!s_last)
-        return;
-    __threadfence();
-    acc = 0.0;
-    n = 0;
+}
+
+__global__ __launch_bounds__(kComposeWG) void k_exposure_final(const double *chunk_sum, const uint32_t *chunk_cnt,
+                                                              uint32_t nchunks, ExposureCtl *ctl) {
+    __shared__ double s_sum[kComposeWG];
+    __shared__ uint32_t s_cnt[kComposeWG];
+    const uint32_t t = threadIdx.x;
+    double acc = 0.0;
+    uint32_t n = 0;
     for (uint32_t k = t; k < nchunks; k += kComposeWG) {
-        acc = acc + __hip_atomic_load(chunk_sum + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        n += __hip_atomic_load(chunk_cnt + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        acc = acc + chunk_sum[k];
+        n += chunk_cnt[k];
     }
     block_tree(s_sum, s_cnt, acc, n);
     if (t == 0) {
         const float mean = static_cast<float>(s_sum[0] / static_cast<double>(s_cnt[0]));
         ctl->exposure = static_cast<float>(exp_d(static_cast<double>(mean)));
         ctl->counted = s_cnt[0];
-        ctl->ticket = 0u;
     }
 }
 
 __global__ __launch_bounds__(kComposeWG) void k_compose(const float4 *film, uint32_t npix, int32_t kind,
                                                        const ExposureCtl *ctl, float fixed_exposure,
-                                                       uint32_t *frame) {
+                                                       const float *thresholds, uint32_t *frame) {
+    __shared__ float t[257];
+    t[threadIdx.x] = thresholds[threadIdx.x];
+    if (threadIdx.x == 0)
+        t[256] = thresholds[256];
     const float e = ctl ? ctl->exposure : fixed_exposure;
+    float4 c[kChunkPixels / kComposeWG];
+    for (uint32_t j = 0; j < kChunkPixels / kComposeWG; j++) {
+        const uint32_t p = blockIdx.x * kChunkPixels + j * kComposeWG + threadIdx.x;
+        c[j] = p < npix ? film[p] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    __syncthreads();
     for (uint32_t j = 0; j < kChunkPixels / kComposeWG; j++) {
         const uint32_t p = blockIdx.x * kChunkPixels + j * kComposeWG + threadIdx.x;
         if (p >= npix)
             return;
-        const float4 c = film[p];
         uint32_t px;
         if (kind == X3D_COMPOSITOR_ACES)
-            px = pixel_of(aces(c.x * e), aces(c.y * e), aces(c.z * e), c.w);
+            px = pixel_of(aces(c[j].x * e), aces(c[j].y * e), aces(c[j].z * e), c[j].w, t);
         else
-            px = pixel_of(at_least0_at_most1(c.x), at_least0_at_most1(c.y), at_least0_at_most1(c.z),
-                          at_least0_at_most1(c.w));
+            px = pixel_of(at_least0_at_most1(c[j].x), at_least0_at_most1(c[j].y), at_least0_at_most1(c[j].z),
+                          at_least0_at_most1(c[j].w), t);
         frame[p] = px;
     }
 }
@@ -285,7 +366,7 @@ struct x3d_compositor {
     float exposure = 0.0f;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    x3d::DevBuf film, frame, chunks, ctl;
+    x3d::DevBuf film, frame, chunks, ctl, thresholds;
     ~x3d_compositor() {
         if (ev0)
             (void)hipEventDestroy(ev0);
@@ -317,21 +398,22 @@ int commit_impl(x3d_compositor *c, const float *d_rgba, uint32_t npix, uint8_t *
     ExposureCtl *ctl = nullptr;
     X3D_CHECK(hipEventRecord(c->ev0, c->stream));
     if (auto_e) {
-        if (!c->ctl.p) {
-            X3D_CHECK(c->ctl.ensure(256));
-            X3D_CHECK(hipMemset(c->ctl.p, 0, 256));
-        }
+        X3D_CHECK(c->ctl.ensure(256));
         X3D_CHECK(c->chunks.ensure(static_cast<size_t>(nchunks) * (sizeof(double) + sizeof(uint32_t))));
         double *chunk_sum = static_cast<double *>(c->chunks.p);
         uint32_t *chunk_cnt = reinterpret_cast<uint32_t *>(chunk_sum + nchunks);
         ctl = static_cast<ExposureCtl *>(c->ctl.p);
-        hipLaunchKernelGGL(k_exposure, dim3(nchunks), dim3(kComposeWG), 0, c->stream, film, npix, nchunks, chunk_sum,
-                           chunk_cnt, ctl);
+        hipLaunchKernelGGL(k_exposure_chunks, dim3(nchunks), dim3(kComposeWG), 0, c->stream, film, npix, chunk_sum,
+                           chunk_cnt);
+        hipLaunchKernelGGL(k_exposure_final, dim3(1), dim3(kComposeWG), 0, c->stream,
+                           static_cast<const double *>(chunk_sum), static_cast<const uint32_t *>(chunk_cnt), nchunks,
+                           ctl);
         X3D_CHECK(hipGetLastError());
     }
     X3D_CHECK(hipEventRecord(c->ev1, c->stream));
     hipLaunchKernelGGL(k_compose, dim3(nchunks), dim3(kComposeWG), 0, c->stream, film, npix, c->kind,
-                       static_cast<const ExposureCtl *>(ctl), c->exposure, reinterpret_cast<uint32_t *>(d_frame));
+                       static_cast<const ExposureCtl *>(ctl), c->exposure, static_cast<const float *>(c->thresholds.p),
+                       reinterpret_cast<uint32_t *>(d_frame));
     X3D_CHECK(hipGetLastError());
     X3D_CHECK(hipEventRecord(c->ev2, c->stream));
     X3D_CHECK(hipEventSynchronize(c->ev2));
@@ -372,10 +454,20 @@ int x3d_compositor_create(int32_t kind, x3d_compositor **out) {
         e = hipEventCreate(&c->ev1);
     if (e == hipSuccess)
         e = hipEventCreate(&c->ev2);
+    std::array<float, 257> const &t = gamma_thresholds();
+    if (e == hipSuccess)
+        e = c->thresholds.ensure(sizeof t);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->thresholds.p, t.data(), sizeof t, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         delete c;
-        return x3d::fail(X3D_ERR_DEVICE, std::string("compositor stream: ") + hipGetErrorString(e));
+        return x3d::fail(X3D_ERR_DEVICE, std::string("compositor set-up: ") + hipGetErrorString(e));
     }
+    for (float x : t)
+        if (x != x) {
+            delete c;
+            return x3d::fail(X3D_ERR_DEVICE, "gamma byte is not a step function near a threshold");
+        }
     *out = c;
     return X3D_OK;
 }

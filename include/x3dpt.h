@@ -191,12 +191,12 @@ int x3d_camera_rays(const x3d_camera *camera, uint32_t width, uint32_t height, f
 typedef struct x3d_renderer x3d_renderer;
 
 /* Tracer types: pathtracer_factory::pt_type values (pathtracerfact.h:12) ... */
-#define X3D_PT_NORMAL 0
-#define X3D_PT_POSITION 1
-#define X3D_PT_DIRECT 2
+#define X3D_PT_NORMAL 0         /* normal_tracer (pathtracer.cpp:492-506) */
+#define X3D_PT_POSITION 1       /* position_tracer (pathtracer.cpp:474-490) */
+#define X3D_PT_DIRECT 2         /* direct_path_tracer (pathtracer.cpp:508-526) */
 #define X3D_PT_UNIDIRECT 3     /* unidirect_path_tracer (pathtracer.cpp:528-585) */
 #define X3D_PT_UNIDIRECT_LT1 4 /* unidirect_lt1_path_tracer (pathtracer.cpp:587-653) */
-#define X3D_PT_BIDIRECT_LT2 5
+#define X3D_PT_BIDIRECT_LT2 5   /* bidirect_lt2_path_tracer (pathtracer.cpp:655-752) */
 #define X3D_PT_BIDIRECT_MIS 6  /* bidirect_mis_path_tracer (pathtracer.cpp:754-803) */
 /* ... plus the fixed-depth unidirectional tracer with light + BSDF sampling combined by the
  * power heuristic (no reference counterpart). */

@@ -36,6 +36,18 @@ void commit(Scene &s) {
         s.light_cum.push_back(s.total_power);
         s.meshes[L.mesh].light = static_cast<int>(l);
     }
+    // if_path_space::load unions each mesh's aabb (trimesh::update_aabb, geometry.cpp:97-103)
+    // into an initially empty box (tensor.h:756-771)
+    bool empty = true;
+    V3f lo, hi;
+    for (Mesh const &m : s.meshes)
+        for (V3f const &v : m.verts) {
+            lo = empty ? v : V3f(std::min(lo.x, v.x), std::min(lo.y, v.y), std::min(lo.z, v.z));
+            hi = empty ? v : V3f(std::max(hi.x, v.x), std::max(hi.y, v.y), std::max(hi.z, v.z));
+            empty = false;
+        }
+    s.space_lo = lo;
+    s.space_range = hi - lo;
     build_bvh(s);
 }
 
@@ -1055,6 +1067,90 @@ template <class T> V3<T> li_lt1(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg
     return L;
 }
 
+// bidirect_lt2_path_tracer (pathtracer.cpp:655-752) evaluated front to back: at each camera
+// vertex join_with_light_paths (pathtracer.cpp:655-699) — one NEE sample p1 and a light path of
+// one vertex (emission sample, closest hit) joined to the camera vertex, p2; the vertex's term
+// is p1 + p2/2 at depth 0 and (p1 + p2)/2 deeper when the join is unoccluded and every cosine
+// positive, else p1, and zero when the light path misses (the reference returns 0 there).
+// sample_indirect_illum (pathtracer.cpp:701-731): Russian roulette p = 0.5 from depth 1; a
+// BSDF sample hitting the scene with cos < 0 at the vertex zeroes the vertex (its term and
+// the continuation); the hit is not tested for facing. A zero BSDF density ends the path with
+// the vertex's term (the reference would divide by zero).
+template <class T> V3<T> li_lt2(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg &g) {
+    Scene const &s = c.s;
+    V3<T> L, beta(T(1));
+    if (!first.valid())
+        return L;
+    Hit<T> x = first;
+    V3<T> wo = -ray_v;
+    for (uint32_t depth = 0; depth < kRefModeMaxDepth; depth++) {
+        if (depth >= 1) {
+            if (g.draw() >= 0.5f)
+                break;
+            beta = beta / T(0.5f);
+        }
+        int mat = mat_of<T>(s, x.geo);
+        V3<T> B;
+        if (!s.lights.empty()) {
+            LightSample<T> ls = sample_light<T>(s, g);
+            V3<T> wl;
+            T r2 = 0, cl = 0;
+            bool ok;
+            V3<T> p1 = direct_from<T>(c, ls, x, wo, &wl, &r2, &cl, &ok);
+            LightSample<T> em = sample_light<T>(s, g);
+            T e0 = T(g.draw());
+            T e1 = T(g.draw());
+            V3<T> w = cos_hemisphere(em.n, e0, e1);
+            T w_dens = inner(em.n, w) / pi_v<T>();
+            Hit<T> lv = trace<T>(c, em.p, w);
+            if (lv.valid()) {
+                T cos_e = inner(em.n, w);
+                V3<T> emitted = cos_e > T(0) ? cvt<T>(s.lights[em.light].rad) * cos_e : V3<T>();
+                V3<T> illum = emitted / (em.area_dens * w_dens);
+                V3<T> tray = -w;
+                V3<T> jp = x.p - lv.p;
+                T dist = norm(jp);
+                jp = jp / dist;
+                T cos_w2 = inner(lv.n, tray);
+                T cos_wo = inner(lv.n, jp);
+                T cos_wi = inner(x.n, -jp);
+                if (cos_wo > T(0) && cos_wi > T(0) && cos_w2 > T(0) &&
+                    !shadow<T>(c, lv.p, jp, T(1e-4f), dist - T(1e-3f))) {
+                    V3<T> f2 = mod(illum, bsdf_eval<T>(s, mat_of<T>(s, lv.geo), lv.n, jp, tray)) * cos_w2;
+                    V3<T> p2 = mod(f2 * cos_wo / (dist * dist), bsdf_eval<T>(s, mat, x.n, wo, -jp)) * cos_wi;
+                    B = depth == 0 ? p1 + T(0.5f) * p2 : T(0.5f) * (p1 + p2);
+                } else {
+                    B = p1;
+                }
+            }
+        }
+        T dens;
+        V3<T> wi = bsdf_sample<T>(s, mat, x.n, wo, g, &dens, true);
+        if (dens == T(0)) {
+            L = L + mod(beta, B);
+            break;
+        }
+        Hit<T> nx = trace<T>(c, x.p, wi);
+        if (!nx.valid()) {
+            L = L + mod(beta, B);
+            break;
+        }
+        T cw = inner(x.n, wi);
+        if (cw < T(0))
+            break;
+        L = L + mod(beta, B);
+        V3<T> f = bsdf_eval<T>(s, mat, x.n, wo, wi);
+        beta = mod(beta, f) * cw / dens;
+        x = nx;
+        wo = -wi;
+    }
+    int l0 = s.meshes[first.geo].light;
+    T cos0 = inner(first.n, -ray_v);
+    if (l0 >= 0 && cos0 > T(0)) // area_light::projected_radiance (light.cpp:54-61)
+        L = L + cvt<T>(s.lights[l0].rad) * cos0;
+    return L;
+}
+
 // Reference mode: unidirect_path_tracer (pathtracer.cpp:528-585), BSDF sampling only,
 // emission at every vertex, Russian roulette p = 0.5 from depth 2.
 template <class T> V3<T> li_unidirect(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg &g) {
@@ -1254,6 +1350,14 @@ V3<T> trace_sample(Scene const &s, RenderOpts const &opt, Hit<T> const &first, V
         return li_direct<T>(c, first, ray_v, g);
     case TR_BIDIRECT_MIS:
         return li_bdpt<T>(c, first, ray_v, g);
+    case TR_BIDIRECT_LT2:
+        return li_lt2<T>(c, first, ray_v, g);
+    case TR_POSITION: { // position_tracer (pathtracer.cpp:474-490): (p - min) / (max - min)
+        if (!first.valid())
+            return V3<T>();
+        V3<T> lo = cvt<T>(s.space_lo), r = cvt<T>(s.space_range);
+        return V3<T>((first.p.x - lo.x) / r.x, (first.p.y - lo.y) / r.y, (first.p.z - lo.z) / r.z);
+    }
     case TR_NORMAL: // normal_tracer (pathtracer.cpp:492-506)
         return first.valid() ? (first.n + V3<T>(T(1.0f))) / T(2.0f) : V3<T>();
     default:

@@ -195,6 +195,8 @@ struct Scene {
     std::vector<Prim> prims;
     // per-prim vertex positions cached for traversal
     std::vector<V3f> pv0, pv1, pv2;
+    // if_path_space::aabb (pathspace.cpp:13-18): lower corner and max - min of all vertices
+    V3f space_lo, space_range;
 };
 
 // BVH build (pathspace.cpp:121-328): SAH with 20 buckets, <=4 prims per leaf, median split by

@@ -34,6 +34,7 @@ CASES = [
     ("indirect_l3", "UNIDIRECT", dict()),
     ("indirect_l3", "DIRECT", dict()),
     ("indirect_l3", "NORMAL", dict()),
+    ("indirect_l3", "BIDIRECT_LT2", dict()),
     ("indirect_l3", "BIDIRECT_MIS", dict(max_path_len=6)),
     ("spheres_small", "BIDIRECT_MIS", dict(max_path_len=5, queue_capacity=4000)),
 ]

@@ -175,6 +175,12 @@ RENDER_CASES = [
     ("box", "UNIDIRECT", {}),
     ("box", "DIRECT", {}),
     ("box", "NORMAL", {}),
+    ("box", "POSITION", {}),
+    ("spheres_small", "POSITION", {}),
+    ("box", "BIDIRECT_LT2", {}),
+    ("indirect_l3", "BIDIRECT_LT2", {}),
+    ("spheres_small", "BIDIRECT_LT2", {}),
+    ("furnace", "BIDIRECT_LT2", {}),
     ("spheres_small", "UNIDIRECT_MIS", dict(max_depth=8)),
     ("spheres_small", "UNIDIRECT_LT1", {}),
     ("box", "BIDIRECT_MIS", dict(max_path_len=8)),

@@ -409,6 +409,93 @@ template <int WG, int LDSD> struct Tracer {
         return rad;
     }
 
+    // bidirect_lt2_path_tracer (pathtracer.cpp:655-752), evaluated front to back. At every
+    // camera vertex, join_with_light_paths: one NEE sample p1 and one light path of a single
+    // vertex (emission sample, closest hit) joined to the camera vertex, p2; their mean (at the
+    // first vertex p1 + p2 / 2). As in the reference, a light path that misses the scene makes
+    // the vertex's term zero (p1 included), and a BSDF sample whose hit lies below the vertex's
+    // surface (cos < 0) ends the path without the vertex's term. Russian roulette p = 0.5 from
+    // depth 1. Where the BSDF sample has zero density the path ends with the vertex's term (the
+    // reference divides by that zero).
+    X3D_DI f3 li_lt2(HitRec const &first, bool valid, f3 ray_v, Pcg &g) {
+        f3 L = mk(0.0f, 0.0f, 0.0f), beta = mk(1.0f, 1.0f, 1.0f);
+        if (!valid)
+            return L;
+        HitRec x = first;
+        f3 wo = -ray_v;
+        for (uint32_t depth = 0; depth < kRefModeMaxDepth; depth++) {
+            if (depth >= 1) {
+                if (g.draw() >= 0.5f)
+                    break;
+                beta = beta / 0.5f;
+            }
+            f3 B = mk(0.0f, 0.0f, 0.0f);
+            if (S.num_lights > 0) {
+                LightSample ls = sample_light(S, g);
+                f3 p1 = mk(0.0f, 0.0f, 0.0f), wl;
+                float r2, cl;
+                if (!direct(ls, x, wo, p1, wl, r2, cl))
+                    p1 = mk(0.0f, 0.0f, 0.0f);
+                LightSample em = sample_light(S, g);
+                float e0 = g.draw();
+                float e1 = g.draw();
+                f3 w = cos_hemisphere(em.n, e0, e1);
+                float w_dens = dot(em.n, w) / kPi;
+                HitRec lv;
+                if (trace(em.p, w, lv)) {
+                    float cos_e = dot(em.n, w);
+                    f3 emitted = cos_e > 0.0f ? light_rad(S, em.light) * cos_e : mk(0.0f, 0.0f, 0.0f);
+                    f3 illum = emitted / (em.area_dens * w_dens);
+                    f3 tray = -w;
+                    f3 jp = x.p - lv.p;
+                    float dist = norm(jp);
+                    jp = jp / dist;
+                    float cos_w2 = dot(lv.n, tray);
+                    float cos_wo = dot(lv.n, jp);
+                    float cos_wi = dot(x.n, -jp);
+                    if (cos_wo > 0.0f && cos_wi > 0.0f && cos_w2 > 0.0f && !occluded(lv.p, jp, 1e-4f, dist - 1e-3f)) {
+                        f3 f2 = mul(illum, bsdf_eval(S.mats, lv.mat, lv.n, jp, tray)) * cos_w2;
+                        f3 p2 = mul(f2 * cos_wo / (dist * dist), bsdf_eval(S.mats, x.mat, x.n, wo, -jp)) * cos_wi;
+                        B = depth == 0 ? p1 + 0.5f * p2 : 0.5f * (p1 + p2);
+                    } else {
+                        B = p1;
+                    }
+                }
+            }
+            float dens;
+            f3 wi = bsdf_sample(S.mats, x.mat, x.n, wo, g, &dens, true);
+            if (dens == 0.0f) {
+                L = L + mul(beta, B);
+                break;
+            }
+            HitRec nx;
+            if (!trace(x.p, wi, nx)) {
+                L = L + mul(beta, B);
+                break;
+            }
+            float cw = dot(x.n, wi);
+            if (cw < 0.0f)
+                break;
+            L = L + mul(beta, B);
+            f3 f = bsdf_eval(S.mats, x.mat, x.n, wo, wi);
+            beta = mul(beta, f) * cw / dens;
+            x = nx;
+            wo = -wi;
+        }
+        if (first.light >= 0 && dot(first.n, -ray_v) > 0.0f)
+            L = L + light_rad(S, first.light) * dot(first.n, -ray_v);
+        return L;
+    }
+
+    // position_tracer (pathtracer.cpp:474-490): the first hit's position in the path space's
+    // bounds, (p - min) / (max - min) per axis.
+    X3D_DI f3 li_position(HitRec const &first, bool valid) const {
+        if (!valid)
+            return mk(0.0f, 0.0f, 0.0f);
+        return mk((first.p.x - S.space_lo[0]) / S.space_range[0], (first.p.y - S.space_lo[1]) / S.space_range[1],
+                  (first.p.z - S.space_lo[2]) / S.space_range[2]);
+    }
+
     // direct_path_tracer (pathtracer.cpp:508-526).
     X3D_DI f3 li_direct(HitRec const &first, bool valid, f3 ray_v, Pcg &g) {
         f3 L = mk(0.0f, 0.0f, 0.0f);

@@ -235,6 +235,10 @@ template <int TRACER> __global__ __launch_bounds__(kWG) void k_render_plain(Rend
                 L = tr.li_unidirect(first, valid, v, g);
             else if (TRACER == X3D_PT_DIRECT)
                 L = tr.li_direct(first, valid, v, g);
+            else if (TRACER == X3D_PT_BIDIRECT_LT2)
+                L = tr.li_lt2(first, valid, v, g);
+            else if (TRACER == X3D_PT_POSITION)
+                L = tr.li_position(first, valid);
             else // X3D_PT_NORMAL (normal_tracer, pathtracer.cpp:492-506)
                 L = valid ? (first.n + mk(1.0f, 1.0f, 1.0f)) / 2.0f : mk(0.0f, 0.0f, 0.0f);
             acc = acc + L;
@@ -423,6 +427,10 @@ __global__ __launch_bounds__(kWG) void k_render_plain_items(RenderParams P, uint
             L = tr.li_unidirect(first, valid, v, g);
         else if (TRACER == X3D_PT_DIRECT)
             L = tr.li_direct(first, valid, v, g);
+        else if (TRACER == X3D_PT_BIDIRECT_LT2)
+            L = tr.li_lt2(first, valid, v, g);
+        else if (TRACER == X3D_PT_POSITION)
+            L = tr.li_position(first, valid);
         else // X3D_PT_NORMAL (normal_tracer, pathtracer.cpp:492-506)
             L = valid ? (first.n + mk(1.0f, 1.0f, 1.0f)) / 2.0f : mk(0.0f, 0.0f, 0.0f);
         samples[q] = make_float4(L.x, L.y, L.z, 0.0f);
@@ -700,6 +708,14 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_DIRECT>, npix * spp)
                                   : persistent_grid(k_render_plain<X3D_PT_DIRECT>, npix);
         break;
+    case X3D_PT_BIDIRECT_LT2:
+        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_BIDIRECT_LT2>, npix * spp)
+                                  : persistent_grid(k_render_plain<X3D_PT_BIDIRECT_LT2>, npix);
+        break;
+    case X3D_PT_POSITION:
+        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_POSITION>, npix * spp)
+                                  : persistent_grid(k_render_plain<X3D_PT_POSITION>, npix);
+        break;
     case X3D_PT_BIDIRECT_MIS: grid_render = persistent_grid(k_render_bdpt, npix * static_cast<uint64_t>(spp)); break;
     default:
         grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_NORMAL>, npix * spp)
@@ -781,6 +797,18 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     case X3D_PT_DIRECT:
         rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_DIRECT>)
                          : (launch_plain<X3D_PT_DIRECT>(grid_render, r->stream, P), X3D_OK);
+        if (rc)
+            return rc;
+        break;
+    case X3D_PT_BIDIRECT_LT2:
+        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_BIDIRECT_LT2>)
+                         : (launch_plain<X3D_PT_BIDIRECT_LT2>(grid_render, r->stream, P), X3D_OK);
+        if (rc)
+            return rc;
+        break;
+    case X3D_PT_POSITION:
+        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_POSITION>)
+                         : (launch_plain<X3D_PT_POSITION>(grid_render, r->stream, P), X3D_OK);
         if (rc)
             return rc;
         break;
@@ -1432,8 +1460,10 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
         return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
     switch (opts->pt_type) {
     case X3D_PT_NORMAL:
+    case X3D_PT_POSITION:
     case X3D_PT_DIRECT:
     case X3D_PT_UNIDIRECT:
+    case X3D_PT_BIDIRECT_LT2:
     case X3D_PT_UNIDIRECT_LT1:
     case X3D_PT_UNIDIRECT_MIS:
     case X3D_PT_BIDIRECT_MIS:

@@ -575,6 +575,24 @@ std::string Scene::upload() {
     dev.num_prims = static_cast<uint32_t>(prims.size());
     dev.num_interior = num_interior;
     dev.far_coords = 0;
+    {
+        // aabb::operator+ from an empty box: min / max over all vertices of all meshes
+        bool empty = true;
+        float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (HostMesh const &m : meshes)
+            for (HostVec3 const &v : m.verts) {
+                float c[3] = {v.x, v.y, v.z};
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = empty ? c[a] : std::min(lo[a], c[a]);
+                    hi[a] = empty ? c[a] : std::max(hi[a], c[a]);
+                }
+                empty = false;
+            }
+        for (int a = 0; a < 3; a++) {
+            dev.space_lo[a] = lo[a];
+            dev.space_range[a] = hi[a] - lo[a];
+        }
+    }
     for (int a = 0; a < 3; a++) {
         float lo = flat.empty() ? 0.0f : flat[0].mn[a], hi = flat.empty() ? 0.0f : flat[0].mx[a];
         dev.box_lo[a] = lo;

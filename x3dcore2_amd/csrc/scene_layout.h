@@ -99,6 +99,10 @@ struct DevScene {
     float box_lo[3];  // scene bounds (root box): lower corner
     float box_inv[3]; // 1 / extent per axis (0 for a flat axis)
     uint32_t far_coords; // some box coordinate exceeds 1e30 in magnitude (slab_may_nan)
+    // if_path_space::aabb (pathspace.cpp:13-18): union of the vertex bounds of every mesh
+    // (trimesh::update_aabb, geometry.cpp:97-103); lower corner and extent max - min
+    float space_lo[3];
+    float space_range[3];
 };
 
 } // namespace x3d

@@ -99,9 +99,50 @@ int x3d_scene_add_obj(x3d_scene *scene, const char *path, int32_t material_id, i
 int x3d_scene_add_area_light(x3d_scene *scene, int32_t mesh_id, const float radiance[3],
                              int32_t *out_light_id);
 
+/* ---- objects by id (replace the three if_obj_actuator back-ends — if_path_space,
+ * if_material_container, if_light_sources — as objdb::push_updates drives them with
+ * load(obj, trans) / unload(obj) / commit(): obj.h:46-55, objdb.cpp:59-87,
+ * pathspace.cpp:13-26, materialcontainer.cpp:20-33, lightsources.cpp:11-24).
+ * A scene takes either these calls or the x3d_scene_add_* calls, not both. Objects are keyed by
+ * the caller's obj_id_t; as in the reference's maps, loading an id that is already loaded of
+ * the same protocol changes nothing (objdb unloads a dirty object before it reloads it), and
+ * x3d_scene_commit orders geometries, materials and lights by id (the hit's `geo` is the
+ * geometry's rank in id order; x3d_scene_geometry_obj_id maps it back). Edits are staged: renders
+ * use the last commit, and a keyed scene may be committed again after edits (a failed commit
+ * leaves it uncommitted). ---- */
+#define X3D_OBJ_GEOMETRY 1 /* obj_protocol_geometry */
+#define X3D_OBJ_LIGHT 3    /* obj_protocol_light */
+#define X3D_OBJ_MATERIAL 4 /* obj_protocol_material */
+
+/* if_path_space::load of a trimesh. transform: column-major 4x4 (NULL = identity), applied with
+ * trimesh::transform's float arithmetic (x3d_transform_mesh). The geometry's material is the
+ * material loaded under material_obj_id when has_material != 0 and that material is loaded at
+ * commit, else the fail-safe Lambertian 0.8 (if_material_container::find,
+ * materialcontainer.cpp:7-17). As in the reference, the path-space bounds that
+ * X3D_PT_POSITION uses grow with every geometry load and never shrink. */
+int x3d_scene_load_geometry(x3d_scene *scene, uint64_t obj_id, const float *verts, const float *normals,
+                            const float *uvs, uint32_t num_verts, const uint32_t *tris, uint32_t num_tris,
+                            int32_t has_material, uint64_t material_obj_id, const float *transform);
+/* if_material_container::load (OREN_NAYAR, COOK_TORR; mixtures are added by index). */
+int x3d_scene_load_material(x3d_scene *scene, uint64_t obj_id, const x3d_material_desc *desc);
+/* if_light_sources::load of an area light on the geometry loaded under geometry_obj_id (which
+ * must be loaded at commit). The light follows its geometry's transform (the reference's
+ * area_light::transform keeps the untransformed geometry, light.cpp:83-85). */
+int x3d_scene_load_light(x3d_scene *scene, uint64_t obj_id, uint64_t geometry_obj_id, const float radiance[3]);
+/* unload(obj) of the given protocol (X3D_OBJ_*); unknown ids are ignored. */
+int x3d_scene_unload(x3d_scene *scene, int32_t protocol, uint64_t obj_id);
+/* obj id of committed mesh `geo` (for add_* scenes: geo itself). */
+int x3d_scene_geometry_obj_id(const x3d_scene *scene, int32_t geo, uint64_t *obj_id);
+
+/* trimesh::transform (geometry.cpp:76-87) on the host: positions (M (x, y, z, 1)).cart(),
+ * normals (~M^-1 (n, 1)).cart(), M^-1 = adjugate / det in e8util::mat's float arithmetic
+ * (tensor.h:426-535). X3D_ERR_INVALID_ARGUMENT when |det| <= 1e-5 (the reference throws). */
+int x3d_transform_mesh(const float transform[16], uint32_t num_verts, const float *verts, const float *normals,
+                       float *out_verts, float *out_normals);
+
 /* Builds the SAH BVH on the host with the reference's layout (bvh_path_space_layout::commit,
  * pathspace.cpp:292-328), the light power CDF (lightsources.cpp:45-52), and uploads the
- * scene to device memory. Geometry and lights are immutable afterwards. */
+ * scene to device memory. Scenes built with x3d_scene_add_* are immutable afterwards. */
 int x3d_scene_commit(x3d_scene *scene);
 
 /* Material parameters of a committed scene (no reference counterpart; for inverse rendering):

@@ -360,3 +360,14 @@ def compose_math(fn, x):
     out = np.zeros_like(x)
     _clib().orc_compose_math(fn, x, x.size, out)
     return out
+
+
+def ref_transform_mesh(transform, verts, normals):
+    """The reference's trimesh::transform (oracle/_ref): (verts, normals) or None if it throws."""
+    lib = C.CDLL(REF_SO)
+    lib.ref_transform_mesh.argtypes = [_f32p, C.c_uint32, _f32p, _f32p, _f32p, _f32p]
+    v, n = _f32(verts).reshape(-1, 3), _f32(normals).reshape(-1, 3)
+    ov, on = np.zeros_like(v), np.zeros_like(n)
+    rc = lib.ref_transform_mesh(_f32(transform).reshape(-1), v.shape[0], v.reshape(-1), n.reshape(-1),
+                                ov.reshape(-1), on.reshape(-1))
+    return None if rc != 0 else (ov, on)

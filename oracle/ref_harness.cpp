@@ -365,4 +365,32 @@ uint32_t ref_render(void *h, int pt_type, float sensor, float focal, float aspec
     return allocated * threads;
 }
 
+// trimesh::transform (geometry.cpp:76-87) of a mesh with the given column-major 4x4 matrix
+// (e8util::mat44's initializer order): out_verts / out_normals float[3*nv]. Returns 0, or -1
+// when the reference throws (matrix not invertible).
+int ref_transform_mesh(float const *trans, uint32_t nv, float const *verts, float const *normals,
+                       float *out_verts, float *out_normals) {
+    e8util::mat44 m({trans[0], trans[1], trans[2], trans[3], trans[4], trans[5], trans[6], trans[7], trans[8],
+                     trans[9], trans[10], trans[11], trans[12], trans[13], trans[14], trans[15]});
+    e8::trimesh mesh("mesh");
+    std::vector<e8util::vec3> vv, nn;
+    for (uint32_t i = 0; i < nv; i++) {
+        vv.push_back(v3(verts + 3 * i));
+        nn.push_back(v3(normals + 3 * i));
+    }
+    mesh.vertices(vv);
+    mesh.normals(nn);
+    try {
+        std::unique_ptr<e8::if_geometry> t = mesh.transform(m);
+        for (uint32_t i = 0; i < nv; i++)
+            for (unsigned c = 0; c < 3; c++) {
+                out_verts[3 * i + c] = t->vertices()[i](c);
+                out_normals[3 * i + c] = t->normals()[i](c);
+            }
+    } catch (std::string const &) {
+        return -1;
+    }
+    return 0;
+}
+
 } // extern "C"

@@ -25,6 +25,7 @@ FORM_PLAIN, FORM_WAVEFRONT = 0, 1
 SORT_DEFAULT, SORT_MATERIAL, SORT_NONE = 0, 1, 2
 RAY_ORDER_DEFAULT, RAY_ORDER_SORTED, RAY_ORDER_APPEND = 0, 1, 2
 COMPOSITOR_CLAMP, COMPOSITOR_ACES = 0, 1
+OBJ_GEOMETRY, OBJ_LIGHT, OBJ_MATERIAL = 1, 3, 4
 TRAVERSAL_CULLED, TRAVERSAL_REFERENCE = 0, 1
 
 
@@ -122,6 +123,8 @@ EXPORTS = [
     "x3d_obj_load", "x3d_obj_free", "x3d_scene_add_obj",
     "x3d_compositor_create", "x3d_compositor_destroy", "x3d_compositor_enable_auto_exposure",
     "x3d_compositor_set_exposure", "x3d_compositor_commit", "x3d_compositor_commit_device",
+    "x3d_scene_load_geometry", "x3d_scene_load_material", "x3d_scene_load_light", "x3d_scene_unload",
+    "x3d_scene_geometry_obj_id", "x3d_transform_mesh",
 ]
 
 
@@ -182,6 +185,12 @@ def lib() -> C.CDLL:
     L.x3d_compositor_set_exposure.argtypes = [vp, C.c_float]
     L.x3d_compositor_commit.argtypes = [vp, vp, u32, u32, vp, C.POINTER(ComposeStats)]
     L.x3d_compositor_commit_device.argtypes = [vp, vp, u32, u32, vp, C.POINTER(ComposeStats)]
+    L.x3d_scene_load_geometry.argtypes = [vp, u64, vp, vp, vp, u32, vp, u32, i32, u64, vp]
+    L.x3d_scene_load_material.argtypes = [vp, u64, C.POINTER(MaterialDesc)]
+    L.x3d_scene_load_light.argtypes = [vp, u64, u64, fp]
+    L.x3d_scene_unload.argtypes = [vp, i32, u64]
+    L.x3d_scene_geometry_obj_id.argtypes = [vp, i32, C.POINTER(u64)]
+    L.x3d_transform_mesh.argtypes = [vp, u32, vp, vp, vp, vp]
     missing = [n for n in EXPORTS if n not in ("x3d_last_error", "x3d_version") and getattr(L, n).argtypes is None]
     if missing:  # a pointer passed without argtypes would be truncated to a C int
         raise ImportError(f"no ctypes signature for {missing}")
@@ -229,6 +238,43 @@ class Scene:
         if getattr(self, "h", None) and _lib is not None:
             _lib.x3d_scene_destroy(self.h)
             self.h = None
+
+    @staticmethod
+    def _desc(m) -> MaterialDesc:
+        d = MaterialDesc()
+        d.kind = m.kind
+        for k in range(3):
+            d.albedo[k] = m.albedo[k]
+        d.sigma, d.roughness = m.sigma, m.roughness
+        d.ior_re, d.ior_im = m.ior
+        d.sub0, d.sub1, d.ratio = m.sub0, m.sub1, m.ratio
+        return d
+
+    # ---- objects by id (the if_obj_actuator protocol: load / unload / commit) ----
+    def load_geometry(self, obj_id, verts, normals, tris, material=None, transform=None):
+        """material: obj id of a material (None: the fail-safe); transform: 4x4 column-major
+        (16 floats) or None."""
+        v, n = _f32(verts), _f32(normals)
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        m = None if transform is None else _f32(transform).reshape(-1)
+        _check(lib().x3d_scene_load_geometry(self.h, int(obj_id), _ptr(v), _ptr(n), None, v.shape[0], _ptr(t),
+                                             t.shape[0], 0 if material is None else 1,
+                                             0 if material is None else int(material), _ptr(m)))
+
+    def load_material(self, obj_id, m):
+        _check(lib().x3d_scene_load_material(self.h, int(obj_id), C.byref(self._desc(m))))
+
+    def load_light(self, obj_id, geometry_obj_id, radiance):
+        r = (C.c_float * 3)(*radiance)
+        _check(lib().x3d_scene_load_light(self.h, int(obj_id), int(geometry_obj_id), r))
+
+    def unload(self, protocol, obj_id):
+        _check(lib().x3d_scene_unload(self.h, int(protocol), int(obj_id)))
+
+    def geometry_obj_id(self, geo) -> int:
+        out = C.c_uint64()
+        _check(lib().x3d_scene_geometry_obj_id(self.h, int(geo), C.byref(out)))
+        return out.value
 
     def add_material(self, m) -> int:
         d = MaterialDesc()
@@ -487,3 +533,12 @@ class Compositor:
         _check(lib().x3d_compositor_commit_device(self.h, C.c_void_p(d_film), width, height, C.c_void_p(d_frame),
                                                   C.byref(st)))
         return st
+
+
+def transform_mesh(transform, verts, normals):
+    """x3d_transform_mesh: trimesh::transform of (verts, normals) by a column-major 4x4."""
+    m = _f32(transform).reshape(-1)
+    v, n = _f32(verts).reshape(-1, 3), _f32(normals).reshape(-1, 3)
+    ov, on = np.zeros_like(v), np.zeros_like(n)
+    _check(lib().x3d_transform_mesh(_ptr(m), v.shape[0], _ptr(v), _ptr(n), _ptr(ov), _ptr(on)))
+    return ov, on

@@ -1020,12 +1020,12 @@ int x3d_scene_destroy(x3d_scene *scene) {
     return X3D_OK;
 }
 
-int x3d_scene_add_material(x3d_scene *scene, const x3d_material_desc *d, int32_t *out_id) {
-    if (!scene || !d)
-        return fail(X3D_ERR_INVALID_ARGUMENT, "scene or desc is null");
-    if (scene->s.committed)
-        return fail(X3D_ERR_STATE, "scene already committed");
-    DevMaterial m{};
+namespace {
+
+// Material description -> device record (oren_nayar, cook_torr, mat_mixture constructors);
+// mixture operands index `existing`.
+int make_material(const x3d_material_desc *d, std::vector<DevMaterial> const &existing, DevMaterial &m) {
+    m = DevMaterial{};
     m.kind = d->kind;
     for (int c = 0; c < 3; c++)
         m.albedo[c] = d->albedo[c];
@@ -1046,10 +1046,10 @@ int x3d_scene_add_material(x3d_scene *scene, const x3d_material_desc *d, int32_t
         m.ior_re = d->ior_re;
         m.ior_im = d->ior_im;
     } else if (d->kind == X3D_MAT_MIXTURE) {
-        int32_t n = static_cast<int32_t>(scene->s.materials.size());
+        int32_t n = static_cast<int32_t>(existing.size());
         if (d->sub0 < 0 || d->sub0 >= n || d->sub1 < 0 || d->sub1 >= n)
             return fail(X3D_ERR_INVALID_ARGUMENT, "mixture operand out of range");
-        if (scene->s.materials[d->sub0].kind == kMatMixture || scene->s.materials[d->sub1].kind == kMatMixture)
+        if (existing[d->sub0].kind == kMatMixture || existing[d->sub1].kind == kMatMixture)
             return fail(X3D_ERR_UNSUPPORTED, "nested mixtures are not supported");
         if (!(d->ratio >= 0.0f && d->ratio <= 1.0f))
             return fail(X3D_ERR_INVALID_ARGUMENT, "mixture ratio must be in [0, 1]");
@@ -1059,6 +1059,72 @@ int x3d_scene_add_material(x3d_scene *scene, const x3d_material_desc *d, int32_t
     } else {
         return fail(X3D_ERR_INVALID_ARGUMENT, "unknown material kind");
     }
+    return X3D_OK;
+}
+
+// Validated mesh arrays -> host mesh. Without a transform the vertices pass through the
+// identity transform the reference loads meshes with (pathspace.cpp:15-19, geometry.cpp:76-87),
+// which maps -0 to +0 (adding +0 does the same); with one, trimesh::transform's arithmetic.
+int make_mesh(const float *verts, const float *normals, uint32_t num_verts, const uint32_t *tris, uint32_t num_tris,
+              const float *transform, HostMesh &m) {
+    if ((!verts && num_verts) || (!normals && num_verts) || (!tris && num_tris))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null mesh array");
+    m.verts.resize(num_verts);
+    m.normals.resize(num_verts);
+    for (uint32_t i = 0; i < num_verts; i++) {
+        if (!finite3(verts + 3 * i) || !finite3(normals + 3 * i))
+            return fail(X3D_ERR_INVALID_ARGUMENT, "vertex " + std::to_string(i) + " is not finite");
+        m.verts[i] = {verts[3 * i] + 0.0f, verts[3 * i + 1] + 0.0f, verts[3 * i + 2] + 0.0f};
+        m.normals[i] = {normals[3 * i] + 0.0f, normals[3 * i + 1] + 0.0f, normals[3 * i + 2] + 0.0f};
+    }
+    if (transform) {
+        for (int k = 0; k < 16; k++)
+            if (!std::isfinite(transform[k]))
+                return fail(X3D_ERR_INVALID_ARGUMENT, "transform is not finite");
+        for (uint32_t i = 0; i < num_verts; i++) {
+            m.verts[i] = {verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+            m.normals[i] = {normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]};
+        }
+        std::string e = x3d::transform_mesh(transform, m.verts, m.normals);
+        if (!e.empty())
+            return fail(X3D_ERR_INVALID_ARGUMENT, e);
+        for (uint32_t i = 0; i < num_verts; i++)
+            if (!finite3(&m.verts[i].x) || !finite3(&m.normals[i].x))
+                return fail(X3D_ERR_INVALID_ARGUMENT, "transformed vertex " + std::to_string(i) + " is not finite");
+    }
+    m.tris.assign(tris, tris + 3ull * num_tris);
+    for (uint32_t v : m.tris)
+        if (v >= num_verts)
+            return fail(X3D_ERR_INVALID_ARGUMENT, "triangle index out of range");
+    return X3D_OK;
+}
+
+int require_indexed(x3d_scene *scene) {
+    if (scene->s.keyed)
+        return fail(X3D_ERR_STATE, "the scene takes objects by id (x3d_scene_load_*)");
+    if (scene->s.committed)
+        return fail(X3D_ERR_STATE, "scene already committed");
+    return X3D_OK;
+}
+
+int require_keyed(x3d_scene *scene) {
+    if (!scene->s.keyed && (!scene->s.meshes.empty() || !scene->s.materials.empty() || !scene->s.lights.empty() ||
+                            scene->s.committed))
+        return fail(X3D_ERR_STATE, "the scene takes objects by index (x3d_scene_add_*)");
+    scene->s.keyed = true;
+    return X3D_OK;
+}
+
+} // namespace
+
+int x3d_scene_add_material(x3d_scene *scene, const x3d_material_desc *d, int32_t *out_id) {
+    if (!scene || !d)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene or desc is null");
+    if (int rc = require_indexed(scene))
+        return rc;
+    DevMaterial m;
+    if (int rc = make_material(d, scene->s.materials, m))
+        return rc;
     scene->s.materials.push_back(m);
     if (out_id)
         *out_id = static_cast<int32_t>(scene->s.materials.size()) - 1;
@@ -1069,31 +1135,118 @@ int x3d_scene_add_mesh(x3d_scene *scene, const float *verts, const float *normal
                        uint32_t num_verts, const uint32_t *tris, uint32_t num_tris, int32_t material_id,
                        int32_t *out_mesh_id) {
     (void)uvs;
-    if (!scene || (!verts && num_verts) || (!normals && num_verts) || (!tris && num_tris))
-        return fail(X3D_ERR_INVALID_ARGUMENT, "null mesh array");
-    if (scene->s.committed)
-        return fail(X3D_ERR_STATE, "scene already committed");
+    if (!scene)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene is null");
+    if (int rc = require_indexed(scene))
+        return rc;
     if (material_id >= static_cast<int32_t>(scene->s.materials.size()))
         return fail(X3D_ERR_INVALID_ARGUMENT, "material id out of range");
     HostMesh m;
-    m.verts.resize(num_verts);
-    m.normals.resize(num_verts);
-    for (uint32_t i = 0; i < num_verts; i++) {
-        if (!finite3(verts + 3 * i) || !finite3(normals + 3 * i))
-            return fail(X3D_ERR_INVALID_ARGUMENT, "vertex " + std::to_string(i) + " is not finite");
-        // The reference loads meshes through an identity transform (pathspace.cpp:15-19,
-        // geometry.cpp:76-87), which maps -0 to +0; adding +0 does the same.
-        m.verts[i] = {verts[3 * i] + 0.0f, verts[3 * i + 1] + 0.0f, verts[3 * i + 2] + 0.0f};
-        m.normals[i] = {normals[3 * i] + 0.0f, normals[3 * i + 1] + 0.0f, normals[3 * i + 2] + 0.0f};
-    }
-    m.tris.assign(tris, tris + 3ull * num_tris);
-    for (uint32_t v : m.tris)
-        if (v >= num_verts)
-            return fail(X3D_ERR_INVALID_ARGUMENT, "triangle index out of range");
+    if (int rc = make_mesh(verts, normals, num_verts, tris, num_tris, nullptr, m))
+        return rc;
     m.material = material_id < 0 ? -1 : material_id;
+    scene->s.grow_space(m.verts);
     scene->s.meshes.push_back(std::move(m));
     if (out_mesh_id)
         *out_mesh_id = static_cast<int32_t>(scene->s.meshes.size()) - 1;
+    return X3D_OK;
+}
+
+int x3d_scene_load_geometry(x3d_scene *scene, uint64_t obj_id, const float *verts, const float *normals,
+                            const float *uvs, uint32_t num_verts, const uint32_t *tris, uint32_t num_tris,
+                            int32_t has_material, uint64_t material_obj_id, const float *transform) {
+    (void)uvs;
+    if (!scene)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene is null");
+    if (int rc = require_keyed(scene))
+        return rc;
+    x3d::KeyedGeometry g;
+    if (int rc = make_mesh(verts, normals, num_verts, tris, num_tris, transform, g.mesh))
+        return rc;
+    g.has_material = has_material != 0;
+    g.material = material_obj_id;
+    // if_path_space::load: the bound grows before the map insert, which keeps an existing entry
+    scene->s.grow_space(g.mesh.verts);
+    scene->s.k_geos.insert(std::make_pair(obj_id, std::move(g)));
+    return X3D_OK;
+}
+
+int x3d_scene_load_material(x3d_scene *scene, uint64_t obj_id, const x3d_material_desc *d) {
+    if (!scene || !d)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene or desc is null");
+    if (int rc = require_keyed(scene))
+        return rc;
+    if (d->kind == X3D_MAT_MIXTURE)
+        return fail(X3D_ERR_UNSUPPORTED, "mixtures are added by index (x3d_scene_add_material)");
+    DevMaterial m;
+    if (int rc = make_material(d, {}, m))
+        return rc;
+    scene->s.k_mats.insert(std::make_pair(obj_id, m));
+    return X3D_OK;
+}
+
+int x3d_scene_load_light(x3d_scene *scene, uint64_t obj_id, uint64_t geometry_obj_id, const float radiance[3]) {
+    if (!scene || !radiance)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene or radiance is null");
+    if (!finite3(radiance) || radiance[0] < 0 || radiance[1] < 0 || radiance[2] < 0)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "radiance must be finite and non-negative");
+    if (int rc = require_keyed(scene))
+        return rc;
+    x3d::KeyedLight l;
+    l.geometry = geometry_obj_id;
+    for (int c = 0; c < 3; c++)
+        l.rad[c] = radiance[c];
+    scene->s.k_lights.insert(std::make_pair(obj_id, l));
+    return X3D_OK;
+}
+
+int x3d_scene_unload(x3d_scene *scene, int32_t protocol, uint64_t obj_id) {
+    if (!scene)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene is null");
+    if (int rc = require_keyed(scene))
+        return rc;
+    if (protocol == X3D_OBJ_GEOMETRY)
+        scene->s.k_geos.erase(obj_id);
+    else if (protocol == X3D_OBJ_MATERIAL)
+        scene->s.k_mats.erase(obj_id);
+    else if (protocol == X3D_OBJ_LIGHT)
+        scene->s.k_lights.erase(obj_id);
+    else
+        return fail(X3D_ERR_INVALID_ARGUMENT, "unknown object protocol");
+    return X3D_OK;
+}
+
+int x3d_scene_geometry_obj_id(const x3d_scene *scene, int32_t geo, uint64_t *obj_id) {
+    if (!scene || !obj_id)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    if (!scene->s.committed)
+        return fail(X3D_ERR_STATE, "scene not committed");
+    if (geo < 0 || static_cast<size_t>(geo) >= scene->s.meshes.size())
+        return fail(X3D_ERR_INVALID_ARGUMENT, "geometry index out of range");
+    *obj_id = scene->s.keyed ? scene->s.geo_ids[geo] : static_cast<uint64_t>(geo);
+    return X3D_OK;
+}
+
+int x3d_transform_mesh(const float transform[16], uint32_t num_verts, const float *verts, const float *normals,
+                       float *out_verts, float *out_normals) {
+    if (!transform || ((!verts || !normals || !out_verts || !out_normals) && num_verts))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<HostVec3> v(num_verts), n(num_verts);
+    for (uint32_t i = 0; i < num_verts; i++) {
+        v[i] = {verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+        n[i] = {normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]};
+    }
+    std::string e = x3d::transform_mesh(transform, v, n);
+    if (!e.empty())
+        return fail(X3D_ERR_INVALID_ARGUMENT, e);
+    for (uint32_t i = 0; i < num_verts; i++) {
+        out_verts[3 * i] = v[i].x;
+        out_verts[3 * i + 1] = v[i].y;
+        out_verts[3 * i + 2] = v[i].z;
+        out_normals[3 * i] = n[i].x;
+        out_normals[3 * i + 1] = n[i].y;
+        out_normals[3 * i + 2] = n[i].z;
+    }
     return X3D_OK;
 }
 
@@ -1152,8 +1305,8 @@ int x3d_scene_add_obj(x3d_scene *scene, const char *path, int32_t material_id, i
 int x3d_scene_add_area_light(x3d_scene *scene, int32_t mesh_id, const float radiance[3], int32_t *out_light_id) {
     if (!scene || !radiance)
         return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
-    if (scene->s.committed)
-        return fail(X3D_ERR_STATE, "scene already committed");
+    if (int rc = require_indexed(scene))
+        return rc;
     if (mesh_id < 0 || mesh_id >= static_cast<int32_t>(scene->s.meshes.size()))
         return fail(X3D_ERR_INVALID_ARGUMENT, "mesh id out of range");
     if (!finite3(radiance))
@@ -1213,6 +1366,12 @@ int x3d_scene_set_material_params(x3d_scene *scene, uint32_t first, uint32_t cou
     }
     X3D_HIP(hipMemcpy(s.buf.mats, s.materials.data(), s.materials.size() * sizeof(DevMaterial),
                       hipMemcpyHostToDevice));
+    if (s.keyed) { // materials are committed in id order: keep the stored objects in step
+        auto it = s.k_mats.begin();
+        std::advance(it, first);
+        for (uint32_t k = 0; k < count && it != s.k_mats.end(); k++, ++it)
+            it->second = s.materials[first + k];
+    }
     return X3D_OK;
 }
 

@@ -226,13 +226,135 @@ inline float bits_f(uint32_t u) {
 
 } // namespace
 
-Scene::~Scene() {
+namespace {
+
+// e8util::mat44 element (row i, column j) of a column-major array.
+inline float at(float const m[16], unsigned i, unsigned j) { return m[4 * j + i]; }
+
+// Determinant of the submatrix of m on `rows` x `cols` (ascending index lists) by cofactor
+// expansion along its first column, summed from 0 in row order with alternating signs: the
+// recursion of mat::det(next_row, curr_col, removed_col) (tensor.h:445-467).
+float cofactor_det(float const m[16], unsigned const *rows, unsigned const *cols, unsigned n) {
+    if (n == 1)
+        return at(m, rows[0], cols[0]);
+    float d = 0.0f;
+    unsigned sub[4];
+    for (unsigned i = 0; i < n; i++) {
+        for (unsigned r = 0, k = 0; r < n; r++)
+            if (r != i)
+                sub[k++] = rows[r];
+        float minor = cofactor_det(m, sub, cols + 1, n - 1);
+        float a = at(m, rows[i], cols[0]);
+        d += (i & 1 ? -a : a) * minor;
+    }
+    return d;
+}
+
+// mat * vec4 (tensor.h:426-435), then vec4::cart (tensor.h:261-268).
+HostVec3 apply(float const m[16], HostVec3 p) {
+    float h[4] = {p.x, p.y, p.z, 1.0f}, r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (unsigned j = 0; j < 4; j++)
+        for (unsigned i = 0; i < 4; i++)
+            r[i] = r[i] + at(m, i, j) * h[j];
+    float inv_w = 1.0f / r[3];
+    return {inv_w * r[0], inv_w * r[1], inv_w * r[2]};
+}
+
+} // namespace
+
+std::string transform_mesh(float const m[16], std::vector<HostVec3> &verts, std::vector<HostVec3> &normals) {
+    // mat::adjugate (tensor.h:478-497): adj(j, i) = (-1)^(i+j) minor(i, j)
+    float adj[16];
+    for (unsigned j = 0; j < 4; j++)
+        for (unsigned i = 0; i < 4; i++) {
+            unsigned rows[3], cols[3];
+            for (unsigned r = 0, k = 0; r < 4; r++)
+                if (r != i)
+                    rows[k++] = r;
+            for (unsigned c = 0, k = 0; c < 4; c++)
+                if (c != j)
+                    cols[k++] = c;
+            float mij = cofactor_det(m, rows, cols, 3);
+            adj[4 * i + j] = (i + j) & 1 ? -mij : mij; // column i, row j
+        }
+    // mat::det(adj) (tensor.h:470-476) and operator^(-1) (tensor.h:511-535)
+    float d = 0.0f;
+    for (unsigned i = 0; i < 4; i++)
+        d += at(m, i, 0) * adj[4 * i + 0];
+    if (std::fabs(d - 0.0f) <= 1e-5f)
+        return "transform is not invertible";
+    float k = 1.0f / d, nt[16];
+    for (unsigned j = 0; j < 4; j++)
+        for (unsigned i = 0; i < 4; i++)
+            nt[4 * i + j] = k * adj[4 * j + i]; // transpose of the inverse
+    for (HostVec3 &v : verts)
+        v = apply(m, v);
+    for (HostVec3 &n : normals)
+        n = apply(nt, n);
+    return "";
+}
+
+void Scene::grow_space(std::vector<HostVec3> const &verts) {
+    for (HostVec3 const &v : verts) {
+        float c[3] = {v.x, v.y, v.z};
+        for (int a = 0; a < 3; a++) {
+            space_lo[a] = space_empty ? c[a] : std::min(space_lo[a], c[a]);
+            space_hi[a] = space_empty ? c[a] : std::max(space_hi[a], c[a]);
+        }
+        space_empty = false;
+    }
+}
+
+void Scene::release() {
     void *ptrs[] = {buf.nodes, buf.nodes4, buf.tris, buf.tri_attr, buf.mats, buf.lights, buf.light_cum_power,
                     buf.light_tris, buf.light_tri_cum};
     for (void *q : ptrs)
         if (q)
             (void)hipFree(q);
+    buf = DeviceBuffers{};
+    dev = DevScene{};
+    flat.clear();
+    prims.clear();
+    committed = false;
 }
+
+// The actuators' maps in id order (bvh_path_space_layout::commit, pathspace.cpp:292-309;
+// basic_light_sources::commit, lightsources.cpp:45-52): materials, then meshes with their
+// material resolved (a missing one is the fail-safe, if_material_container::find,
+// materialcontainer.cpp:7-17), then lights on their geometry's mesh.
+std::string Scene::flatten_keyed() {
+    materials.clear();
+    meshes.clear();
+    lights.clear();
+    geo_ids.clear();
+    std::map<uint64_t, int32_t> mat_index, geo_index;
+    for (auto const &kv : k_mats) {
+        mat_index[kv.first] = static_cast<int32_t>(materials.size());
+        materials.push_back(kv.second);
+    }
+    for (auto const &kv : k_geos) {
+        HostMesh m = kv.second.mesh;
+        auto it = kv.second.has_material ? mat_index.find(kv.second.material) : mat_index.end();
+        m.material = it == mat_index.end() ? -1 : it->second;
+        geo_index[kv.first] = static_cast<int32_t>(meshes.size());
+        geo_ids.push_back(kv.first);
+        meshes.push_back(std::move(m));
+    }
+    for (auto const &kv : k_lights) {
+        auto it = geo_index.find(kv.second.geometry);
+        if (it == geo_index.end())
+            return "light " + std::to_string(kv.first) + ": its geometry " + std::to_string(kv.second.geometry) +
+                   " is not loaded";
+        HostLight l;
+        l.mesh = it->second;
+        for (int c = 0; c < 3; c++)
+            l.rad[c] = kv.second.rad[c];
+        lights.push_back(l);
+    }
+    return "";
+}
+
+Scene::~Scene() { release(); }
 
 void Scene::build_bvh() {
     std::vector<BuildPrim> p;
@@ -265,9 +387,15 @@ void Scene::build_bvh() {
 }
 
 std::string Scene::commit() {
-    if (committed)
+    if (committed && !keyed)
         return "scene already committed";
     auto t0 = std::chrono::steady_clock::now();
+    if (keyed) {
+        release();
+        std::string e = flatten_keyed();
+        if (!e.empty())
+            return e;
+    }
     size_t total_tris = 0;
     for (auto const &m : meshes)
         total_tris += m.tris.size() / 3;
@@ -575,23 +703,9 @@ std::string Scene::upload() {
     dev.num_prims = static_cast<uint32_t>(prims.size());
     dev.num_interior = num_interior;
     dev.far_coords = 0;
-    {
-        // aabb::operator+ from an empty box: min / max over all vertices of all meshes
-        bool empty = true;
-        float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-        for (HostMesh const &m : meshes)
-            for (HostVec3 const &v : m.verts) {
-                float c[3] = {v.x, v.y, v.z};
-                for (int a = 0; a < 3; a++) {
-                    lo[a] = empty ? c[a] : std::min(lo[a], c[a]);
-                    hi[a] = empty ? c[a] : std::max(hi[a], c[a]);
-                }
-                empty = false;
-            }
-        for (int a = 0; a < 3; a++) {
-            dev.space_lo[a] = lo[a];
-            dev.space_range[a] = hi[a] - lo[a];
-        }
+    for (int a = 0; a < 3; a++) {
+        dev.space_lo[a] = space_lo[a];
+        dev.space_range[a] = space_hi[a] - space_lo[a];
     }
     for (int a = 0; a < 3; a++) {
         float lo = flat.empty() ? 0.0f : flat[0].mn[a], hi = flat.empty() ? 0.0f : flat[0].mx[a];

@@ -35,6 +35,7 @@ struct alignas(256) WfCounter {
 struct WfCtl {
     WfCounter act_count[2][4];
     WfCounter sq_count, sq_head, eq_count, eq_head, fin_count;
+    WfCounter eq_count_b, fin_count_b; // second parity of the fused form's double-buffered queues
     uint32_t gen;
     uint32_t pad[2];
 };
@@ -75,6 +76,15 @@ struct WfArgs {
     uint32_t *ray_key[2], *ray_rank[2]; // per queue entry
     uint32_t *bin_mat[2];               // [kSortBlocks][kSortBins] counts, then offsets
     uint32_t *bin_tot[2];               // [kSortBins]
+    // Consumer (in) and producer (out) sides of the extension and finishing queues: the same
+    // buffers and counters, or, in the fused form, the two parities of a pair double-buffered by
+    // bounce (shade reads bounce k's queues while it appends bounce k + 1's).
+    const uint32_t *fin_in;
+    uint32_t *eq_out, *fin_out;
+    WfCounter *eq_cnt_in, *fin_cnt_in, *eq_cnt_out, *fin_cnt_out;
+    // Fused form (no material sort, no ray sort): generate and shade also run the next vertex's
+    // light sample and BSDF sample (no nee_bsdf launch, no active lists).
+    int fused;
 };
 
 // Ray keys: direction octant (3 bits) above the Morton code of the origin's cell in an 8^3 grid
@@ -224,6 +234,22 @@ __global__ void k_wf_reset(WfCtl *ctl, int parity) {
         ctl->eq_count.v = 0;
         ctl->eq_head.v = 0;
         ctl->fin_count.v = 0;
+        if (parity < 0) {
+            ctl->eq_count_b.v = 0;
+            ctl->fin_count_b.v = 0;
+        }
+    }
+}
+
+// Fused form, between a bounce's extension kernel and its shade: the shadow queue and both
+// fetch cursors restart, and the next bounce's extension and finishing queues start empty.
+__global__ void k_wf_reset_fused(WfCtl *ctl, WfCounter *eq_out, WfCounter *fin_out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->sq_count.v = 0;
+        ctl->sq_head.v = 0;
+        ctl->eq_head.v = 0;
+        eq_out->v = 0;
+        fin_out->v = 0;
     }
 }
 
@@ -233,7 +259,85 @@ X3D_DI uint32_t gen_count(WfArgs const &A, uint32_t gen) {
     return left < A.gen_samples ? left : A.gen_samples;
 }
 
-__global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
+// Next-event estimation and BSDF sampling of a live path at vertex x (Tracer::direct with the MIS
+// weight of Tracer::mis_bounce, then the BSDF sample): draws the light sample, then the BSDF
+// sample, from the path's PCG state; stores the advanced state and the sampled direction.
+struct NeeSample {
+    bool has_shadow, ok;
+    f3 sc, sdir, edir;
+    float sdist;
+};
+
+X3D_DI NeeSample nee_sample(WfArgs const &A, uint32_t path, HitRec const &x, f3 wo, f3 beta, uint32_t pix,
+                            uint64_t state) {
+    NeeSample r;
+    r.has_shadow = false;
+    r.sc = r.sdir = mk(0.0f, 0.0f, 0.0f);
+    r.sdist = 0.0f;
+    Pcg g;
+    g.state = state;
+    g.inc = (static_cast<uint64_t>(pix) << 1u) | 1u;
+    if (A.S.num_lights > 0) {
+        LightSample ls = sample_light(A.S, g);
+        f3 l = x.p - ls.p;
+        float r2 = dot(l, l);
+        f3 inv = l / __builtin_sqrtf(r2);
+        float co = dot(ls.n, inv);
+        float ci = dot(x.n, -inv);
+        if (co > 0 && ci > 0) {
+            f3 illum = ((light_rad(A.S, ls.light) * ci) * co) / r2;
+            if (!near0(illum)) {
+                float dist = norm(l);
+                f3 wl = -l / dist;
+                f3 f = bsdf_eval(A.S.mats, x.mat, x.n, wo, wl);
+                f3 d = mul(illum, f) / ls.area_dens;
+                float cl = dot(ls.n, -wl);
+                float pl = ls.area_dens * r2 / cl;
+                float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
+                float w = pl * pl / (pl * pl + pb * pb);
+                r.sc = mul(beta, d * w);
+                r.sdist = dist;
+                r.sdir = wl;
+                r.has_shadow = true;
+            }
+        }
+    }
+    float dens;
+    f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, false);
+    A.rng[path] = g.state;
+    r.ok = dens > 0.0f;
+    if (r.ok)
+        A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
+    r.edir = wi;
+    return r;
+}
+
+// Queues a sampled vertex's shadow ray and extension ray (or finishes the path when the BSDF
+// sample failed). Block-wide: every thread of the block calls it at one uniform call site
+// (block_append synchronizes); `scratch` holds 6 zeroed words.
+X3D_DI void push_rays(WfArgs const &A, bool in, uint32_t path, f3 xp3, NeeSample const &r, uint32_t *scratch) {
+    uint32_t *const queues[3] = {&A.ctl->sq_count.v, &A.eq_cnt_out->v, &A.fin_cnt_out->v};
+    const bool want[3] = {in && r.has_shadow, in && r.ok, in && !r.ok};
+    uint32_t idx[3];
+    block_append<3>(queues, want, idx, scratch);
+    if (want[0]) {
+        uint32_t q = idx[0];
+        A.sq[3 * q] = make_float4(xp3.x, xp3.y, xp3.z, r.sdist - 1e-3f);
+        A.sq[3 * q + 1] = make_float4(r.sdir.x, r.sdir.y, r.sdir.z, __uint_as_float(path));
+        A.sq[3 * q + 2] = make_float4(r.sc.x, r.sc.y, r.sc.z, 0.0f);
+        if (A.sort_rays)
+            A.ray_key[0][q] = ray_key(A.S, xp3, r.sdir);
+    }
+    if (want[1]) {
+        A.eq_out[idx[1]] = path;
+        if (A.sort_rays)
+            A.ray_key[1][idx[1]] = ray_key(A.S, xp3, r.edir);
+    }
+    if (want[2])
+        A.fin_out[idx[2]] = path;
+}
+
+template <bool FUSED> __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
     __shared__ uint32_t scratch[2 * kKinds];
     clear_scratch(scratch, 2 * kKinds);
     uint32_t gen = A.ctl->gen;
@@ -243,6 +347,9 @@ __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
         uint32_t id = b + threadIdx.x;
         bool live = false;
         int kind = 0;
+        NeeSample r;
+        r.has_shadow = r.ok = false;
+        f3 xp3 = mk(0.0f, 0.0f, 0.0f);
         if (id < n) {
             uint32_t s_local = id / A.npix, pix = id - s_local * A.npix;
             uint32_t s = A.s_begin + gen * A.gen_samples + s_local;
@@ -267,11 +374,19 @@ __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
                 A.wo[id] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
                 A.beta[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
                 A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
-                A.rng[id] = g.state;
-                kind = kind_of(A.S, x.mat);
+                if (FUSED) {
+                    xp3 = x.p;
+                    r = nee_sample(A, id, x, wo, mk(1.0f, 1.0f, 1.0f), pix, g.state);
+                } else {
+                    A.rng[id] = g.state;
+                    kind = kind_of(A.S, x.mat);
+                }
             }
         }
-        push_active(A, 0, kind, id, live, scratch);
+        if (FUSED)
+            push_rays(A, live, id, xp3, r, scratch);
+        else
+            push_active(A, 0, kind, id, live, scratch);
     }
 }
 
@@ -281,16 +396,14 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
     uint32_t c0 = A.ctl->act_count[parity][0].v, c1 = A.ctl->act_count[parity][1].v,
              c2 = A.ctl->act_count[parity][2].v;
     uint32_t total = c0 + c1 + c2;
-    uint32_t *const queues[3] = {&A.ctl->sq_count.v, &A.ctl->eq_count.v, &A.ctl->fin_count.v};
     // block-uniform trip count (block_append synchronizes the block)
     for (uint32_t b = blockIdx.x * blockDim.x; b < total; b += gridDim.x * blockDim.x) {
         uint32_t t = b + threadIdx.x;
         bool in = t < total;
-        bool has_shadow = false, ok = false;
         uint32_t path = 0;
-        f3 xp3 = mk(0.0f, 0.0f, 0.0f), sc = mk(0.0f, 0.0f, 0.0f), sdir = mk(0.0f, 0.0f, 0.0f);
-        f3 edir = mk(0.0f, 0.0f, 0.0f);
-        float sdist = 0.0f;
+        f3 xp3 = mk(0.0f, 0.0f, 0.0f);
+        NeeSample r;
+        r.has_shadow = r.ok = false;
         if (in) {
             uint32_t kind = t < c0 ? 0u : (t < c0 + c1 ? 1u : 2u);
             uint32_t j = t - (kind == 0 ? 0u : (kind == 1 ? c0 : c0 + c1));
@@ -302,71 +415,17 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
             x.mat = static_cast<int32_t>(__float_as_uint(xp.w));
             x.light = static_cast<int32_t>(__float_as_uint(xn.w));
             xp3 = x.p;
-            f3 wo = xyz(wo4), beta = xyz(b4);
-            uint32_t pix = __float_as_uint(wo4.w);
-            Pcg g;
-            g.state = A.rng[path];
-            g.inc = (static_cast<uint64_t>(pix) << 1u) | 1u;
-            // next-event estimation (Tracer::direct + the MIS weight of Tracer::mis_bounce)
-            if (A.S.num_lights > 0) {
-                LightSample ls = sample_light(A.S, g);
-                f3 l = x.p - ls.p;
-                float r2 = dot(l, l);
-                f3 inv = l / __builtin_sqrtf(r2);
-                float co = dot(ls.n, inv);
-                float ci = dot(x.n, -inv);
-                if (co > 0 && ci > 0) {
-                    f3 illum = ((light_rad(A.S, ls.light) * ci) * co) / r2;
-                    if (!near0(illum)) {
-                        float dist = norm(l);
-                        f3 wl = -l / dist;
-                        f3 f = bsdf_eval(A.S.mats, x.mat, x.n, wo, wl);
-                        f3 d = mul(illum, f) / ls.area_dens;
-                        float cl = dot(ls.n, -wl);
-                        float pl = ls.area_dens * r2 / cl;
-                        float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
-                        float w = pl * pl / (pl * pl + pb * pb);
-                        sc = mul(beta, d * w);
-                        sdist = dist;
-                        sdir = wl;
-                        has_shadow = true;
-                    }
-                }
-            }
-            float dens;
-            f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, false);
-            A.rng[path] = g.state;
-            ok = dens > 0.0f;
-            if (ok)
-                A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
-            edir = wi;
+            r = nee_sample(A, path, x, xyz(wo4), xyz(b4), __float_as_uint(wo4.w), A.rng[path]);
         }
         // one call site with all lanes active: block_append synchronizes the block
-        const bool want[3] = {in && has_shadow, in && ok, in && !ok};
-        uint32_t idx[3];
-        block_append<3>(queues, want, idx, scratch);
-        if (want[0]) {
-            uint32_t q = idx[0];
-            A.sq[3 * q] = make_float4(xp3.x, xp3.y, xp3.z, sdist - 1e-3f);
-            A.sq[3 * q + 1] = make_float4(sdir.x, sdir.y, sdir.z, __uint_as_float(path));
-            A.sq[3 * q + 2] = make_float4(sc.x, sc.y, sc.z, 0.0f);
-            if (A.sort_rays)
-                A.ray_key[0][q] = ray_key(A.S, xp3, sdir);
-        }
-        if (want[1]) {
-            A.eq[idx[1]] = path;
-            if (A.sort_rays)
-                A.ray_key[1][idx[1]] = ray_key(A.S, xp3, edir);
-        }
-        if (want[2])
-            A.fin[idx[2]] = path;
+        push_rays(A, in, path, xp3, r, scratch);
     }
 }
 
 // Counting sort of both ray queues by key (blockIdx.y = queue), in four passes: per-block key
 // counts over contiguous chunks (with each entry's rank among equal keys of its block), column
 // totals, per-(block, key) output offsets, scatter. No global atomics.
-X3D_DI uint32_t queue_count(WfArgs const &A, int q) { return q == 0 ? A.ctl->sq_count.v : A.ctl->eq_count.v; }
+X3D_DI uint32_t queue_count(WfArgs const &A, int q) { return q == 0 ? A.ctl->sq_count.v : A.eq_cnt_in->v; }
 
 __global__ __launch_bounds__(kSortWG) void k_wf_sort_count(WfArgs A) {
     __shared__ uint32_t hist[kSortBins];
@@ -582,7 +641,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
         __syncthreads();
     }
     const float4 *tris = static_cast<const float4 *>(S.tris);
-    const uint32_t n = A.ctl->eq_count.v;
+    const uint32_t n = A.eq_cnt_in->v;
     uint32_t pops = 0, tests = 0;
     uint32_t *slot = cursor + 2 * (threadIdx.x >> 6);
     if (__lane_id() == 0) {
@@ -668,10 +727,11 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     }
 }
 
+template <bool FUSED>
 __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t bounce) {
     __shared__ uint32_t scratch[2 * kKinds];
     clear_scratch(scratch, 2 * kKinds);
-    uint32_t ne = A.ctl->eq_count.v, nf = A.ctl->fin_count.v;
+    uint32_t ne = A.eq_cnt_in->v, nf = A.fin_cnt_in->v;
     uint32_t total = ne + nf;
     // block-uniform trip count (block_append synchronizes the block)
     for (uint32_t b = blockIdx.x * blockDim.x; b < total; b += gridDim.x * blockDim.x) {
@@ -679,10 +739,13 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
         bool live = false;
         int kind = 0;
         uint32_t path = 0;
+        NeeSample r;
+        r.has_shadow = r.ok = false;
+        f3 xp3 = mk(0.0f, 0.0f, 0.0f);
         if (t >= total) {
             // no item: only takes part in the block-wide append below
         } else if (t >= ne) {
-            path = A.fin[t - ne];
+            path = A.fin_in[t - ne];
             A.lbuf[path] = A.L[path];
         } else {
             path = A.eq_in[t];
@@ -732,11 +795,19 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
                     A.beta[path] = make_float4(beta.x, beta.y, beta.z, b4.w);
                     A.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
                     live = true;
-                    kind = kind_of(A.S, nx.mat);
+                    if (FUSED) {
+                        xp3 = nx.p;
+                        r = nee_sample(A, path, nx, -wi, beta, __float_as_uint(wo4.w), A.rng[path]);
+                    } else {
+                        kind = kind_of(A.S, nx.mat);
+                    }
                 }
             }
         }
-        push_active(A, parity ^ 1, kind, path, live, scratch);
+        if (FUSED)
+            push_rays(A, live, path, xp3, r, scratch);
+        else
+            push_active(A, parity ^ 1, kind, path, live, scratch);
     }
 }
 
@@ -767,9 +838,9 @@ __global__ __launch_bounds__(256) void k_wf_accumulate(WfArgs A) {
 }
 
 // Ray-query counts (one closest-hit per extension ray, one any-hit per shadow ray).
-__global__ void k_wf_count(WfCtl *ctl, Counters *cnt) {
+__global__ void k_wf_count(WfCtl *ctl, WfCounter const *eq_cnt, Counters *cnt) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && cnt) {
-        atomicAdd(&cnt->closest, static_cast<unsigned long long>(ctl->eq_count.v));
+        atomicAdd(&cnt->closest, static_cast<unsigned long long>(eq_cnt->v));
         atomicAdd(&cnt->anyhit, static_cast<unsigned long long>(ctl->sq_count.v));
     }
 }
@@ -779,6 +850,7 @@ __global__ void k_wf_count(WfCtl *ctl, Counters *cnt) {
 // ---------------------------------------------------------------------------------------------
 struct WavefrontEngine {
     DevBuf xp, xn, wo, beta, L, wi, rng, lbuf, act0, act1, sq, eq, fin, eh, ctl, spill;
+    DevBuf eq_b, fin_b; // the fused form's second queue parity
     DevBuf sq_sorted, eq_sorted, ray_key, ray_rank, bin_mat, bin_tot; // ray reordering
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -872,8 +944,14 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     WF_HIP(e->wi.ensure(f4));
     WF_HIP(e->lbuf.ensure(f4));
     WF_HIP(e->rng.ensure(sizeof(uint64_t) * cap));
-    WF_HIP(e->act0.ensure(sizeof(uint32_t) * kKinds * static_cast<size_t>(cap)));
-    WF_HIP(e->act1.ensure(sizeof(uint32_t) * kKinds * static_cast<size_t>(cap)));
+    const bool fused = !tune.sort_by_material && !tune.sort_rays;
+    if (fused) {
+        WF_HIP(e->eq_b.ensure(sizeof(uint32_t) * cap));
+        WF_HIP(e->fin_b.ensure(sizeof(uint32_t) * cap));
+    } else {
+        WF_HIP(e->act0.ensure(sizeof(uint32_t) * kKinds * static_cast<size_t>(cap)));
+        WF_HIP(e->act1.ensure(sizeof(uint32_t) * kKinds * static_cast<size_t>(cap)));
+    }
     WF_HIP(e->sq.ensure(3 * f4));
     WF_HIP(e->eq.ensure(sizeof(uint32_t) * cap));
     WF_HIP(e->fin.ensure(sizeof(uint32_t) * cap));
@@ -937,6 +1015,12 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     A.firefly = P.firefly;
     A.sq_in = A.sq;
     A.eq_in = A.eq;
+    A.fin_in = A.fin;
+    A.eq_out = A.eq;
+    A.fin_out = A.fin;
+    A.eq_cnt_in = A.eq_cnt_out = &A.ctl->eq_count;
+    A.fin_cnt_in = A.fin_cnt_out = &A.ctl->fin_count;
+    A.fused = fused ? 1 : 0;
     if (tune.sort_rays) {
         A.sq_sorted = static_cast<float4 *>(e->sq_sorted.p);
         A.eq_sorted = static_cast<uint32_t *>(e->eq_sorted.p);
@@ -952,42 +1036,78 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
 
     // Grid-strided per-path kernels get exactly the blocks that are resident at once: a larger
     // grid would run its surplus waves as a second, late round.
-    uint32_t g_gen = grid_for(k_wf_generate, 256), g_nee = grid_for(k_wf_nee_bsdf, 256),
-             g_shade = grid_for(k_wf_shade, 256), g_acc = grid_for(k_wf_accumulate, 256);
+    uint32_t g_gen = fused ? grid_for(k_wf_generate<true>, 256) : grid_for(k_wf_generate<false>, 256),
+             g_nee = grid_for(k_wf_nee_bsdf, 256),
+             g_shade = fused ? grid_for(k_wf_shade<true>, 256) : grid_for(k_wf_shade<false>, 256),
+             g_acc = grid_for(k_wf_accumulate, 256);
+    // The fused form's queue parities: bounce k consumes parity k & 1 and its shade fills the other.
+    auto parity_args = [&](uint32_t k) {
+        WfArgs B = A;
+        uint32_t *eqs[2] = {static_cast<uint32_t *>(e->eq.p), static_cast<uint32_t *>(e->eq_b.p)};
+        uint32_t *fins[2] = {static_cast<uint32_t *>(e->fin.p), static_cast<uint32_t *>(e->fin_b.p)};
+        WfCounter *eqc[2] = {&A.ctl->eq_count, &A.ctl->eq_count_b};
+        WfCounter *finc[2] = {&A.ctl->fin_count, &A.ctl->fin_count_b};
+        const uint32_t p = k & 1u, q = p ^ 1u;
+        B.eq = B.eq_out = eqs[q];
+        B.eq_in = eqs[p];
+        B.fin = B.fin_out = fins[q];
+        B.fin_in = fins[p];
+        B.eq_cnt_in = eqc[p];
+        B.eq_cnt_out = eqc[q];
+        B.fin_cnt_in = finc[p];
+        B.fin_cnt_out = finc[q];
+        return B;
+    };
     uint32_t generations = (P.s_end - P.s_begin + gen_samples - 1) / gen_samples;
 
     bool profile = tune.profile != 0;
+    auto trace_bounce = [&](hipStream_t s, WfArgs const &B) {
+        if (profile)
+            (void)hipEventRecord(e->ev[0], s);
+        hipLaunchKernelGGL(tk.shadow, dim3(g_shadow), dim3(wg), 0, s, B);
+        if (profile)
+            (void)hipEventRecord(e->ev[1], s);
+        hipLaunchKernelGGL(tk.extend, dim3(g_extend), dim3(wg), 0, s, B);
+        if (profile) {
+            (void)hipEventRecord(e->ev[2], s);
+            float a = 0, b = 0;
+            (void)hipEventSynchronize(e->ev[2]);
+            (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
+            (void)hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
+            e->shadow_ms += a;
+            e->extend_ms += b;
+            e->trace_launches++;
+        }
+        hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, s, B.ctl, static_cast<const WfCounter *>(B.eq_cnt_in),
+                           B.cnt);
+    };
     auto record_generation = [&](hipStream_t s) {
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, -1);
-        hipLaunchKernelGGL(k_wf_generate, dim3(g_gen), dim3(256), 0, s, A);
-        for (uint32_t k = 0; k < P.max_depth; k++) {
-            int par = static_cast<int>(k & 1u);
-            hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, par);
-            hipLaunchKernelGGL(k_wf_nee_bsdf, dim3(g_nee), dim3(256), 0, s, A, par);
-            if (A.sort_rays) {
-                hipLaunchKernelGGL(k_wf_sort_count, dim3(kSortBlocks, 2), dim3(kSortWG), 0, s, A);
-                hipLaunchKernelGGL(k_wf_sort_totals, dim3(kSortBins / 256, 2), dim3(256), 0, s, A);
-                hipLaunchKernelGGL(k_wf_sort_offsets, dim3(kSortBins / 256, 2), dim3(256), 0, s, A);
-                hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSortBlocks, 2), dim3(kSortWG), 0, s, A);
+        if (fused) {
+            // generate fills bounce 0's queues (parity 0); bounce k's shade fills bounce k + 1's
+            WfArgs G = parity_args(1);
+            hipLaunchKernelGGL(k_wf_generate<true>, dim3(g_gen), dim3(256), 0, s, G);
+            for (uint32_t k = 0; k < P.max_depth; k++) {
+                WfArgs B = parity_args(k);
+                trace_bounce(s, B);
+                hipLaunchKernelGGL(k_wf_reset_fused, dim3(1), dim3(64), 0, s, B.ctl, B.eq_cnt_out, B.fin_cnt_out);
+                hipLaunchKernelGGL(k_wf_shade<true>, dim3(g_shade), dim3(256), 0, s, B, 0, k);
             }
-            if (profile)
-                (void)hipEventRecord(e->ev[0], s);
-            hipLaunchKernelGGL(tk.shadow, dim3(g_shadow), dim3(wg), 0, s, A);
-            if (profile)
-                (void)hipEventRecord(e->ev[1], s);
-            hipLaunchKernelGGL(tk.extend, dim3(g_extend), dim3(wg), 0, s, A);
-            if (profile) {
-                (void)hipEventRecord(e->ev[2], s);
-                float a = 0, b = 0;
-                (void)hipEventSynchronize(e->ev[2]);
-                (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
-                (void)hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
-                e->shadow_ms += a;
-                e->extend_ms += b;
-                e->trace_launches++;
+        } else {
+            hipLaunchKernelGGL(k_wf_generate<false>, dim3(g_gen), dim3(256), 0, s, A);
+            for (uint32_t k = 0; k < P.max_depth; k++) {
+                int par = static_cast<int>(k & 1u);
+                hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, par);
+                hipLaunchKernelGGL(k_wf_nee_bsdf, dim3(g_nee), dim3(256), 0, s, A, par);
+                if (A.sort_rays) {
+                    hipLaunchKernelGGL(k_wf_sort_count, dim3(kSortBlocks, 2), dim3(kSortWG), 0, s, A);
+                    hipLaunchKernelGGL(k_wf_sort_totals, dim3(kSortBins / 256, 2), dim3(256), 0, s, A);
+                    hipLaunchKernelGGL(k_wf_sort_offsets, dim3(kSortBins / 256, 2), dim3(256), 0, s, A);
+                    hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSortBlocks, 2), dim3(kSortWG), 0, s, A);
+                }
+                trace_bounce(s, A);
+                hipLaunchKernelGGL(k_wf_shade<false>, dim3(g_shade), dim3(256), 0, s, A, par, k);
             }
-            hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, s, A.ctl, A.cnt);
-            hipLaunchKernelGGL(k_wf_shade, dim3(g_shade), dim3(256), 0, s, A, par, k);
         }
         hipLaunchKernelGGL(k_wf_accumulate, dim3(g_acc), dim3(256), 0, s, A);
         hipLaunchKernelGGL(k_wf_next_gen, dim3(1), dim3(64), 0, s, A.ctl);

@@ -58,7 +58,21 @@ def build_library(verbose: bool = True) -> str:
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
           f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
+    build_host()
     return LIB
+
+
+HOST = os.path.join(PKG, "host", "render_cornell")
+
+
+def build_host() -> str:
+    """Builds the C++ host program (host/render_cornell.cpp) against the C ABI and libx3dpt.so."""
+    gxx = shutil.which("g++") or "g++"
+    tmp = HOST + ".tmp"
+    _run([gxx, "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), HOST + ".cpp", "-o", tmp,
+          "-L" + PKG, "-lx3dpt", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, HOST)
+    return HOST
 
 
 def build_oracle(with_reference: bool = True) -> None:

@@ -79,6 +79,7 @@ def build_oracle(with_reference: bool = True) -> None:
     """Builds the CPU checkers under oracle/ (test infrastructure)."""
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir, "-j8"])
+    _run(["make", "-s", "-C", odir, "-j8", "sanitize"])
     if with_reference and os.path.isdir("/root/reference/corelib/src"):
         _run(["make", "-s", "-C", odir, "-j8", "ref"])
 

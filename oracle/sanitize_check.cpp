@@ -2,6 +2,10 @@
 // UndefinedBehaviorSanitizer (built by `make -C oracle sanitize`, run by tests/test_sanitizers.py):
 //   - the library's Wavefront OBJ reader (x3dcore2_amd/csrc/objfile.cpp) on the reference's
 //     Cornell box files and on malformed files;
+//   - the library's host scene code (x3dcore2_amd/csrc/scene.cpp): the reference-layout BVH
+//     build, the 4-wide tree collapse and its consistency check, the device packing, objects by
+//     id with transforms (without a GPU the commit stops at its first device allocation, after
+//     all host work);
 //   - the oracle's scene build (BVH, light CDFs), every tracer, the firefly filter, ray queries
 //     and the compositors, on the Cornell box loaded through that reader.
 // Any sanitizer report aborts the process (-fno-sanitize-recover=all); a failed check returns 1.
@@ -16,6 +20,7 @@
 #include <vector>
 
 #include "../x3dcore2_amd/csrc/objfile.h"
+#include "../x3dcore2_amd/csrc/scene.h"
 
 extern "C" {
 void *orc_scene_new();
@@ -91,6 +96,50 @@ int main(int argc, char **argv) {
     {
         x3d::ObjMesh m;
         expect(!x3d::load_obj((scratch + "/absent.obj").c_str(), m).empty(), "absent file accepted");
+    }
+
+    // The library's host scene: indexed, then by id with a transform and a second commit.
+    {
+        x3d::Scene sc;
+        for (int k = 0; k < 8; k++) {
+            x3d::HostMesh hm;
+            x3d::ObjMesh const &m = meshes[k];
+            for (size_t v = 0; v < m.verts.size() / 3; v++) {
+                hm.verts.push_back({m.verts[3 * v], m.verts[3 * v + 1], m.verts[3 * v + 2]});
+                hm.normals.push_back({m.normals[3 * v], m.normals[3 * v + 1], m.normals[3 * v + 2]});
+            }
+            hm.tris = m.tris;
+            sc.grow_space(hm.verts);
+            sc.meshes.push_back(hm);
+        }
+        sc.lights.push_back({7, {13.665f, 9.9f, 5.175f}});
+        std::string e = sc.commit();
+        expect(e.empty() || e.find("hip") != std::string::npos, "indexed commit: " + e);
+        expect(!sc.flat.empty() && sc.prims.size() > 1000 && sc.num_nodes4 > 0, "indexed scene: no BVH");
+
+        x3d::Scene kd;
+        kd.keyed = true;
+        const float t[16] = {0.8f, 0.0f, -0.1f, 0.0f, 0.0f, 1.1f, 0.0f, 0.0f,
+                             0.1f, 0.0f, 0.9f,  0.0f, 0.05f, -0.02f, 0.1f, 1.0f};
+        for (int k = 0; k < 8; k++) {
+            x3d::KeyedGeometry g;
+            g.mesh = sc.meshes[k];
+            if (k == 5)
+                expect(x3d::transform_mesh(t, g.mesh.verts, g.mesh.normals).empty(), "transform");
+            g.has_material = k < 4;
+            g.material = 100 + k;
+            kd.grow_space(g.mesh.verts);
+            kd.k_geos.insert({static_cast<uint64_t>(900 - 7 * k), g});
+        }
+        kd.k_lights.insert({1u, x3d::KeyedLight{900 - 7 * 7, {1.0f, 1.0f, 1.0f}}});
+        e = kd.commit();
+        expect(e.empty() || e.find("hip") != std::string::npos, "keyed commit: " + e);
+        kd.k_geos.erase(900 - 7 * 3);
+        e = kd.commit();
+        expect(e.empty() || e.find("hip") != std::string::npos, "keyed recommit: " + e);
+        expect(kd.meshes.size() == 7 && kd.geo_ids.size() == 7, "keyed scene after unload");
+        kd.k_lights.insert({2u, x3d::KeyedLight{12345, {1.0f, 1.0f, 1.0f}}});
+        expect(!kd.commit().empty(), "a light on an absent geometry was accepted");
     }
 
     // Oracle scene from the reader's meshes.

@@ -1,5 +1,6 @@
 """Host code under AddressSanitizer + UndefinedBehaviorSanitizer (no GPU): the library's OBJ
-reader on the reference's Cornell box files and on malformed files, and the oracle's scene build,
+reader on the reference's Cornell box files and on malformed files, the library's host scene
+code (BVH build, wide-tree collapse, packing, objects by id), and the oracle's scene build,
 tracers, firefly filter, ray queries and compositors on that scene (oracle/sanitize_check.cpp,
 built by `make -C oracle sanitize`). Any sanitizer report aborts the program."""
 

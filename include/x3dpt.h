@@ -245,7 +245,8 @@ typedef struct x3d_renderer x3d_renderer;
 
 /* Execution forms of the same estimator; every form gives the bit-identical film. */
 #define X3D_FORM_PLAIN 0     /* one thread per pixel path, one kernel (the drop-in) */
-#define X3D_FORM_WAVEFRONT 1 /* ray queues sorted by material/depth, bounce loop in a hipGraph */
+#define X3D_FORM_WAVEFRONT 1 /* ray queues per bounce, bounce loop in a hipGraph; runs
+                                UNIDIRECT_MIS, UNIDIRECT_LT1 and UNIDIRECT (the last two unsorted) */
 
 typedef struct x3d_render_options {
     int32_t pt_type;       /* X3D_PT_* */

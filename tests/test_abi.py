@@ -64,6 +64,9 @@ def test_render_option_errors_without_gpu(x3d):
         dict(pt_type=7),
         dict(pt_type=x3d.PT_BIDIRECT_LT2, form=x3d.FORM_WAVEFRONT),
         dict(pt_type=x3d.PT_POSITION, form=x3d.FORM_WAVEFRONT),
+        dict(pt_type=x3d.PT_DIRECT, form=x3d.FORM_WAVEFRONT),
+        dict(pt_type=x3d.PT_UNIDIRECT_LT1, form=x3d.FORM_WAVEFRONT, sort_key=1),   # sorting: MIS only
+        dict(pt_type=x3d.PT_UNIDIRECT, form=x3d.FORM_WAVEFRONT, ray_order=1),
         dict(pt_type=x3d.PT_UNIDIRECT_MIS, workgroup_size=96),
         dict(pt_type=x3d.PT_UNIDIRECT_MIS, stack_depth=12),
         dict(pt_type=x3d.PT_UNIDIRECT_MIS, queue_capacity=(1 << 26) + 1),

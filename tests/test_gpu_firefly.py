@@ -1,7 +1,7 @@
 """Films with the firefly filter (x3d_render's firefly_filter, sampling_task::run,
 renderer.cpp:46-83) bit-identical to the oracle's (orc_render_firefly: every sample image
 rendered, filtered, summed per pixel in sample order), for every tracer and both forms of the
-MIS tracer, in one generation and in several. The filter only lowers samples, so the filtered
+MIS, lt1 and unidirect tracers, in one generation and in several. The filter only lowers samples, so the filtered
 film is at most the unfiltered one everywhere and below it on the pixels it touched; the scenes
 and sizes are chosen so the oracle caps samples on dozens of pixels.
 """
@@ -32,6 +32,8 @@ CASES = [
     ("indirect_l3", "UNIDIRECT_LT1", dict()),
     ("spheres_small", "UNIDIRECT_LT1", dict(queue_capacity=2000)),
     ("indirect_l3", "UNIDIRECT", dict()),
+    ("indirect_l3", "UNIDIRECT_LT1", dict(form="wavefront")),
+    ("spheres_small", "UNIDIRECT", dict(form="wavefront", queue_capacity=3000)),
     ("indirect_l3", "DIRECT", dict()),
     ("indirect_l3", "NORMAL", dict()),
     ("indirect_l3", "BIDIRECT_LT2", dict()),

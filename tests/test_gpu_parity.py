@@ -241,6 +241,47 @@ def test_wavefront_film_identical_to_plain(x3d, scene_name, kw):
         assert st.anyhit_queries == sp.anyhit_queries
 
 
+REF_WAVEFRONT_CASES = [
+    dict(),                                                   # one generation, graph
+    dict(workgroup_size=128, stack_depth=8, queue_capacity=3000),  # generations of 1 sample
+    dict(profile=1),                                          # launched without a graph
+]
+
+
+@pytest.mark.parametrize("kw", REF_WAVEFRONT_CASES)
+@pytest.mark.parametrize("tracer", ["UNIDIRECT_LT1", "UNIDIRECT"])
+@pytest.mark.parametrize("scene_name", ["box", "spheres_small"])
+def test_wavefront_reference_tracers_identical_to_plain(x3d, scene_name, tracer, kw):
+    """The reference-mode lt1 and unidirect tracers (Russian roulette, up to 64 vertices) in the
+    wavefront form: the same film bits and the same query counts as the plain kernels."""
+    desc = QUERY_SCENES[scene_name]()
+    print(f"[gpu] wavefront {tracer} {scene_name} {kw}", flush=True)
+    gs = x3d.Scene(desc)
+    pt = getattr(x3d, "PT_" + tracer)
+    w, h, spp = 48, 40, 5
+    plain, sp = x3d.Renderer(pt, seed=23).render(gs, desc.camera, w, h, spp)
+    wf = x3d.Renderer(pt, form=x3d.FORM_WAVEFRONT, seed=23, **kw)
+    for _ in range(2):  # second call replays the cached graph
+        img, st = wf.render(gs, desc.camera, w, h, spp)
+        assert np.array_equal(_bits(img), _bits(plain)), np.abs(img - plain).max()
+        assert st.closest_queries == sp.closest_queries
+        assert st.anyhit_queries == sp.anyhit_queries
+    assert img[:, :3].max() > 0
+
+
+def test_wavefront_lt1_autotune(x3d):
+    """The autotuner of an lt1 wavefront renderer tries the unsorted configurations only."""
+    desc = QUERY_SCENES["spheres_small"]()
+    gs = x3d.Scene(desc)
+    w, h, spp = 64, 48, 3
+    plain, _ = x3d.Renderer(x3d.PT_UNIDIRECT_LT1, seed=5).render(gs, desc.camera, w, h, spp)
+    wf = x3d.Renderer(x3d.PT_UNIDIRECT_LT1, form=x3d.FORM_WAVEFRONT, seed=5)
+    rep = wf.autotune(gs, desc.camera, w, h, 1)
+    assert rep.candidates == 6 and rep.sort_key == x3d.SORT_NONE
+    img, _ = wf.render(gs, desc.camera, w, h, spp)
+    assert np.array_equal(_bits(img), _bits(plain))
+
+
 def test_wavefront_autotune_and_profile(x3d):
     desc = QUERY_SCENES["spheres_small"]()
     gs = x3d.Scene(desc)

@@ -641,6 +641,9 @@ template <int T> void launch_plain(uint32_t grid, hipStream_t s, RenderParams co
 // Wavefront tuning from the renderer's options (0 / DEFAULT fields keep the defaults).
 x3d::WavefrontTuning wf_tuning(x3d_render_options const &o) {
     x3d::WavefrontTuning tune;
+    tune.tracer = o.pt_type == X3D_PT_UNIDIRECT_LT1 ? x3d::kWfLt1
+                  : o.pt_type == X3D_PT_UNIDIRECT   ? x3d::kWfUnidirect
+                                                    : x3d::kWfMis;
     if (o.workgroup_size)
         tune.trace_wg = static_cast<int>(o.workgroup_size);
     if (o.queue_capacity)
@@ -1635,8 +1638,13 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
                     "max_path_len must be in [1, " + std::to_string(x3d::kMaxPathLen) + "] for BIDIRECT_MIS");
     if (opts->form != X3D_FORM_PLAIN && opts->form != X3D_FORM_WAVEFRONT)
         return fail(X3D_ERR_INVALID_ARGUMENT, "unknown render form " + std::to_string(opts->form));
-    if (opts->form == X3D_FORM_WAVEFRONT && opts->pt_type != X3D_PT_UNIDIRECT_MIS)
-        return fail(X3D_ERR_UNSUPPORTED, "the wavefront form runs the UNIDIRECT_MIS tracer only");
+    if (opts->form == X3D_FORM_WAVEFRONT && opts->pt_type != X3D_PT_UNIDIRECT_MIS &&
+        opts->pt_type != X3D_PT_UNIDIRECT_LT1 && opts->pt_type != X3D_PT_UNIDIRECT)
+        return fail(X3D_ERR_UNSUPPORTED,
+                    "the wavefront form runs the UNIDIRECT_MIS, UNIDIRECT_LT1 and UNIDIRECT tracers only");
+    if (opts->form == X3D_FORM_WAVEFRONT && opts->pt_type != X3D_PT_UNIDIRECT_MIS &&
+        (opts->sort_key == X3D_SORT_MATERIAL || opts->ray_order == X3D_RAY_ORDER_SORTED))
+        return fail(X3D_ERR_UNSUPPORTED, "material and ray sorting apply to the wavefront UNIDIRECT_MIS tracer only");
     if (opts->workgroup_size != 0 && opts->workgroup_size != 64 && opts->workgroup_size != 128 &&
         opts->workgroup_size != 256)
         return fail(X3D_ERR_INVALID_ARGUMENT, "workgroup_size must be 0, 64, 128 or 256");
@@ -1754,7 +1762,10 @@ int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_cam
     // probe would clamp it to one generation and could not tell it apart from a smaller one.
     const uint32_t wgs[] = {64, 128, 256};
     const uint32_t depths[] = {8, 16};
-    const int32_t sorts[] = {X3D_SORT_MATERIAL, X3D_SORT_NONE};
+    const int32_t sorts_mis[] = {X3D_SORT_MATERIAL, X3D_SORT_NONE};
+    // the lt1 / unidirect wavefront runs unsorted only
+    const bool mis = r->opt.pt_type == X3D_PT_UNIDIRECT_MIS;
+    const std::vector<int32_t> sorts(sorts_mis + (mis ? 0 : 1), sorts_mis + 2);
     const uint32_t caps[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
     const uint64_t probe_paths = npix * probe_spp;
     x3d_autotune_report rep{};

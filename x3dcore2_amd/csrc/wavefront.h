@@ -1,4 +1,4 @@
-// Wavefront form of the MIS tracer (host interface).
+// Wavefront form of the MIS tracer and of the reference-mode lt1 / unidirect tracers (host interface).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,7 +14,12 @@ struct WavefrontEngine;
 constexpr uint32_t kDefaultQueueCapacity = 1u << 26; // 64M paths: ~13.5 GiB of path state and queues
 constexpr uint32_t kMaxQueueCapacity = 1u << 26;
 
+// Estimator a wavefront render reproduces (bit for bit, against its plain kernel): the MIS tracer,
+// unidirect_lt1_path_tracer or unidirect_path_tracer.
+enum WfTracer : int { kWfMis = 0, kWfLt1 = 1, kWfUnidirect = 2 };
+
 struct WavefrontTuning {
+    int tracer = kWfMis;
     uint32_t paths_in_flight = kDefaultQueueCapacity; // queue capacity (paths of one generation)
     int trace_wg = 64;                   // workgroup size of the traversal kernels (64, 128, 256)
     int stack_depth = 16;                // LDS stack entries per lane of the traversal kernels (8, 16)

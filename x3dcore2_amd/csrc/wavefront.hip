@@ -1,4 +1,5 @@
-// Wavefront form of the fixed-depth MIS tracer.
+// Wavefront form of the fixed-depth MIS tracer, and of the reference-mode lt1 and unidirect
+// tracers (Russian roulette, no depth limit below kRefModeMaxDepth) in the fused form.
 //
 // Paths live in HBM as structure-of-arrays state; each bounce runs as separate kernels:
 //   reset      clear the queue counters of this bounce
@@ -85,6 +86,8 @@ struct WfArgs {
     // Fused form (no material sort, no ray sort): generate and shade also run the next vertex's
     // light sample and BSDF sample (no nee_bsdf launch, no active lists).
     int fused;
+    // lt1: the first hit's emission per path, added when the path ends (w = 1 when there is one).
+    float4 *le0;
 };
 
 // Ray keys: direction octant (3 bits) above the Morton code of the origin's cell in an 8^3 grid
@@ -268,16 +271,17 @@ struct NeeSample {
     float sdist;
 };
 
-X3D_DI NeeSample nee_sample(WfArgs const &A, uint32_t path, HitRec const &x, f3 wo, f3 beta, uint32_t pix,
-                            uint64_t state) {
+// TR = kWfMis: the light sample's contribution carries the power-heuristic weight and the BSDF is
+// sampled with the evaluated mixture densities (Tracer::mis_bounce); kWfLt1: unweighted
+// (Tracer::direct as li_lt1 adds it) and the reference's mixture sampling; kWfUnidirect: BSDF
+// sample only (no light sample).
+template <int TR>
+X3D_DI NeeSample nee_sample(WfArgs const &A, uint32_t path, HitRec const &x, f3 wo, f3 beta, Pcg &g) {
     NeeSample r;
     r.has_shadow = false;
     r.sc = r.sdir = mk(0.0f, 0.0f, 0.0f);
     r.sdist = 0.0f;
-    Pcg g;
-    g.state = state;
-    g.inc = (static_cast<uint64_t>(pix) << 1u) | 1u;
-    if (A.S.num_lights > 0) {
+    if (TR != kWfUnidirect && A.S.num_lights > 0) {
         LightSample ls = sample_light(A.S, g);
         f3 l = x.p - ls.p;
         float r2 = dot(l, l);
@@ -291,11 +295,15 @@ X3D_DI NeeSample nee_sample(WfArgs const &A, uint32_t path, HitRec const &x, f3 
                 f3 wl = -l / dist;
                 f3 f = bsdf_eval(A.S.mats, x.mat, x.n, wo, wl);
                 f3 d = mul(illum, f) / ls.area_dens;
-                float cl = dot(ls.n, -wl);
-                float pl = ls.area_dens * r2 / cl;
-                float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
-                float w = pl * pl / (pl * pl + pb * pb);
-                r.sc = mul(beta, d * w);
+                if (TR == kWfMis) {
+                    float cl = dot(ls.n, -wl);
+                    float pl = ls.area_dens * r2 / cl;
+                    float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
+                    float w = pl * pl / (pl * pl + pb * pb);
+                    r.sc = mul(beta, d * w);
+                } else {
+                    r.sc = mul(beta, d);
+                }
                 r.sdist = dist;
                 r.sdir = wl;
                 r.has_shadow = true;
@@ -303,13 +311,32 @@ X3D_DI NeeSample nee_sample(WfArgs const &A, uint32_t path, HitRec const &x, f3 
         }
     }
     float dens;
-    f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, false);
+    f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, TR != kWfMis);
     A.rng[path] = g.state;
-    r.ok = dens > 0.0f;
+    // li_mis stops on !(dens > 0); li_lt1 / li_unidirect on dens == 0
+    r.ok = TR == kWfMis ? dens > 0.0f : dens != 0.0f;
     if (r.ok)
         A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
     r.edir = wi;
     return r;
+}
+
+X3D_DI Pcg path_pcg(uint64_t state, uint32_t pix) {
+    Pcg g;
+    g.state = state;
+    g.inc = (static_cast<uint64_t>(pix) << 1u) | 1u;
+    return g;
+}
+
+// A path's radiance into the generation's sample buffer; lt1 adds its first hit's emission last,
+// as li_lt1 does after its loop.
+template <int TR> X3D_DI void finish_path(WfArgs const &A, uint32_t path, f3 L) {
+    if (TR == kWfLt1) {
+        float4 e = A.le0[path];
+        if (e.w != 0.0f)
+            L = L + xyz(e);
+    }
+    A.lbuf[path] = make_float4(L.x, L.y, L.z, 0.0f);
 }
 
 // Queues a sampled vertex's shadow ray and extension ray (or finishes the path when the BSDF
@@ -337,7 +364,7 @@ X3D_DI void push_rays(WfArgs const &A, bool in, uint32_t path, f3 xp3, NeeSample
         A.fin_out[idx[2]] = path;
 }
 
-template <bool FUSED> __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
+template <bool FUSED, int TR> __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
     __shared__ uint32_t scratch[2 * kKinds];
     clear_scratch(scratch, 2 * kKinds);
     uint32_t gen = A.ctl->gen;
@@ -359,13 +386,25 @@ template <bool FUSED> __global__ __launch_bounds__(256) void k_wf_generate(WfArg
             RP.first = A.first;
             bool valid;
             HitRec x = load_first(RP, pix, valid);
-            // Tracer::mis_begin
+            // Tracer::mis_begin / the first vertex of li_lt1, li_unidirect
             Pcg g = pcg_for(A.seed, pix, s);
             f3 L = mk(0.0f, 0.0f, 0.0f);
             f3 wo = -v;
-            if (valid && x.light >= 0)
-                L = L + light_radiance(A.S, x.light, wo, x.n);
-            live = valid && A.max_depth > 0;
+            if (TR == kWfMis) {
+                if (valid && x.light >= 0)
+                    L = L + light_radiance(A.S, x.light, wo, x.n);
+                live = valid && A.max_depth > 0;
+            } else if (TR == kWfLt1) {
+                live = valid;
+                if (valid) {
+                    f3 e = x.light >= 0 ? light_radiance(A.S, x.light, wo, x.n) : mk(0.0f, 0.0f, 0.0f);
+                    A.le0[id] = make_float4(e.x, e.y, e.z, x.light >= 0 ? 1.0f : 0.0f);
+                }
+            } else {
+                live = valid;
+                if (valid && x.light >= 0)
+                    L = L + mul(mk(1.0f, 1.0f, 1.0f), light_radiance(A.S, x.light, wo, x.n));
+            }
             if (!live) {
                 A.lbuf[id] = make_float4(L.x, L.y, L.z, 0.0f);
             } else {
@@ -376,7 +415,7 @@ template <bool FUSED> __global__ __launch_bounds__(256) void k_wf_generate(WfArg
                 A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
                 if (FUSED) {
                     xp3 = x.p;
-                    r = nee_sample(A, id, x, wo, mk(1.0f, 1.0f, 1.0f), pix, g.state);
+                    r = nee_sample<TR>(A, id, x, wo, mk(1.0f, 1.0f, 1.0f), g);
                 } else {
                     A.rng[id] = g.state;
                     kind = kind_of(A.S, x.mat);
@@ -415,7 +454,8 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
             x.mat = static_cast<int32_t>(__float_as_uint(xp.w));
             x.light = static_cast<int32_t>(__float_as_uint(xn.w));
             xp3 = x.p;
-            r = nee_sample(A, path, x, xyz(wo4), xyz(b4), __float_as_uint(wo4.w), A.rng[path]);
+            Pcg g = path_pcg(A.rng[path], __float_as_uint(wo4.w));
+            r = nee_sample<kWfMis>(A, path, x, xyz(wo4), xyz(b4), g);
         }
         // one call site with all lanes active: block_append synchronizes the block
         push_rays(A, in, path, xp3, r, scratch);
@@ -727,7 +767,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     }
 }
 
-template <bool FUSED>
+template <bool FUSED, int TR>
 __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t bounce) {
     __shared__ uint32_t scratch[2 * kKinds];
     clear_scratch(scratch, 2 * kKinds);
@@ -746,7 +786,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
             // no item: only takes part in the block-wide append below
         } else if (t >= ne) {
             path = A.fin_in[t - ne];
-            A.lbuf[path] = A.L[path];
+            finish_path<TR>(A, path, xyz(A.L[path]));
         } else {
             path = A.eq_in[t];
             ClosestHit h = A.eh[t];
@@ -769,7 +809,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
                 hit = !(dot(nx.n, -wi) <= 0.0f);
             }
             if (!hit) {
-                A.lbuf[path] = L4;
+                finish_path<TR>(A, path, L);
             } else {
                 float dens = wi4.w;
                 int32_t xmat = static_cast<int32_t>(__float_as_uint(xp4.w));
@@ -777,15 +817,38 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
                 float cw = dot(xnrm, wi);
                 f3 f = bsdf_eval(A.S.mats, xmat, xnrm, wo, wi);
                 beta = mul(beta, f) * cw / dens;
-                if (nx.light >= 0) {
-                    float cl = dot(nx.n, -wi);
-                    DevLight const &Ln = A.S.lights[nx.light];
-                    float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
-                    float w = dens * dens / (dens * dens + pl * pl);
-                    L = L + mul(beta, light_radiance(A.S, nx.light, -wi, nx.n) * w);
+                bool go;
+                Pcg g = path_pcg(0, __float_as_uint(wo4.w));
+                if (TR == kWfMis) {
+                    if (nx.light >= 0) {
+                        float cl = dot(nx.n, -wi);
+                        DevLight const &Ln = A.S.lights[nx.light];
+                        float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
+                        float w = dens * dens / (dens * dens + pl * pl);
+                        L = L + mul(beta, light_radiance(A.S, nx.light, -wi, nx.n) * w);
+                    }
+                    go = bounce + 1 < A.max_depth;
+                    if (go && FUSED)
+                        g.state = A.rng[path];
+                } else {
+                    // li_lt1 / li_unidirect at depth bounce + 1: Russian roulette p = 0.5 from depth 2,
+                    // then (unidirect) the vertex's emission; the loop stops after kRefModeMaxDepth vertices
+                    const uint32_t depth = bounce + 1;
+                    go = depth < kRefModeMaxDepth;
+                    if (go) {
+                        g.state = A.rng[path];
+                        if (depth >= 2) {
+                            if (g.draw() >= 0.5f)
+                                go = false;
+                            else
+                                beta = beta / 0.5f;
+                        }
+                    }
+                    if (go && TR == kWfUnidirect && nx.light >= 0)
+                        L = L + mul(beta, light_radiance(A.S, nx.light, -wi, nx.n));
                 }
-                if (bounce + 1 >= A.max_depth) {
-                    A.lbuf[path] = make_float4(L.x, L.y, L.z, 0.0f);
+                if (!go) {
+                    finish_path<TR>(A, path, L);
                 } else {
                     A.xp[path] =
                         make_float4(nx.p.x, nx.p.y, nx.p.z, __uint_as_float(static_cast<uint32_t>(nx.mat)));
@@ -797,7 +860,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
                     live = true;
                     if (FUSED) {
                         xp3 = nx.p;
-                        r = nee_sample(A, path, nx, -wi, beta, __float_as_uint(wo4.w), A.rng[path]);
+                        r = nee_sample<TR>(A, path, nx, -wi, beta, g);
                     } else {
                         kind = kind_of(A.S, nx.mat);
                     }
@@ -851,6 +914,7 @@ __global__ void k_wf_count(WfCtl *ctl, WfCounter const *eq_cnt, Counters *cnt) {
 struct WavefrontEngine {
     DevBuf xp, xn, wo, beta, L, wi, rng, lbuf, act0, act1, sq, eq, fin, eh, ctl, spill;
     DevBuf eq_b, fin_b; // the fused form's second queue parity
+    DevBuf le0;         // lt1's first-hit emission per path
     DevBuf sq_sorted, eq_sorted, ray_key, ray_rank, bin_mat, bin_tot; // ray reordering
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -914,7 +978,7 @@ bool same_config(WfArgs const &a, WfArgs const &b, WavefrontTuning const &ta, Wa
     return std::memcmp(&a, &b, sizeof(WfArgs)) == 0 && ta.paths_in_flight == tb.paths_in_flight &&
            ta.trace_wg == tb.trace_wg && ta.stack_depth == tb.stack_depth &&
            ta.sort_by_material == tb.sort_by_material && ta.sort_rays == tb.sort_rays &&
-           ta.refill_below == tb.refill_below && ta.leaf_below == tb.leaf_below;
+           ta.refill_below == tb.refill_below && ta.leaf_below == tb.leaf_below && ta.tracer == tb.tracer;
 }
 
 } // namespace
@@ -945,6 +1009,10 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     WF_HIP(e->lbuf.ensure(f4));
     WF_HIP(e->rng.ensure(sizeof(uint64_t) * cap));
     const bool fused = !tune.sort_by_material && !tune.sort_rays;
+    if (tune.tracer != kWfMis && !fused)
+        return "the lt1 and unidirect tracers run the wavefront form without material or ray sorting only";
+    if (tune.tracer == kWfLt1)
+        WF_HIP(e->le0.ensure(f4));
     if (fused) {
         WF_HIP(e->eq_b.ensure(sizeof(uint32_t) * cap));
         WF_HIP(e->fin_b.ensure(sizeof(uint32_t) * cap));
@@ -1021,6 +1089,7 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     A.eq_cnt_in = A.eq_cnt_out = &A.ctl->eq_count;
     A.fin_cnt_in = A.fin_cnt_out = &A.ctl->fin_count;
     A.fused = fused ? 1 : 0;
+    A.le0 = tune.tracer == kWfLt1 ? static_cast<float4 *>(e->le0.p) : nullptr;
     if (tune.sort_rays) {
         A.sq_sorted = static_cast<float4 *>(e->sq_sorted.p);
         A.eq_sorted = static_cast<uint32_t *>(e->eq_sorted.p);
@@ -1036,9 +1105,23 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
 
     // Grid-strided per-path kernels get exactly the blocks that are resident at once: a larger
     // grid would run its surplus waves as a second, late round.
-    uint32_t g_gen = fused ? grid_for(k_wf_generate<true>, 256) : grid_for(k_wf_generate<false>, 256),
-             g_nee = grid_for(k_wf_nee_bsdf, 256),
-             g_shade = fused ? grid_for(k_wf_shade<true>, 256) : grid_for(k_wf_shade<false>, 256),
+    using GenKernel = void (*)(WfArgs);
+    using ShadeKernel = void (*)(WfArgs, int, uint32_t);
+    GenKernel gen_k = k_wf_generate<false, kWfMis>;
+    ShadeKernel shade_k = k_wf_shade<false, kWfMis>;
+    if (fused) {
+        gen_k = tune.tracer == kWfLt1         ? k_wf_generate<true, kWfLt1>
+                : tune.tracer == kWfUnidirect ? k_wf_generate<true, kWfUnidirect>
+                                              : k_wf_generate<true, kWfMis>;
+        shade_k = tune.tracer == kWfLt1         ? k_wf_shade<true, kWfLt1>
+                  : tune.tracer == kWfUnidirect ? k_wf_shade<true, kWfUnidirect>
+                                                : k_wf_shade<true, kWfMis>;
+    }
+    // MIS: max_depth bounces; lt1 / unidirect: the reference-mode loop's kRefModeMaxDepth vertices
+    // (Russian roulette ends nearly every path long before; bounces with empty queues cost a few
+    // microseconds of launch each).
+    const uint32_t bounces = tune.tracer == kWfMis ? P.max_depth : kRefModeMaxDepth;
+    uint32_t g_gen = grid_for(gen_k, 256), g_nee = grid_for(k_wf_nee_bsdf, 256), g_shade = grid_for(shade_k, 256),
              g_acc = grid_for(k_wf_accumulate, 256);
     // The fused form's queue parities: bounce k consumes parity k & 1 and its shade fills the other.
     auto parity_args = [&](uint32_t k) {
@@ -1086,15 +1169,15 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
         if (fused) {
             // generate fills bounce 0's queues (parity 0); bounce k's shade fills bounce k + 1's
             WfArgs G = parity_args(1);
-            hipLaunchKernelGGL(k_wf_generate<true>, dim3(g_gen), dim3(256), 0, s, G);
-            for (uint32_t k = 0; k < P.max_depth; k++) {
+            hipLaunchKernelGGL(gen_k, dim3(g_gen), dim3(256), 0, s, G);
+            for (uint32_t k = 0; k < bounces; k++) {
                 WfArgs B = parity_args(k);
                 trace_bounce(s, B);
                 hipLaunchKernelGGL(k_wf_reset_fused, dim3(1), dim3(64), 0, s, B.ctl, B.eq_cnt_out, B.fin_cnt_out);
-                hipLaunchKernelGGL(k_wf_shade<true>, dim3(g_shade), dim3(256), 0, s, B, 0, k);
+                hipLaunchKernelGGL(shade_k, dim3(g_shade), dim3(256), 0, s, B, 0, k);
             }
         } else {
-            hipLaunchKernelGGL(k_wf_generate<false>, dim3(g_gen), dim3(256), 0, s, A);
+            hipLaunchKernelGGL(gen_k, dim3(g_gen), dim3(256), 0, s, A);
             for (uint32_t k = 0; k < P.max_depth; k++) {
                 int par = static_cast<int>(k & 1u);
                 hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, par);
@@ -1106,7 +1189,7 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
                     hipLaunchKernelGGL(k_wf_sort_scatter, dim3(kSortBlocks, 2), dim3(kSortWG), 0, s, A);
                 }
                 trace_bounce(s, A);
-                hipLaunchKernelGGL(k_wf_shade<false>, dim3(g_shade), dim3(256), 0, s, A, par, k);
+                hipLaunchKernelGGL(shade_k, dim3(g_shade), dim3(256), 0, s, A, par, k);
             }
         }
         hipLaunchKernelGGL(k_wf_accumulate, dim3(g_acc), dim3(256), 0, s, A);

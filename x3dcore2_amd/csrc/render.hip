@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kWG) void k_first_hit(RenderParams P) {
     }
 }
 
-template <int TRACER> __global__ __launch_bounds__(kWG) void k_render_plain(RenderParams P) {
+template <int TRACER> __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k_render_plain(RenderParams P) {
     __shared__ uint2 lds[kLdsDepth * kWG];
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
     Tracer<kWG, kLdsDepth> tr{P.S, st};
@@ -328,7 +328,7 @@ constexpr uint64_t kGenItems = 1u << 24; // (pixel, sample) items per generation
 // although subpath lengths and connection counts vary widely between samples. Each sample's
 // radiance is stored; k_sum_samples then adds every pixel's samples in sample order, as the
 // oracle does.
-__global__ __launch_bounds__(kWG) void k_render_bdpt(RenderParams P, uint32_t gen_begin, uint32_t gen_samples,
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k_render_bdpt(RenderParams P, uint32_t gen_begin, uint32_t gen_samples,
                                                      float4 *samples, uint32_t *next_item) {
     __shared__ uint2 lds[kLdsDepth * kWG];
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kWG) void k_render_mis_items(RenderParams P, uint32
 // Reference-mode tracers over (pixel, sample) items (for the firefly filter, which needs whole
 // sample images): item q = pixel q % npix, sample gen_begin + q / npix, radiance to samples[q].
 template <int TRACER>
-__global__ __launch_bounds__(kWG) void k_render_plain_items(RenderParams P, uint32_t gen_begin, uint32_t gen_samples,
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k_render_plain_items(RenderParams P, uint32_t gen_begin, uint32_t gen_samples,
                                                            float4 *samples, uint32_t *next_item) {
     __shared__ uint2 lds[kLdsDepth * kWG];
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);

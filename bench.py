@@ -7,8 +7,9 @@ per pixel, at most 8 bounces, FP32. One step = one render of the whole film; the
 triangles, materials, lights) is resident in HBM before the timed region and the film stays in
 HBM. Both execution forms are timed on the same workload — the plain one-thread-per-path kernel
 (the drop-in) and the autotuned wavefront form — and their films are checked to be bit-identical
-(so the RMSE against any reference is equal). `value` is the wavefront form's samples/s. Prints
-one JSON line.
+(so the RMSE against any reference is equal). `value` is the wavefront form's samples/s. The
+reference's own lt1 tracer is also timed in both forms at 16 spp (`lt1_tracer`). Prints one JSON
+line.
 
     python bench.py [--steps K] [--warmup W] [--workload spheres|box|indirect|batch|adjoint] [--spp N]
                     [--no-cpu-baseline] [--no-autotune | --wf-config WG,DEPTH,SORT,CAP]
@@ -423,9 +424,30 @@ def main():
         "tri_tests_per_launch": tests,
     }
     out["compose"] = bench_compose(x3d, DeviceArray, R["film"], films[best], w, h, args.no_cpu_baseline)
+    out["lt1_tracer"] = bench_lt1(x3d, scene, desc, R["film"])
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(desc)
     print(json.dumps(out), flush=True)
+
+
+def bench_lt1(x3d, scene, desc, film_d, spp=16):
+    """The reference's own unidirect_lt1 tracer (NEE without MIS, Russian roulette) on the same
+    scene and camera at 16 spp: the plain kernel and the wavefront form side by side (device
+    kernel time, best of two renders after a warm-up), and whether their films are identical."""
+    w, h = desc.width, desc.height
+    res = {"tracer": "unidirect_lt1", "spp": spp}
+    films = {}
+    for form, f in (("plain", x3d.FORM_PLAIN), ("wavefront", x3d.FORM_WAVEFRONT)):
+        r = x3d.Renderer(x3d.PT_UNIDIRECT_LT1, form=f, seed=1)
+        films[form], _ = r.render(scene, desc.camera, w, h, spp)
+        ms = min(r.render_device(scene, desc.camera, w, h, spp, film_d.ptr).kernel_ms for _ in range(2))
+        res[form + "_ms"] = ms
+        res[form + "_msamples_per_s"] = w * h * spp / ms / 1e3
+        log(f"lt1 {form}: {ms:.2f} ms, {w * h * spp / ms / 1e3:.1f} Msamples/s")
+    res["speedup"] = res["plain_ms"] / res["wavefront_ms"]
+    res["films_bit_identical"] = bool(np.array_equal(films["plain"].view(np.uint32),
+                                                     films["wavefront"].view(np.uint32)))
+    return res
 
 
 def bench_compose(x3d, DeviceArray, film_d, film_host, w, h, no_cpu, calls=20):

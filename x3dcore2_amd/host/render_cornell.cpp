@@ -6,8 +6,9 @@
 // exposure off, exposure 1; pipeline.cpp:116-117), as pt_render_pipeline::render_frame does
 // (pipeline.cpp:73-101).
 //
-//   render_cornell OBJ_DIR OUT_PREFIX WIDTH HEIGHT SPP TRACER [SEED]
-//     TRACER: lt1 | mis | bdpt | lt2
+//   render_cornell OBJ_DIR OUT_PREFIX WIDTH HEIGHT SPP TRACER [SEED [FORM]]
+//     TRACER: lt1 | unidirect | mis | bdpt | lt2
+//     FORM: plain (default) | wavefront (lt1, unidirect, mis; the same film)
 //     writes OUT_PREFIX.film (float32 rgba), OUT_PREFIX.rgba (uint8 rgba), OUT_PREFIX.ppm
 //   render_cornell OBJ_DIR --host-only
 //     builds the scene on the host only and reports the device error of the commit (for
@@ -105,14 +106,21 @@ int main(int argc, char **argv) {
         return 0;
     }
     if (argc < 7) {
-        std::fprintf(stderr, "usage: %s OBJ_DIR OUT_PREFIX WIDTH HEIGHT SPP lt1|mis|bdpt|lt2 [SEED]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s OBJ_DIR OUT_PREFIX WIDTH HEIGHT SPP lt1|unidirect|mis|bdpt|lt2 [SEED [FORM]]\n",
+                     argv[0]);
         return 2;
     }
     const std::string dir = argv[1], out = argv[2], tracer = argv[6];
     const uint32_t w = static_cast<uint32_t>(std::atoi(argv[3])), h = static_cast<uint32_t>(std::atoi(argv[4])),
                    spp = static_cast<uint32_t>(std::atoi(argv[5]));
     const uint64_t seed = argc > 7 ? std::strtoull(argv[7], nullptr, 10) : 1;
-    int32_t pt = tracer == "lt1" ? X3D_PT_UNIDIRECT_LT1
+    const std::string form = argc > 8 ? argv[8] : "plain";
+    if (form != "plain" && form != "wavefront") {
+        std::fprintf(stderr, "unknown form %s\n", form.c_str());
+        return 2;
+    }
+    int32_t pt = tracer == "lt1"         ? X3D_PT_UNIDIRECT_LT1
+                 : tracer == "unidirect" ? X3D_PT_UNIDIRECT
                  : tracer == "mis" ? X3D_PT_UNIDIRECT_MIS
                  : tracer == "bdpt" ? X3D_PT_BIDIRECT_MIS
                  : tracer == "lt2" ? X3D_PT_BIDIRECT_LT2
@@ -126,6 +134,7 @@ int main(int argc, char **argv) {
         s.commit();
         x3d_render_options o = x3d::renderer::defaults(pt);
         o.seed = seed;
+        o.form = form == "wavefront" ? X3D_FORM_WAVEFRONT : X3D_FORM_PLAIN;
         x3d::renderer r(o);
         x3d_render_stats st{};
         std::vector<float> film = r.render(s, camera(), w, h, spp, false, &st);
@@ -140,8 +149,8 @@ int main(int argc, char **argv) {
         ppm << "P6\n" << w << " " << h << "\n255\n";
         for (size_t p = 0; p < static_cast<size_t>(w) * h; p++)
             ppm.write(reinterpret_cast<const char *>(&px[4 * p]), 3);
-        std::printf("%ux%u x %u spp (%s): %.2f ms render, exposure %.5f, compose %.3f ms\n", w, h, spp,
-                    tracer.c_str(), st.total_ms, cs.exposure, cs.exposure_ms + cs.compose_ms);
+        std::printf("%ux%u x %u spp (%s, %s): %.2f ms render, exposure %.5f, compose %.3f ms\n", w, h, spp,
+                    tracer.c_str(), form.c_str(), st.total_ms, cs.exposure, cs.exposure_ms + cs.compose_ms);
     } catch (x3d::error const &e) {
         std::fprintf(stderr, "error %d: %s\n", e.code(), e.what());
         return 1;

@@ -269,6 +269,27 @@ def test_wavefront_reference_tracers_identical_to_plain(x3d, scene_name, tracer,
     assert img[:, :3].max() > 0
 
 
+@pytest.mark.parametrize("tracer", ["UNIDIRECT_LT1", "UNIDIRECT", "UNIDIRECT_MIS"])
+def test_wavefront_without_lights_and_ragged(x3d, tracer):
+    """Edge cases of the wavefront form: a scene without lights (no light sample is drawn, the film
+    is black) and a ragged film (7 x 3 pixels, 3 samples, one sample per generation) match the plain
+    kernel bit for bit."""
+    import dataclasses
+
+    desc = QUERY_SCENES["box"]()
+    pt = getattr(x3d, "PT_" + tracer)
+    dark = dataclasses.replace(desc, lights=[])
+    for d, (w, h, spp, cap) in ((dark, (24, 16, 2, 0)), (desc, (7, 3, 3, 1))):
+        print(f"[gpu] wavefront edge {tracer} lights={len(d.lights)} {w}x{h}x{spp} cap={cap}", flush=True)
+        gs = x3d.Scene(d)
+        plain, sp = x3d.Renderer(pt, seed=9).render(gs, d.camera, w, h, spp)
+        img, st = x3d.Renderer(pt, form=x3d.FORM_WAVEFRONT, seed=9, queue_capacity=cap).render(gs, d.camera, w, h, spp)
+        assert np.array_equal(_bits(img), _bits(plain)), np.abs(img - plain).max()
+        assert st.closest_queries == sp.closest_queries and st.anyhit_queries == sp.anyhit_queries
+        if not d.lights:
+            assert not img[:, :3].any() and st.anyhit_queries == 0
+
+
 def test_wavefront_lt1_autotune(x3d):
     """The autotuner of an lt1 wavefront renderer tries the unsorted configurations only."""
     desc = QUERY_SCENES["spheres_small"]()

@@ -387,7 +387,7 @@ def main():
         nodes = (s0.closest_nodes - s0.first_hit_nodes) + s0.anyhit_nodes
         tests = (s0.closest_tests - s0.first_hit_tests) + s0.anyhit_tests
         launch_ms = float(np.mean([s.trace_ms for s in R["stats"]]))
-        kernel = "k_render_mis"
+        kernel = "k_render_mis_items"
         traffic = traffic_from_profile(f"traffic_{desc.name}_plain.md", kernel)
     if best == "wavefront" and not args.no_ablation:
         # ablation: the same configuration with both ray queues sorted by key before traversal

@@ -257,7 +257,9 @@ typedef struct x3d_render_options {
     uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default (64) */
     uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default = max = 64M
                               (216 B per path; a render allocates for min(capacity, w*h*spp));
-                              BIDIRECT_MIS: (pixel, sample) items per generation (default 16M) */
+                              plain UNIDIRECT_MIS, BIDIRECT_MIS and firefly-filtered renders:
+                              (pixel, sample) items per generation (default 16M, 256 MiB of
+                              sample radiance) */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
     uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16 */
     int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */

@@ -258,68 +258,6 @@ template <int TRACER> __global__ __launch_bounds__(kWG) __attribute__((amdgpu_wa
     }
 }
 
-// MIS tracer of the plain form with path regeneration: every loop iteration advances each lane's
-// path by one bounce; a lane whose path ends adds it to its pixel's sum and starts the pixel's
-// next sample, and a lane whose pixel is complete writes it and takes the next unclaimed pixel
-// from a global counter. Lanes therefore stay busy while paths of different lengths finish,
-// and each pixel's samples are still summed in order, as in the oracle.
-__global__ __launch_bounds__(kWG) void k_render_mis(RenderParams P, uint32_t *next_pixel) {
-    __shared__ uint2 lds[kLdsDepth * kWG];
-    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
-    Tracer<kWG, kLdsDepth> tr{P.S, st};
-    uint32_t npix = P.width * P.height;
-    uint32_t grid_threads = gridDim.x * kWG;
-    float scale = 1.0f / static_cast<float>(P.spp_total);
-    uint32_t pix = blockIdx.x * kWG + threadIdx.x;
-    if (pix < npix && P.s_begin < P.s_end) {
-        f3 o, v;
-        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
-        bool valid;
-        HitRec first = load_first(P, pix, valid);
-        f3 acc = mk(0.0f, 0.0f, 0.0f);
-        if (P.s_begin > 0) {
-            float4 q = P.acc[pix];
-            acc = mk(q.x, q.y, q.z);
-        }
-        uint32_t s = P.s_begin;
-        PathState ps;
-        tr.mis_begin(ps, first, valid, v, pcg_for(P.seed, pix, s), P.max_depth);
-        for (;;) {
-            if (ps.alive)
-                tr.mis_bounce(ps, P.max_depth);
-            if (!ps.alive) {
-                acc = acc + ps.L;
-                if (++s == P.s_end) {
-                    if (P.finalize)
-                        P.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
-                    else
-                        P.acc[pix] = make_float4(acc.x, acc.y, acc.z, 0.0f);
-                    pix = grid_threads + atomicAdd(next_pixel, 1u);
-                    if (pix >= npix)
-                        break;
-                    camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
-                    first = load_first(P, pix, valid);
-                    acc = mk(0.0f, 0.0f, 0.0f);
-                    if (P.s_begin > 0) {
-                        float4 q = P.acc[pix];
-                        acc = mk(q.x, q.y, q.z);
-                    }
-                    s = P.s_begin;
-                }
-                tr.mis_begin(ps, first, valid, v, pcg_for(P.seed, pix, s), P.max_depth);
-            }
-        }
-    }
-    if (P.cnt) {
-        flush_counter(&P.cnt->closest, tr.nclosest);
-        flush_counter(&P.cnt->anyhit, tr.nany);
-        flush_counter(&P.cnt->pops, tr.pops);
-        flush_counter(&P.cnt->tests, tr.tests);
-        flush_counter(&P.cnt->spops, tr.spops);
-        flush_counter(&P.cnt->stests, tr.stests);
-    }
-}
-
 constexpr uint64_t kGenItems = 1u << 24; // (pixel, sample) items per generation: 256 MiB of sample radiance
 
 // Bidirectional tracer. Work items are the (pixel, sample) pairs of one generation of samples,
@@ -357,11 +295,13 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 }
 
-// Plain MIS tracer for films with fewer pixels than the chip holds lanes (one pixel per lane
-// would leave most lanes idle): work items are the (pixel, sample) pairs of a generation, as in
+// Plain MIS tracer: work items are the (pixel, sample) pairs of a generation, as in
 // k_render_bdpt; a lane whose path ends stores its radiance and starts the next unclaimed item
-// at once (per-bounce regeneration, as in k_render_mis). k_sum_samples then adds every pixel's
-// samples in sample order, so the film equals k_render_mis's bit for bit.
+// at once (per-bounce path regeneration), so lanes stay busy while paths of different lengths
+// finish. k_sum_samples then adds every pixel's samples in sample order, as the oracle does.
+// (Per-pixel lanes that regenerate the pixel's next sample in place measured slower: 559 vs
+// 496 ms at 1024^2 x 64 spp and 204 vs 129 ms at 512^2 x 64 spp on the 1.3M-triangle spheres
+// scene; the item order keeps neighbouring lanes on neighbouring pixels of one sample.)
 __global__ __launch_bounds__(kWG) void k_render_mis_items(RenderParams P, uint32_t gen_begin, uint32_t gen_samples,
                                                           float4 *samples, uint32_t *next_item) {
     __shared__ uint2 lds[kLdsDepth * kWG];
@@ -688,16 +628,13 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     int tracer = r->opt.pt_type;
     uint32_t grid_first = persistent_grid(k_first_hit, npix);
     uint32_t grid_render = 1;
-    // Films with fewer pixels than 4x the lanes the plain MIS kernel keeps resident run the
-    // MIS tracer over (pixel, sample) items instead (same film).
-    // The firefly filter works on whole sample images: every tracer then runs over items.
-    bool mis_items = tracer == X3D_PT_UNIDIRECT_MIS && !wf && !d_dl &&
-                     (firefly || npix < 4ull * kWG * persistent_grid(k_render_mis, ~0ull >> 8));
+    // The plain MIS tracer runs over (pixel, sample) items. The firefly filter works on whole
+    // sample images: every tracer then runs over items.
+    const bool mis_items = tracer == X3D_PT_UNIDIRECT_MIS && !wf && !d_dl;
     const bool plain_items = firefly && tracer != X3D_PT_UNIDIRECT_MIS && tracer != X3D_PT_BIDIRECT_MIS;
     switch (tracer) {
     case X3D_PT_UNIDIRECT_MIS:
-        grid_render = mis_items ? persistent_grid(k_render_mis_items, npix * static_cast<uint64_t>(spp))
-                                : persistent_grid(k_render_mis, npix);
+        grid_render = mis_items ? persistent_grid(k_render_mis_items, npix * static_cast<uint64_t>(spp)) : 1;
         break;
     case X3D_PT_UNIDIRECT_LT1:
         grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_UNIDIRECT_LT1>, npix * spp)
@@ -777,13 +714,9 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     switch (tracer) {
     case -1: break;
     case X3D_PT_UNIDIRECT_MIS:
-        if (mis_items) {
-            rc = run_generations(k_render_mis_items);
-            if (rc)
-                return rc;
-        } else {
-            hipLaunchKernelGGL(k_render_mis, dim3(grid_render), dim3(kWG), 0, r->stream, P, next_pixel);
-        }
+        rc = run_generations(k_render_mis_items);
+        if (rc)
+            return rc;
         break;
     case X3D_PT_UNIDIRECT_LT1:
         rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_UNIDIRECT_LT1>)

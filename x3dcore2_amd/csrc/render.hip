@@ -201,63 +201,6 @@ __global__ __launch_bounds__(kWG) void k_first_hit(RenderParams P) {
     }
 }
 
-template <int TRACER> __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k_render_plain(RenderParams P) {
-    __shared__ uint2 lds[kLdsDepth * kWG];
-    TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
-    Tracer<kWG, kLdsDepth> tr{P.S, st};
-    uint32_t npix = P.width * P.height;
-    float scale = 1.0f / static_cast<float>(P.spp_total);
-    for (uint32_t pix = blockIdx.x * kWG + threadIdx.x; pix < npix; pix += gridDim.x * kWG) {
-        f3 o, v;
-        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
-        float4 a = P.first[3 * pix], b = P.first[3 * pix + 1], c = P.first[3 * pix + 2];
-        HitRec first;
-        first.p = mk(a.x, a.y, a.z);
-        first.t = a.w;
-        first.n = mk(b.x, b.y, b.z);
-        first.geo = static_cast<int32_t>(__float_as_uint(b.w));
-        first.mat = static_cast<int32_t>(__float_as_uint(c.x));
-        first.light = static_cast<int32_t>(__float_as_uint(c.y));
-        bool valid = c.z != 0.0f;
-        f3 acc = mk(0.0f, 0.0f, 0.0f);
-        if (P.s_begin > 0) {
-            float4 q = P.acc[pix];
-            acc = mk(q.x, q.y, q.z);
-        }
-        for (uint32_t s = P.s_begin; s < P.s_end; s++) {
-            Pcg g = pcg_for(P.seed, pix, s);
-            f3 L;
-            if (TRACER == X3D_PT_UNIDIRECT_MIS)
-                L = tr.li_mis(first, valid, v, g, P.max_depth);
-            else if (TRACER == X3D_PT_UNIDIRECT_LT1)
-                L = tr.li_lt1(first, valid, v, g);
-            else if (TRACER == X3D_PT_UNIDIRECT)
-                L = tr.li_unidirect(first, valid, v, g);
-            else if (TRACER == X3D_PT_DIRECT)
-                L = tr.li_direct(first, valid, v, g);
-            else if (TRACER == X3D_PT_BIDIRECT_LT2)
-                L = tr.li_lt2(first, valid, v, g);
-            else if (TRACER == X3D_PT_POSITION)
-                L = tr.li_position(first, valid);
-            else // X3D_PT_NORMAL (normal_tracer, pathtracer.cpp:492-506)
-                L = valid ? (first.n + mk(1.0f, 1.0f, 1.0f)) / 2.0f : mk(0.0f, 0.0f, 0.0f);
-            acc = acc + L;
-        }
-        if (P.finalize)
-            P.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
-        else
-            P.acc[pix] = make_float4(acc.x, acc.y, acc.z, 0.0f);
-    }
-    if (P.cnt) {
-        flush_counter(&P.cnt->closest, tr.nclosest);
-        flush_counter(&P.cnt->anyhit, tr.nany);
-        flush_counter(&P.cnt->pops, tr.pops);
-        flush_counter(&P.cnt->tests, tr.tests);
-        flush_counter(&P.cnt->spops, tr.spops);
-        flush_counter(&P.cnt->stests, tr.stests);
-    }
-}
-
 constexpr uint64_t kGenItems = 1u << 24; // (pixel, sample) items per generation: 256 MiB of sample radiance
 
 // Bidirectional tracer. Work items are the (pixel, sample) pairs of one generation of samples,
@@ -572,10 +515,6 @@ int run_queries(const x3d_scene *scene, const float *d_rays, const float *d_tmin
 
 namespace {
 
-template <int T> void launch_plain(uint32_t grid, hipStream_t s, RenderParams const &P) {
-    hipLaunchKernelGGL(k_render_plain<T>, dim3(grid), dim3(kWG), 0, s, P);
-}
-
 // Renders into d_out; with d_dl (dLoss/dfilm) it runs the adjoint of the MIS tracer instead and
 // writes the material gradient to d_grad.
 // Wavefront tuning from the renderer's options (0 / DEFAULT fields keep the defaults).
@@ -631,35 +570,32 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     // The plain MIS tracer runs over (pixel, sample) items. The firefly filter works on whole
     // sample images: every tracer then runs over items.
     const bool mis_items = tracer == X3D_PT_UNIDIRECT_MIS && !wf && !d_dl;
-    const bool plain_items = firefly && tracer != X3D_PT_UNIDIRECT_MIS && tracer != X3D_PT_BIDIRECT_MIS;
+    // The reference-mode tracers run over (pixel, sample) items as well (same films; per-pixel
+    // lanes measured 1.3-1.9x slower: lt1 98.2 vs 75.0 ms, direct 16.1 vs 8.7 ms at 1024^2 x 16 spp
+    // on the 1.3M-triangle spheres scene).
+    const bool plain_items = tracer != X3D_PT_UNIDIRECT_MIS && tracer != X3D_PT_BIDIRECT_MIS;
     switch (tracer) {
     case X3D_PT_UNIDIRECT_MIS:
         grid_render = mis_items ? persistent_grid(k_render_mis_items, npix * static_cast<uint64_t>(spp)) : 1;
         break;
     case X3D_PT_UNIDIRECT_LT1:
-        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_UNIDIRECT_LT1>, npix * spp)
-                                  : persistent_grid(k_render_plain<X3D_PT_UNIDIRECT_LT1>, npix);
+        grid_render = persistent_grid(k_render_plain_items<X3D_PT_UNIDIRECT_LT1>, npix * spp);
         break;
     case X3D_PT_UNIDIRECT:
-        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_UNIDIRECT>, npix * spp)
-                                  : persistent_grid(k_render_plain<X3D_PT_UNIDIRECT>, npix);
+        grid_render = persistent_grid(k_render_plain_items<X3D_PT_UNIDIRECT>, npix * spp);
         break;
     case X3D_PT_DIRECT:
-        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_DIRECT>, npix * spp)
-                                  : persistent_grid(k_render_plain<X3D_PT_DIRECT>, npix);
+        grid_render = persistent_grid(k_render_plain_items<X3D_PT_DIRECT>, npix * spp);
         break;
     case X3D_PT_BIDIRECT_LT2:
-        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_BIDIRECT_LT2>, npix * spp)
-                                  : persistent_grid(k_render_plain<X3D_PT_BIDIRECT_LT2>, npix);
+        grid_render = persistent_grid(k_render_plain_items<X3D_PT_BIDIRECT_LT2>, npix * spp);
         break;
     case X3D_PT_POSITION:
-        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_POSITION>, npix * spp)
-                                  : persistent_grid(k_render_plain<X3D_PT_POSITION>, npix);
+        grid_render = persistent_grid(k_render_plain_items<X3D_PT_POSITION>, npix * spp);
         break;
     case X3D_PT_BIDIRECT_MIS: grid_render = persistent_grid(k_render_bdpt, npix * static_cast<uint64_t>(spp)); break;
     default:
-        grid_render = plain_items ? persistent_grid(k_render_plain_items<X3D_PT_NORMAL>, npix * spp)
-                                  : persistent_grid(k_render_plain<X3D_PT_NORMAL>, npix);
+        grid_render = persistent_grid(k_render_plain_items<X3D_PT_NORMAL>, npix * spp);
         break;
     }
     uint32_t grid = grid_first > grid_render ? grid_first : grid_render;
@@ -719,32 +655,27 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
             return rc;
         break;
     case X3D_PT_UNIDIRECT_LT1:
-        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_UNIDIRECT_LT1>)
-                         : (launch_plain<X3D_PT_UNIDIRECT_LT1>(grid_render, r->stream, P), X3D_OK);
+        rc = run_generations(k_render_plain_items<X3D_PT_UNIDIRECT_LT1>);
         if (rc)
             return rc;
         break;
     case X3D_PT_UNIDIRECT:
-        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_UNIDIRECT>)
-                         : (launch_plain<X3D_PT_UNIDIRECT>(grid_render, r->stream, P), X3D_OK);
+        rc = run_generations(k_render_plain_items<X3D_PT_UNIDIRECT>);
         if (rc)
             return rc;
         break;
     case X3D_PT_DIRECT:
-        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_DIRECT>)
-                         : (launch_plain<X3D_PT_DIRECT>(grid_render, r->stream, P), X3D_OK);
+        rc = run_generations(k_render_plain_items<X3D_PT_DIRECT>);
         if (rc)
             return rc;
         break;
     case X3D_PT_BIDIRECT_LT2:
-        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_BIDIRECT_LT2>)
-                         : (launch_plain<X3D_PT_BIDIRECT_LT2>(grid_render, r->stream, P), X3D_OK);
+        rc = run_generations(k_render_plain_items<X3D_PT_BIDIRECT_LT2>);
         if (rc)
             return rc;
         break;
     case X3D_PT_POSITION:
-        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_POSITION>)
-                         : (launch_plain<X3D_PT_POSITION>(grid_render, r->stream, P), X3D_OK);
+        rc = run_generations(k_render_plain_items<X3D_PT_POSITION>);
         if (rc)
             return rc;
         break;
@@ -754,8 +685,7 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
             return rc;
         break;
     default:
-        rc = plain_items ? run_generations(k_render_plain_items<X3D_PT_NORMAL>)
-                         : (launch_plain<X3D_PT_NORMAL>(grid_render, r->stream, P), X3D_OK);
+        rc = run_generations(k_render_plain_items<X3D_PT_NORMAL>);
         if (rc)
             return rc;
         break;

@@ -99,6 +99,18 @@ int x3d_scene_add_obj(x3d_scene *scene, const char *path, int32_t material_id, i
 int x3d_scene_add_area_light(x3d_scene *scene, int32_t mesh_id, const float radiance[3],
                              int32_t *out_light_id);
 
+/* Adds the sky light (replaces sky_light, light.cpp:87-163, which the reference's glTF loader
+ * adds as its virtual light, resource.cpp:518-523). It has no geometry: rays never hit it, and
+ * it is reached only by light sampling. At x3d_scene_commit it is placed by
+ * sky_light::set_scene_boundary (light.cpp:96-100) with the path space's bounds (the union of
+ * every mesh's vertex bounds): diameter = 2 * (|max - min| / 2), reference point = the bounds'
+ * centroid with z = 0 (the reference never calls set_scene_boundary, leaving both unset). A
+ * light sample picks u cosine-distributed about +z, the point (u + ref) * dia with normal -u and
+ * density (z . u) / pi; it sends L max(cos_light, 0) max(cos_target, 0) (no distance
+ * fall-off). Its power, for the light choice, is pi * dia^2 / 2 per channel. In
+ * X3D_PT_UNIDIRECT_MIS its light samples carry MIS weight 1 (no BSDF sample can reach it). */
+int x3d_scene_add_sky_light(x3d_scene *scene, const float radiance[3], int32_t *out_light_id);
+
 /* ---- objects by id (replace the three if_obj_actuator back-ends — if_path_space,
  * if_material_container, if_light_sources — as objdb::push_updates drives them with
  * load(obj, trans) / unload(obj) / commit(): obj.h:46-55, objdb.cpp:59-87,
@@ -129,6 +141,9 @@ int x3d_scene_load_material(x3d_scene *scene, uint64_t obj_id, const x3d_materia
  * must be loaded at commit). The light follows its geometry's transform (the reference's
  * area_light::transform keeps the untransformed geometry, light.cpp:83-85). */
 int x3d_scene_load_light(x3d_scene *scene, uint64_t obj_id, uint64_t geometry_obj_id, const float radiance[3]);
+/* if_light_sources::load of a sky light (see x3d_scene_add_sky_light; light.cpp:87-163),
+ * placed at each commit with the path space's bounds at that time. */
+int x3d_scene_load_sky_light(x3d_scene *scene, uint64_t obj_id, const float radiance[3]);
 /* unload(obj) of the given protocol (X3D_OBJ_*); unknown ids are ignored. */
 int x3d_scene_unload(x3d_scene *scene, int32_t protocol, uint64_t obj_id);
 /* obj id of committed mesh `geo` (for add_* scenes: geo itself). */
@@ -244,7 +259,9 @@ typedef struct x3d_renderer x3d_renderer;
 #define X3D_PT_UNIDIRECT_MIS 16
 
 /* Execution forms of the same estimator; every form gives the bit-identical film. */
-#define X3D_FORM_PLAIN 0     /* one thread per pixel path, one kernel (the drop-in) */
+#define X3D_FORM_PLAIN 0     /* persistent kernels over the (pixel, sample) items of a generation,
+                                each lane taking the next item when its path ends, then an in-order
+                                per-pixel sum (the drop-in) */
 #define X3D_FORM_WAVEFRONT 1 /* ray queues per bounce, bounce loop in a hipGraph; runs
                                 UNIDIRECT_MIS, UNIDIRECT_LT1 and UNIDIRECT (the last two unsorted) */
 
@@ -257,9 +274,9 @@ typedef struct x3d_render_options {
     uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default (64) */
     uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default = max = 64M
                               (216 B per path; a render allocates for min(capacity, w*h*spp));
-                              plain UNIDIRECT_MIS, BIDIRECT_MIS and firefly-filtered renders:
-                              (pixel, sample) items per generation (default 16M, 256 MiB of
-                              sample radiance) */
+                              PLAIN (every tracer): (pixel, sample) items per generation, whole
+                              samples of every pixel (default 16M; the sample buffer takes 16 B
+                              per item, 256 MiB at the default, min(capacity, w*h*spp) items) */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
     uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16 */
     int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */
@@ -331,12 +348,13 @@ int x3d_renderer_autotune(x3d_renderer *r, const x3d_scene *scene, const x3d_cam
                           x3d_autotune_report *report);
 
 /* Renders the film: rgba float[4*w*h], pixel (x, y) at index x + y*w, row 0 = top; rgb is the
- * mean radiance over spp samples, alpha = 1/spp (renderer.cpp:270-295 with one task).
+ * mean radiance over spp samples, alpha = 1/spp (pt_image_renderer::render with one task: renderer.cpp:140-159).
  * firefly_filter != 0 applies sampling_task::run's filter (renderer.cpp:46-83) to every sample
  * image before the per-pixel sum: an interior pixel whose squared norm exceeds each of its 8
- * neighbours' by more than 1.5 is capped, per component, at their mean squared norm + 1.5. Every
- * tracer then keeps whole sample images in device memory (generations of at most queue_capacity
- * (pixel, sample) items: 16M by default, 64M in the wavefront form). stats may be NULL. */
+ * neighbours' by more than 1.5 is capped, per component, at their mean squared norm + 1.5. The
+ * filter acts on whole sample images, which every form keeps in device memory anyway (plain:
+ * generations of at most queue_capacity (pixel, sample) items, 16M by default; wavefront: one
+ * image per generation sample). stats may be NULL. */
 int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
                uint32_t height, uint32_t spp, int32_t firefly_filter, float *rgba,
                x3d_render_stats *stats);

@@ -64,8 +64,25 @@ class _Backend:
                                          np.ascontiguousarray(mesh.tris, np.uint32), mesh.tris.shape[0],
                                          mat_ids[mesh.material] if mesh.material >= 0 else -1)
         for mi, rad in desc.lights:
-            getattr(lib, p + "add_area_light")(self.h, mi, _f32(rad))
+            if mi < 0:  # the sky
+                getattr(lib, p + "add_sky_light")(self.h, _f32(rad))
+            else:
+                getattr(lib, p + "add_area_light")(self.h, mi, _f32(rad))
         getattr(lib, p + "commit")(self.h)
+
+    def light_info(self, light):
+        """(power [3], sky reference point [3], sky diameter) after commit."""
+        out = np.zeros(7, np.float32)
+        getattr(self.lib, self.prefix + "light_info")(self.h, light, out)
+        return out[:3], out[3:6], out[6]
+
+    def light_eval(self, light, l, nl, nt):
+        """if_light::eval of light `light` at connections l (target - light point), light normals
+        nl and target normals nt ([n, 3] each) -> [n, 3] float32."""
+        l, nl, nt = _f32(l), _f32(nl), _f32(nt)
+        out = np.zeros_like(l)
+        getattr(self.lib, self.prefix + "light_eval")(self.h, light, l.shape[0], l, nl, nt, out)
+        return out
 
     def bvh(self):
         nn, np_ = C.c_uint32(), C.c_uint32()
@@ -90,6 +107,9 @@ class Oracle(_Backend):
         lib.orc_add_mixture.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float]
         lib.orc_add_mesh.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, _u32p, C.c_uint32, C.c_int]
         lib.orc_add_area_light.argtypes = [C.c_void_p, C.c_int, _f32p]
+        lib.orc_add_sky_light.argtypes = [C.c_void_p, _f32p]
+        lib.orc_light_info.argtypes = [C.c_void_p, C.c_int, _f32p]
+        lib.orc_light_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _f32p, _f32p, _f32p, _f32p]
         lib.orc_commit.argtypes = [C.c_void_p]
         lib.orc_bvh_size.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         lib.orc_bvh_dump.argtypes = [C.c_void_p, _f32p, _u32p, _i32p]
@@ -253,6 +273,9 @@ class Reference(_Backend):
         lib.ref_add_mixture.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float]
         lib.ref_add_mesh.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, _u32p, C.c_uint32, C.c_int]
         lib.ref_add_area_light.argtypes = [C.c_void_p, C.c_int, _f32p]
+        lib.ref_add_sky_light.argtypes = [C.c_void_p, _f32p]
+        lib.ref_light_info.argtypes = [C.c_void_p, C.c_int, _f32p]
+        lib.ref_light_eval.argtypes = [C.c_void_p, C.c_int, C.c_uint32, _f32p, _f32p, _f32p, _f32p]
         lib.ref_commit.argtypes = [C.c_void_p]
         lib.ref_bvh_size.argtypes = [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         lib.ref_bvh_dump.argtypes = [C.c_void_p, _f32p, _u32p, _i32p]

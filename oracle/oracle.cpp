@@ -27,15 +27,6 @@ void commit(Scene &s) {
         m.area = cum;
         m.light = -1;
     }
-    s.light_cum.clear();
-    s.total_power = 0;
-    for (size_t l = 0; l < s.lights.size(); l++) {
-        Light &L = s.lights[l];
-        L.power = (static_cast<float>(M_PI) * s.meshes[L.mesh].area) * L.rad;
-        s.total_power += norm(L.power);
-        s.light_cum.push_back(s.total_power);
-        s.meshes[L.mesh].light = static_cast<int>(l);
-    }
     // if_path_space::load unions each mesh's aabb (trimesh::update_aabb, geometry.cpp:97-103)
     // into an initially empty box (tensor.h:756-771)
     bool empty = true;
@@ -48,6 +39,25 @@ void commit(Scene &s) {
         }
     s.space_lo = lo;
     s.space_range = hi - lo;
+    s.light_cum.clear();
+    s.total_power = 0;
+    for (size_t l = 0; l < s.lights.size(); l++) {
+        Light &L = s.lights[l];
+        if (L.sky()) {
+            // sky_light::set_scene_boundary (light.cpp:96-100) with the path space's bounds:
+            // enclosing_radius = |max - min| / 2, centroid = (max + min) / 2 (tensor.h:811-813)
+            L.sky_dia = 2 * (norm(hi - lo) / 2);
+            L.sky_ref = (hi + lo) / 2.0f;
+            L.sky_ref.z = 0.0f;
+            // sky_light::power (light.cpp:151): pi * dia^2 / 2 in every channel
+            L.power = V3f(static_cast<float>(M_PI) * (L.sky_dia * L.sky_dia / 2));
+        } else {
+            L.power = (static_cast<float>(M_PI) * s.meshes[L.mesh].area) * L.rad;
+            s.meshes[L.mesh].light = static_cast<int>(l);
+        }
+        s.total_power += norm(L.power);
+        s.light_cum.push_back(s.total_power);
+    }
     build_bvh(s);
 }
 
@@ -727,11 +737,20 @@ template <class T> struct LightSample {
     int light;
     V3<T> p, n;
     T area_dens; // area density times light selection probability (pathtracer.cpp:206-207)
+    bool sky;
 };
 
-// basic_light_sources::sample_light (lightsources.cpp:59-75) + trimesh::sample
-// (geometry.cpp:44-66) with the triangle drawn by area instead of by index.
-// Draws consumed: 4.
+// Selection probability |power| / total power (lightsources.cpp:73); 0 when every light has zero
+// power (the reference's 0 / 0).
+inline float pick_prob(Scene const &s, Light const &L) {
+    return s.total_power > 0.0f ? norm(L.power) / s.total_power : 0.0f;
+}
+
+// basic_light_sources::sample_light (lightsources.cpp:59-75), then a point on the light: on an
+// area light trimesh::sample (geometry.cpp:44-66) with the triangle drawn by area instead of by
+// index; on the sky sky_light::sample_emssion_surface (light.cpp:117-129): u cosine-distributed
+// about +z, p = (u + ref) * dia, n = -u, density (z . u) / pi. Draws consumed: 4 (the sky
+// skips the second).
 template <class T> LightSample<T> sample_light(Scene const &s, Pcg &g) {
     LightSample<T> ls;
     float e = g.draw() * s.total_power;
@@ -746,7 +765,20 @@ template <class T> LightSample<T> sample_light(Scene const &s, Pcg &g) {
     if (lo >= s.light_cum.size())
         lo = static_cast<unsigned>(s.light_cum.size()) - 1;
     Light const &L = s.lights[lo];
-    float prob = norm(L.power) / s.total_power;
+    float prob = pick_prob(s, L);
+    ls.light = static_cast<int>(lo);
+    ls.sky = L.sky();
+    if (L.sky()) {
+        (void)g.draw();
+        T e0 = T(g.draw());
+        T e1 = T(g.draw());
+        V3<T> z(T(0), T(0), T(1));
+        V3<T> u = cos_hemisphere(z, e0, e1);
+        ls.n = -u;
+        ls.p = (u + cvt<T>(L.sky_ref)) * T(L.sky_dia);
+        ls.area_dens = (inner(z, u) / pi_v<T>()) * T(prob);
+        return ls;
+    }
     Mesh const &m = s.meshes[L.mesh];
     unsigned nt = static_cast<unsigned>(m.cum_area.size());
     float ea = g.draw() * m.area;
@@ -767,7 +799,6 @@ template <class T> LightSample<T> sample_light(Scene const &s, Pcg &g) {
     T b1 = r * v;
     T b2 = T(1) - b0 - b1;
     uint32_t i0 = m.tris[3 * a], i1 = m.tris[3 * a + 1], i2 = m.tris[3 * a + 2];
-    ls.light = static_cast<int>(lo);
     ls.p = b0 * cvt<T>(m.verts[i0]) + b1 * cvt<T>(m.verts[i1]) + b2 * cvt<T>(m.verts[i2]);
     ls.n = normalize(b0 * cvt<T>(m.normals[i0]) + b1 * cvt<T>(m.normals[i1]) +
                      b2 * cvt<T>(m.normals[i2]));
@@ -775,8 +806,16 @@ template <class T> LightSample<T> sample_light(Scene const &s, Pcg &g) {
     return ls;
 }
 
-// area_light::eval (light.cpp:41-52): radiance through the connection l = target - p_light.
+// area_light::eval (light.cpp:41-52): radiance through the connection l = target - p_light;
+// sky_light::eval (light.cpp:131-137): no distance fall-off, cosines clamped at zero.
 template <class T> V3<T> light_eval(Light const &L, V3<T> l, V3<T> nl, V3<T> nt, T *r2_out) {
+    if (L.sky()) {
+        V3<T> in = normalize(l);
+        T cos_o = inner(nl, in);
+        T cos_i = inner(nt, -in);
+        *r2_out = T(0);
+        return (cvt<T>(L.rad) * smax(cos_o, T(0))) * smax(cos_i, T(0));
+    }
     T r2 = inner(l, l);
     V3<T> in = l / std::sqrt(r2);
     T cos_o = inner(nl, in);
@@ -792,8 +831,19 @@ template <class T> V3<T> light_radiance(Light const &L, V3<T> w, V3<T> n) {
     return inner(n, w) > 0 ? cvt<T>(L.rad) : V3<T>();
 }
 
-template <class T> T light_pick_prob(Scene const &s, int light) {
-    return T(norm(s.lights[light].power) / s.total_power);
+template <class T> T light_pick_prob(Scene const &s, int light) { return T(pick_prob(s, s.lights[light])); }
+
+// Emission direction of a light sample and its solid-angle density: cosine-distributed about the
+// area light's normal (area_light::sample_emssion, light.cpp:26-32), or the sky's inward normal
+// with density 1 (sky_light::sample_emssion, light.cpp:102-115).
+template <class T> V3<T> emission_dir(LightSample<T> const &em, T e0, T e1, T *w_dens) {
+    if (em.sky) {
+        *w_dens = T(1);
+        return em.n;
+    }
+    V3<T> w = cos_hemisphere(em.n, e0, e1);
+    *w_dens = inner(em.n, w) / pi_v<T>();
+    return w;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -890,9 +940,13 @@ V3<T> li_mis(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg &g, AdjVertT<T> *r
             bool ok;
             V3<T> d = direct_from<T>(c, ls, x, wo, &wi, &r2, &cl, &ok);
             if (ok) {
-                T pl = ls.area_dens * r2 / cl;
-                T pb = bsdf_pdf<T>(s, mat, x.n, wo, wi);
-                T w = pl * pl / (pl * pl + pb * pb);
+                // a BSDF sample never reaches the sky (it has no surface): weight 1
+                T w = T(1);
+                if (!ls.sky) {
+                    T pl = ls.area_dens * r2 / cl;
+                    T pb = bsdf_pdf<T>(s, mat, x.n, wo, wi);
+                    w = pl * pl / (pl * pl + pb * pb);
+                }
                 V3<T> add = mod(beta, d * w);
                 L = L + add;
                 if (r) {
@@ -1100,13 +1154,14 @@ template <class T> V3<T> li_lt2(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pcg
             LightSample<T> em = sample_light<T>(s, g);
             T e0 = T(g.draw());
             T e1 = T(g.draw());
-            V3<T> w = cos_hemisphere(em.n, e0, e1);
-            T w_dens = inner(em.n, w) / pi_v<T>();
+            T w_dens;
+            V3<T> w = emission_dir(em, e0, e1, &w_dens);
             Hit<T> lv = trace<T>(c, em.p, w);
             if (lv.valid()) {
                 T cos_e = inner(em.n, w);
                 V3<T> emitted = cos_e > T(0) ? cvt<T>(s.lights[em.light].rad) * cos_e : V3<T>();
-                V3<T> illum = emitted / (em.area_dens * w_dens);
+                // a light of zero selection probability carries no power
+                V3<T> illum = em.area_dens > T(0) ? emitted / (em.area_dens * w_dens) : V3<T>();
                 V3<T> tray = -w;
                 V3<T> jp = x.p - lv.p;
                 T dist = norm(jp);
@@ -1272,8 +1327,8 @@ template <class T> V3<T> li_bdpt(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pc
     LightSample<T> em = sample_light<T>(s, g);
     T e0 = T(g.draw());
     T e1 = T(g.draw());
-    V3<T> w = cos_hemisphere(em.n, e0, e1);
-    T w_dens = inner(em.n, w) / pi_v<T>();
+    T w_dens;
+    V3<T> w = emission_dir(em, e0, e1, &w_dens);
     Light const &light = s.lights[em.light];
     uint32_t light_len = 0;
     Hit<T> v0 = trace<T>(c, em.p, w);
@@ -1286,7 +1341,7 @@ template <class T> V3<T> li_bdpt(Ctx<T> &c, Hit<T> const &first, V3<T> ray_v, Pc
     // light.projected_radiance(w, n) / (dens0 * area_dens) (pathtracer.cpp:418-420)
     T cos_e = inner(em.n, w);
     V3<T> emitted = cos_e > T(0) ? cvt<T>(light.rad) * cos_e : V3<T>();
-    V3<T> light_emission = emitted / (w_dens * em.area_dens);
+    V3<T> light_emission = em.area_dens > T(0) ? emitted / (w_dens * em.area_dens) : V3<T>();
 
     for (uint32_t plen = 1; plen <= cam_len + light_len + 1; plen++) {
         int cp = static_cast<int>(std::min(plen - 1, cam_len));
@@ -1467,6 +1522,31 @@ int orc_add_area_light(void *h, int mesh, float const *rad) {
     s->lights.push_back({mesh, ld3(rad), V3f()});
     return static_cast<int>(s->lights.size()) - 1;
 }
+// sky_light (light.cpp:87-163), placed at commit by set_scene_boundary with the path space's
+// bounds.
+int orc_add_sky_light(void *h, float const *rad) {
+    Scene *s = static_cast<Scene *>(h);
+    s->lights.push_back({-1, ld3(rad), V3f()});
+    return static_cast<int>(s->lights.size()) - 1;
+}
+// Light state after commit: power (3), sky reference point (3), sky diameter.
+void orc_light_info(void *h, int light, float *out) {
+    Light const &L = static_cast<Scene *>(h)->lights.at(static_cast<size_t>(light));
+    float v[7] = {L.power.x, L.power.y, L.power.z, L.sky_ref.x, L.sky_ref.y, L.sky_ref.z, L.sky_dia};
+    std::memcpy(out, v, sizeof v);
+}
+// Light evaluation for tests: light_eval of light `light` at connections l (float[3n]), light
+// normals nl and target normals nt -> out float[3n].
+void orc_light_eval(void *h, int light, uint32_t n, float const *l, float const *nl, float const *nt, float *out) {
+    Light const &L = static_cast<Scene *>(h)->lights.at(static_cast<size_t>(light));
+    for (uint32_t k = 0; k < n; k++) {
+        float r2;
+        V3f v = light_eval<float>(L, ld3(l + 3 * k), ld3(nl + 3 * k), ld3(nt + 3 * k), &r2);
+        out[3 * k] = v.x;
+        out[3 * k + 1] = v.y;
+        out[3 * k + 2] = v.z;
+    }
+}
 int orc_commit(void *h) {
     commit(*static_cast<Scene *>(h));
     return 0;
@@ -1613,8 +1693,8 @@ void orc_camera_rays(float sensor, float focal, float aspect, float const *trans
 }
 
 // Renders the listed pixels (indices i + j*w; all pixels when pixels == nullptr) with `spp`
-// samples each, accumulating sequentially per pixel and scaling by 1/spp (renderer.cpp:171-226,
-// 289-295 with one task). rgba: float[4*npix] (alpha = 1/spp as in renderer.cpp:284,293),
+// samples each, accumulating sequentially per pixel and scaling by 1/spp (renderer.cpp:43-45,
+// 140-159 with one task). rgba: float[4*npix] (alpha = 1/spp as in renderer.cpp:148, 154-157),
 // or double when prec == 1. Work is spread over `threads` CPU threads by pixel; results do not
 // depend on the thread count. stats: uint64[6] (see orc_intersect), may be null.
 void orc_render(void *h, float const *cam, int tracer, uint32_t max_depth, uint32_t max_path_len,

@@ -170,10 +170,16 @@ struct Mesh {
     float area = 0.0f;
 };
 
+// area_light (light.cpp:17-85) on mesh `mesh`, or sky_light (light.cpp:87-163) when mesh < 0.
+// The sky's reference point and diameter are set at commit from the path space's bounds by
+// sky_light::set_scene_boundary (light.cpp:96-100), which the reference never calls itself.
 struct Light {
     int mesh;
     V3f rad;
     V3f power;
+    bool sky() const { return mesh < 0; }
+    V3f sky_ref;        // m_ref_p: the bounds' centroid with z = 0
+    float sky_dia = 0;  // m_dia: twice the bounds' enclosing radius
 };
 
 struct FlatNode { // bvh_path_space_layout::flattened_node (pathspace.h:130-150)

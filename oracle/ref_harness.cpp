@@ -4,7 +4,7 @@
 // under the reference checkout, into oracle/_ref/libx3dref.so (see oracle/Makefile). It only
 // drives reference code through its public classes; it contains no restatement of the
 // reference's algorithms except the 40-line sample accumulation of pt_image_renderer::render
-// (corelib/src/renderer.cpp:171-226, 242-301), which cannot be compiled here because
+// (corelib/src/renderer.cpp:35-166), which cannot be compiled here because
 // renderer.h pulls in Qt through frame.h (frame.h:6).
 //
 // Used only by tests/ and by bench.py's CPU-baseline leg. Never part of the product.
@@ -38,17 +38,18 @@
 #include <utility>
 #include <vector>
 
-// Read-only access to the flattened BVH, primitive array and geometry map for layout dumps
-// (all standard headers are included above, so only the reference's classes are affected).
+// Read-only access to the flattened BVH, primitive array and geometry map for layout dumps, and
+// to the sky light's reference point and diameter (all standard headers are included above, so
+// only the reference's classes are affected).
 #define private public
 #define protected public
 #include "pathspace.h"
+#include "light.h"
 #undef protected
 #undef private
 
 #include "camera.h"
 #include "geometry.h"
-#include "light.h"
 #include "lightsources.h"
 #include "material.h"
 #include "materialcontainer.h"
@@ -167,6 +168,14 @@ int ref_add_area_light(void *h, int mesh, float const *rad) {
     return static_cast<int>(s->lights.size()) - 1;
 }
 
+// sky_light (light.cpp:87-163). Nothing in the reference calls set_scene_boundary; ref_commit
+// calls it with the path space's bounds before the light sources are committed.
+int ref_add_sky_light(void *h, float const *rad) {
+    ref_scene *s = static_cast<ref_scene *>(h);
+    s->lights.push_back(std::make_shared<e8::sky_light>(v3(rad)));
+    return static_cast<int>(s->lights.size()) - 1;
+}
+
 int ref_commit(void *h) {
     ref_scene *s = static_cast<ref_scene *>(h);
     e8util::mat44 id = e8util::mat44_scale(1.0f);
@@ -191,8 +200,10 @@ int ref_commit(void *h) {
     for (auto const &p : s->linear->m_geometries) {
         s->geo_index[p.second.get()] = static_cast<uint32_t>(std::lround(p.second->texcoords()[0](1)));
     }
-    for (auto const &l : s->lights)
+    for (auto const &l : s->lights) {
+        l->set_scene_boundary(s->bvh->aabb());
         s->light_sources->load(*l, id);
+    }
     if (!s->lights.empty())
         s->light_sources->commit();
     s->committed = true;
@@ -310,7 +321,7 @@ void ref_camera_rays(float sensor, float focal, float aspect, float const *trans
 }
 
 // Renders with reference tracer `pt_type` (pathtracerfact.h:12 order) using `threads` sampling
-// tasks, each seeded i*1361+33 and drawing ceil(spp/threads) samples (renderer.cpp:232-262).
+// tasks, each seeded i*1361+33 and drawing ceil(spp/threads) samples (renderer.cpp:96-104, 124-132).
 // rgba: float[4*w*h], row 0 = top. Returns the number of samples actually taken per pixel.
 uint32_t ref_render(void *h, int pt_type, float sensor, float focal, float aspect,
                     float const *trans, uint32_t w, uint32_t hh, uint32_t spp, uint32_t threads,
@@ -363,6 +374,29 @@ uint32_t ref_render(void *h, int pt_type, float sensor, float focal, float aspec
             rgba[4 * p + c] = film[p](c);
     }
     return allocated * threads;
+}
+
+// Light state after commit: power() (3), and for a sky light m_ref_p (3) and m_dia; zeros
+// for an area light.
+void ref_light_info(void *h, int light, float *out) {
+    ref_scene *s = static_cast<ref_scene *>(h);
+    e8::if_light const &L = *s->lights.at(static_cast<size_t>(light));
+    put3(out, L.power());
+    out[3] = out[4] = out[5] = out[6] = 0.0f;
+    if (auto const *sky = dynamic_cast<e8::sky_light const *>(&L)) {
+        put3(out + 3, sky->m_ref_p);
+        out[6] = sky->m_dia;
+    }
+}
+
+// if_light::eval (light.cpp:41-52, 131-137) of light `light`: l float[3n] (target - light point),
+// nl / nt the light and target normals -> out float[3n].
+void ref_light_eval(void *h, int light, uint32_t n, float const *l, float const *nl, float const *nt,
+                    float *out) {
+    ref_scene *s = static_cast<ref_scene *>(h);
+    e8::if_light const &L = *s->lights.at(static_cast<size_t>(light));
+    for (uint32_t k = 0; k < n; k++)
+        put3(out + 3 * k, L.eval(v3(l + 3 * k), v3(nl + 3 * k), v3(nt + 3 * k)));
 }
 
 // trimesh::transform (geometry.cpp:76-87) of a mesh with the given column-major 4x4 matrix

@@ -177,3 +177,79 @@ def test_position_bounds_only_grow(x3d):
     front = got.any(1)
     assert front.mean() > 0.8
     assert np.array_equal(_bits(got[front]), _bits(want[front]))
+
+
+def test_material_params_follow_ids_after_edits(x3d):
+    """x3d_scene_set_material_params on a keyed scene writes back by obj id: after
+    load -> commit -> load_material(new lower id) -> set_material_params -> commit, the changed
+    material is the one the caller addressed (committed index -> id of the last commit), and the
+    film equals an indexed scene with the same parameters."""
+    desc = scenes.box_scene()
+    keyed = x3d.Scene()
+    ids = [500 + 10 * k for k in range(len(desc.materials))]
+    for i, m in zip(ids, desc.materials):
+        keyed.load_material(i, m)
+    for k, m in enumerate(desc.meshes):
+        keyed.load_geometry(100 + k, m.verts, m.normals, m.tris, material=ids[m.material])
+    for k, (mi, rad) in enumerate(desc.lights):
+        keyed.load_light(900 + k, 100 + mi, rad)
+    keyed.commit()
+    # a new material under a lower id than all others: its rank is 0 at the next commit
+    keyed.load_material(1, scenes.Material(scenes.OREN_NAYAR, (0.2, 0.9, 0.2)))
+    new_albedo = np.array([[0.9, 0.1, 0.1]], np.float32)
+    keyed.set_material_params(1, new_albedo)  # committed index 1 = obj id ids[1] (the red wall)
+    keyed.commit()
+    mats = [scenes.Material(scenes.OREN_NAYAR, (0.2, 0.9, 0.2))] + list(desc.materials)
+    mats[2] = scenes.Material(scenes.OREN_NAYAR, tuple(float(a) for a in new_albedo[0]))
+    indexed = _indexed(x3d, mats, [(m.verts, m.normals, m.tris, m.material + 1) for m in desc.meshes],
+                       desc.lights)
+    a, _ = _films(x3d, keyed, desc.camera)
+    b, _ = _films(x3d, indexed, desc.camera)
+    for fa, fb in zip(a, b):
+        assert np.array_equal(_bits(fa), _bits(fb))
+
+
+def test_keyed_sky_light_equals_indexed(x3d):
+    """x3d_scene_load_sky_light by id gives the film of x3d_scene_add_sky_light; the sky is placed
+    from the path space's bounds at commit."""
+    desc = scenes.sky_scene()
+    indexed = x3d.Scene(desc)
+    keyed = x3d.Scene()
+    for k, m in enumerate(desc.materials):
+        keyed.load_material(10 + k, m)
+    for k, m in enumerate(desc.meshes):
+        keyed.load_geometry(100 + k, m.verts, m.normals, m.tris, material=10 + m.material)
+    for k, (mi, rad) in enumerate(desc.lights):
+        if mi < 0:
+            keyed.load_sky_light(900 + k, rad)
+        else:
+            keyed.load_light(900 + k, 100 + mi, rad)
+    keyed.commit()
+    for pt in (x3d.PT_UNIDIRECT_MIS, x3d.PT_BIDIRECT_MIS, x3d.PT_DIRECT):
+        a, _ = x3d.Renderer(pt, seed=3).render(keyed, desc.camera, 40, 32, 3)
+        b, _ = x3d.Renderer(pt, seed=3).render(indexed, desc.camera, 40, 32, 3)
+        assert np.array_equal(_bits(a), _bits(b)) and a[:, :3].max() > 0
+
+
+@pytest.mark.parametrize("tracer", ["UNIDIRECT_MIS", "DIRECT", "UNIDIRECT_LT1", "UNIDIRECT", "BIDIRECT_LT2",
+                                    "BIDIRECT_MIS"])
+def test_zero_power_lights_give_black_finite_films(x3d, oracle_mod, tracer):
+    import dataclasses
+
+    desc = scenes.multi_light_scene()
+    dark = dataclasses.replace(desc, lights=[(m, (0.0, 0.0, 0.0)) for m, _ in desc.lights])
+    pt = getattr(x3d, "PT_" + tracer)
+    img, _ = x3d.Renderer(pt, seed=2).render(x3d.Scene(dark), dark.camera, 24, 20, 4)
+    assert np.isfinite(img).all() and not img[:, :3].any()
+    ref = oracle_mod.Oracle(dark).render(pt, 24, 20, 4, seed=2)
+    assert np.array_equal(_bits(img), _bits(ref))
+
+
+def test_negative_radiance_rejected(x3d):
+    s = x3d.Scene()
+    m = scenes.quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), -1)
+    mi = s.add_mesh(m.verts, m.normals, m.tris)
+    with pytest.raises(x3d.X3DError):
+        s.add_area_light(mi, (1.0, -1.0, 1.0))
+    with pytest.raises(x3d.X3DError):
+        s.add_sky_light((0.5, float("nan"), 0.5))

@@ -24,6 +24,8 @@ QUERY_SCENES = {
     "furnace": lambda: scenes.furnace_sphere(),
     "indirect_l3": lambda: scenes.indirect_scene(level=3),
     "spheres_small": lambda: scenes.spheres_scene(count=12, target_tris=30000, seed=77),
+    "multi_light": lambda: scenes.multi_light_scene(),
+    "sky": lambda: scenes.sky_scene(),
 }
 
 
@@ -188,6 +190,15 @@ RENDER_CASES = [
     ("indirect_l3", "BIDIRECT_MIS", dict(max_path_len=6)),
     ("spheres_small", "BIDIRECT_MIS", dict(max_path_len=5)),
     ("furnace", "BIDIRECT_MIS", dict(max_path_len=16)),
+] + [
+    # several lights of unequal power and area (one a fan of unequal triangles), and the sky light
+    # beside an area light: light choice by power, triangle choice by area, the sky's sampling and
+    # evaluation, its MIS weight, its emission for the light subpaths
+    (name, tracer, kw)
+    for name in ("multi_light", "sky")
+    for tracer, kw in (("UNIDIRECT_MIS", dict(max_depth=6)), ("UNIDIRECT_LT1", {}), ("UNIDIRECT", {}),
+                       ("DIRECT", {}), ("NORMAL", {}), ("POSITION", {}), ("BIDIRECT_LT2", {}),
+                       ("BIDIRECT_MIS", dict(max_path_len=6)))
 ]
 
 
@@ -226,7 +237,7 @@ WAVEFRONT_CASES = [
 
 
 @pytest.mark.parametrize("kw", WAVEFRONT_CASES)
-@pytest.mark.parametrize("scene_name", ["box", "spheres_small"])
+@pytest.mark.parametrize("scene_name", ["box", "spheres_small", "multi_light", "sky"])
 def test_wavefront_film_identical_to_plain(x3d, scene_name, kw):
     desc = QUERY_SCENES[scene_name]()
     print(f"[gpu] wavefront {scene_name} {kw}", flush=True)
@@ -250,7 +261,7 @@ REF_WAVEFRONT_CASES = [
 
 @pytest.mark.parametrize("kw", REF_WAVEFRONT_CASES)
 @pytest.mark.parametrize("tracer", ["UNIDIRECT_LT1", "UNIDIRECT"])
-@pytest.mark.parametrize("scene_name", ["box", "spheres_small"])
+@pytest.mark.parametrize("scene_name", ["box", "spheres_small", "multi_light", "sky"])
 def test_wavefront_reference_tracers_identical_to_plain(x3d, scene_name, tracer, kw):
     """The reference-mode lt1 and unidirect tracers (Russian roulette, up to 64 vertices) in the
     wavefront form: the same film bits and the same query counts as the plain kernels."""
@@ -343,8 +354,9 @@ def test_bdpt_generations_and_edge_cases(x3d, oracle_mod):
 
 
 def test_plain_mis_both_work_shapes_bit_exact(x3d, oracle_mod):
-    """The plain MIS tracer runs one pixel per lane on large films and (pixel, sample) items on
-    films smaller than the chip's lane count; both give the oracle's film."""
+    """The plain MIS tracer runs over (pixel, sample) items of generations sized by the queue
+    capacity: a film with more pixels than the chip has lanes and a small film with more samples
+    per pixel both give the oracle's film."""
     desc = QUERY_SCENES["spheres_small"]()
     gs = x3d.Scene(desc)
     orc = oracle_mod.Oracle(desc)

@@ -114,7 +114,8 @@ class ComposeStats(C.Structure):
 # Every exported symbol declared in include/x3dpt.h.
 EXPORTS = [
     "x3d_last_error", "x3d_version", "x3d_scene_create", "x3d_scene_destroy", "x3d_scene_add_material",
-    "x3d_scene_add_mesh", "x3d_scene_add_area_light", "x3d_scene_commit", "x3d_scene_get_info",
+    "x3d_scene_add_mesh", "x3d_scene_add_area_light", "x3d_scene_add_sky_light", "x3d_scene_commit",
+    "x3d_scene_get_info",
     "x3d_scene_dump_bvh", "x3d_intersect", "x3d_occluded", "x3d_intersect_device", "x3d_occluded_device",
     "x3d_bsdf_eval", "x3d_bsdf_pdf", "x3d_bsdf_sample", "x3d_camera_rays", "x3d_render_options_default",
     "x3d_renderer_create", "x3d_renderer_destroy", "x3d_render", "x3d_render_device", "x3d_pcg_draws",
@@ -123,7 +124,8 @@ EXPORTS = [
     "x3d_obj_load", "x3d_obj_free", "x3d_scene_add_obj",
     "x3d_compositor_create", "x3d_compositor_destroy", "x3d_compositor_enable_auto_exposure",
     "x3d_compositor_set_exposure", "x3d_compositor_commit", "x3d_compositor_commit_device",
-    "x3d_scene_load_geometry", "x3d_scene_load_material", "x3d_scene_load_light", "x3d_scene_unload",
+    "x3d_scene_load_geometry", "x3d_scene_load_material", "x3d_scene_load_light", "x3d_scene_load_sky_light",
+    "x3d_scene_unload",
     "x3d_scene_geometry_obj_id", "x3d_transform_mesh",
 ]
 
@@ -153,6 +155,7 @@ def lib() -> C.CDLL:
     L.x3d_obj_free.argtypes = [C.POINTER(ObjMesh)]
     L.x3d_scene_add_obj.argtypes = [vp, C.c_char_p, i32, C.POINTER(i32)]
     L.x3d_scene_add_area_light.argtypes = [vp, i32, fp, C.POINTER(i32)]
+    L.x3d_scene_add_sky_light.argtypes = [vp, fp, C.POINTER(i32)]
     L.x3d_scene_commit.argtypes = [vp]
     L.x3d_scene_get_info.argtypes = [vp, C.POINTER(SceneInfo)]
     L.x3d_scene_dump_bvh.argtypes = [vp, vp, vp, vp]
@@ -188,6 +191,7 @@ def lib() -> C.CDLL:
     L.x3d_scene_load_geometry.argtypes = [vp, u64, vp, vp, vp, u32, vp, u32, i32, u64, vp]
     L.x3d_scene_load_material.argtypes = [vp, u64, C.POINTER(MaterialDesc)]
     L.x3d_scene_load_light.argtypes = [vp, u64, u64, fp]
+    L.x3d_scene_load_sky_light.argtypes = [vp, u64, fp]
     L.x3d_scene_unload.argtypes = [vp, i32, u64]
     L.x3d_scene_geometry_obj_id.argtypes = [vp, i32, C.POINTER(u64)]
     L.x3d_transform_mesh.argtypes = [vp, u32, vp, vp, vp, vp]
@@ -268,6 +272,10 @@ class Scene:
         r = (C.c_float * 3)(*radiance)
         _check(lib().x3d_scene_load_light(self.h, int(obj_id), int(geometry_obj_id), r))
 
+    def load_sky_light(self, obj_id, radiance):
+        r = (C.c_float * 3)(*radiance)
+        _check(lib().x3d_scene_load_sky_light(self.h, int(obj_id), r))
+
     def unload(self, protocol, obj_id):
         _check(lib().x3d_scene_unload(self.h, int(protocol), int(obj_id)))
 
@@ -308,12 +316,22 @@ class Scene:
         _check(lib().x3d_scene_add_area_light(self.h, int(mesh_id), r, C.byref(out)))
         return out.value
 
+    def add_sky_light(self, radiance) -> int:
+        r = (C.c_float * 3)(*radiance)
+        out = C.c_int32()
+        _check(lib().x3d_scene_add_sky_light(self.h, r, C.byref(out)))
+        return out.value
+
     def load(self, desc):
+        """Loads a scenes.SceneDesc; a light entry with mesh index < 0 is the sky light."""
         ids = [self.add_material(m) for m in desc.materials]
         for mesh in desc.meshes:
             self.add_mesh(mesh.verts, mesh.normals, mesh.tris, ids[mesh.material] if mesh.material >= 0 else -1)
         for mi, rad in desc.lights:
-            self.add_area_light(mi, rad)
+            if mi < 0:
+                self.add_sky_light(rad)
+            else:
+                self.add_area_light(mi, rad)
 
     def commit(self):
         _check(lib().x3d_scene_commit(self.h))

@@ -13,11 +13,14 @@ struct LightSample {
     int32_t light;
     f3 p, n;
     float area_dens; // area density times light selection probability
+    bool sky;
 };
 
 // basic_light_sources::sample_light (lightsources.cpp:59-75), then a point on the chosen
-// emitter: triangle by area, sqrt-barycentric point and interpolated normal
-// (geometry.cpp:44-66). Draws consumed: 4.
+// light: on an area light a triangle by area, sqrt-barycentric point and interpolated normal
+// (geometry.cpp:44-66); on the sky sky_light::sample_emssion_surface (light.cpp:117-129):
+// u cosine-distributed about +z, p = (u + ref) * dia, n = -u, density (z . u) / pi. Draws
+// consumed: 4 (the sky skips the second).
 X3D_DI LightSample sample_light(DevScene const &S, Pcg &g) {
     LightSample ls;
     float e = g.draw() * S.total_power;
@@ -32,6 +35,19 @@ X3D_DI LightSample sample_light(DevScene const &S, Pcg &g) {
     if (lo >= S.num_lights)
         lo = S.num_lights - 1;
     DevLight const L = S.lights[lo];
+    ls.light = static_cast<int32_t>(lo);
+    ls.sky = L.kind == kLightSky;
+    if (ls.sky) {
+        (void)g.draw();
+        float e0 = g.draw();
+        float e1 = g.draw();
+        const f3 z = mk(0.0f, 0.0f, 1.0f);
+        f3 u = cos_hemisphere(z, e0, e1);
+        ls.n = -u;
+        ls.p = (u + mk(L.sky_ref[0], L.sky_ref[1], L.sky_ref[2])) * L.sky_dia;
+        ls.area_dens = (dot(z, u) / kPi) * L.prob;
+        return ls;
+    }
     float ea = g.draw() * L.area;
     uint32_t a = 0, b = L.tri_count;
     const float *cum = S.light_tri_cum + L.tri_begin;
@@ -51,7 +67,6 @@ X3D_DI LightSample sample_light(DevScene const &S, Pcg &g) {
     float b1 = r * v;
     float b2 = 1.0f - b0 - b1;
     const float4 *lt = static_cast<const float4 *>(S.light_tris) + 6u * (L.tri_begin + a);
-    ls.light = static_cast<int32_t>(lo);
     ls.p = b0 * xyz(lt[0]) + b1 * xyz(lt[1]) + b2 * xyz(lt[2]);
     ls.n = normalize(b0 * xyz(lt[3]) + b1 * xyz(lt[4]) + b2 * xyz(lt[5]));
     ls.area_dens = (1.0f / L.area) * L.prob;
@@ -61,6 +76,44 @@ X3D_DI LightSample sample_light(DevScene const &S, Pcg &g) {
 X3D_DI f3 light_rad(DevScene const &S, int32_t l) {
     DevLight const &L = S.lights[l];
     return mk(L.rad[0], L.rad[1], L.rad[2]);
+}
+
+// Radiance a light sample sends to point xp of normal xn, before visibility: area_light::eval
+// (light.cpp:41-52, L cos_i cos_o / r^2 when both cosines are positive) or sky_light::eval
+// (light.cpp:131-137, L max(cos_o, 0) max(cos_i, 0), no fall-off). False when the result is
+// within 1e-5 of zero (transport_illum_source, pathtracer.cpp:160-162). l = xp - ls.p and, for
+// an area light, r2 = |l|^2 out.
+X3D_DI bool light_arrives(DevScene const &S, LightSample const &ls, f3 xp, f3 xn, f3 &illum, f3 &l, float &r2) {
+    l = xp - ls.p;
+    if (ls.sky) {
+        f3 in = normalize(l);
+        float co = dot(ls.n, in);
+        float ci = dot(xn, -in);
+        r2 = 0.0f;
+        illum = (light_rad(S, ls.light) * smax(co, 0.0f)) * smax(ci, 0.0f);
+        return !near0(illum);
+    }
+    r2 = dot(l, l);
+    f3 in = l / __builtin_sqrtf(r2);
+    float co = dot(ls.n, in);
+    float ci = dot(xn, -in);
+    if (!(co > 0 && ci > 0))
+        return false;
+    illum = ((light_rad(S, ls.light) * ci) * co) / r2;
+    return !near0(illum);
+}
+
+// Emission direction of a light sample and its solid-angle density: cosine-distributed about
+// the area light's normal (area_light::sample_emssion, light.cpp:26-32), or the sky's inward
+// normal -u with density 1 (sky_light::sample_emssion, light.cpp:102-115).
+X3D_DI f3 emission_dir(LightSample const &em, float e0, float e1, float &w_dens) {
+    if (em.sky) {
+        w_dens = 1.0f;
+        return em.n;
+    }
+    f3 w = cos_hemisphere(em.n, e0, e1);
+    w_dens = dot(em.n, w) / kPi;
+    return w;
 }
 
 // area_light::radiance (light.cpp:62-71): one-sided.
@@ -119,15 +172,8 @@ template <int WG, int LDSD> struct Tracer {
     // the direction to the light, the squared distance and the cosine at the light.
     X3D_DI bool direct(LightSample const &ls, f3 xp, f3 xn, int32_t xmat, f3 wo, f3 &contrib, f3 &wi, float &r2,
                        float &cos_l) {
-        f3 l = xp - ls.p;
-        r2 = dot(l, l);
-        f3 in = l / __builtin_sqrtf(r2);
-        float co = dot(ls.n, in);
-        float ci = dot(xn, -in);
-        if (!(co > 0 && ci > 0))
-            return false; // area_light::eval is zero (light.cpp:41-52)
-        f3 illum = ((light_rad(S, ls.light) * ci) * co) / r2;
-        if (near0(illum))
+        f3 l, illum;
+        if (!light_arrives(S, ls, xp, xn, illum, l, r2))
             return false;
         float dist = norm(l);
         wi = -l / dist;
@@ -179,9 +225,13 @@ template <int WG, int LDSD> struct Tracer {
             f3 d, wi;
             float r2, cl;
             if (direct(ls, x, ps.wo, d, wi, r2, cl)) {
-                float pl = ls.area_dens * r2 / cl;
-                float pb = bsdf_pdf(S.mats, x.mat, x.n, ps.wo, wi);
-                float w = pl * pl / (pl * pl + pb * pb);
+                // a BSDF sample never reaches the sky (it has no surface): weight 1
+                float w = 1.0f;
+                if (!ls.sky) {
+                    float pl = ls.area_dens * r2 / cl;
+                    float pb = bsdf_pdf(S.mats, x.mat, x.n, ps.wo, wi);
+                    w = pl * pl / (pl * pl + pb * pb);
+                }
                 f3 add = mul(ps.beta, d * w);
                 ps.L = ps.L + add;
                 if constexpr (REC) {
@@ -350,8 +400,8 @@ template <int WG, int LDSD> struct Tracer {
         LightSample em = sample_light(S, g);
         float e0 = g.draw();
         float e1 = g.draw();
-        f3 w = cos_hemisphere(em.n, e0, e1);
-        float w_dens = dot(em.n, w) / kPi;
+        float w_dens;
+        f3 w = emission_dir(em, e0, e1, w_dens);
         uint32_t light_len = 0;
         HitRec v0;
         if (trace(em.p, w, v0) && dot(v0.n, -w) > 0.0f) {
@@ -364,7 +414,8 @@ template <int WG, int LDSD> struct Tracer {
         }
         float cos_e = dot(em.n, w);
         f3 emitted = cos_e > 0.0f ? light_rad(S, em.light) * cos_e : mk(0.0f, 0.0f, 0.0f);
-        f3 light_emission = emitted / (w_dens * em.area_dens);
+        // a light of zero selection probability carries no power
+        f3 light_emission = em.area_dens > 0.0f ? emitted / (w_dens * em.area_dens) : mk(0.0f, 0.0f, 0.0f);
 
         for (uint32_t plen = 1; plen <= cam_len + light_len + 1; plen++) {
             int cp = static_cast<int>(plen - 1 < cam_len ? plen - 1 : cam_len);
@@ -439,13 +490,13 @@ template <int WG, int LDSD> struct Tracer {
                 LightSample em = sample_light(S, g);
                 float e0 = g.draw();
                 float e1 = g.draw();
-                f3 w = cos_hemisphere(em.n, e0, e1);
-                float w_dens = dot(em.n, w) / kPi;
+                float w_dens;
+                f3 w = emission_dir(em, e0, e1, w_dens);
                 HitRec lv;
                 if (trace(em.p, w, lv)) {
                     float cos_e = dot(em.n, w);
                     f3 emitted = cos_e > 0.0f ? light_rad(S, em.light) * cos_e : mk(0.0f, 0.0f, 0.0f);
-                    f3 illum = emitted / (em.area_dens * w_dens);
+                    f3 illum = em.area_dens > 0.0f ? emitted / (em.area_dens * w_dens) : mk(0.0f, 0.0f, 0.0f);
                     f3 tray = -w;
                     f3 jp = x.p - lv.p;
                     float dist = norm(jp);

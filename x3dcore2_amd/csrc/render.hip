@@ -573,7 +573,6 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     // The reference-mode tracers run over (pixel, sample) items as well (same films; per-pixel
     // lanes measured 1.3-1.9x slower: lt1 98.2 vs 75.0 ms, direct 16.1 vs 8.7 ms at 1024^2 x 16 spp
     // on the 1.3M-triangle spheres scene).
-    const bool plain_items = tracer != X3D_PT_UNIDIRECT_MIS && tracer != X3D_PT_BIDIRECT_MIS;
     switch (tracer) {
     case X3D_PT_UNIDIRECT_MIS:
         grid_render = mis_items ? persistent_grid(k_render_mis_items, npix * static_cast<uint64_t>(spp)) : 1;
@@ -1066,6 +1065,22 @@ int x3d_scene_load_light(x3d_scene *scene, uint64_t obj_id, uint64_t geometry_ob
     return X3D_OK;
 }
 
+int x3d_scene_load_sky_light(x3d_scene *scene, uint64_t obj_id, const float radiance[3]) {
+    if (!scene || !radiance)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "scene or radiance is null");
+    if (!finite3(radiance) || radiance[0] < 0 || radiance[1] < 0 || radiance[2] < 0)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "radiance must be finite and non-negative");
+    if (int rc = require_keyed(scene))
+        return rc;
+    x3d::KeyedLight l;
+    l.geometry = 0;
+    l.sky = true;
+    for (int c = 0; c < 3; c++)
+        l.rad[c] = radiance[c];
+    scene->s.k_lights.insert(std::make_pair(obj_id, l));
+    return X3D_OK;
+}
+
 int x3d_scene_unload(x3d_scene *scene, int32_t protocol, uint64_t obj_id) {
     if (!scene)
         return fail(X3D_ERR_INVALID_ARGUMENT, "scene is null");
@@ -1175,13 +1190,30 @@ int x3d_scene_add_area_light(x3d_scene *scene, int32_t mesh_id, const float radi
         return rc;
     if (mesh_id < 0 || mesh_id >= static_cast<int32_t>(scene->s.meshes.size()))
         return fail(X3D_ERR_INVALID_ARGUMENT, "mesh id out of range");
-    if (!finite3(radiance))
-        return fail(X3D_ERR_INVALID_ARGUMENT, "radiance must be finite");
+    if (!finite3(radiance) || radiance[0] < 0 || radiance[1] < 0 || radiance[2] < 0)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "radiance must be finite and non-negative");
     HostLight l;
     l.mesh = mesh_id;
     l.rad[0] = radiance[0];
     l.rad[1] = radiance[1];
     l.rad[2] = radiance[2];
+    scene->s.lights.push_back(l);
+    if (out_light_id)
+        *out_light_id = static_cast<int32_t>(scene->s.lights.size()) - 1;
+    return X3D_OK;
+}
+
+int x3d_scene_add_sky_light(x3d_scene *scene, const float radiance[3], int32_t *out_light_id) {
+    if (!scene || !radiance)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    if (int rc = require_indexed(scene))
+        return rc;
+    if (!finite3(radiance) || radiance[0] < 0 || radiance[1] < 0 || radiance[2] < 0)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "radiance must be finite and non-negative");
+    HostLight l;
+    l.mesh = -1;
+    for (int c = 0; c < 3; c++)
+        l.rad[c] = radiance[c];
     scene->s.lights.push_back(l);
     if (out_light_id)
         *out_light_id = static_cast<int32_t>(scene->s.lights.size()) - 1;
@@ -1232,11 +1264,15 @@ int x3d_scene_set_material_params(x3d_scene *scene, uint32_t first, uint32_t cou
     }
     X3D_HIP(hipMemcpy(s.buf.mats, s.materials.data(), s.materials.size() * sizeof(DevMaterial),
                       hipMemcpyHostToDevice));
-    if (s.keyed) { // materials are committed in id order: keep the stored objects in step
-        auto it = s.k_mats.begin();
-        std::advance(it, first);
-        for (uint32_t k = 0; k < count && it != s.k_mats.end(); k++, ++it)
-            it->second = s.materials[first + k];
+    if (s.keyed) {
+        // Keep the stored objects in step by id (committed material k has obj id mat_ids[k]), so
+        // that a later commit keeps the parameters even after loads or unloads since this
+        // commit; a material unloaded since the commit is not brought back.
+        for (uint32_t k = 0; k < count; k++) {
+            auto it = s.k_mats.find(s.mat_ids[first + k]);
+            if (it != s.k_mats.end())
+                it->second = s.materials[first + k];
+        }
     }
     return X3D_OK;
 }

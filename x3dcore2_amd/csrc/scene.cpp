@@ -327,9 +327,11 @@ std::string Scene::flatten_keyed() {
     meshes.clear();
     lights.clear();
     geo_ids.clear();
+    mat_ids.clear();
     std::map<uint64_t, int32_t> mat_index, geo_index;
     for (auto const &kv : k_mats) {
         mat_index[kv.first] = static_cast<int32_t>(materials.size());
+        mat_ids.push_back(kv.first);
         materials.push_back(kv.second);
     }
     for (auto const &kv : k_geos) {
@@ -341,6 +343,14 @@ std::string Scene::flatten_keyed() {
         meshes.push_back(std::move(m));
     }
     for (auto const &kv : k_lights) {
+        if (kv.second.sky) {
+            HostLight l;
+            l.mesh = -1;
+            for (int c = 0; c < 3; c++)
+                l.rad[c] = kv.second.rad[c];
+            lights.push_back(l);
+            continue;
+        }
         auto it = geo_index.find(kv.second.geometry);
         if (it == geo_index.end())
             return "light " + std::to_string(kv.first) + ": its geometry " + std::to_string(kv.second.geometry) +
@@ -404,6 +414,8 @@ std::string Scene::commit() {
     for (auto &m : meshes)
         m.light = -1;
     for (size_t l = 0; l < lights.size(); l++) {
+        if (lights[l].mesh < 0)
+            continue; // the sky has no geometry
         HostMesh &m = meshes[lights[l].mesh];
         if (m.light >= 0)
             return "mesh " + std::to_string(lights[l].mesh) + " carries two lights";
@@ -626,6 +638,36 @@ std::string Scene::upload() {
     std::vector<F4> ltris;
     float total_power = 0;
     for (auto const &L : lights) {
+        if (L.mesh < 0) {
+            // sky_light::set_scene_boundary (light.cpp:96-100) applied at commit with the path
+            // space's bounds: m_dia = 2 * enclosing_radius = 2 * (|max - min| / 2), m_ref_p =
+            // (max + min) / 2 with z = 0 (tensor.h:811-813); power = pi * dia^2 / 2 per channel
+            // (light.cpp:151).
+            DevLight d{};
+            d.kind = kLightSky;
+            d.mesh = kNoMesh;
+            float dx = space_hi[0] - space_lo[0], dy = space_hi[1] - space_lo[1], dz = space_hi[2] - space_lo[2];
+            float nn = 0.0f;
+            nn += dx * dx;
+            nn += dy * dy;
+            nn += dz * dz;
+            d.sky_dia = 2 * (std::sqrt(nn) / 2);
+            d.sky_ref[0] = (space_hi[0] + space_lo[0]) / 2;
+            d.sky_ref[1] = (space_hi[1] + space_lo[1]) / 2;
+            d.sky_ref[2] = 0.0f;
+            float p = static_cast<float>(M_PI) * (d.sky_dia * d.sky_dia / 2);
+            float pn = 0.0f;
+            pn += p * p;
+            pn += p * p;
+            pn += p * p;
+            total_power += std::sqrt(pn);
+            cum_power.push_back(total_power);
+            for (int c = 0; c < 3; c++)
+                d.rad[c] = L.rad[c];
+            d.prob = std::sqrt(pn); // divided by the total below
+            dl.push_back(d);
+            continue;
+        }
         HostMesh const &m = meshes[L.mesh];
         DevLight d{};
         d.tri_begin = static_cast<uint32_t>(tri_cum.size());
@@ -665,11 +707,14 @@ std::string Scene::upload() {
         d.rad[1] = L.rad[1];
         d.rad[2] = L.rad[2];
         d.mesh = static_cast<uint32_t>(L.mesh);
+        d.kind = kLightArea;
         d.prob = std::sqrt(pn); // divided by the total below
         dl.push_back(d);
     }
+    // basic_light_sources::sample_light's probability (lightsources.cpp:73); lights of zero
+    // total power get probability 0 (the reference's 0 / 0 would make every estimate NaN)
     for (auto &d : dl)
-        d.prob = d.prob / total_power;
+        d.prob = total_power > 0.0f ? d.prob / total_power : 0.0f;
 
     std::string e;
     buf.bytes = 0;

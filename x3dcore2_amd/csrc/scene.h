@@ -22,7 +22,7 @@ struct HostMesh {
 };
 
 struct HostLight {
-    int32_t mesh;
+    int32_t mesh; // -1: the sky (sky_light, light.cpp:87-163)
     float rad[3];
 };
 
@@ -53,6 +53,7 @@ struct KeyedGeometry {
 struct KeyedLight {
     uint64_t geometry; // obj id of the emitting geometry
     float rad[3];
+    bool sky = false; // sky_light: no geometry
 };
 
 // trimesh::transform (geometry.cpp:76-87) with a column-major 4x4 matrix m[4 * col + row]:
@@ -86,6 +87,7 @@ class Scene {
     std::map<uint64_t, DevMaterial> k_mats;
     std::map<uint64_t, KeyedLight> k_lights;
     std::vector<uint64_t> geo_ids; // obj id of each committed mesh (keyed scenes)
+    std::vector<uint64_t> mat_ids; // obj id of each committed material (keyed scenes)
 
     // if_path_space::m_bound (pathspace.cpp:13-19): the union of every geometry ever loaded; it
     // grows with each load and is never reduced.

@@ -42,14 +42,22 @@ struct DevMaterial { // 64 B
 };
 static_assert(sizeof(DevMaterial) == 64, "DevMaterial layout");
 
-struct DevLight { // 32 B
+constexpr uint32_t kLightArea = 0; // area_light (light.cpp:17-85)
+constexpr uint32_t kLightSky = 1;  // sky_light (light.cpp:87-163)
+constexpr uint32_t kNoMesh = 0xFFFFFFFFu;
+
+struct DevLight { // 64 B
     uint32_t tri_begin, tri_count;
-    float area; // sum of triangle areas (geometry.cpp:105-118)
-    float prob; // |power| / total power (lightsources.cpp:73)
+    float area; // sum of triangle areas (geometry.cpp:105-118); 0 for the sky
+    float prob; // |power| / total power (lightsources.cpp:73); 0 when the total is 0
     float rad[3];
-    uint32_t mesh;
+    uint32_t mesh; // kNoMesh for the sky
+    uint32_t kind; // kLightArea / kLightSky
+    float sky_dia; // sky_light::m_dia, set at commit by set_scene_boundary (light.cpp:96-100)
+    float sky_ref[3]; // sky_light::m_ref_p: centroid of the path-space bounds with z = 0
+    uint32_t pad[3];
 };
-static_assert(sizeof(DevLight) == 32, "DevLight layout");
+static_assert(sizeof(DevLight) == 64, "DevLight layout");
 
 // Child reference encoding: interior node index, or a leaf:
 // bit 31 = leaf, bits 28..29 = primitive count - 1 (1..4 primitives), bits 0..27 = first prim.

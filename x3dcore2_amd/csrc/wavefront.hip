@@ -283,31 +283,27 @@ X3D_DI NeeSample nee_sample(WfArgs const &A, uint32_t path, HitRec const &x, f3 
     r.sdist = 0.0f;
     if (TR != kWfUnidirect && A.S.num_lights > 0) {
         LightSample ls = sample_light(A.S, g);
-        f3 l = x.p - ls.p;
-        float r2 = dot(l, l);
-        f3 inv = l / __builtin_sqrtf(r2);
-        float co = dot(ls.n, inv);
-        float ci = dot(x.n, -inv);
-        if (co > 0 && ci > 0) {
-            f3 illum = ((light_rad(A.S, ls.light) * ci) * co) / r2;
-            if (!near0(illum)) {
-                float dist = norm(l);
-                f3 wl = -l / dist;
-                f3 f = bsdf_eval(A.S.mats, x.mat, x.n, wo, wl);
-                f3 d = mul(illum, f) / ls.area_dens;
-                if (TR == kWfMis) {
-                    float cl = dot(ls.n, -wl);
-                    float pl = ls.area_dens * r2 / cl;
-                    float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
-                    float w = pl * pl / (pl * pl + pb * pb);
-                    r.sc = mul(beta, d * w);
-                } else {
-                    r.sc = mul(beta, d);
-                }
-                r.sdist = dist;
-                r.sdir = wl;
-                r.has_shadow = true;
+        f3 l, illum;
+        float r2;
+        if (light_arrives(A.S, ls, x.p, x.n, illum, l, r2)) {
+            float dist = norm(l);
+            f3 wl = -l / dist;
+            f3 f = bsdf_eval(A.S.mats, x.mat, x.n, wo, wl);
+            f3 d = mul(illum, f) / ls.area_dens;
+            if (TR == kWfMis && !ls.sky) {
+                float cl = dot(ls.n, -wl);
+                float pl = ls.area_dens * r2 / cl;
+                float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
+                float w = pl * pl / (pl * pl + pb * pb);
+                r.sc = mul(beta, d * w);
+            } else {
+                // lt1 adds it unweighted; the sky, which a BSDF sample never reaches, has MIS
+                // weight 1 (d * 1 == d)
+                r.sc = mul(beta, d);
             }
+            r.sdist = dist;
+            r.sdir = wl;
+            r.has_shadow = true;
         }
     }
     float dens;

@@ -226,6 +226,66 @@ def box_scene(width=256, height=256, spp=16, max_depth=5) -> SceneDesc:
     return SceneDesc("box", mats, meshes, lights, translate_camera(0.5, 0.5, 2.0), width, height, spp, max_depth)
 
 
+def fan(points, normal, material) -> Mesh:
+    """Planar polygon as a triangle fan from its first corner (triangles of unequal area unless
+    the polygon is regular)."""
+    v = np.array(points, dtype=np.float32)
+    n = np.repeat(np.array([normal], dtype=np.float32), v.shape[0], axis=0)
+    t = np.array([[0, k, k + 1] for k in range(1, v.shape[0] - 1)], dtype=np.uint32)
+    return Mesh(v, n, t, material)
+
+
+def multi_light_scene(width=256, height=256, spp=16, max_depth=5, unequal=True) -> SceneDesc:
+    """The box room with three emitters of unequal power and area: the 0.25 x 0.25 ceiling quad
+    (17, 12, 4), a 0.3 x 0.15 panel in front of the back wall facing +z (2, 4, 6) and a
+    pentagon in front of the left wall facing +x (6, 6, 3). With unequal=True the pentagon is a
+    fan of three triangles of different areas; with unequal=False it is split into equal halves
+    (two triangles of one quad), the case where the reference's choice of emitter triangle by
+    index (geometry.cpp:47-48, 63) is unbiased."""
+    mats: List[Material] = []
+    meshes: List[Mesh] = []
+    lights = []
+    white, _, _ = _room(mats, meshes)
+    meshes.append(box((0.33, 0.15, 0.62), (0.3, 0.3, 0.3), math.radians(-17.0), white))
+    meshes.append(box((0.66, 0.3, 0.35), (0.3, 0.6, 0.3), math.radians(17.0), white))
+    _ceiling_emitter(mats, meshes, lights)
+    black = len(mats) - 1
+    z = 0.002
+    meshes.append(quad((0.2, 0.55, z), (0.5, 0.55, z), (0.5, 0.7, z), (0.2, 0.7, z), (0, 0, 1), black))
+    lights.append((len(meshes) - 1, (2.0, 4.0, 6.0)))
+    x = 0.002
+    if unequal:
+        pent = [(x, 0.45, 0.3), (x, 0.42, 0.55), (x, 0.6, 0.72), (x, 0.78, 0.6), (x, 0.72, 0.35)]
+        meshes.append(fan(pent, (1, 0, 0), black))
+    else:
+        meshes.append(quad((x, 0.45, 0.3), (x, 0.45, 0.7), (x, 0.75, 0.7), (x, 0.75, 0.3), (1, 0, 0), black))
+    lights.append((len(meshes) - 1, (6.0, 6.0, 3.0)))
+    return SceneDesc("multi_light" if unequal else "multi_light_eq", mats, meshes, lights,
+                     translate_camera(0.5, 0.5, 2.0), width, height, spp, max_depth)
+
+
+SKY_RADIANCE = (0.529, 0.808, 0.922)  # the reference's default sky (resource.cpp:518-523)
+
+
+def sky_scene(width=256, height=256, spp=16, max_depth=5, area_light=True) -> SceneDesc:
+    """An open scene under the sky light: a ground quad in the plane z = 0 facing +z (the sky's
+    zenith), a block, a tall block and a GGX sphere on it, seen from above; optionally a small
+    area emitter facing the ground as a second light of another power."""
+    mats: List[Material] = [Material(OREN_NAYAR, (0.6, 0.6, 0.55)), Material(OREN_NAYAR, (0.7, 0.3, 0.2)),
+                            Material(COOK_TORR, (0.8, 0.8, 0.8), roughness=0.3, ior=GLOSSY_IOR),
+                            Material(OREN_NAYAR, (0.0, 0.0, 0.0))]
+    meshes: List[Mesh] = [quad((-0.5, -0.5, 0), (1.5, -0.5, 0), (1.5, 1.5, 0), (-0.5, 1.5, 0), (0, 0, 1), 0)]
+    m = box((0.3, 0.35, 0.15), (0.3, 0.3, 0.3), math.radians(25.0), 1)
+    meshes.append(m)
+    meshes.append(box((0.7, 0.6, 0.3), (0.2, 0.2, 0.6), math.radians(-10.0), 0))
+    meshes.append(displaced_sphere((0.45, 0.75, 0.15), 0.15, 3, 0.0, 5, 2))
+    lights = [(-1, SKY_RADIANCE)]
+    if area_light:
+        meshes.append(quad((0.6, 0.1, 0.8), (0.6, 0.3, 0.8), (0.8, 0.3, 0.8), (0.8, 0.1, 0.8), (0, 0, -1), 3))
+        lights.append((len(meshes) - 1, (8.0, 8.0, 6.0)))
+    return SceneDesc("sky", mats, meshes, lights, translate_camera(0.5, 0.5, 2.0), width, height, spp, max_depth)
+
+
 def sphere_levels(rng: np.random.Generator, count: int, target_tris: int) -> np.ndarray:
     """Subdivision levels in {3, 4, 5} whose face total is closest to target_tris (seeded)."""
     levels = rng.choice([3, 4, 5], size=count, p=[0.48, 0.30, 0.22])

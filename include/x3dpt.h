@@ -364,6 +364,22 @@ int x3d_render_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera 
                       uint32_t width, uint32_t height, uint32_t spp, float *d_rgba,
                       x3d_render_stats *stats);
 
+/* Tracer-level entry (replaces if_path_tracer::compute_first_hit + sample, pathtracer.h:44-63,
+ * pathtracer.cpp:456-472, as tst_pathtracer.cpp:58-86 drives them): radiance estimates for the
+ * caller's rays instead of a camera's. rays: float[6*n] (origin, direction; the direction is used
+ * as given, as e8util::ray does). The first hits are computed once per call (closest hit, a hit
+ * seen from behind dropped, the hit's light recorded), then samples [sample_begin,
+ * sample_begin + num_samples) are drawn for every ray with the renderer's tracer and options:
+ * sample s of ray k uses PCG stream (seed, k, s), so calls with consecutive sample ranges
+ * continue one sequence. radiance: float[3 * n * num_samples], sample-major (sample
+ * sample_begin + j of ray k at 3 * (j * n + k)). Always the plain form, no firefly filter.
+ * stats may be NULL. */
+int x3d_trace_rays(x3d_renderer *r, const x3d_scene *scene, const float *rays, uint32_t n, uint32_t sample_begin,
+                   uint32_t num_samples, float *radiance, x3d_render_stats *stats);
+/* Same with device pointers (d_rays: float[6*n], d_radiance: float[3*n*num_samples]). */
+int x3d_trace_rays_device(x3d_renderer *r, const x3d_scene *scene, const float *d_rays, uint32_t n,
+                          uint32_t sample_begin, uint32_t num_samples, float *d_radiance, x3d_render_stats *stats);
+
 /* PCG draws of stream (seed, pixel, sample) computed on the device (for RNG parity tests). */
 int x3d_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out);
 

@@ -133,6 +133,8 @@ class Oracle(_Backend):
         lib.orc_render_backward.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
                                             C.c_uint32, _f32p, C.c_int, C.c_uint32, _f64p]
         lib.orc_firefly_accumulate.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
+        lib.orc_trace_rays.argtypes = [C.c_void_p, _f32p, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint64,
+                                       C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
         lib.orc_render_firefly.argtypes = [C.c_void_p, _f32p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint64,
                                            C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]
         self.h = C.c_void_p(lib.orc_scene_new())
@@ -155,6 +157,16 @@ class Oracle(_Backend):
         spp, h, w, _ = x.shape
         out = np.zeros((h, w, 3), np.float32)
         self.lib.orc_firefly_accumulate(x, w, h, spp, out)
+        return out
+
+    def trace_rays(self, tracer, rays, num_samples, sample_begin=0, seed=1, max_depth=5, max_path_len=8,
+                   threads=None):
+        """Radiance of samples [sample_begin, sample_begin + num_samples) of caller rays
+        (float32 [n, 6]), sample s of ray k on PCG stream (seed, k, s): [num_samples, n, 3]."""
+        r = _f32(rays).reshape(-1, 6)
+        out = np.zeros((num_samples, r.shape[0], 3), np.float32)
+        self.lib.orc_trace_rays(self.h, r.reshape(-1), r.shape[0], tracer, max_depth, max_path_len, seed,
+                                sample_begin, num_samples, threads or os.cpu_count() or 1, out.reshape(-1))
         return out
 
     def render_firefly(self, tracer, width, height, spp, seed=1, max_depth=5, max_path_len=8, threads=None):

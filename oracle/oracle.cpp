@@ -1697,6 +1697,42 @@ void orc_camera_rays(float sensor, float focal, float aspect, float const *trans
 // 140-159 with one task). rgba: float[4*npix] (alpha = 1/spp as in renderer.cpp:148, 154-157),
 // or double when prec == 1. Work is spread over `threads` CPU threads by pixel; results do not
 // depend on the thread count. stats: uint64[6] (see orc_intersect), may be null.
+// if_path_tracer::compute_first_hit + sample (pathtracer.h:44-63, pathtracer.cpp:456-472) on
+// caller rays (float[6*n]): samples [s_begin, s_begin + num) of every ray, sample s of ray k on
+// PCG stream (seed, k, s). out: float[3 * n * num], sample-major. FP32 only.
+void orc_trace_rays(void *h, float const *rays, uint32_t n, int tracer, uint32_t max_depth, uint32_t max_path_len,
+                    uint64_t seed, uint32_t s_begin, uint32_t num, uint32_t threads, float *out) {
+    Scene *s = static_cast<Scene *>(h);
+    RenderOpts opt;
+    opt.tracer = tracer;
+    opt.max_depth = max_depth;
+    opt.max_path_len = max_path_len;
+    opt.seed = seed;
+    opt.traversal = TRAV_CULLED;
+    if (threads == 0)
+        threads = 1;
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            uint32_t k = next.fetch_add(1);
+            if (k >= n)
+                break;
+            V3f o = ld3(rays + 6 * k), v = ld3(rays + 6 * k + 3);
+            Hit<float> first = intersect<float>(*s, o, v, TRAV_CULLED, nullptr);
+            if (inner(first.n, -v) <= 0.0f)
+                first = Hit<float>();
+            for (uint32_t j = 0; j < num; j++)
+                st3(out + 3 * (static_cast<size_t>(j) * n + k),
+                    trace_sample<float>(*s, opt, first, v, k, s_begin + j, nullptr));
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < threads; t++)
+        pool.emplace_back(work);
+    for (auto &t : pool)
+        t.join();
+}
+
 void orc_render(void *h, float const *cam, int tracer, uint32_t max_depth, uint32_t max_path_len,
                 uint64_t seed, int traversal, uint32_t w, uint32_t hh, uint32_t spp,
                 uint32_t const *pixels, uint32_t npix, int prec, uint32_t threads, void *rgba,

@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kWG) void k_first_hit(RenderParams P) {
     uint32_t npix = P.width * P.height;
     for (uint32_t pix = blockIdx.x * kWG + threadIdx.x; pix < npix; pix += gridDim.x * kWG) {
         f3 o, v;
-        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        pixel_ray(P, pix, o, v);
         nq++;
         ClosestHit h = closest_hit<kWG, kLdsDepth, false>(P.S, o, v, st, pops, tests);
         float4 a = make_float4(0, 0, 0, 0), b = make_float4(0, 0, 0, 0), c = make_float4(0, 0, 0, 0);
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (uint32_t q = blockIdx.x * kWG + threadIdx.x; q < total; q = grid_threads + atomicAdd(next_item, 1u)) {
         uint32_t pix = q % npix, s = gen_begin + q / npix;
         f3 o, v;
-        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        pixel_ray(P, pix, o, v);
         bool valid;
         HitRec first = load_first(P, pix, valid);
         Pcg g = pcg_for(P.seed, pix, s);
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kWG) void k_render_mis_items(RenderParams P, uint32
     auto begin = [&](uint32_t item) {
         uint32_t pix = item % npix, s = gen_begin + item / npix;
         f3 o, v;
-        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        pixel_ray(P, pix, o, v);
         bool valid;
         HitRec first = load_first(P, pix, valid);
         tr.mis_begin(ps, first, valid, v, pcg_for(P.seed, pix, s), P.max_depth);
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (uint32_t q = blockIdx.x * kWG + threadIdx.x; q < total; q = gridDim.x * kWG + atomicAdd(next_item, 1u)) {
         uint32_t pix = q % npix, s = gen_begin + q / npix;
         f3 o, v;
-        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+        pixel_ray(P, pix, o, v);
         bool valid;
         HitRec first = load_first(P, pix, valid);
         Pcg g = pcg_for(P.seed, pix, s);
@@ -354,6 +354,16 @@ __global__ __launch_bounds__(256) void k_sum_samples(RenderParams P, uint32_t ge
             P.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
         else
             P.acc[pix] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+    }
+}
+
+// Sample radiance of a generation (float4 per item) to packed rgb (x3d_trace_rays).
+__global__ __launch_bounds__(256) void k_pack_rgb(const float4 *samples, uint64_t n, float *out) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += gridDim.x * 256ull) {
+        float4 q = samples[i];
+        out[3 * i] = q.x;
+        out[3 * i + 1] = q.y;
+        out[3 * i + 2] = q.z;
     }
 }
 
@@ -441,6 +451,7 @@ struct x3d_renderer {
     std::vector<x3d::DevBuf> batch_films; // device films of x3d_render_batch
     x3d::AdjointEngine *adj = nullptr;
     x3d::DevBuf adj_dl, adj_grad;          // host-pointer variant of x3d_render_backward
+    x3d::DevBuf trace_rays, trace_out;     // host-pointer variant of x3d_trace_rays
     ~x3d_renderer() {
         if (adj)
             x3d::adjoint_destroy(adj);
@@ -539,14 +550,26 @@ x3d::WavefrontTuning wf_tuning(x3d_render_options const &o) {
 
 // One render. `wf` (with `wf_tune`) selects the wavefront form of the MIS tracer, null the plain
 // kernels; d_dl / d_grad select the adjoint pass.
+// Ray-list mode of render_impl (x3d_trace_rays): the caller's rays replace the camera's (width =
+// ray count, height = 1), samples [s_first, s_first + spp) are drawn, and every sample's
+// radiance is written to d_out (rgb, sample-major) instead of being summed into a film.
+struct RayList {
+    const float *d_rays; // float[6 * n]
+    uint32_t s_first;
+    float *d_out; // float[3 * n * spp]
+};
+
 int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
                 uint32_t height, uint32_t spp, float *d_out, x3d_render_stats *stats,
                 x3d::WavefrontEngine *wf = nullptr, x3d::WavefrontTuning const *wf_tune = nullptr,
-                const float4 *d_dl = nullptr, float *d_grad = nullptr, bool firefly = false) {
+                const float4 *d_dl = nullptr, float *d_grad = nullptr, bool firefly = false,
+                RayList const *rl = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     RenderParams P{};
     P.S = scene->s.dev;
-    P.cam = cam_params(*camera);
+    if (camera)
+        P.cam = cam_params(*camera);
+    P.rays = rl ? rl->d_rays : nullptr;
     P.width = width;
     P.height = height;
     P.spp_total = spp;
@@ -636,13 +659,20 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         X3D_HIP(r->samples.ensure(static_cast<size_t>(gen) * npix * sizeof(float4)));
         float4 *samples = static_cast<float4 *>(r->samples.p);
         uint32_t g_sum = device_info().cus * 4u;
+        const uint32_t s_first = rl ? rl->s_first : 0u;
         for (uint32_t s0 = 0; s0 < spp; s0 += gen) {
             uint32_t cnt = spp - s0 < gen ? spp - s0 : gen;
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream,
                                reinterpret_cast<unsigned long long *>(next_pixel), 1u);
-            hipLaunchKernelGGL(kernel, dim3(grid_render), dim3(kWG), 0, r->stream, P, s0, cnt, samples, next_pixel);
-            hipLaunchKernelGGL(k_sum_samples, dim3(g_sum), dim3(256), 0, r->stream, P, s0, cnt,
-                               static_cast<const float4 *>(samples));
+            hipLaunchKernelGGL(kernel, dim3(grid_render), dim3(kWG), 0, r->stream, P, s_first + s0, cnt, samples,
+                               next_pixel);
+            if (rl)
+                hipLaunchKernelGGL(k_pack_rgb, dim3(g_sum), dim3(256), 0, r->stream,
+                                   static_cast<const float4 *>(samples), static_cast<uint64_t>(cnt) * npix,
+                                   rl->d_out + 3ull * s0 * npix);
+            else
+                hipLaunchKernelGGL(k_sum_samples, dim3(g_sum), dim3(256), 0, r->stream, P, s0, cnt,
+                                   static_cast<const float4 *>(samples));
         }
         return X3D_OK;
     };
@@ -1587,6 +1617,56 @@ int x3d_render_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera 
         return fail(X3D_ERR_INVALID_ARGUMENT, "d_rgba is null");
     x3d::WavefrontTuning tune = wf_tuning(r->opt);
     return render_impl(r, scene, camera, width, height, spp, d_rgba, stats, r->wf, &tune);
+}
+
+namespace {
+int check_trace_args(x3d_renderer *r, const x3d_scene *scene, const void *rays, uint32_t n, uint32_t num_samples,
+                     const void *radiance) {
+    if (!r || !rays || !radiance)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_scene(scene);
+    if (rc)
+        return rc;
+    if (n == 0 || num_samples == 0)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "n and num_samples must be > 0");
+    if (n > (1u << 30) || static_cast<uint64_t>(n) * num_samples > (1ull << 34))
+        return fail(X3D_ERR_INVALID_ARGUMENT, "too many rays or samples");
+    return X3D_OK;
+}
+} // namespace
+
+int x3d_trace_rays_device(x3d_renderer *r, const x3d_scene *scene, const float *d_rays, uint32_t n,
+                          uint32_t sample_begin, uint32_t num_samples, float *d_radiance, x3d_render_stats *stats) {
+    int rc = check_trace_args(r, scene, d_rays, n, num_samples, d_radiance);
+    if (rc)
+        return rc;
+    RayList rl{d_rays, sample_begin, d_radiance};
+    return render_impl(r, scene, nullptr, n, 1, num_samples, nullptr, stats, nullptr, nullptr, nullptr, nullptr,
+                       false, &rl);
+}
+
+int x3d_trace_rays(x3d_renderer *r, const x3d_scene *scene, const float *rays, uint32_t n, uint32_t sample_begin,
+                   uint32_t num_samples, float *radiance, x3d_render_stats *stats) {
+    int rc = check_trace_args(r, scene, rays, n, num_samples, radiance);
+    if (rc)
+        return rc;
+    for (uint64_t k = 0; k < 6ull * n; k++)
+        if (!std::isfinite(rays[k]))
+            return fail(X3D_ERR_INVALID_ARGUMENT, "rays must be finite");
+    auto t0 = std::chrono::steady_clock::now();
+    const size_t out_bytes = 3ull * n * num_samples * sizeof(float);
+    X3D_HIP(r->trace_rays.ensure(6ull * n * sizeof(float)));
+    X3D_HIP(r->trace_out.ensure(out_bytes));
+    X3D_HIP(hipMemcpy(r->trace_rays.p, rays, 6ull * n * sizeof(float), hipMemcpyHostToDevice));
+    RayList rl{static_cast<const float *>(r->trace_rays.p), sample_begin, static_cast<float *>(r->trace_out.p)};
+    rc = render_impl(r, scene, nullptr, n, 1, num_samples, nullptr, stats, nullptr, nullptr, nullptr, nullptr, false,
+                     &rl);
+    if (rc)
+        return rc;
+    X3D_HIP(hipMemcpy(radiance, r->trace_out.p, out_bytes, hipMemcpyDeviceToHost));
+    if (stats)
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return X3D_OK;
 }
 
 int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width, uint32_t height,

@@ -104,7 +104,20 @@ struct RenderParams {
     int32_t finalize;
     Counters *cnt;
     int32_t firefly; // firefly filter on the sample images (sampling_task::run, renderer.cpp:46-83)
+    const float *rays; // caller rays (origin, direction: 6 floats per "pixel") instead of the
+                       // camera's, or null (x3d_trace_rays)
 };
+
+// The primary ray of pixel `pix`: the caller's ray in ray-list mode, else the camera's.
+X3D_DI void pixel_ray(RenderParams const &P, uint32_t pix, f3 &o, f3 &v) {
+    if (P.rays) {
+        const float *r = P.rays + 6u * static_cast<size_t>(pix);
+        o = mk(r[0], r[1], r[2]);
+        v = mk(r[3], r[4], r[5]);
+    } else {
+        camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
+    }
+}
 
 // Firefly filter of sampling_task::run (renderer.cpp:46-83) for pixel (x, y) of one sample image
 // `img` (rgb per pixel, row-major, width w, height h): an interior pixel whose squared norm

@@ -380,6 +380,21 @@ int x3d_trace_rays(x3d_renderer *r, const x3d_scene *scene, const float *rays, u
 int x3d_trace_rays_device(x3d_renderer *r, const x3d_scene *scene, const float *d_rays, uint32_t n,
                           uint32_t sample_begin, uint32_t num_samples, float *d_radiance, x3d_render_stats *stats);
 
+/* No counterpart in the reference: x3d_render_device for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same film as x3d_render_device bit for bit (any tracer; always the plain form, no firefly
+ * filter), but the call allocates nothing: its working memory (first hits 48 B, running sums
+ * 16 B per pixel, traversal-stack spill, one or more sample images of 16 B per pixel) comes from
+ * a fixed 128 MiB device array that the library holds on every GPU it runs on until the process
+ * ends, with fewer resident workgroups where the stack spill would not fit. Films of more than
+ * about 1.6M pixels (fewer for scenes with very deep trees) are refused with
+ * X3D_ERR_INVALID_ARGUMENT before anything is launched. Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (from several threads or renderers) race and
+ * must be serialised by the caller. The scene must already be committed (its upload allocates),
+ * and the renderer already created. */
+int x3d_render_device_noalloc(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                              uint32_t height, uint32_t spp, float *d_rgba, x3d_render_stats *stats);
+
 /* PCG draws of stream (seed, pixel, sample) computed on the device (for RNG parity tests). */
 int x3d_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out);
 

@@ -119,7 +119,7 @@ EXPORTS = [
     "x3d_scene_dump_bvh", "x3d_intersect", "x3d_occluded", "x3d_intersect_device", "x3d_occluded_device",
     "x3d_bsdf_eval", "x3d_bsdf_pdf", "x3d_bsdf_sample", "x3d_camera_rays", "x3d_render_options_default",
     "x3d_renderer_create", "x3d_renderer_destroy", "x3d_render", "x3d_render_device", "x3d_pcg_draws",
-    "x3d_trace_rays", "x3d_trace_rays_device",
+    "x3d_trace_rays", "x3d_trace_rays_device", "x3d_render_device_noalloc",
     "x3d_renderer_autotune", "x3d_render_batch", "x3d_render_batch_device", "x3d_renderer_autotune_batch",
     "x3d_scene_set_material_params", "x3d_render_forward", "x3d_render_backward", "x3d_render_backward_device",
     "x3d_obj_load", "x3d_obj_free", "x3d_scene_add_obj",
@@ -174,6 +174,8 @@ def lib() -> C.CDLL:
     L.x3d_render.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, i32, vp, C.POINTER(RenderStats)]
     L.x3d_render_device.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp, C.POINTER(RenderStats)]
     L.x3d_pcg_draws.argtypes = [u64, u64, u32, u32, vp]
+    L.x3d_render_device_noalloc.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, vp,
+                                            C.POINTER(RenderStats)]
     L.x3d_trace_rays.argtypes = [vp, vp, vp, u32, u32, u32, vp, C.POINTER(RenderStats)]
     L.x3d_trace_rays_device.argtypes = [vp, vp, vp, u32, u32, u32, vp, C.POINTER(RenderStats)]
     L.x3d_renderer_autotune.argtypes = [vp, vp, C.POINTER(CameraDesc), u32, u32, u32, C.POINTER(AutotuneReport)]

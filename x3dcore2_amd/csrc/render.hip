@@ -433,6 +433,32 @@ template <class K> uint32_t persistent_grid(K kernel, uint64_t items) {
 
 uint32_t spill_entries(DevScene const &S) { return spill_entries_for(S, kLdsDepth); }
 
+// Fixed working memory of x3d_render_device_noalloc: a file-scope device array that every GPU the
+// library runs on holds (its own copy, at its own address) until the process ends. Host code
+// never uses the array's name as an address: the per-device address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+constexpr int kMaxScratchDevices = 64;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+int scratch_for_current_device(void **out) {
+    static std::mutex mu;
+    static void *ptr[kMaxScratchDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices)
+        return fail(X3D_ERR_DEVICE, "hipGetDevice failed or device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(X3D_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        ptr[dev] = p;
+    }
+    *out = ptr[dev];
+    return X3D_OK;
+}
+
 } // namespace x3d
 
 struct x3d_scene {
@@ -559,11 +585,27 @@ struct RayList {
     float *d_out; // float[3 * n * spp]
 };
 
+// Device memory carved out of one fixed block (x3d_render_device_noalloc): bump allocation in
+// 256-byte steps, no HIP allocation.
+struct ScratchArena {
+    char *base;
+    size_t bytes, used;
+    static size_t up(size_t n) { return (n + 255) & ~static_cast<size_t>(255); }
+    void *take(size_t n) {
+        size_t a = up(used);
+        if (a > bytes || n > bytes - a)
+            return nullptr;
+        used = a + n;
+        return base + a;
+    }
+    size_t left() const { return up(used) < bytes ? bytes - up(used) : 0; }
+};
+
 int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
                 uint32_t height, uint32_t spp, float *d_out, x3d_render_stats *stats,
                 x3d::WavefrontEngine *wf = nullptr, x3d::WavefrontTuning const *wf_tune = nullptr,
                 const float4 *d_dl = nullptr, float *d_grad = nullptr, bool firefly = false,
-                RayList const *rl = nullptr) {
+                RayList const *rl = nullptr, ScratchArena *arena = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     RenderParams P{};
     P.S = scene->s.dev;
@@ -579,14 +621,32 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     P.spill_per_thread = spill_entries(P.S);
     P.firefly = firefly ? 1 : 0;
     uint64_t npix = static_cast<uint64_t>(width) * height;
-    X3D_HIP(r->first.ensure(npix * 3 * sizeof(float4)));
-    X3D_HIP(r->acc.ensure(npix * sizeof(float4)));
+    // Working memory: the renderer's growable buffers, or (arena) pieces of a fixed block.
+    auto buffer = [&](DevBuf &b, size_t n, void **p) -> int {
+        if (arena) {
+            *p = arena->take(n);
+            if (!*p)
+                return fail(X3D_ERR_INVALID_ARGUMENT, "the film needs more than the " +
+                                                          std::to_string(arena->bytes >> 20) +
+                                                          " MiB of fixed scratch memory");
+            return X3D_OK;
+        }
+        X3D_HIP(b.ensure(n));
+        *p = b.p;
+        return X3D_OK;
+    };
+    void *p_first, *p_acc, *p_cnt, *p_spill;
     // counters: [0] first-hit pass, [1] sample kernels, then the pixel work counter
-    X3D_HIP(r->counters.ensure(2 * sizeof(Counters) + 64));
-    P.first = static_cast<float4 *>(r->first.p);
-    P.acc = static_cast<float4 *>(r->acc.p);
+    if (int rc = buffer(r->counters, 2 * sizeof(Counters) + 64, &p_cnt))
+        return rc;
+    if (int rc = buffer(r->first, npix * 3 * sizeof(float4), &p_first))
+        return rc;
+    if (int rc = buffer(r->acc, npix * sizeof(float4), &p_acc))
+        return rc;
+    P.first = static_cast<float4 *>(p_first);
+    P.acc = static_cast<float4 *>(p_acc);
     P.out = reinterpret_cast<float4 *>(d_out);
-    P.cnt = static_cast<Counters *>(r->counters.p);
+    P.cnt = static_cast<Counters *>(p_cnt);
     int tracer = r->opt.pt_type;
     uint32_t grid_first = persistent_grid(k_first_hit, npix);
     uint32_t grid_render = 1;
@@ -620,9 +680,24 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         grid_render = persistent_grid(k_render_plain_items<X3D_PT_NORMAL>, npix * spp);
         break;
     }
+    const size_t spill_block = static_cast<size_t>(kWG) * P.spill_per_thread * sizeof(uint2);
+    if (arena) {
+        // Keep room for one sample per generation; the traversal-stack spill of the resident
+        // lanes takes at most half of the rest, fewer workgroups where it would not fit (every
+        // kernel fetches its work dynamically, so any grid size gives the same film).
+        const size_t one_sample = ScratchArena::up(npix * sizeof(float4)) + 256;
+        const size_t left = arena->left();
+        const size_t cap = left > one_sample ? (left - one_sample) / 2 / spill_block : 0;
+        if (cap == 0)
+            return fail(X3D_ERR_INVALID_ARGUMENT, "the film needs more than the " + std::to_string(arena->bytes >> 20) +
+                                                      " MiB of fixed scratch memory");
+        grid_first = grid_first < cap ? grid_first : static_cast<uint32_t>(cap);
+        grid_render = grid_render < cap ? grid_render : static_cast<uint32_t>(cap);
+    }
     uint32_t grid = grid_first > grid_render ? grid_first : grid_render;
-    X3D_HIP(r->spill.ensure(static_cast<size_t>(grid) * kWG * P.spill_per_thread * sizeof(uint2)));
-    P.spill = static_cast<uint2 *>(r->spill.p);
+    if (int rc = buffer(r->spill, static_cast<size_t>(grid) * spill_block, &p_spill))
+        return rc;
+    P.spill = static_cast<uint2 *>(p_spill);
     // Counters are cleared by a kernel, not a memset, so the sequence stays capturable in a graph.
     uint32_t *next_pixel = reinterpret_cast<uint32_t *>(P.cnt + 2);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream, reinterpret_cast<unsigned long long *>(P.cnt),
@@ -654,10 +729,14 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     // items each, rendered by `kernel` into the sample buffer and summed per pixel in order.
     auto run_generations = [&](void (*kernel)(RenderParams, uint32_t, uint32_t, float4 *, uint32_t *)) -> int {
         uint64_t items = r->opt.queue_capacity ? r->opt.queue_capacity : kGenItems;
+        if (arena && items > arena->left() / sizeof(float4))
+            items = arena->left() / sizeof(float4);
         uint32_t gen = static_cast<uint32_t>(items / npix);
         gen = gen < 1 ? 1 : (gen > spp ? spp : gen);
-        X3D_HIP(r->samples.ensure(static_cast<size_t>(gen) * npix * sizeof(float4)));
-        float4 *samples = static_cast<float4 *>(r->samples.p);
+        void *p_samples;
+        if (int rc = buffer(r->samples, static_cast<size_t>(gen) * npix * sizeof(float4), &p_samples))
+            return rc;
+        float4 *samples = static_cast<float4 *>(p_samples);
         uint32_t g_sum = device_info().cus * 4u;
         const uint32_t s_first = rl ? rl->s_first : 0u;
         for (uint32_t s0 = 0; s0 < spp; s0 += gen) {
@@ -1667,6 +1746,21 @@ int x3d_trace_rays(x3d_renderer *r, const x3d_scene *scene, const float *rays, u
     if (stats)
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return X3D_OK;
+}
+
+int x3d_render_device_noalloc(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,
+                              uint32_t height, uint32_t spp, float *d_rgba, x3d_render_stats *stats) {
+    int rc = check_render_args(r, scene, camera, width, height, spp);
+    if (rc)
+        return rc;
+    if (!d_rgba)
+        return fail(X3D_ERR_INVALID_ARGUMENT, "d_rgba is null");
+    void *base = nullptr;
+    if (int rc2 = scratch_for_current_device(&base))
+        return rc2;
+    ScratchArena arena{static_cast<char *>(base), kScratchBytes, 0};
+    return render_impl(r, scene, camera, width, height, spp, d_rgba, stats, nullptr, nullptr, nullptr, nullptr, false,
+                       nullptr, &arena);
 }
 
 int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width, uint32_t height,

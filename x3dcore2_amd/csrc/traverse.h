@@ -117,6 +117,17 @@ X3D_DI uint32_t visit_interior(DevScene const &S, uint32_t ref, f3 o, f3 vinv, f
     return st.pop_next(REF_ORDER ? __builtin_inff() : cut);
 }
 
+// Compare-exchange of two (entry distance, ref) pairs held in named variables, ascending distance.
+X3D_DI void cswap2(float &ta, uint32_t &ra, float &tb, uint32_t &rb) {
+    bool sw = tb < ta;
+    float lo = sw ? tb : ta, hi = sw ? ta : tb;
+    uint32_t rl = sw ? rb : ra, rh = sw ? ra : rb;
+    ta = lo;
+    tb = hi;
+    ra = rl;
+    rb = rh;
+}
+
 // Compare-exchange of two (entry distance, ref) slots, ascending distance.
 X3D_DI void cswap(float *t, uint32_t *r, int a, int b) {
     bool sw = t[b] < t[a];
@@ -186,11 +197,24 @@ X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, float
     }
     // sorting network, ascending entry distance
     if (kTreeWidth == 4) {
-        cswap(t, r, 0, 1);
-        cswap(t, r, 2, 3);
-        cswap(t, r, 0, 2);
-        cswap(t, r, 1, 3);
-        cswap(t, r, 1, 2);
+        // on named variables: the same network over the arrays let the compiler track the
+        // permutation as select chains of indices (about twice the instructions)
+        float t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
+        uint32_t r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+        cswap2(t0, r0, t1, r1);
+        cswap2(t2, r2, t3, r3);
+        cswap2(t0, r0, t2, r2);
+        cswap2(t1, r1, t3, r3);
+        cswap2(t1, r1, t2, r2);
+        if (r3 != kDone)
+            st.push(r3, t3);
+        if (r2 != kDone)
+            st.push(r2, t2);
+        if (r1 != kDone)
+            st.push(r1, t1);
+        if (r0 != kDone)
+            return r0;
+        return st.pop_next(cut);
     } else {
         constexpr int kNet[19][2] = {{0, 2}, {1, 3}, {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 1}, {2, 3},
                                      {4, 5}, {6, 7}, {2, 4}, {3, 5}, {1, 4}, {3, 6}, {1, 2}, {3, 4}, {5, 6}};

@@ -583,7 +583,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     uint32_t i = wave_fetch_batched(&A.ctl->sq_head.v, slot);
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
     float tmax = 0.0f;
-    uint32_t ref = kDone, leaf = kDone;
+    uint32_t ref = kDone, leaf = kDone, cur = 0; // cur: next triangle of the parked leaf
     bool blocked = false, wide = false;
     auto init = [&](uint32_t k) {
         float4 a = A.sq_in[3 * k], b = A.sq_in[3 * k + 1];
@@ -598,6 +598,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
         leaf = kDone;
         if (ref != kEmptyRef && !is_interior(ref)) {
             leaf = ref;
+            cur = leaf_start(leaf);
             ref = kDone;
         }
     };
@@ -614,31 +615,35 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
             }
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
+                cur = leaf_start(leaf);
                 ref = st.pop_next(__builtin_inff());
             }
             // descend until fewer than refill_below lanes still look for their first leaf
             if (__popcll(__ballot(leaf == kDone)) < A.refill_below)
                 break;
         }
+        // One triangle per lane and iteration: a lane whose leaf is done takes its next stacked
+        // leaf at once instead of waiting for lanes with larger leaves.
         while (leaf != kDone) {
-            uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
-            for (uint32_t p = s0; p < s0 + cnt; p++) {
+            {
+                const uint32_t p = cur;
                 float4 a = tris[3 * p], b = tris[3 * p + 1], c = tris[3 * p + 2];
                 float c1, c2, t0;
                 tests++;
-                if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), 1e-4f, tmax, &c1, &c2, &t0)) {
-                    blocked = true;
-                    break;
-                }
+                blocked = tri_hit(o, v, xyz(a), xyz(b), xyz(c), 1e-4f, tmax, &c1, &c2, &t0);
             }
+            cur++;
             if (blocked) {
                 leaf = kDone;
                 ref = kDone;
-            } else if (is_leaf_ref(ref)) {
-                leaf = ref;
-                ref = st.pop_next(__builtin_inff());
-            } else {
-                leaf = kDone;
+            } else if (cur == leaf_start(leaf) + leaf_count(leaf)) {
+                if (is_leaf_ref(ref)) {
+                    leaf = ref;
+                    cur = leaf_start(leaf);
+                    ref = st.pop_next(__builtin_inff());
+                } else {
+                    leaf = kDone;
+                }
             }
             // test leaves until fewer than leaf_below lanes still hold one (the rest keep theirs)
             if (__popcll(__ballot(leaf != kDone)) < A.leaf_below)
@@ -687,7 +692,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     uint32_t i = wave_fetch_batched(&A.ctl->eq_head.v, slot);
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
     ClosestHit h{-1, __builtin_inff(), 0.0f, 0.0f};
-    uint32_t best_leaf = 0, ref = kDone, leaf = kDone;
+    uint32_t best_leaf = 0, ref = kDone, leaf = kDone, cur = 0; // cur: next triangle of the parked leaf
     bool wide = false;
     auto init = [&](uint32_t k) {
         uint32_t path = A.eq_in[k];
@@ -702,6 +707,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
         leaf = kDone;
         if (ref != kEmptyRef && !is_interior(ref)) {
             leaf = ref;
+            cur = leaf_start(leaf);
             ref = kDone;
         }
     };
@@ -719,15 +725,18 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
             }
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
+                cur = leaf_start(leaf);
                 ref = st.pop_next(cut);
             }
             // descend until fewer than refill_below lanes still look for their first leaf
             if (__popcll(__ballot(leaf == kDone)) < A.refill_below)
                 break;
         }
+        // One triangle per lane and iteration: a lane whose leaf is done takes its next stacked
+        // leaf at once instead of waiting for lanes with larger leaves.
         while (leaf != kDone) {
-            uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
-            for (uint32_t p = s0; p < s0 + cnt; p++) {
+            const uint32_t s0 = leaf_start(leaf), p = cur;
+            {
                 float4 a = tris[3 * p], b = tris[3 * p + 1], c = tris[3 * p + 2];
                 float c1, c2, t0;
                 tests++;
@@ -740,11 +749,15 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
                     best_leaf = s0;
                 }
             }
-            if (is_leaf_ref(ref)) {
-                leaf = ref;
-                ref = st.pop_next(h.t + h.t * kCullSlack);
-            } else {
-                leaf = kDone;
+            cur++;
+            if (cur == s0 + leaf_count(leaf)) {
+                if (is_leaf_ref(ref)) {
+                    leaf = ref;
+                    cur = leaf_start(leaf);
+                    ref = st.pop_next(h.t + h.t * kCullSlack);
+                } else {
+                    leaf = kDone;
+                }
             }
             // test leaves until fewer than leaf_below lanes still hold one (the rest keep theirs)
             if (__popcll(__ballot(leaf != kDone)) < A.leaf_below)

@@ -9,9 +9,11 @@
 // specialised to BSDF parameters). Walking the recorded vertices backwards, `after` holds the
 // contributions made after the BSDF sample of the current vertex, which all contain its f.
 //
-// Gradients go to per-wave slots in LDS (one float per material parameter), then per block in
-// wave order, then over blocks in block order: with the static (pixel, sample) assignment the
-// result does not depend on timing.
+// Gradients are summed in 64-bit fixed point (2^-32 resolution, |sum| < 2^31): each contribution
+// is rounded once to an integer multiple of 2^-32 and added with integer atomics into per-wave
+// slots in LDS, then per block and over blocks. Integer addition is associative, so the result
+// does not depend on the order of the additions, the lane order of same-address atomics, the
+// grid size or the number of CUs: it is the same on every run and every device.
 
 #include "adjoint.h"
 
@@ -61,9 +63,21 @@ X3D_DI void lobe_scalar(DevMaterial const &m, f3 n, f3 o, f3 i, float *sv, float
     *ds_dr = *sv * dlog * 4.0f * m.roughness;
 }
 
+constexpr double kAdjFixScale = 4294967296.0;      // 2^32
+constexpr float kAdjFixMax = 9.2e18f;                // clamp below 2^63
+
+// A contribution in 64-bit fixed point (round to nearest), as the unsigned bit pattern the LDS
+// atomics add.
+X3D_DI unsigned long long to_fixed(float v) {
+    double x = static_cast<double>(v) * kAdjFixScale;
+    x = x > kAdjFixMax ? kAdjFixMax : (x < -kAdjFixMax ? -kAdjFixMax : x); // NaN stays NaN
+    long long q = x == x ? __double2ll_rn(x) : 0ll;
+    return static_cast<unsigned long long>(q);
+}
+
 // Adds Q-weighted d log f_c (per channel) of material `mat` at (n, o, i) to the slots
 // (4 per material: d albedo rgb, d roughness): f_c = sum over lobes w_l s_l a_{l,c}.
-X3D_DI void add_bsdf_grad(DevScene const &S, int32_t mat, f3 n, f3 o, f3 i, f3 Q, float *slots) {
+X3D_DI void add_bsdf_grad(DevScene const &S, int32_t mat, f3 n, f3 o, f3 i, f3 Q, unsigned long long *slots) {
     if (mat < 0 || static_cast<uint32_t>(mat) >= S.fail_safe_mat)
         return; // the fail-safe material has no parameters
     f3 f = bsdf_eval(S.mats, mat, n, o, i);
@@ -78,11 +92,11 @@ X3D_DI void add_bsdf_grad(DevScene const &S, int32_t mat, f3 n, f3 o, f3 i, f3 Q
         for (int c = 0; c < 3; c++) {
             if (fc[c] == 0.0f)
                 continue;
-            atomicAdd(&slots[4 * lm + c], qc[c] * wgt * sv / fc[c]);
+            atomicAdd(&slots[4 * lm + c], to_fixed(qc[c] * wgt * sv / fc[c]));
             gr += qc[c] * wgt * L.albedo[c] * ds / fc[c];
         }
         if (L.kind == kMatCookTorr)
-            atomicAdd(&slots[4 * lm + 3], gr);
+            atomicAdd(&slots[4 * lm + 3], to_fixed(gr));
     };
     DevMaterial const &m = S.mats[mat];
     if (m.kind == kMatMixture) {
@@ -96,15 +110,15 @@ X3D_DI void add_bsdf_grad(DevScene const &S, int32_t mat, f3 n, f3 o, f3 i, f3 Q
 constexpr int kAdjWaves = kWG / 64;
 
 __global__ __launch_bounds__(kWG) void k_adjoint(RenderParams P, const float4 *dl, uint32_t num_mats,
-                                                 float *partial) {
+                                                 unsigned long long *partial) {
     __shared__ uint2 lds[kLdsDepth * kWG];
-    __shared__ float slots[kAdjWaves][4 * kAdjMaxMats];
+    __shared__ unsigned long long slots[kAdjWaves][4 * kAdjMaxMats];
     for (uint32_t i = threadIdx.x; i < kAdjWaves * 4 * kAdjMaxMats; i += kWG)
-        (&slots[0][0])[i] = 0.0f;
+        (&slots[0][0])[i] = 0ull;
     __syncthreads();
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
     Tracer<kWG, kLdsDepth> tr{P.S, st};
-    float *my = slots[threadIdx.x >> 6];
+    unsigned long long *my = slots[threadIdx.x >> 6];
     uint32_t npix = P.width * P.height;
     uint64_t total = static_cast<uint64_t>(npix) * P.spp_total;
     float inv = 1.0f / static_cast<float>(P.spp_total);
@@ -146,21 +160,32 @@ __global__ __launch_bounds__(kWG) void k_adjoint(RenderParams P, const float4 *d
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < 4 * num_mats; i += kWG) {
-        float acc = 0.0f;
+        unsigned long long acc = 0ull;
         for (int w = 0; w < kAdjWaves; w++)
             acc += slots[w][i];
         partial[static_cast<size_t>(blockIdx.x) * 4 * num_mats + i] = acc;
     }
 }
 
-__global__ void k_adjoint_reduce(const float *partial, uint32_t blocks, uint32_t n, float *grad) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    double acc = 0.0;
-    for (uint32_t b = 0; b < blocks; b++)
-        acc += static_cast<double>(partial[static_cast<size_t>(b) * n + i]);
-    grad[i] = static_cast<float>(acc);
+// Fixed-point sums over blocks (exact, wrap-around two's complement) -> float gradient: one
+// block per parameter, its threads striding over the per-block partials, then a tree in LDS
+// (integer sums: the grouping does not change the result).
+__global__ __launch_bounds__(256) void k_adjoint_reduce(const unsigned long long *partial, uint32_t blocks,
+                                                        uint32_t n, float *grad) {
+    __shared__ unsigned long long red[256];
+    const uint32_t i = blockIdx.x;
+    unsigned long long acc = 0ull;
+    for (uint32_t b = threadIdx.x; b < blocks; b += 256)
+        acc += partial[static_cast<size_t>(b) * n + i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        grad[i] = static_cast<float>(static_cast<double>(static_cast<long long>(red[0])) / kAdjFixScale);
 }
 
 } // namespace
@@ -190,7 +215,8 @@ std::string adjoint_render(AdjointEngine *e, RenderParams const &P, const float4
     if (grid < 1)
         grid = 1;
     uint32_t n = 4 * (num_mats ? num_mats : 1);
-    if (hipError_t err = e->partial.ensure(sizeof(float) * static_cast<size_t>(grid) * n); err != hipSuccess)
+    if (hipError_t err = e->partial.ensure(sizeof(unsigned long long) * static_cast<size_t>(grid) * n);
+        err != hipSuccess)
         return std::string("hipMalloc: ") + hipGetErrorString(err);
     RenderParams Q = P;
     Q.spill_per_thread = spill_entries_for(P.S, kLdsDepth);
@@ -199,9 +225,10 @@ std::string adjoint_render(AdjointEngine *e, RenderParams const &P, const float4
         return std::string("hipMalloc: ") + hipGetErrorString(err);
     Q.spill = static_cast<uint2 *>(e->spill.p);
     hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(kWG), 0, stream, Q, d_dl, num_mats,
-                       static_cast<float *>(e->partial.p));
-    hipLaunchKernelGGL(k_adjoint_reduce, dim3((n + 255) / 256), dim3(256), 0, stream,
-                       static_cast<const float *>(e->partial.p), grid, 4 * num_mats, d_grad);
+                       static_cast<unsigned long long *>(e->partial.p));
+    if (num_mats > 0)
+        hipLaunchKernelGGL(k_adjoint_reduce, dim3(4 * num_mats), dim3(256), 0, stream,
+                           static_cast<const unsigned long long *>(e->partial.p), grid, 4 * num_mats, d_grad);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess)
         return std::string("adjoint launch: ") + hipGetErrorString(err);

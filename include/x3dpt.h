@@ -258,7 +258,9 @@ typedef struct x3d_renderer x3d_renderer;
  * power heuristic (no reference counterpart). */
 #define X3D_PT_UNIDIRECT_MIS 16
 
-/* Execution forms of the same estimator; every form gives the bit-identical film. */
+/* Execution forms of the same estimator; every form gives the bit-identical film.
+ * x3d_render_options_default picks the wavefront form for UNIDIRECT_MIS, UNIDIRECT_LT1 and
+ * UNIDIRECT, the plain form for the other tracers. */
 #define X3D_FORM_PLAIN 0     /* persistent kernels over the (pixel, sample) items of a generation,
                                 each lane taking the next item when its path ends, then an in-order
                                 per-pixel sum (the drop-in) */
@@ -402,7 +404,7 @@ int x3d_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, ui
  * renderer.h:57-59). Jobs (scene, camera, film size, spp) with fewer than batch_split samples
  * share one persistent launch, captured as a hipGraph and replayed while the batch keeps its
  * shape; larger jobs then run one after another in the wavefront form (fewest samples first).
- * The renderer must be a plain-form UNIDIRECT_MIS renderer; each job's film is bit-identical
+ * The renderer must be a UNIDIRECT_MIS renderer (of either form); each job's film is bit-identical
  * to x3d_render of that job. Options used: max_depth, seed, batch_split; for the shared launch
  * workgroup_size (64/128/256), stack_depth (8/16), sort_key (NONE: hand tiles out in job order;
  * otherwise most expensive job first); for the large jobs queue_capacity. Jobs may share
@@ -434,7 +436,8 @@ int x3d_render_batch_device(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *j
 
 /* ---- differentiation (no reference counterpart: the reference has no differentiable
  * renderer). Gradient of a film loss with respect to material parameters for the
- * UNIDIRECT_MIS tracer (plain form, max_depth <= 8, at most 64 materials). ---- */
+ * UNIDIRECT_MIS tracer (either form: the backward pass always replays paths one per lane;
+ * max_depth <= 8, at most 64 materials). ---- */
 
 /* Forward pass of the pair: the film of x3d_render (firefly filter off). */
 int x3d_render_forward(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera, uint32_t width,

@@ -86,3 +86,16 @@ def test_every_export_has_a_ctypes_signature(x3d):
     for name in x3d.EXPORTS:
         if name not in ("x3d_last_error", "x3d_version"):
             assert getattr(lib, name).argtypes is not None, name
+
+
+def test_option_defaults_pick_the_wavefront_form_where_it_exists(x3d):
+    """x3d_render_options_default: the wavefront form (same film, faster) for the tracers that
+    have one, the plain form for the others; the reference's m_max_path_len of 8."""
+    import ctypes as C
+
+    for pt, form in ((x3d.PT_UNIDIRECT_MIS, x3d.FORM_WAVEFRONT), (x3d.PT_UNIDIRECT_LT1, x3d.FORM_WAVEFRONT),
+                     (x3d.PT_UNIDIRECT, x3d.FORM_WAVEFRONT), (x3d.PT_BIDIRECT_MIS, x3d.FORM_PLAIN),
+                     (x3d.PT_DIRECT, x3d.FORM_PLAIN), (x3d.PT_NORMAL, x3d.FORM_PLAIN)):
+        o = x3d.RenderOptions()
+        assert x3d.lib().x3d_render_options_default(pt, C.byref(o)) == 0
+        assert o.form == form and o.max_path_len == 8 and o.pt_type == pt

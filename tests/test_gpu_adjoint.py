@@ -44,6 +44,10 @@ def test_adjoint_matches_oracle(x3d, oracle_mod):
               f"max |gpu - fp64| {np.abs(g - g64).max() / scale:.2e} (relative to max |grad|)", flush=True)
         assert st.num_samples == w * h * spp
         assert np.abs(g - g32).max() <= 1e-4 * scale
+        # a renderer of the wavefront form (the default) runs the same replay
+        gw, _ = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=depth, seed=7).backward(
+            sc, desc.camera, w, h, spp, dl)
+        assert np.array_equal(gw.view(np.uint32), g.view(np.uint32))
         assert np.abs(g - g64).max() <= 2e-3 * scale
 
 

@@ -72,3 +72,12 @@ def test_batch_autotune_and_errors(x3d, batch):
         r.render_batch([(built[0], descs[0].camera, 1, 8, 1)])
     films, st, lat = r.render_batch([])
     assert films == [] and st.num_samples == 0
+
+
+def test_batch_through_a_wavefront_form_renderer(x3d, batch):
+    """A renderer of the wavefront form (the default for UNIDIRECT_MIS) batches the same films."""
+    descs, built, items = batch
+    plain, _, _ = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=9).render_batch(items)
+    wf, _, _ = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=6, seed=9).render_batch(items)
+    for a, b in zip(plain, wf):
+        assert np.array_equal(_bits(a), _bits(b))

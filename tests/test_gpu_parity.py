@@ -280,7 +280,39 @@ def test_wavefront_reference_tracers_identical_to_plain(x3d, scene_name, tracer,
     assert img[:, :3].max() > 0
 
 
-@pytest.mark.parametrize("tracer", ["UNIDIRECT_LT1", "UNIDIRECT", "UNIDIRECT_MIS"])
+BDPT_WAVEFRONT_CASES = [
+    dict(max_path_len=6),                                          # one generation
+    dict(max_path_len=1),                                          # emission ray only
+    dict(max_path_len=2, queue_capacity=3000),                     # generations of 1 sample
+    dict(max_path_len=8, workgroup_size=128, stack_depth=8),
+    dict(max_path_len=16, profile=1),                              # deepest subpaths, timed launches
+]
+
+
+@pytest.mark.parametrize("kw", BDPT_WAVEFRONT_CASES)
+@pytest.mark.parametrize("scene_name", ["box", "indirect_l3", "spheres_small", "multi_light", "sky", "furnace"])
+def test_wavefront_bdpt_identical_to_plain(x3d, scene_name, kw):
+    """The bidirectional tracer in the wavefront form (subpath vertices as structure-of-arrays
+    rows, connections traced by the any-hit kernel): the plain kernel's film bits and query
+    counts, in one or several generations."""
+    desc = QUERY_SCENES[scene_name]()
+    print(f"[gpu] wavefront BDPT {scene_name} {kw}", flush=True)
+    gs = x3d.Scene(desc)
+    w, h, spp = 48, 40, 4
+    plain, sp = x3d.Renderer(x3d.PT_BIDIRECT_MIS, seed=29, max_path_len=kw["max_path_len"]).render(
+        gs, desc.camera, w, h, spp)
+    wf = x3d.Renderer(x3d.PT_BIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, seed=29, **kw)
+    for _ in range(2):
+        img, st = wf.render(gs, desc.camera, w, h, spp)
+        assert np.array_equal(_bits(img), _bits(plain)), np.abs(img - plain).max()
+        assert st.closest_queries == sp.closest_queries
+        assert st.anyhit_queries == sp.anyhit_queries
+    assert img[:, :3].max() > 0
+    if kw.get("profile"):
+        assert st.trace_launches == 2 * kw["max_path_len"] and st.extend_ms > 0 and st.shadow_ms > 0
+
+
+@pytest.mark.parametrize("tracer", ["UNIDIRECT_LT1", "UNIDIRECT", "UNIDIRECT_MIS", "BIDIRECT_MIS"])
 def test_wavefront_without_lights_and_ragged(x3d, tracer):
     """Edge cases of the wavefront form: a scene without lights (no light sample is drawn, the film
     is black) and a ragged film (7 x 3 pixels, 3 samples, one sample per generation) match the plain

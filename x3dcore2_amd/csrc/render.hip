@@ -675,7 +675,10 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     case X3D_PT_POSITION:
         grid_render = persistent_grid(k_render_plain_items<X3D_PT_POSITION>, npix * spp);
         break;
-    case X3D_PT_BIDIRECT_MIS: grid_render = persistent_grid(k_render_bdpt, npix * static_cast<uint64_t>(spp)); break;
+    case X3D_PT_BIDIRECT_MIS:
+        // (the wavefront form launches its own kernels)
+        grid_render = wf ? 1 : persistent_grid(k_render_bdpt, npix * static_cast<uint64_t>(spp));
+        break;
     default:
         grid_render = persistent_grid(k_render_plain_items<X3D_PT_NORMAL>, npix * spp);
         break;
@@ -718,7 +721,7 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         if (!e.empty())
             return fail(e.find("hip") != std::string::npos ? X3D_ERR_DEVICE : X3D_ERR_UNSUPPORTED, e);
         tracer = -1; // launched
-    } else if (wf) {
+    } else if (wf && tracer != X3D_PT_BIDIRECT_MIS) {
         std::string e = x3d::wf_render(wf, P, *wf_tune, r->stream);
         if (!e.empty())
             return fail(X3D_ERR_DEVICE, e);
@@ -726,11 +729,20 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     }
     int rc = X3D_OK;
     // Generations of whole samples, at most queue_capacity (default kGenItems) (pixel, sample)
-    // items each, rendered by `kernel` into the sample buffer and summed per pixel in order.
+    // items each, rendered by `kernel` (or, for the bidirectional tracer's wavefront form, by its
+    // kernel sequence) into the sample buffer and summed per pixel in order.
+    const bool bd_wf = wf && tracer == X3D_PT_BIDIRECT_MIS;
     auto run_generations = [&](void (*kernel)(RenderParams, uint32_t, uint32_t, float4 *, uint32_t *)) -> int {
         uint64_t items = r->opt.queue_capacity ? r->opt.queue_capacity : kGenItems;
         if (arena && items > arena->left() / sizeof(float4))
             items = arena->left() / sizeof(float4);
+        if (bd_wf) {
+            const uint64_t cap = x3d::bd_max_items(P.max_path_len);
+            if (cap < npix)
+                return fail(X3D_ERR_INVALID_ARGUMENT, "the film has more pixels than one bidirectional wavefront "
+                                                      "generation holds; use the plain form");
+            items = items < cap ? items : cap;
+        }
         uint32_t gen = static_cast<uint32_t>(items / npix);
         gen = gen < 1 ? 1 : (gen > spp ? spp : gen);
         void *p_samples;
@@ -741,10 +753,16 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         const uint32_t s_first = rl ? rl->s_first : 0u;
         for (uint32_t s0 = 0; s0 < spp; s0 += gen) {
             uint32_t cnt = spp - s0 < gen ? spp - s0 : gen;
-            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream,
-                               reinterpret_cast<unsigned long long *>(next_pixel), 1u);
-            hipLaunchKernelGGL(kernel, dim3(grid_render), dim3(kWG), 0, r->stream, P, s_first + s0, cnt, samples,
-                               next_pixel);
+            if (bd_wf) {
+                std::string e = x3d::bd_generation(wf, P, *wf_tune, s_first + s0, cnt, samples, s0 == 0, r->stream);
+                if (!e.empty())
+                    return fail(X3D_ERR_DEVICE, e);
+            } else {
+                hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream,
+                                   reinterpret_cast<unsigned long long *>(next_pixel), 1u);
+                hipLaunchKernelGGL(kernel, dim3(grid_render), dim3(kWG), 0, r->stream, P, s_first + s0, cnt, samples,
+                                   next_pixel);
+            }
             if (rl)
                 hipLaunchKernelGGL(k_pack_rgb, dim3(g_sum), dim3(256), 0, r->stream,
                                    static_cast<const float4 *>(samples), static_cast<uint64_t>(cnt) * npix,
@@ -1650,9 +1668,10 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
     if (opts->form != X3D_FORM_PLAIN && opts->form != X3D_FORM_WAVEFRONT)
         return fail(X3D_ERR_INVALID_ARGUMENT, "unknown render form " + std::to_string(opts->form));
     if (opts->form == X3D_FORM_WAVEFRONT && opts->pt_type != X3D_PT_UNIDIRECT_MIS &&
-        opts->pt_type != X3D_PT_UNIDIRECT_LT1 && opts->pt_type != X3D_PT_UNIDIRECT)
-        return fail(X3D_ERR_UNSUPPORTED,
-                    "the wavefront form runs the UNIDIRECT_MIS, UNIDIRECT_LT1 and UNIDIRECT tracers only");
+        opts->pt_type != X3D_PT_UNIDIRECT_LT1 && opts->pt_type != X3D_PT_UNIDIRECT &&
+        opts->pt_type != X3D_PT_BIDIRECT_MIS)
+        return fail(X3D_ERR_UNSUPPORTED, "the wavefront form runs the UNIDIRECT_MIS, UNIDIRECT_LT1, UNIDIRECT and "
+                                         "BIDIRECT_MIS tracers only");
     if (opts->form == X3D_FORM_WAVEFRONT && opts->pt_type != X3D_PT_UNIDIRECT_MIS &&
         (opts->sort_key == X3D_SORT_MATERIAL || opts->ray_order == X3D_RAY_ORDER_SORTED))
         return fail(X3D_ERR_UNSUPPORTED, "material and ray sorting apply to the wavefront UNIDIRECT_MIS tracer only");

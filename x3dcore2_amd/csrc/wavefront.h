@@ -45,4 +45,17 @@ void wf_last_timing(WavefrontEngine const *e, double *shadow_ms, double *extend_
 // samples; writes P.out (P.finalize) or P.acc. Returns an empty string or an error message.
 std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, hipStream_t stream);
 
+// Wavefront form of the bidirectional tracer. Working memory per (pixel, sample) item of a
+// generation for subpaths of up to max_len vertices (bd_item_bytes), and the largest generation
+// (items) that fits the form's budget.
+constexpr uint64_t kBdBudgetBytes = 24ull << 30;
+uint64_t bd_item_bytes(uint32_t max_len);
+uint32_t bd_max_items(uint32_t max_len);
+// One generation: samples [s_first, s_first + gen_samples) of every pixel of a render whose
+// first-hit pass has filled P.first; sample radiance of item j (pixel j % npix, sample
+// s_first + j / npix) to samples[j], bit-identical to k_render_bdpt's. first_gen restarts the
+// profiled traversal timings. Returns an empty string or an error message.
+std::string bd_generation(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, uint32_t s_first,
+                          uint32_t gen_samples, float4 *samples, bool first_gen, hipStream_t stream);
+
 } // namespace x3d

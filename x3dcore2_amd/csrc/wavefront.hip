@@ -88,6 +88,8 @@ struct WfArgs {
     int fused;
     // lt1: the first hit's emission per path, added when the path ends (w = 1 when there is one).
     float4 *le0;
+    // Bidirectional form: the shadow kernel writes each connection ray's visibility here.
+    uint8_t *vis;
 };
 
 // Ray keys: direction octant (3 bits) above the Morton code of the origin's cell in an 8^3 grid
@@ -559,7 +561,13 @@ __global__ __launch_bounds__(kSortWG) void k_wf_sort_scatter(WfArgs A) {
     }
 }
 
-template <int WG, int LDSD>
+// Persistent any-hit traversal over the shadow queue. MIS / lt1 rays: 3 x float4 (o, tmax),
+// (d, path), (contribution, -), tmin 1e-4; an unoccluded ray adds its contribution to the path's
+// radiance. BD (bidirectional connections): 2 x float4 (o, tmax), (d, slot | kBdLongTmin); the
+// visibility (1 = unoccluded) goes to vis[slot & ~kBdLongTmin]; tmin is 1e-3 where the flag is
+// set (subpath connections), else 1e-4 (light samples).
+constexpr uint32_t kBdLongTmin = 0x80000000u;
+template <int WG, int LDSD, bool BD = false>
 __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     __shared__ uint2 lds[LDSD * WG];
     __shared__ uint32_t cursor[2 * (WG / 64)];
@@ -582,14 +590,17 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     }
     uint32_t i = wave_fetch_batched(&A.ctl->sq_head.v, slot);
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
-    float tmax = 0.0f;
+    float tmin = 1e-4f, tmax = 0.0f;
     uint32_t ref = kDone, leaf = kDone, cur = 0; // cur: next triangle of the parked leaf
     bool blocked = false, wide = false;
+    constexpr uint32_t kStride = BD ? 2u : 3u;
     auto init = [&](uint32_t k) {
-        float4 a = A.sq_in[3 * k], b = A.sq_in[3 * k + 1];
+        float4 a = A.sq_in[kStride * k], b = A.sq_in[kStride * k + 1];
         o = xyz(a);
         tmax = a.w;
         v = xyz(b);
+        if (BD)
+            tmin = (__float_as_uint(b.w) & kBdLongTmin) ? 1e-3f : 1e-4f;
         vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
         wide = !slab_may_nan(o, vinv, S.far_coords != 0);
         blocked = false;
@@ -608,10 +619,10 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
         while (is_interior(ref)) {
             if (wide) {
                 pops += kTreeWidth / 2; // a wide node counts as W/2 binary nodes (same bytes and tests)
-                ref = visit_wide<false>(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st, top);
+                ref = visit_wide<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, top);
             } else {
                 pops++;
-                ref = visit_interior<false>(S, ref, o, vinv, 1e-4f, tmax, __builtin_inff(), st, true);
+                ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, true);
             }
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
@@ -630,7 +641,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
                 float4 a = tris[3 * p], b = tris[3 * p + 1], c = tris[3 * p + 2];
                 float c1, c2, t0;
                 tests++;
-                blocked = tri_hit(o, v, xyz(a), xyz(b), xyz(c), 1e-4f, tmax, &c1, &c2, &t0);
+                blocked = tri_hit(o, v, xyz(a), xyz(b), xyz(c), tmin, tmax, &c1, &c2, &t0);
             }
             cur++;
             if (blocked) {
@@ -650,7 +661,9 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
                 break;
         }
         if (ref == kDone && leaf == kDone) {
-            if (!blocked) {
+            if (BD) {
+                A.vis[__float_as_uint(A.sq_in[2 * i + 1].w) & ~kBdLongTmin] = blocked ? 0 : 1;
+            } else if (!blocked) {
                 uint32_t path = __float_as_uint(A.sq_in[3 * i + 1].w);
                 float4 c = A.sq_in[3 * i + 2];
                 float4 L = A.L[path];
@@ -917,6 +930,405 @@ __global__ void k_wf_count(WfCtl *ctl, WfCounter const *eq_cnt, Counters *cnt) {
     }
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Wavefront form of the bidirectional tracer (Tracer::li_bdpt). One generation holds n (pixel,
+// sample) items, item j = pixel j % npix, sample s_first + j / npix, as in k_render_bdpt:
+//   bd_start    camera vertex 0 from the first-hit cache, first BSDF sample of the camera walk
+//   extend      closest hits of the queued walk rays (the MIS form's traversal kernel)
+//   bd_shade    per hit: the next camera or light vertex and the walk's next BSDF sample; when
+//               a sample's camera walk ends, its emission sample and emission ray (the light
+//               walk), as li_bdpt draws them from the same PCG stream
+//   bd_connect  per (item, strategy): the shadow ray of each light-sample and subpath
+//               connection strategy that passes its geometric test
+//   shadow      any-hit traversal of those rays (visibility per strategy)
+//   bd_accumulate per item: li_bdpt's sums over path lengths and strategies, in its order and
+//               with its float expressions, from the stored vertices and visibilities
+// Subpath vertices live in HBM as structure-of-arrays ([vertex][item] float4 rows: position and
+// material, normal, direction to the previous vertex, prefix transport) instead of 1680 B of
+// scratch per lane. Each item draws the same numbers from its PCG stream and evaluates the same
+// expressions as li_bdpt, so the sample radiance is bit-identical to the plain kernel's.
+// ---------------------------------------------------------------------------------------------
+constexpr uint32_t kBdCam = 0, kBdLight0 = 1, kBdLight = 2, kBdDone = 3;
+constexpr uint32_t kBdDead = 1u << 20;
+X3D_DI uint32_t bd_cam_len(uint32_t s) { return s & 31u; }
+X3D_DI uint32_t bd_light_len(uint32_t s) { return (s >> 8) & 31u; }
+X3D_DI uint32_t bd_phase(uint32_t s) { return (s >> 16) & 3u; }
+X3D_DI uint32_t bd_state(uint32_t cl, uint32_t ll, uint32_t ph) { return cl | (ll << 8) | (ph << 16); }
+
+struct BdArgs {
+    DevScene S;
+    CamParams cam;
+    uint32_t width, height, npix, n;
+    uint32_t s_first, max_len, nstrat; // nstrat = (max_len + 1)^2 strategy slots per item
+    uint64_t seed;
+    const float4 *first; // first-hit cache (RenderParams::first)
+    uint32_t *state;     // per item: lengths, phase, dead flag
+    uint64_t *rng;       // per item: PCG state (the stream's increment follows from the pixel)
+    float4 *ray_o, *ray_d; // per item: the walk's queued ray (d.w = its BSDF density)
+    float4 *em0, *em1, *emis; // per item: emission sample (p, area_dens), (n, light | sky << 31),
+                              // light_emission of li_bdpt
+    float4 *cv[4], *lv[4];    // camera / light vertices [k * n + j]: (p, mat), n, v, transport
+    uint32_t *eq_in, *eq_out;
+    WfCounter *eq_cnt_in, *eq_cnt_out;
+    const ClosestHit *eh;
+    float4 *sq;          // shadow rays, 2 x float4
+    WfCounter *sq_count;
+    uint8_t *vis;        // [strategy * n + j]
+    float4 *samples;     // sample radiance [j]
+};
+
+struct BdVert {
+    f3 p, n, v, t;
+    int32_t mat;
+};
+
+X3D_DI BdVert bd_load(float4 *const *a, uint32_t n, uint32_t k, uint32_t j) {
+    size_t q = static_cast<size_t>(k) * n + j;
+    float4 p = a[0][q], nn = a[1][q], v = a[2][q], t = a[3][q];
+    return BdVert{xyz(p), xyz(nn), xyz(v), xyz(t), static_cast<int32_t>(__float_as_uint(p.w))};
+}
+X3D_DI void bd_store(float4 *const *a, uint32_t n, uint32_t k, uint32_t j, f3 p, int32_t mat, f3 nn, f3 v, f3 t) {
+    size_t q = static_cast<size_t>(k) * n + j;
+    a[0][q] = make_float4(p.x, p.y, p.z, __uint_as_float(static_cast<uint32_t>(mat)));
+    a[1][q] = make_float4(nn.x, nn.y, nn.z, 0.0f);
+    a[2][q] = make_float4(v.x, v.y, v.z, 0.0f);
+    a[3][q] = make_float4(t.x, t.y, t.z, 0.0f);
+}
+
+X3D_DI LightSample bd_em(BdArgs const &B, uint32_t j) {
+    float4 a = B.em0[j], b = B.em1[j];
+    uint32_t w = __float_as_uint(b.w);
+    LightSample ls;
+    ls.p = xyz(a);
+    ls.area_dens = a.w;
+    ls.n = xyz(b);
+    ls.light = static_cast<int32_t>(w & 0x7FFFFFFFu);
+    ls.sky = (w >> 31) != 0;
+    return ls;
+}
+
+// The walk of item j after its last vertex: the next BSDF sample of the current subpath
+// (extend_walk's loop; `ended`: the walk already stopped at a miss or a back-facing hit), or,
+// once the camera walk is over, the emission sample that starts the light walk; once the light
+// walk is over, the item is done. Returns true when a ray is queued (ray_o / ray_d written) and
+// stores the new state and PCG state.
+X3D_DI bool bd_advance(BdArgs const &B, uint32_t j, Pcg &g, uint32_t cl, uint32_t ll, uint32_t ph, bool ended) {
+    DevScene const &S = B.S;
+    bool queued = false;
+    if (ended) {
+        // no further BSDF sample on this subpath
+    } else if (ph == kBdCam && cl < B.max_len) {
+        BdVert cur = bd_load(B.cv, B.n, cl - 1, j);
+        float dens;
+        f3 i = bsdf_sample(S.mats, cur.mat, cur.n, cur.v, g, &dens, true);
+        if (!near0(dens)) {
+            B.ray_o[j] = make_float4(cur.p.x, cur.p.y, cur.p.z, 0.0f);
+            B.ray_d[j] = make_float4(i.x, i.y, i.z, dens);
+            queued = true;
+        }
+    } else if (ph == kBdLight && ll < B.max_len) {
+        BdVert cur = bd_load(B.lv, B.n, ll - 1, j);
+        float dens;
+        f3 i = bsdf_sample(S.mats, cur.mat, cur.n, cur.v, g, &dens, true);
+        if (!near0(dens)) {
+            B.ray_o[j] = make_float4(cur.p.x, cur.p.y, cur.p.z, 0.0f);
+            B.ray_d[j] = make_float4(i.x, i.y, i.z, dens);
+            queued = true;
+        }
+    }
+    if (!queued && ph == kBdCam) {
+        // camera walk over: emission sample (li_bdpt after extend_walk<false>)
+        LightSample em = sample_light(S, g);
+        float e0 = g.draw();
+        float e1 = g.draw();
+        float w_dens;
+        f3 w = emission_dir(em, e0, e1, w_dens);
+        float cos_e = dot(em.n, w);
+        f3 emitted = cos_e > 0.0f ? light_rad(S, em.light) * cos_e : mk(0.0f, 0.0f, 0.0f);
+        f3 light_emission = em.area_dens > 0.0f ? emitted / (w_dens * em.area_dens) : mk(0.0f, 0.0f, 0.0f);
+        B.em0[j] = make_float4(em.p.x, em.p.y, em.p.z, em.area_dens);
+        B.em1[j] = make_float4(em.n.x, em.n.y, em.n.z,
+                               __uint_as_float(static_cast<uint32_t>(em.light) | (em.sky ? 0x80000000u : 0u)));
+        B.emis[j] = make_float4(light_emission.x, light_emission.y, light_emission.z, 0.0f);
+        B.ray_o[j] = make_float4(em.p.x, em.p.y, em.p.z, 0.0f);
+        B.ray_d[j] = make_float4(w.x, w.y, w.z, 0.0f);
+        ph = kBdLight0;
+        queued = true;
+    } else if (!queued) {
+        ph = kBdDone;
+    }
+    B.state[j] = bd_state(cl, ll, ph);
+    B.rng[j] = g.state;
+    return queued;
+}
+
+__global__ __launch_bounds__(256) void k_bd_start(BdArgs B) {
+    __shared__ uint32_t scratch[2];
+    clear_scratch(scratch, 2);
+    DevScene const &S = B.S;
+    for (uint32_t b = blockIdx.x * blockDim.x; b < B.n; b += gridDim.x * blockDim.x) {
+        const uint32_t j = b + threadIdx.x;
+        bool queued = false;
+        if (j < B.n) {
+            const uint32_t pix = j % B.npix, s = B.s_first + j / B.npix;
+            f3 o, v;
+            camera_ray(B.cam, pix % B.width, pix / B.width, B.width, B.height, o, v);
+            float4 fa = B.first[3 * pix], fb = B.first[3 * pix + 1], fc = B.first[3 * pix + 2];
+            const bool valid = fc.z != 0.0f;
+            if (!valid || B.max_len == 0 || S.num_lights == 0) {
+                B.state[j] = kBdDead;
+            } else {
+                Pcg g = pcg_for(B.seed, pix, s);
+                bd_store(B.cv, B.n, 0, j, xyz(fa), static_cast<int32_t>(__float_as_uint(fc.x)), xyz(fb), -v,
+                         mk(1.0f, 1.0f, 1.0f));
+                queued = bd_advance(B, j, g, 1, 0, kBdCam, false);
+            }
+        }
+        uint32_t *const c[1] = {&B.eq_cnt_out->v};
+        const bool p[1] = {queued};
+        uint32_t idx[1];
+        block_append<1>(c, p, idx, scratch);
+        if (queued)
+            B.eq_out[idx[0]] = j;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bd_shade(BdArgs B) {
+    __shared__ uint32_t scratch[2];
+    clear_scratch(scratch, 2);
+    DevScene const &S = B.S;
+    const uint32_t total = B.eq_cnt_in->v;
+    for (uint32_t b = blockIdx.x * blockDim.x; b < total; b += gridDim.x * blockDim.x) {
+        const uint32_t t = b + threadIdx.x;
+        bool queued = false;
+        uint32_t j = 0;
+        if (t < total) {
+            j = B.eq_in[t];
+            const uint32_t st = B.state[j], pix = j % B.npix;
+            uint32_t cl = bd_cam_len(st), ll = bd_light_len(st), ph = bd_phase(st);
+            Pcg g = path_pcg(B.rng[j], pix);
+            const ClosestHit h = B.eh[t];
+            const float4 rd = B.ray_d[j];
+            const f3 i = xyz(rd);
+            HitRec nx;
+            bool hit = h.prim >= 0;
+            if (hit) {
+                nx = finish_hit(S, h);
+                hit = dot(nx.n, -i) > 0.0f;
+            }
+            if (ph == kBdLight0) {
+                // the emission ray: the light walk's first vertex, if it hits a surface facing it
+                if (hit) {
+                    bd_store(B.lv, B.n, 0, j, nx.p, nx.mat, nx.n, -i, mk(1.0f, 1.0f, 1.0f));
+                    ll = 1;
+                    ph = kBdLight;
+                } else {
+                    ph = kBdDone;
+                }
+            } else if (hit) {
+                const bool cam = ph == kBdCam;
+                const uint32_t k = cam ? cl : ll;
+                BdVert cur = bd_load(cam ? B.cv : B.lv, B.n, k - 1, j);
+                float cos_w = dot(cur.n, i);
+                // extend_walk: projected_adjoint_brdf on the camera walk, projected_brdf on the
+                // light walk
+                f3 f = cam ? bsdf_eval(S.mats, cur.mat, cur.n, cur.v, i) : bsdf_eval(S.mats, cur.mat, cur.n, i, cur.v);
+                bd_store(cam ? B.cv : B.lv, B.n, k, j, nx.p, nx.mat, nx.n, -i, mul(cur.t, (f * cos_w) / rd.w));
+                if (cam)
+                    cl++;
+                else
+                    ll++;
+            }
+            if (ph == kBdDone) {
+                B.state[j] = bd_state(cl, ll, kBdDone);
+            } else {
+                // a camera walk that stopped here goes on to its emission sample; a light walk
+                // that stopped here is done
+                queued = bd_advance(B, j, g, cl, ll, ph, !hit);
+            }
+        }
+        uint32_t *const c[1] = {&B.eq_cnt_out->v};
+        const bool p[1] = {queued};
+        uint32_t idx[1];
+        block_append<1>(c, p, idx, scratch);
+        if (queued)
+            B.eq_out[idx[0]] = j;
+    }
+}
+
+// Reserves `cnt` consecutive entries of a queue for each thread of the block: one global atomic
+// per block (wave-level inclusive scans, then the waves' totals in LDS). Every thread of the
+// block must call it; `scratch` holds WG / 64 + 1 words. Returns the thread's first entry.
+template <int WG> X3D_DI uint32_t block_reserve(uint32_t cnt, uint32_t *counter, uint32_t *scratch) {
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint32_t incl = cnt;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint32_t up = __shfl_up(incl, d, 64);
+        if (lane >= d)
+            incl += up;
+    }
+    if (lane == 63)
+        scratch[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < WG / 64; w++) {
+            uint32_t c = scratch[w];
+            scratch[w] = tot;
+            tot += c;
+        }
+        scratch[WG / 64] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = scratch[WG / 64] + scratch[wave] + incl - cnt;
+    __syncthreads(); // scratch is reused by the next call
+    return base;
+}
+
+// Shadow rays of item j's strategies, in li_bdpt's tests: light-sample strategies (lp = 0:
+// Tracer::direct's ray from camera vertex cp - 1 towards the emission sample,
+// [1e-4, dist - 1e-3]) and subpath connections (lj -> cj, [1e-3, dist - 1e-3]) whose cosines
+// are both positive. WRITE = false counts them; WRITE = true stores them from queue entry `at`
+// on, tagged with their strategy slot k = cp * (max_len + 1) + lp (visibility index k * n + j).
+template <bool WRITE> X3D_DI uint32_t bd_connections(BdArgs const &B, uint32_t j, uint32_t st, uint32_t at) {
+    DevScene const &S = B.S;
+    const uint32_t side = B.max_len + 1, cam_len = bd_cam_len(st), light_len = bd_light_len(st);
+    const LightSample em = bd_em(B, j);
+    uint32_t cnt = 0;
+    auto emit = [&](f3 o, f3 d, float tmax, uint32_t slot) {
+        if (WRITE) {
+            const size_t e = 2 * static_cast<size_t>(at + cnt);
+            B.sq[e] = make_float4(o.x, o.y, o.z, tmax);
+            B.sq[e + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
+        }
+        cnt++;
+    };
+    for (uint32_t cp = 1; cp <= cam_len; cp++) {
+        const size_t qc = static_cast<size_t>(cp - 1) * B.n + j;
+        const f3 cpos = xyz(B.cv[0][qc]), cnrm = xyz(B.cv[1][qc]);
+        f3 l, illum;
+        float r2;
+        if (light_arrives(S, em, cpos, cnrm, illum, l, r2)) {
+            float dist = norm(l);
+            emit(cpos, -l / dist, dist - 1e-3f, (cp * side) * B.n + j);
+        }
+        for (uint32_t lp = 1; lp <= light_len; lp++) {
+            const size_t ql = static_cast<size_t>(lp - 1) * B.n + j;
+            const f3 lpos = xyz(B.lv[0][ql]), lnrm = xyz(B.lv[1][ql]);
+            f3 join = cpos - lpos;
+            float dist = norm(join);
+            join = join / dist;
+            float cos_wo = dot(lnrm, join);
+            float cos_wi = dot(cnrm, -join);
+            if (cos_wo > 0.0f && cos_wi > 0.0f)
+                emit(lpos, join, dist - 1e-3f, ((cp * side + lp) * B.n + j) | kBdLongTmin);
+        }
+    }
+    return cnt;
+}
+
+// Queues every item's shadow rays (counted, one reservation per block, then written).
+__global__ __launch_bounds__(256) void k_bd_connect(BdArgs B) {
+    __shared__ uint32_t scratch[256 / 64 + 1];
+    for (uint32_t b = blockIdx.x * blockDim.x; b < B.n; b += gridDim.x * blockDim.x) {
+        const uint32_t j = b + threadIdx.x;
+        uint32_t st = kBdDead;
+        if (j < B.n)
+            st = B.state[j];
+        const uint32_t cnt = (st & kBdDead) ? 0u : bd_connections<false>(B, j, st, 0);
+        const uint32_t at = block_reserve<256>(cnt, &B.sq_count->v, scratch);
+        if (cnt)
+            bd_connections<true>(B, j, st, at);
+    }
+}
+
+// li_bdpt's transport_all_connectible_subpaths for each item, with the visibilities the shadow
+// kernel found; the sample's radiance goes to samples[j].
+__global__ __launch_bounds__(256) void k_bd_accumulate(BdArgs B) {
+    DevScene const &S = B.S;
+    const uint32_t side = B.max_len + 1;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < B.n; j += gridDim.x * blockDim.x) {
+        const uint32_t st = B.state[j];
+        f3 rad = mk(0.0f, 0.0f, 0.0f);
+        if (!(st & kBdDead)) {
+            const uint32_t cam_len = bd_cam_len(st), light_len = bd_light_len(st), pix = j % B.npix;
+            const int32_t first_light = static_cast<int32_t>(__float_as_uint(B.first[3 * pix + 2].y));
+            const LightSample em = bd_em(B, j);
+            const f3 light_emission = xyz(B.emis[j]);
+            for (uint32_t plen = 1; plen <= cam_len + light_len + 1; plen++) {
+                int cp = static_cast<int>(plen - 1 < cam_len ? plen - 1 : cam_len);
+                uint32_t lp = plen - 1 - static_cast<uint32_t>(cp);
+                f3 psum = mk(0.0f, 0.0f, 0.0f);
+                float wsum = 0.0f;
+                for (; cp >= 0 && lp <= light_len; lp++, cp--) {
+                    const size_t slot = static_cast<size_t>(static_cast<uint32_t>(cp) * side + lp) * B.n + j;
+                    if (lp == 0 && cp == 0) {
+                        if (first_light >= 0) {
+                            BdVert c0 = bd_load(B.cv, B.n, 0, j);
+                            psum = psum + light_radiance(S, first_light, c0.v, c0.n);
+                        }
+                        wsum = wsum + 1.0f;
+                    } else if (lp == 0) {
+                        BdVert cj = bd_load(B.cv, B.n, static_cast<uint32_t>(cp) - 1, j);
+                        f3 tr = mk(0.0f, 0.0f, 0.0f), l, illum;
+                        float r2;
+                        if (light_arrives(S, em, cj.p, cj.n, illum, l, r2) && B.vis[slot]) {
+                            float dist = norm(l);
+                            f3 wi = -l / dist;
+                            f3 f = bsdf_eval(S.mats, cj.mat, cj.n, cj.v, wi);
+                            tr = mul(illum, f) / em.area_dens;
+                        }
+                        psum = psum + mul(tr, cj.t);
+                        wsum = wsum + 1.0f;
+                    } else if (cp > 0) {
+                        BdVert lj = bd_load(B.lv, B.n, lp - 1, j);
+                        BdVert cj = bd_load(B.cv, B.n, static_cast<uint32_t>(cp) - 1, j);
+                        f3 join = cj.p - lj.p;
+                        float dist = norm(join);
+                        join = join / dist;
+                        float cos_wo = dot(lj.n, join);
+                        float cos_wi = dot(cj.n, -join);
+                        if (cos_wo > 0.0f && cos_wi > 0.0f && B.vis[slot]) {
+                            f3 sub = mul(light_emission, lj.t);
+                            float geo = cos_wi * cos_wo / (dist * dist);
+                            f3 fl = bsdf_eval(S.mats, lj.mat, lj.n, join, lj.v);
+                            f3 fc = bsdf_eval(S.mats, cj.mat, cj.n, cj.v, -join);
+                            f3 tr = mul(mul(sub, fl), fc) * geo;
+                            psum = psum + mul(tr, cj.t);
+                        }
+                        wsum = wsum + 1.0f;
+                    }
+                }
+                if (wsum > 0.0f)
+                    rad = rad + psum / wsum;
+            }
+        }
+        B.samples[j] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+    }
+}
+
+// Before each walk bounce's shade: the traversal cursor restarts and the next bounce's queue
+// starts empty. Before the connections: the shadow queue and its cursor start empty.
+__global__ void k_bd_reset(WfCtl *ctl, WfCounter *eq_out, int shadow) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->eq_head.v = 0;
+        if (eq_out)
+            eq_out->v = 0;
+        if (shadow) {
+            ctl->sq_count.v = 0;
+            ctl->sq_head.v = 0;
+        }
+    }
+}
+
+// Query counts of the walk bounce (closest hits) or of the connections (any hits).
+__global__ void k_bd_count(WfCounter const *c, Counters *cnt, int anyhit) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && cnt)
+        atomicAdd(anyhit ? &cnt->anyhit : &cnt->closest, static_cast<unsigned long long>(c->v));
+}
+
 // ---------------------------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------------------------
@@ -925,6 +1337,9 @@ struct WavefrontEngine {
     DevBuf eq_b, fin_b; // the fused form's second queue parity
     DevBuf le0;         // lt1's first-hit emission per path
     DevBuf sq_sorted, eq_sorted, ray_key, ray_rank, bin_mat, bin_tot; // ray reordering
+    // bidirectional form: per item state, PCG state, walk ray, emission sample, vertices,
+    // visibilities
+    DevBuf bd_state, bd_rng, bd_ro, bd_rd, bd_em, bd_cv, bd_lv, bd_vis;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     WfArgs key{};
@@ -967,11 +1382,13 @@ template <class K> uint32_t grid_for(K kernel, int wg) {
 using TraceKernel = void (*)(WfArgs);
 
 struct TraceKernels {
-    TraceKernel shadow, extend;
+    TraceKernel shadow, extend, bd_shadow;
     int wg, ldsd;
 };
 
-template <int WG, int D> TraceKernels trace_kernels() { return {k_wf_shadow<WG, D>, k_wf_extend<WG, D>, WG, D}; }
+template <int WG, int D> TraceKernels trace_kernels() {
+    return {k_wf_shadow<WG, D>, k_wf_extend<WG, D>, k_wf_shadow<WG, D, true>, WG, D};
+}
 
 // Workgroup sizes 64 / 128 / 256 and LDS stack depths 8 / 16 (deeper entries spill to memory).
 TraceKernels pick_trace(int wg, int ldsd) {
@@ -991,6 +1408,162 @@ bool same_config(WfArgs const &a, WfArgs const &b, WavefrontTuning const &ta, Wa
 }
 
 } // namespace
+
+
+uint64_t bd_item_bytes(uint32_t max_len) {
+    const uint64_t m = max_len, strat = (m + 1) * (m + 1);
+    return 4 + 8 + 5 * 16          // state, PCG state, walk ray, emission sample
+           + 2 * 4 * 16 * m        // camera and light vertices
+           + 2 * 4 + sizeof(ClosestHit) // both queue parities, hit records
+           + 2 * 16 * (m + m * m)  // shadow rays (every light-sample and connection strategy)
+           + strat                 // visibilities
+           + 16;                   // the sample radiance (render's sample buffer)
+}
+
+uint32_t bd_max_items(uint32_t max_len) {
+    uint64_t cap = kBdBudgetBytes / bd_item_bytes(max_len);
+    const uint64_t strat = static_cast<uint64_t>(max_len + 1) * (max_len + 1);
+    const uint64_t slot_cap = (static_cast<uint64_t>(1) << 31) / strat; // slot ids below kBdLongTmin
+    cap = cap < slot_cap ? cap : slot_cap;
+    return static_cast<uint32_t>(cap < (1u << 26) ? cap : (1u << 26));
+}
+
+std::string bd_generation(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, uint32_t s_first,
+                          uint32_t gen_samples, float4 *samples, bool first_gen, hipStream_t stream) {
+#define BD_HIP(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t err_ = (call);                                                                    \
+        if (err_ != hipSuccess)                                                                      \
+            return std::string(#call) + ": " + hipGetErrorString(err_);                              \
+    } while (0)
+    const uint32_t npix = P.width * P.height, m = P.max_path_len;
+    const uint64_t n64 = static_cast<uint64_t>(gen_samples) * npix;
+    if (n64 == 0)
+        return "";
+    if (m < 1 || m > kMaxPathLen || n64 > bd_max_items(m))
+        return "bidirectional generation of " + std::to_string(n64) + " items exceeds its working memory";
+    const uint32_t n = static_cast<uint32_t>(n64), side = m + 1, nstrat = side * side;
+    const size_t f4 = sizeof(float4) * static_cast<size_t>(n);
+    BD_HIP(e->bd_state.ensure(sizeof(uint32_t) * static_cast<size_t>(n)));
+    BD_HIP(e->bd_rng.ensure(sizeof(uint64_t) * static_cast<size_t>(n)));
+    BD_HIP(e->bd_ro.ensure(f4));
+    BD_HIP(e->bd_rd.ensure(f4));
+    BD_HIP(e->bd_em.ensure(3 * f4));
+    BD_HIP(e->bd_cv.ensure(4 * m * f4));
+    BD_HIP(e->bd_lv.ensure(4 * m * f4));
+    BD_HIP(e->bd_vis.ensure(static_cast<size_t>(n) * nstrat));
+    BD_HIP(e->eq.ensure(sizeof(uint32_t) * static_cast<size_t>(n)));
+    BD_HIP(e->eq_b.ensure(sizeof(uint32_t) * static_cast<size_t>(n)));
+    BD_HIP(e->eh.ensure(sizeof(ClosestHit) * static_cast<size_t>(n)));
+    BD_HIP(e->sq.ensure(2 * f4 * (m + static_cast<size_t>(m) * m)));
+    BD_HIP(e->ctl.ensure(sizeof(WfCtl)));
+    TraceKernels tk = pick_trace(tune.trace_wg, tune.stack_depth);
+    const int wg = tk.wg;
+    const uint32_t g_shadow = grid_for(tk.bd_shadow, wg), g_extend = grid_for(tk.extend, wg);
+    const uint32_t trace_threads = (g_shadow > g_extend ? g_shadow : g_extend) * wg;
+    const uint32_t spill_per_thread = spill_entries_for(P.S, tk.ldsd);
+    BD_HIP(e->spill.ensure(sizeof(uint2) * static_cast<size_t>(trace_threads) * spill_per_thread));
+
+    WfCtl *ctl = static_cast<WfCtl *>(e->ctl.p);
+    uint32_t *qs[2] = {static_cast<uint32_t *>(e->eq.p), static_cast<uint32_t *>(e->eq_b.p)};
+    WfCounter *qc[2] = {&ctl->eq_count, &ctl->eq_count_b};
+
+    BdArgs B{};
+    B.S = P.S;
+    B.cam = P.cam;
+    B.width = P.width;
+    B.height = P.height;
+    B.npix = npix;
+    B.n = n;
+    B.s_first = s_first;
+    B.max_len = m;
+    B.nstrat = nstrat;
+    B.seed = P.seed;
+    B.first = P.first;
+    B.state = static_cast<uint32_t *>(e->bd_state.p);
+    B.rng = static_cast<uint64_t *>(e->bd_rng.p);
+    B.ray_o = static_cast<float4 *>(e->bd_ro.p);
+    B.ray_d = static_cast<float4 *>(e->bd_rd.p);
+    B.em0 = static_cast<float4 *>(e->bd_em.p);
+    B.em1 = B.em0 + n;
+    B.emis = B.em1 + n;
+    for (int a = 0; a < 4; a++) {
+        B.cv[a] = static_cast<float4 *>(e->bd_cv.p) + static_cast<size_t>(a) * m * n;
+        B.lv[a] = static_cast<float4 *>(e->bd_lv.p) + static_cast<size_t>(a) * m * n;
+    }
+    B.eh = static_cast<const ClosestHit *>(e->eh.p);
+    B.sq = static_cast<float4 *>(e->sq.p);
+    B.sq_count = &ctl->sq_count;
+    B.vis = static_cast<uint8_t *>(e->bd_vis.p);
+    B.samples = samples;
+
+    WfArgs X{};
+    X.S = P.S;
+    X.ctl = ctl;
+    X.spill = static_cast<uint2 *>(e->spill.p);
+    X.spill_per_thread = spill_per_thread;
+    X.cnt = P.cnt;
+    X.refill_below = static_cast<uint32_t>(tune.refill_below);
+    X.leaf_below = static_cast<uint32_t>(tune.leaf_below);
+    X.xp = B.ray_o;
+    X.wi = B.ray_d;
+    X.eh = static_cast<ClosestHit *>(e->eh.p);
+    X.sq_in = B.sq;
+    X.vis = B.vis;
+
+    const uint32_t g_item = grid_for(k_bd_shade, 256), g_start = grid_for(k_bd_start, 256),
+                   g_conn = grid_for(k_bd_connect, 256), g_acc = grid_for(k_bd_accumulate, 256);
+    const bool profile = tune.profile != 0;
+    if (first_gen) {
+        e->shadow_ms = e->extend_ms = 0.0;
+        e->trace_launches = 0;
+    }
+    if (profile && !e->ev[0])
+        for (auto &ev : e->ev)
+            BD_HIP(hipEventCreate(&ev));
+    auto timed = [&](TraceKernel k, uint32_t grid, WfArgs const &A, double &acc) {
+        if (profile)
+            (void)hipEventRecord(e->ev[0], stream);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(wg), 0, stream, A);
+        if (profile) {
+            (void)hipEventRecord(e->ev[1], stream);
+            float ms = 0;
+            (void)hipEventSynchronize(e->ev[1]);
+            (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
+            acc += ms;
+        }
+    };
+    // walks: at most max_len - 1 camera extensions, then the emission ray and max_len - 1 light
+    // extensions
+    const uint32_t bounces = 2 * m - 1;
+    hipLaunchKernelGGL(k_bd_reset, dim3(1), dim3(64), 0, stream, ctl, qc[0], 1);
+    B.eq_out = qs[0];
+    B.eq_cnt_out = qc[0];
+    hipLaunchKernelGGL(k_bd_start, dim3(g_start), dim3(256), 0, stream, B);
+    for (uint32_t k = 0; k < bounces; k++) {
+        const uint32_t p = k & 1u, q = p ^ 1u;
+        X.eq_in = qs[p];
+        X.eq_cnt_in = qc[p];
+        timed(tk.extend, g_extend, X, e->extend_ms);
+        hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, stream, static_cast<const WfCounter *>(qc[p]), P.cnt, 0);
+        hipLaunchKernelGGL(k_bd_reset, dim3(1), dim3(64), 0, stream, ctl, qc[q], 0);
+        B.eq_in = qs[p];
+        B.eq_cnt_in = qc[p];
+        B.eq_out = qs[q];
+        B.eq_cnt_out = qc[q];
+        hipLaunchKernelGGL(k_bd_shade, dim3(g_item), dim3(256), 0, stream, B);
+    }
+    hipLaunchKernelGGL(k_bd_reset, dim3(1), dim3(64), 0, stream, ctl, static_cast<WfCounter *>(nullptr), 1);
+    hipLaunchKernelGGL(k_bd_connect, dim3(g_conn), dim3(256), 0, stream, B);
+    timed(tk.bd_shadow, g_shadow, X, e->shadow_ms);
+    hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, stream, static_cast<const WfCounter *>(&ctl->sq_count), P.cnt, 1);
+    hipLaunchKernelGGL(k_bd_accumulate, dim3(g_acc), dim3(256), 0, stream, B);
+    if (profile)
+        e->trace_launches += bounces + 1;
+    BD_HIP(hipGetLastError());
+    return "";
+#undef BD_HIP
+}
 
 std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, hipStream_t stream) {
 #define WF_HIP(call)                                                                                 \

@@ -174,7 +174,8 @@ def traffic_from_profile(name, kernel, config):
 def leg_bdpt(x3d, steps, warmup, spp=0, cpu=True, budget_s=8.0):
     """The bidirectional tracer with the reference's uniform strategy weights on the indirectly lit
     room with four 20k-triangle GGX spheres, 512 x 512, 32 spp, subpaths of up to 6 vertices (film in
-    HBM); beside it the reference's own bidirect_mis_path_tracer (oracle/_ref, its fixed 8-vertex
+    HBM), in the wavefront form (the default) and the plain form, whose films must be identical;
+    beside it the reference's own bidirect_mis_path_tracer (oracle/_ref, its fixed 8-vertex
     subpaths) on the host cores at a reduced resolution sized to ~budget_s."""
     from x3dcore2_amd import scenes
     from x3dcore2_amd.device import DeviceArray, synchronize
@@ -182,20 +183,31 @@ def leg_bdpt(x3d, steps, warmup, spp=0, cpu=True, budget_s=8.0):
     desc = scenes.indirect_scene()
     scene = x3d.Scene(desc)
     w, h, spp = desc.width, desc.height, spp or desc.spp
-    r = x3d.Renderer(x3d.PT_BIDIRECT_MIS, max_path_len=6, seed=1)
-    film = DeviceArray(w * h * 16)
-    for _ in range(warmup):
-        r.render_device(scene, desc.camera, w, h, spp, film.ptr)
-    synchronize()
-    t0 = time.perf_counter()
-    stats = [r.render_device(scene, desc.camera, w, h, spp, film.ptr) for _ in range(steps)]
-    synchronize()
-    dt = time.perf_counter() - t0
+
+    def timed(form, n):
+        r = x3d.Renderer(x3d.PT_BIDIRECT_MIS, form=form, max_path_len=6, seed=1)
+        film = DeviceArray(w * h * 16)
+        for _ in range(warmup):
+            r.render_device(scene, desc.camera, w, h, spp, film.ptr)
+        synchronize()
+        t0 = time.perf_counter()
+        stats = [r.render_device(scene, desc.camera, w, h, spp, film.ptr) for _ in range(n)]
+        synchronize()
+        return time.perf_counter() - t0, stats, film.to_host(np.float32, (w * h, 4))
+
+    dt, stats, img = timed(x3d.FORM_WAVEFRONT, steps)
+    dp, _, img_p = timed(x3d.FORM_PLAIN, max(1, steps // 2))
     rays = sum(st.closest_queries + st.anyhit_queries for st in stats)
     res = {"workload": f"indirect room + 4 GGX spheres ({desc.num_tris} tris)", "width": w, "height": h, "spp": spp,
-           "max_path_len": 6, "msamples_per_s": w * h * spp * steps / dt / 1e6, "ms_per_step": dt * 1e3 / steps,
-           "mrays_per_s": rays / dt / 1e6, "rays_per_sample": rays / (w * h * spp * steps)}
-    log(f"bdpt leg: {res['ms_per_step']:.1f} ms per render, {res['msamples_per_s']:.1f} Msamples/s")
+           "max_path_len": 6, "form": "wavefront", "msamples_per_s": w * h * spp * steps / dt / 1e6,
+           "ms_per_step": dt * 1e3 / steps, "mrays_per_s": rays / dt / 1e6,
+           "rays_per_sample": rays / (w * h * spp * steps), "plain_ms_per_step": dp * 1e3 / max(1, steps // 2),
+           "films_bit_identical": bool(np.array_equal(img.view(np.uint32), img_p.view(np.uint32)))}
+    res["speedup_vs_plain"] = res["plain_ms_per_step"] / res["ms_per_step"]
+    if not res["films_bit_identical"]:
+        raise SystemExit("plain and wavefront bidirectional films differ")
+    log(f"bdpt leg: {res['ms_per_step']:.1f} ms per render ({res['plain_ms_per_step']:.1f} plain), "
+        f"{res['msamples_per_s']:.1f} Msamples/s")
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import bindings
@@ -272,9 +284,11 @@ def bench_secondary(args):
         res = leg_bdpt(x3d, args.steps, args.warmup, args.spp, cpu=not args.no_cpu_baseline)
         out.update(metric="bidirectional path tracing Msamples/s", unit="Msamples/s", value=res["msamples_per_s"],
                    ms_per_step=res["ms_per_step"], mrays_per_s=res["mrays_per_s"],
-                   rays_per_sample=res["rays_per_sample"], cpu_reference=res.get("cpu_reference"),
+                   rays_per_sample=res["rays_per_sample"], plain_ms_per_step=res["plain_ms_per_step"],
+                   speedup_vs_plain=res["speedup_vs_plain"], films_bit_identical=res["films_bit_identical"],
+                   cpu_reference=res.get("cpu_reference"),
                    config={"workload": res["workload"], "width": res["width"], "height": res["height"],
-                           "spp": res["spp"], "max_path_len": 6,
+                           "spp": res["spp"], "max_path_len": 6, "form": "wavefront",
                            "tracer": "bidirectional, reference uniform strategy weights"})
         print(json.dumps(out), flush=True)
         return

@@ -259,13 +259,18 @@ typedef struct x3d_renderer x3d_renderer;
 #define X3D_PT_UNIDIRECT_MIS 16
 
 /* Execution forms of the same estimator; every form gives the bit-identical film.
- * x3d_render_options_default picks the wavefront form for UNIDIRECT_MIS, UNIDIRECT_LT1 and
- * UNIDIRECT, the plain form for the other tracers. */
+ * x3d_render_options_default picks the wavefront form for UNIDIRECT_MIS, UNIDIRECT_LT1,
+ * UNIDIRECT and BIDIRECT_MIS, the plain form for the other tracers. */
 #define X3D_FORM_PLAIN 0     /* persistent kernels over the (pixel, sample) items of a generation,
                                 each lane taking the next item when its path ends, then an in-order
                                 per-pixel sum (the drop-in) */
 #define X3D_FORM_WAVEFRONT 1 /* ray queues per bounce, bounce loop in a hipGraph; runs
-                                UNIDIRECT_MIS, UNIDIRECT_LT1 and UNIDIRECT (the last two unsorted) */
+                                UNIDIRECT_MIS, UNIDIRECT_LT1 and UNIDIRECT (the last two unsorted);
+                                BIDIRECT_MIS: subpath vertices in structure-of-arrays rows, walk
+                                rays per bounce, then every connection ray in one any-hit pass
+                                (unsorted, no graph; generations of whole samples sized by
+                                queue_capacity and a 24 GiB working-memory budget, about 2.4 KB
+                                per (pixel, sample) item at 6-vertex subpaths) */
 
 typedef struct x3d_render_options {
     int32_t pt_type;       /* X3D_PT_* */
@@ -276,9 +281,10 @@ typedef struct x3d_render_options {
     uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default (64) */
     uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default = max = 64M
                               (216 B per path; a render allocates for min(capacity, w*h*spp));
-                              PLAIN (every tracer): (pixel, sample) items per generation, whole
-                              samples of every pixel (default 16M; the sample buffer takes 16 B
-                              per item, 256 MiB at the default, min(capacity, w*h*spp) items) */
+                              PLAIN (every tracer) and WAVEFRONT BIDIRECT_MIS: (pixel, sample)
+                              items per generation, whole samples of every pixel (default 16M;
+                              the sample buffer takes 16 B per item, 256 MiB at the default,
+                              min(capacity, w*h*spp) items) */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
     uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16 */
     int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */

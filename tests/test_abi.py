@@ -60,7 +60,7 @@ def test_render_option_errors_without_gpu(x3d):
     bad = [
         dict(pt_type=x3d.PT_BIDIRECT_MIS, max_path_len=0),
         dict(pt_type=x3d.PT_BIDIRECT_MIS, max_path_len=17),
-        dict(pt_type=x3d.PT_BIDIRECT_MIS, form=x3d.FORM_WAVEFRONT),
+        dict(pt_type=x3d.PT_BIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, sort_key=1),   # sorting: MIS only
         dict(pt_type=7),
         dict(pt_type=x3d.PT_BIDIRECT_LT2, form=x3d.FORM_WAVEFRONT),
         dict(pt_type=x3d.PT_POSITION, form=x3d.FORM_WAVEFRONT),
@@ -94,7 +94,7 @@ def test_option_defaults_pick_the_wavefront_form_where_it_exists(x3d):
     import ctypes as C
 
     for pt, form in ((x3d.PT_UNIDIRECT_MIS, x3d.FORM_WAVEFRONT), (x3d.PT_UNIDIRECT_LT1, x3d.FORM_WAVEFRONT),
-                     (x3d.PT_UNIDIRECT, x3d.FORM_WAVEFRONT), (x3d.PT_BIDIRECT_MIS, x3d.FORM_PLAIN),
+                     (x3d.PT_UNIDIRECT, x3d.FORM_WAVEFRONT), (x3d.PT_BIDIRECT_MIS, x3d.FORM_WAVEFRONT),
                      (x3d.PT_DIRECT, x3d.FORM_PLAIN), (x3d.PT_NORMAL, x3d.FORM_PLAIN)):
         o = x3d.RenderOptions()
         assert x3d.lib().x3d_render_options_default(pt, C.byref(o)) == 0

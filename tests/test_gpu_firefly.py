@@ -1,7 +1,7 @@
 """Films with the firefly filter (x3d_render's firefly_filter, sampling_task::run,
 renderer.cpp:46-83) bit-identical to the oracle's (orc_render_firefly: every sample image
 rendered, filtered, summed per pixel in sample order), for every tracer and both forms of the
-MIS, lt1 and unidirect tracers, in one generation and in several. The filter only lowers samples, so the filtered
+MIS, lt1, unidirect and bidirectional tracers, in one generation and in several. The filter only lowers samples, so the filtered
 film is at most the unfiltered one everywhere and below it on the pixels it touched; the scenes
 and sizes are chosen so the oracle caps samples on dozens of pixels.
 """
@@ -39,6 +39,8 @@ CASES = [
     ("indirect_l3", "BIDIRECT_LT2", dict()),
     ("indirect_l3", "BIDIRECT_MIS", dict(max_path_len=6)),
     ("spheres_small", "BIDIRECT_MIS", dict(max_path_len=5, queue_capacity=4000)),
+    ("indirect_l3", "BIDIRECT_MIS", dict(form="wavefront", max_path_len=6)),
+    ("spheres_small", "BIDIRECT_MIS", dict(form="wavefront", max_path_len=5, queue_capacity=4000)),
 ]
 
 

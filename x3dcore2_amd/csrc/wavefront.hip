@@ -88,8 +88,6 @@ struct WfArgs {
     int fused;
     // lt1: the first hit's emission per path, added when the path ends (w = 1 when there is one).
     float4 *le0;
-    // Bidirectional form: the shadow kernel writes each connection ray's visibility here.
-    uint8_t *vis;
 };
 
 // Ray keys: direction octant (3 bits) above the Morton code of the origin's cell in an 8^3 grid
@@ -563,9 +561,9 @@ __global__ __launch_bounds__(kSortWG) void k_wf_sort_scatter(WfArgs A) {
 
 // Persistent any-hit traversal over the shadow queue. MIS / lt1 rays: 3 x float4 (o, tmax),
 // (d, path), (contribution, -), tmin 1e-4; an unoccluded ray adds its contribution to the path's
-// radiance. BD (bidirectional connections): 2 x float4 (o, tmax), (d, slot | kBdLongTmin); the
-// visibility (1 = unoccluded) goes to vis[slot & ~kBdLongTmin]; tmin is 1e-3 where the flag is
-// set (subpath connections), else 1e-4 (light samples).
+// radiance. BD (bidirectional connections): 2 x float4 (o, tmax), (d, slot | kBdLongTmin); tmin
+// is 1e-3 where the flag is set (subpath connections), else 1e-4 (light samples), and the ray's
+// visibility (1 = unoccluded, 0 = occluded) replaces its tmax.
 constexpr uint32_t kBdLongTmin = 0x80000000u;
 template <int WG, int LDSD, bool BD = false>
 __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
@@ -662,7 +660,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
         }
         if (ref == kDone && leaf == kDone) {
             if (BD) {
-                A.vis[__float_as_uint(A.sq_in[2 * i + 1].w) & ~kBdLongTmin] = blocked ? 0 : 1;
+                A.sq[2 * static_cast<size_t>(i)].w = blocked ? 0.0f : 1.0f;
             } else if (!blocked) {
                 uint32_t path = __float_as_uint(A.sq_in[3 * i + 1].w);
                 float4 c = A.sq_in[3 * i + 2];
@@ -941,9 +939,10 @@ __global__ void k_wf_count(WfCtl *ctl, WfCounter const *eq_cnt, Counters *cnt) {
 //               walk), as li_bdpt draws them from the same PCG stream
 //   bd_connect  per (item, strategy): the shadow ray of each light-sample and subpath
 //               connection strategy that passes its geometric test
-//   shadow      any-hit traversal of those rays (visibility per strategy)
-//   bd_accumulate per item: li_bdpt's sums over path lengths and strategies, in its order and
-//               with its float expressions, from the stored vertices and visibilities
+//   shadow      any-hit traversal of those rays (visibility per ray)
+//   bd_contrib  per ray: the value li_bdpt adds for its strategy (the connection's transport
+//               and BSDFs, or the light sample's estimate), with li_bdpt's float expressions
+//   bd_accumulate per item: li_bdpt's sums over path lengths and strategies, in its order
 // Subpath vertices live in HBM as structure-of-arrays ([vertex][item] float4 rows: position and
 // material, normal, direction to the previous vertex, prefix transport) instead of 1680 B of
 // scratch per lane. Each item draws the same numbers from its PCG stream and evaluates the same
@@ -951,6 +950,7 @@ __global__ void k_wf_count(WfCtl *ctl, WfCounter const *eq_cnt, Counters *cnt) {
 // ---------------------------------------------------------------------------------------------
 constexpr uint32_t kBdCam = 0, kBdLight0 = 1, kBdLight = 2, kBdDone = 3;
 constexpr uint32_t kBdDead = 1u << 20;
+constexpr uint32_t kBdNoRay = 0xFFFFFFFFu;
 X3D_DI uint32_t bd_cam_len(uint32_t s) { return s & 31u; }
 X3D_DI uint32_t bd_light_len(uint32_t s) { return (s >> 8) & 31u; }
 X3D_DI uint32_t bd_phase(uint32_t s) { return (s >> 16) & 3u; }
@@ -974,7 +974,7 @@ struct BdArgs {
     const ClosestHit *eh;
     float4 *sq;          // shadow rays, 2 x float4
     WfCounter *sq_count;
-    uint8_t *vis;        // [strategy * n + j]
+    uint32_t *rix;       // [strategy * n + j]: queue entry of the strategy's shadow ray, or kBdNoRay
     float4 *samples;     // sample radiance [j]
 };
 
@@ -1008,18 +1008,19 @@ X3D_DI LightSample bd_em(BdArgs const &B, uint32_t j) {
     return ls;
 }
 
-// The walk of item j after its last vertex: the next BSDF sample of the current subpath
-// (extend_walk's loop; `ended`: the walk already stopped at a miss or a back-facing hit), or,
-// once the camera walk is over, the emission sample that starts the light walk; once the light
-// walk is over, the item is done. Returns true when a ray is queued (ray_o / ray_d written) and
-// stores the new state and PCG state.
-X3D_DI bool bd_advance(BdArgs const &B, uint32_t j, Pcg &g, uint32_t cl, uint32_t ll, uint32_t ph, bool ended) {
+// The walk of item j after its last vertex `last` (of the subpath that `ph` names): the next
+// BSDF sample of that subpath (extend_walk's loop; `ended`: the walk already stopped at a miss
+// or a back-facing hit), or, once the camera walk is over, the emission sample that starts the
+// light walk; once the light walk is over, the item is done. Returns true when a ray is queued
+// (ray_o / ray_d written) and stores the new state and PCG state.
+X3D_DI bool bd_advance(BdArgs const &B, uint32_t j, Pcg &g, uint32_t cl, uint32_t ll, uint32_t ph, bool ended,
+                       BdVert const &last) {
     DevScene const &S = B.S;
     bool queued = false;
     if (ended) {
         // no further BSDF sample on this subpath
     } else if (ph == kBdCam && cl < B.max_len) {
-        BdVert cur = bd_load(B.cv, B.n, cl - 1, j);
+        BdVert const &cur = last;
         float dens;
         f3 i = bsdf_sample(S.mats, cur.mat, cur.n, cur.v, g, &dens, true);
         if (!near0(dens)) {
@@ -1028,7 +1029,7 @@ X3D_DI bool bd_advance(BdArgs const &B, uint32_t j, Pcg &g, uint32_t cl, uint32_
             queued = true;
         }
     } else if (ph == kBdLight && ll < B.max_len) {
-        BdVert cur = bd_load(B.lv, B.n, ll - 1, j);
+        BdVert const &cur = last;
         float dens;
         f3 i = bsdf_sample(S.mats, cur.mat, cur.n, cur.v, g, &dens, true);
         if (!near0(dens)) {
@@ -1080,9 +1081,9 @@ __global__ __launch_bounds__(256) void k_bd_start(BdArgs B) {
                 B.state[j] = kBdDead;
             } else {
                 Pcg g = pcg_for(B.seed, pix, s);
-                bd_store(B.cv, B.n, 0, j, xyz(fa), static_cast<int32_t>(__float_as_uint(fc.x)), xyz(fb), -v,
-                         mk(1.0f, 1.0f, 1.0f));
-                queued = bd_advance(B, j, g, 1, 0, kBdCam, false);
+                const BdVert v0{xyz(fa), xyz(fb), -v, mk(1.0f, 1.0f, 1.0f), static_cast<int32_t>(__float_as_uint(fc.x))};
+                bd_store(B.cv, B.n, 0, j, v0.p, v0.mat, v0.n, v0.v, v0.t);
+                queued = bd_advance(B, j, g, 1, 0, kBdCam, false, v0);
             }
         }
         uint32_t *const c[1] = {&B.eq_cnt_out->v};
@@ -1117,10 +1118,12 @@ __global__ __launch_bounds__(256) void k_bd_shade(BdArgs B) {
                 nx = finish_hit(S, h);
                 hit = dot(nx.n, -i) > 0.0f;
             }
+            BdVert nv{};
             if (ph == kBdLight0) {
                 // the emission ray: the light walk's first vertex, if it hits a surface facing it
                 if (hit) {
-                    bd_store(B.lv, B.n, 0, j, nx.p, nx.mat, nx.n, -i, mk(1.0f, 1.0f, 1.0f));
+                    nv = BdVert{nx.p, nx.n, -i, mk(1.0f, 1.0f, 1.0f), nx.mat};
+                    bd_store(B.lv, B.n, 0, j, nv.p, nv.mat, nv.n, nv.v, nv.t);
                     ll = 1;
                     ph = kBdLight;
                 } else {
@@ -1134,7 +1137,8 @@ __global__ __launch_bounds__(256) void k_bd_shade(BdArgs B) {
                 // extend_walk: projected_adjoint_brdf on the camera walk, projected_brdf on the
                 // light walk
                 f3 f = cam ? bsdf_eval(S.mats, cur.mat, cur.n, cur.v, i) : bsdf_eval(S.mats, cur.mat, cur.n, i, cur.v);
-                bd_store(cam ? B.cv : B.lv, B.n, k, j, nx.p, nx.mat, nx.n, -i, mul(cur.t, (f * cos_w) / rd.w));
+                nv = BdVert{nx.p, nx.n, -i, mul(cur.t, (f * cos_w) / rd.w), nx.mat};
+                bd_store(cam ? B.cv : B.lv, B.n, k, j, nv.p, nv.mat, nv.n, nv.v, nv.t);
                 if (cam)
                     cl++;
                 else
@@ -1145,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_bd_shade(BdArgs B) {
             } else {
                 // a camera walk that stopped here goes on to its emission sample; a light walk
                 // that stopped here is done
-                queued = bd_advance(B, j, g, cl, ll, ph, !hit);
+                queued = bd_advance(B, j, g, cl, ll, ph, !hit, nv);
             }
         }
         uint32_t *const c[1] = {&B.eq_cnt_out->v};
@@ -1191,17 +1195,19 @@ template <int WG> X3D_DI uint32_t block_reserve(uint32_t cnt, uint32_t *counter,
 // Tracer::direct's ray from camera vertex cp - 1 towards the emission sample,
 // [1e-4, dist - 1e-3]) and subpath connections (lj -> cj, [1e-3, dist - 1e-3]) whose cosines
 // are both positive. WRITE = false counts them; WRITE = true stores them from queue entry `at`
-// on, tagged with their strategy slot k = cp * (max_len + 1) + lp (visibility index k * n + j).
+// on, tagged with their strategy slot index (cp * (max_len + 1) + lp) * n + j, and records in
+// rix every strategy's queue entry (kBdNoRay where none was queued).
 template <bool WRITE> X3D_DI uint32_t bd_connections(BdArgs const &B, uint32_t j, uint32_t st, uint32_t at) {
     DevScene const &S = B.S;
     const uint32_t side = B.max_len + 1, cam_len = bd_cam_len(st), light_len = bd_light_len(st);
     const LightSample em = bd_em(B, j);
     uint32_t cnt = 0;
-    auto emit = [&](f3 o, f3 d, float tmax, uint32_t slot) {
+    auto emit = [&](f3 o, f3 d, float tmax, uint32_t slot, uint32_t tag) {
         if (WRITE) {
             const size_t e = 2 * static_cast<size_t>(at + cnt);
             B.sq[e] = make_float4(o.x, o.y, o.z, tmax);
-            B.sq[e + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(slot));
+            B.sq[e + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(slot | tag));
+            B.rix[slot] = at + cnt;
         }
         cnt++;
     };
@@ -1210,9 +1216,12 @@ template <bool WRITE> X3D_DI uint32_t bd_connections(BdArgs const &B, uint32_t j
         const f3 cpos = xyz(B.cv[0][qc]), cnrm = xyz(B.cv[1][qc]);
         f3 l, illum;
         float r2;
+        const uint32_t row = cp * side * B.n + j;
         if (light_arrives(S, em, cpos, cnrm, illum, l, r2)) {
             float dist = norm(l);
-            emit(cpos, -l / dist, dist - 1e-3f, (cp * side) * B.n + j);
+            emit(cpos, -l / dist, dist - 1e-3f, row, 0u);
+        } else if (WRITE) {
+            B.rix[row] = kBdNoRay;
         }
         for (uint32_t lp = 1; lp <= light_len; lp++) {
             const size_t ql = static_cast<size_t>(lp - 1) * B.n + j;
@@ -1223,7 +1232,9 @@ template <bool WRITE> X3D_DI uint32_t bd_connections(BdArgs const &B, uint32_t j
             float cos_wo = dot(lnrm, join);
             float cos_wi = dot(cnrm, -join);
             if (cos_wo > 0.0f && cos_wi > 0.0f)
-                emit(lpos, join, dist - 1e-3f, ((cp * side + lp) * B.n + j) | kBdLongTmin);
+                emit(lpos, join, dist - 1e-3f, row + lp * B.n, kBdLongTmin);
+            else if (WRITE)
+                B.rix[row + lp * B.n] = kBdNoRay;
         }
     }
     return cnt;
@@ -1239,13 +1250,64 @@ __global__ __launch_bounds__(256) void k_bd_connect(BdArgs B) {
             st = B.state[j];
         const uint32_t cnt = (st & kBdDead) ? 0u : bd_connections<false>(B, j, st, 0);
         const uint32_t at = block_reserve<256>(cnt, &B.sq_count->v, scratch);
-        if (cnt)
+        if (!(st & kBdDead))
             bd_connections<true>(B, j, st, at);
     }
 }
 
-// li_bdpt's transport_all_connectible_subpaths for each item, with the visibilities the shadow
-// kernel found; the sample's radiance goes to samples[j].
+// Per shadow ray (one thread each): the value li_bdpt adds to its path length's sum for the
+// ray's strategy, written over the ray's first float4 (w = 1: add it, w = 0: add nothing).
+// Light sample (lp = 0): mul(tr, cj.transport), tr = Tracer::direct's estimate if the ray is
+// unoccluded, else 0 (it is added either way). Connection: mul(tr, cj.transport) with the
+// connection's transport tr if unoccluded; an occluded connection adds nothing.
+__global__ __launch_bounds__(256) void k_bd_contrib(BdArgs B) {
+    DevScene const &S = B.S;
+    const uint32_t total = B.sq_count->v, side = B.max_len + 1;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const size_t e = 2 * static_cast<size_t>(i);
+        const bool visible = B.sq[e].w != 0.0f;
+        const uint32_t slot = __float_as_uint(B.sq[e + 1].w) & ~kBdLongTmin;
+        const uint32_t k = slot / B.n, j = slot - k * B.n, cp = k / side, lp = k - cp * side;
+        const BdVert cj = bd_load(B.cv, B.n, cp - 1, j);
+        f3 val = mk(0.0f, 0.0f, 0.0f);
+        float add = 1.0f;
+        if (lp == 0) {
+            f3 tr = mk(0.0f, 0.0f, 0.0f);
+            if (visible) {
+                const LightSample em = bd_em(B, j);
+                f3 l, illum;
+                float r2;
+                (void)light_arrives(S, em, cj.p, cj.n, illum, l, r2); // held when the ray was queued
+                float dist = norm(l);
+                f3 wi = -l / dist;
+                f3 f = bsdf_eval(S.mats, cj.mat, cj.n, cj.v, wi);
+                tr = mul(illum, f) / em.area_dens;
+            }
+            val = mul(tr, cj.t);
+        } else if (visible) {
+            const BdVert lj = bd_load(B.lv, B.n, lp - 1, j);
+            f3 join = cj.p - lj.p;
+            float dist = norm(join);
+            join = join / dist;
+            float cos_wo = dot(lj.n, join);
+            float cos_wi = dot(cj.n, -join);
+            f3 sub = mul(xyz(B.emis[j]), lj.t);
+            float geo = cos_wi * cos_wo / (dist * dist);
+            f3 fl = bsdf_eval(S.mats, lj.mat, lj.n, join, lj.v);
+            f3 fc = bsdf_eval(S.mats, cj.mat, cj.n, cj.v, -join);
+            f3 tr = mul(mul(sub, fl), fc) * geo;
+            val = mul(tr, cj.t);
+        } else {
+            add = 0.0f;
+        }
+        B.sq[e] = make_float4(val.x, val.y, val.z, add);
+    }
+}
+
+// li_bdpt's transport_all_connectible_subpaths for each item: per path length, the sum over its
+// strategies in li_bdpt's order (the values of k_bd_contrib; a light-sample strategy without a
+// ray adds mul(0, cj.transport), as li_bdpt does) divided by the strategy count; the sample's
+// radiance goes to samples[j].
 __global__ __launch_bounds__(256) void k_bd_accumulate(BdArgs B) {
     DevScene const &S = B.S;
     const uint32_t side = B.max_len + 1;
@@ -1255,48 +1317,34 @@ __global__ __launch_bounds__(256) void k_bd_accumulate(BdArgs B) {
         if (!(st & kBdDead)) {
             const uint32_t cam_len = bd_cam_len(st), light_len = bd_light_len(st), pix = j % B.npix;
             const int32_t first_light = static_cast<int32_t>(__float_as_uint(B.first[3 * pix + 2].y));
-            const LightSample em = bd_em(B, j);
-            const f3 light_emission = xyz(B.emis[j]);
             for (uint32_t plen = 1; plen <= cam_len + light_len + 1; plen++) {
                 int cp = static_cast<int>(plen - 1 < cam_len ? plen - 1 : cam_len);
                 uint32_t lp = plen - 1 - static_cast<uint32_t>(cp);
                 f3 psum = mk(0.0f, 0.0f, 0.0f);
                 float wsum = 0.0f;
                 for (; cp >= 0 && lp <= light_len; lp++, cp--) {
-                    const size_t slot = static_cast<size_t>(static_cast<uint32_t>(cp) * side + lp) * B.n + j;
+                    const uint32_t slot = (static_cast<uint32_t>(cp) * side + lp) * B.n + j;
                     if (lp == 0 && cp == 0) {
                         if (first_light >= 0) {
-                            BdVert c0 = bd_load(B.cv, B.n, 0, j);
-                            psum = psum + light_radiance(S, first_light, c0.v, c0.n);
+                            const f3 v0 = xyz(B.cv[2][j]), n0 = xyz(B.cv[1][j]);
+                            psum = psum + light_radiance(S, first_light, v0, n0);
                         }
                         wsum = wsum + 1.0f;
                     } else if (lp == 0) {
-                        BdVert cj = bd_load(B.cv, B.n, static_cast<uint32_t>(cp) - 1, j);
-                        f3 tr = mk(0.0f, 0.0f, 0.0f), l, illum;
-                        float r2;
-                        if (light_arrives(S, em, cj.p, cj.n, illum, l, r2) && B.vis[slot]) {
-                            float dist = norm(l);
-                            f3 wi = -l / dist;
-                            f3 f = bsdf_eval(S.mats, cj.mat, cj.n, cj.v, wi);
-                            tr = mul(illum, f) / em.area_dens;
+                        const uint32_t r = B.rix[slot];
+                        if (r == kBdNoRay) {
+                            const f3 t = xyz(B.cv[3][static_cast<size_t>(cp - 1) * B.n + j]);
+                            psum = psum + mul(mk(0.0f, 0.0f, 0.0f), t);
+                        } else {
+                            psum = psum + xyz(B.sq[2 * static_cast<size_t>(r)]);
                         }
-                        psum = psum + mul(tr, cj.t);
                         wsum = wsum + 1.0f;
                     } else if (cp > 0) {
-                        BdVert lj = bd_load(B.lv, B.n, lp - 1, j);
-                        BdVert cj = bd_load(B.cv, B.n, static_cast<uint32_t>(cp) - 1, j);
-                        f3 join = cj.p - lj.p;
-                        float dist = norm(join);
-                        join = join / dist;
-                        float cos_wo = dot(lj.n, join);
-                        float cos_wi = dot(cj.n, -join);
-                        if (cos_wo > 0.0f && cos_wi > 0.0f && B.vis[slot]) {
-                            f3 sub = mul(light_emission, lj.t);
-                            float geo = cos_wi * cos_wo / (dist * dist);
-                            f3 fl = bsdf_eval(S.mats, lj.mat, lj.n, join, lj.v);
-                            f3 fc = bsdf_eval(S.mats, cj.mat, cj.n, cj.v, -join);
-                            f3 tr = mul(mul(sub, fl), fc) * geo;
-                            psum = psum + mul(tr, cj.t);
+                        const uint32_t r = B.rix[slot];
+                        if (r != kBdNoRay) {
+                            const float4 c = B.sq[2 * static_cast<size_t>(r)];
+                            if (c.w != 0.0f)
+                                psum = psum + xyz(c);
                         }
                         wsum = wsum + 1.0f;
                     }
@@ -1339,7 +1387,7 @@ struct WavefrontEngine {
     DevBuf sq_sorted, eq_sorted, ray_key, ray_rank, bin_mat, bin_tot; // ray reordering
     // bidirectional form: per item state, PCG state, walk ray, emission sample, vertices,
     // visibilities
-    DevBuf bd_state, bd_rng, bd_ro, bd_rd, bd_em, bd_cv, bd_lv, bd_vis;
+    DevBuf bd_state, bd_rng, bd_ro, bd_rd, bd_em, bd_cv, bd_lv, bd_rix;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     WfArgs key{};
@@ -1416,7 +1464,7 @@ uint64_t bd_item_bytes(uint32_t max_len) {
            + 2 * 4 * 16 * m        // camera and light vertices
            + 2 * 4 + sizeof(ClosestHit) // both queue parities, hit records
            + 2 * 16 * (m + m * m)  // shadow rays (every light-sample and connection strategy)
-           + strat                 // visibilities
+           + 4 * strat             // queue entry per strategy
            + 16;                   // the sample radiance (render's sample buffer)
 }
 
@@ -1451,7 +1499,7 @@ std::string bd_generation(WavefrontEngine *e, RenderParams const &P, WavefrontTu
     BD_HIP(e->bd_em.ensure(3 * f4));
     BD_HIP(e->bd_cv.ensure(4 * m * f4));
     BD_HIP(e->bd_lv.ensure(4 * m * f4));
-    BD_HIP(e->bd_vis.ensure(static_cast<size_t>(n) * nstrat));
+    BD_HIP(e->bd_rix.ensure(sizeof(uint32_t) * static_cast<size_t>(n) * nstrat));
     BD_HIP(e->eq.ensure(sizeof(uint32_t) * static_cast<size_t>(n)));
     BD_HIP(e->eq_b.ensure(sizeof(uint32_t) * static_cast<size_t>(n)));
     BD_HIP(e->eh.ensure(sizeof(ClosestHit) * static_cast<size_t>(n)));
@@ -1494,7 +1542,7 @@ std::string bd_generation(WavefrontEngine *e, RenderParams const &P, WavefrontTu
     B.eh = static_cast<const ClosestHit *>(e->eh.p);
     B.sq = static_cast<float4 *>(e->sq.p);
     B.sq_count = &ctl->sq_count;
-    B.vis = static_cast<uint8_t *>(e->bd_vis.p);
+    B.rix = static_cast<uint32_t *>(e->bd_rix.p);
     B.samples = samples;
 
     WfArgs X{};
@@ -1508,11 +1556,12 @@ std::string bd_generation(WavefrontEngine *e, RenderParams const &P, WavefrontTu
     X.xp = B.ray_o;
     X.wi = B.ray_d;
     X.eh = static_cast<ClosestHit *>(e->eh.p);
+    X.sq = B.sq;
     X.sq_in = B.sq;
-    X.vis = B.vis;
 
     const uint32_t g_item = grid_for(k_bd_shade, 256), g_start = grid_for(k_bd_start, 256),
-                   g_conn = grid_for(k_bd_connect, 256), g_acc = grid_for(k_bd_accumulate, 256);
+                   g_conn = grid_for(k_bd_connect, 256), g_contrib = grid_for(k_bd_contrib, 256),
+                   g_acc = grid_for(k_bd_accumulate, 256);
     const bool profile = tune.profile != 0;
     if (first_gen) {
         e->shadow_ms = e->extend_ms = 0.0;
@@ -1557,6 +1606,7 @@ std::string bd_generation(WavefrontEngine *e, RenderParams const &P, WavefrontTu
     hipLaunchKernelGGL(k_bd_connect, dim3(g_conn), dim3(256), 0, stream, B);
     timed(tk.bd_shadow, g_shadow, X, e->shadow_ms);
     hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, stream, static_cast<const WfCounter *>(&ctl->sq_count), P.cnt, 1);
+    hipLaunchKernelGGL(k_bd_contrib, dim3(g_contrib), dim3(256), 0, stream, B);
     hipLaunchKernelGGL(k_bd_accumulate, dim3(g_acc), dim3(256), 0, stream, B);
     if (profile)
         e->trace_launches += bounces + 1;

@@ -109,8 +109,12 @@ X3D_DI void add_bsdf_grad(DevScene const &S, int32_t mat, f3 n, f3 o, f3 i, f3 Q
 
 constexpr int kAdjWaves = kWG / 64;
 
-__global__ __launch_bounds__(kWG) void k_adjoint(RenderParams P, const float4 *dl, uint32_t num_mats,
-                                                 unsigned long long *partial) {
+// Built for 4 waves per SIMD (128 VGPRs, a few spilled): the compiler's own choice, 3 waves
+// without spills, measured 10% slower.
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(RenderParams P,
+                                                                                        const float4 *dl,
+                                                                                        uint32_t num_mats,
+                                                 unsigned long long *partial, float4 *records) {
     __shared__ uint2 lds[kLdsDepth * kWG];
     __shared__ unsigned long long slots[kAdjWaves][4 * kAdjMaxMats];
     for (uint32_t i = threadIdx.x; i < kAdjWaves * 4 * kAdjMaxMats; i += kWG)
@@ -122,7 +126,7 @@ __global__ __launch_bounds__(kWG) void k_adjoint(RenderParams P, const float4 *d
     uint32_t npix = P.width * P.height;
     uint64_t total = static_cast<uint64_t>(npix) * P.spp_total;
     float inv = 1.0f / static_cast<float>(P.spp_total);
-    AdjVert rec[kAdjMaxDepth];
+    const AdjRecord rec = AdjRecord::of_lane(records, (blockIdx.x * kWG + threadIdx.x) >> 6);
     for (uint64_t q = blockIdx.x * static_cast<uint64_t>(kWG) + threadIdx.x; q < total;
          q += static_cast<uint64_t>(gridDim.x) * kWG) {
         uint32_t pix = static_cast<uint32_t>(q % npix), s = static_cast<uint32_t>(q / npix);
@@ -137,17 +141,20 @@ __global__ __launch_bounds__(kWG) void k_adjoint(RenderParams P, const float4 *d
         uint32_t nrec = 0;
         while (ps.alive) {
             nrec = ps.k + 1;
-            tr.mis_bounce<true>(ps, P.max_depth, rec);
+            tr.mis_bounce<true>(ps, P.max_depth, &rec);
         }
         f3 tail = mk(0.0f, 0.0f, 0.0f);
         for (int k = static_cast<int>(nrec) - 1; k >= 0; k--) {
-            AdjVert const &r = rec[k];
-            f3 after = r.c_em + tail;
-            if (r.flags & kAdjCont)
-                add_bsdf_grad(P.S, r.mat, r.n, r.wo, r.wi, mul(gw, after), my);
-            if (r.flags & kAdjNee)
-                add_bsdf_grad(P.S, r.mat, r.n, r.wo, r.wl, mul(gw, r.c_nee), my);
-            tail = after + r.c_nee;
+            const float4 r0 = rec.row(k, 0), r1 = rec.row(k, 1);
+            const int32_t mat = static_cast<int32_t>(__float_as_uint(r0.w));
+            const uint32_t flags = __float_as_uint(r1.w);
+            const f3 n = xyz(r0), wo = xyz(r1), c_nee = xyz(rec.row(k, 4));
+            f3 after = xyz(rec.row(k, 5)) + tail;
+            if (flags & kAdjCont)
+                add_bsdf_grad(P.S, mat, n, wo, xyz(rec.row(k, 2)), mul(gw, after), my);
+            if (flags & kAdjNee)
+                add_bsdf_grad(P.S, mat, n, wo, xyz(rec.row(k, 3)), mul(gw, c_nee), my);
+            tail = after + c_nee;
         }
     }
     if (P.cnt) {
@@ -191,7 +198,7 @@ __global__ __launch_bounds__(256) void k_adjoint_reduce(const unsigned long long
 } // namespace
 
 struct AdjointEngine {
-    DevBuf partial, spill;
+    DevBuf partial, spill, records;
 };
 
 AdjointEngine *adjoint_create() { return new AdjointEngine(); }
@@ -224,8 +231,12 @@ std::string adjoint_render(AdjointEngine *e, RenderParams const &P, const float4
         err != hipSuccess)
         return std::string("hipMalloc: ") + hipGetErrorString(err);
     Q.spill = static_cast<uint2 *>(e->spill.p);
+    // path records: kAdjRows float4 per vertex and resident lane (~150 MB for 256 CUs)
+    if (hipError_t err = e->records.ensure(sizeof(float4) * kAdjRows * kAdjMaxDepth * static_cast<size_t>(grid) * kWG);
+        err != hipSuccess)
+        return std::string("hipMalloc: ") + hipGetErrorString(err);
     hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(kWG), 0, stream, Q, d_dl, num_mats,
-                       static_cast<unsigned long long *>(e->partial.p));
+                       static_cast<unsigned long long *>(e->partial.p), static_cast<float4 *>(e->records.p));
     if (num_mats > 0)
         hipLaunchKernelGGL(k_adjoint_reduce, dim3(4 * num_mats), dim3(256), 0, stream,
                            static_cast<const unsigned long long *>(e->partial.p), grid, 4 * num_mats, d_grad);

@@ -129,16 +129,43 @@ struct Pathlet {
     int32_t mat;
 };
 
-// One vertex of a recorded MIS path (for the adjoint): shading normal, outgoing direction,
-// sampled BSDF direction, light-sample direction, the NEE contribution made at the vertex and
-// the emission reached through the BSDF sample (both weighted by throughput and MIS weight).
-struct AdjVert {
-    f3 n, wo, wi, wl, c_nee, c_em;
-    int32_t mat;
-    uint32_t flags; // kAdjNee: NEE contributed; kAdjCont: the BSDF sample continued the path
+// One lane's subpath in global memory: four float4 rows per vertex ((p, mat), n, v, transport),
+// in one contiguous block per wave (the 64 lanes' rows of one (vertex, field) pair side by side),
+// so the subpaths take no private (scratch) memory.
+constexpr uint32_t kPathletRows = 4;
+struct PathletRows {
+    float4 *base; // this lane's row of vertex 0, field 0 (rows of one lane 64 float4 apart)
+    X3D_DI float4 &row(uint32_t k, uint32_t f) const { return base[(kPathletRows * k + f) * 64u]; }
+    X3D_DI Pathlet load(uint32_t k) const {
+        float4 a = row(k, 0), b = row(k, 1), c = row(k, 2), d = row(k, 3);
+        return Pathlet{xyz(a), xyz(b), xyz(c), xyz(d), static_cast<int32_t>(__float_as_uint(a.w))};
+    }
+    X3D_DI void store(uint32_t k, Pathlet const &v) const {
+        row(k, 0) = make_float4(v.p.x, v.p.y, v.p.z, __uint_as_float(static_cast<uint32_t>(v.mat)));
+        row(k, 1) = make_float4(v.n.x, v.n.y, v.n.z, 0.0f);
+        row(k, 2) = make_float4(v.v.x, v.v.y, v.v.z, 0.0f);
+        row(k, 3) = make_float4(v.transport.x, v.transport.y, v.transport.z, 0.0f);
+    }
 };
+
+// Record of a replayed MIS path (for the adjoint), per vertex: shading normal and material,
+// outgoing direction and flags (kAdjNee: NEE contributed; kAdjCont: the BSDF sample continued the
+// path), sampled BSDF direction, light-sample direction, the NEE contribution made at the vertex
+// and the emission reached through the BSDF sample (both weighted by throughput and MIS weight).
+// The rows live in global memory, one float4 each, in one contiguous block per wave (the 64
+// lanes' rows of one (vertex, field) pair side by side), so the records take no private
+// (scratch) memory and a wave's record stays within a few pages.
 constexpr uint32_t kAdjNee = 1u, kAdjCont = 2u;
 constexpr uint32_t kAdjMaxDepth = 8; // deepest path the adjoint records
+constexpr uint32_t kAdjRows = 6;     // float4 rows per recorded vertex
+struct AdjRecord {
+    float4 *base; // this lane's row of vertex 0, field 0 (rows of one lane 64 float4 apart)
+    X3D_DI float4 &row(uint32_t k, uint32_t f) const { return base[(kAdjRows * k + f) * 64u]; }
+    // the record of the calling lane in `buf`, given its wave's global index
+    X3D_DI static AdjRecord of_lane(float4 *buf, size_t wave) {
+        return AdjRecord{buf + wave * (kAdjRows * kAdjMaxDepth * 64u) + __lane_id()};
+    }
+};
 
 // State of one MIS path between bounces (see Tracer::mis_begin / mis_bounce).
 struct PathState {
@@ -207,19 +234,22 @@ template <int WG, int LDSD> struct Tracer {
             ps.L = ps.L + light_radiance(S, first.light, ps.wo, first.n);
     }
 
-    // With REC, the vertex is recorded in rec[ps.k] (the adjoint's replay).
-    template <bool REC = false> X3D_DI void mis_bounce(PathState &ps, uint32_t max_depth, AdjVert *rec = nullptr) {
+    // With REC, the vertex is recorded as vertex ps.k of `rec` (the adjoint's replay).
+    template <bool REC = false>
+    X3D_DI void mis_bounce(PathState &ps, uint32_t max_depth, AdjRecord const *rec = nullptr) {
         HitRec const &x = ps.x;
-        AdjVert *r = nullptr;
-        if constexpr (REC) {
-            r = rec + ps.k;
-            r->n = x.n;
-            r->wo = ps.wo;
-            r->mat = x.mat;
-            r->flags = 0;
-            r->c_nee = mk(0.0f, 0.0f, 0.0f);
-            r->c_em = mk(0.0f, 0.0f, 0.0f);
-        }
+        // rows are written as soon as their values are known (nothing extra stays live across
+        // the traversal calls)
+        uint32_t flags = 0;
+        const uint32_t k = ps.k;
+        auto put = [&](uint32_t f, f3 v, float w) {
+            if constexpr (REC)
+                rec->row(k, f) = make_float4(v.x, v.y, v.z, w);
+        };
+        const f3 zero = mk(0.0f, 0.0f, 0.0f);
+        put(0, x.n, __uint_as_float(static_cast<uint32_t>(x.mat)));
+        put(4, zero, 0.0f);
+        put(5, zero, 0.0f);
         if (S.num_lights > 0) {
             LightSample ls = sample_light(S, ps.g);
             f3 d, wi;
@@ -234,27 +264,25 @@ template <int WG, int LDSD> struct Tracer {
                 }
                 f3 add = mul(ps.beta, d * w);
                 ps.L = ps.L + add;
-                if constexpr (REC) {
-                    r->flags |= kAdjNee;
-                    r->wl = wi;
-                    r->c_nee = add;
-                }
+                flags |= kAdjNee;
+                put(3, wi, 0.0f);
+                put(4, add, 0.0f);
             }
         }
         float dens;
         f3 wi = bsdf_sample(S.mats, x.mat, x.n, ps.wo, ps.g, &dens, false);
         HitRec nx;
         if (!(dens > 0.0f) || !trace(x.p, wi, nx) || dot(nx.n, -wi) <= 0.0f) {
+            put(1, ps.wo, __uint_as_float(flags));
             ps.alive = false;
             return;
         }
         float cw = dot(x.n, wi);
         f3 f = bsdf_eval(S.mats, x.mat, x.n, ps.wo, wi);
         ps.beta = mul(ps.beta, f) * cw / dens;
-        if constexpr (REC) {
-            r->flags |= kAdjCont;
-            r->wi = wi;
-        }
+        flags |= kAdjCont;
+        put(1, ps.wo, __uint_as_float(flags));
+        put(2, wi, 0.0f);
         if (nx.light >= 0) {
             float cl = dot(nx.n, -wi);
             DevLight const &Ln = S.lights[nx.light];
@@ -262,8 +290,7 @@ template <int WG, int LDSD> struct Tracer {
             float w = dens * dens / (dens * dens + pl * pl);
             f3 add = mul(ps.beta, light_radiance(S, nx.light, -wi, nx.n) * w);
             ps.L = ps.L + add;
-            if constexpr (REC)
-                r->c_em = add;
+            put(5, add, 0.0f);
         }
         ps.x = nx;
         ps.wo = -wi;
@@ -356,9 +383,10 @@ template <int WG, int LDSD> struct Tracer {
     // mixture densities) until max_len vertices, a density within 1e-5 of zero, a miss or a
     // back-facing hit, accumulating the prefix transport (f * cos) / dens of light_transport_info
     // (pathtracer.cpp:254-274); IMPORTANCE: projected_brdf, else projected_adjoint_brdf.
-    template <bool IMPORTANCE> X3D_DI uint32_t extend_walk(Pathlet *path, uint32_t len, uint32_t max_len, Pcg &g) {
+    // `cur` is path vertex len - 1 on entry.
+    template <bool IMPORTANCE>
+    X3D_DI uint32_t extend_walk(PathletRows const &path, Pathlet cur, uint32_t len, uint32_t max_len, Pcg &g) {
         while (len < max_len) {
-            Pathlet const cur = path[len - 1];
             float dens;
             f3 i = bsdf_sample(S.mats, cur.mat, cur.n, cur.v, g, &dens, true);
             if (near0(dens))
@@ -368,12 +396,14 @@ template <int WG, int LDSD> struct Tracer {
                 break;
             float cos_w = dot(cur.n, i);
             f3 f = IMPORTANCE ? bsdf_eval(S.mats, cur.mat, cur.n, i, cur.v) : bsdf_eval(S.mats, cur.mat, cur.n, cur.v, i);
-            Pathlet &nxt = path[len];
+            Pathlet nxt;
             nxt.p = nx.p;
             nxt.n = nx.n;
             nxt.v = -i;
             nxt.transport = mul(cur.transport, (f * cos_w) / dens);
             nxt.mat = nx.mat;
+            path.store(len, nxt);
+            cur = nxt;
             len++;
         }
         return len;
@@ -385,17 +415,14 @@ template <int WG, int LDSD> struct Tracer {
     // transport_all_connectible_subpaths (pathtracer.cpp:351-452): per total length, the
     // average over its strategies with uniform weights; (0,0), every light_plen == 0 and every
     // connection strategy counts (occluded or not), cam_plen == 0 does not.
-    X3D_DI f3 li_bdpt(HitRec const &first, bool valid, f3 ray_v, Pcg &g, uint32_t max_len, Pathlet *cam,
-                      Pathlet *lig) {
+    X3D_DI f3 li_bdpt(HitRec const &first, bool valid, f3 ray_v, Pcg &g, uint32_t max_len, PathletRows const &cam,
+                      PathletRows const &lig) {
         f3 rad = mk(0.0f, 0.0f, 0.0f);
         if (!valid || max_len == 0 || S.num_lights == 0)
             return rad;
-        cam[0].p = first.p;
-        cam[0].n = first.n;
-        cam[0].v = -ray_v;
-        cam[0].transport = mk(1.0f, 1.0f, 1.0f);
-        cam[0].mat = first.mat;
-        uint32_t cam_len = extend_walk<false>(cam, 1, max_len, g);
+        const Pathlet c0{first.p, first.n, -ray_v, mk(1.0f, 1.0f, 1.0f), first.mat};
+        cam.store(0, c0);
+        uint32_t cam_len = extend_walk<false>(cam, c0, 1, max_len, g);
 
         LightSample em = sample_light(S, g);
         float e0 = g.draw();
@@ -405,12 +432,9 @@ template <int WG, int LDSD> struct Tracer {
         uint32_t light_len = 0;
         HitRec v0;
         if (trace(em.p, w, v0) && dot(v0.n, -w) > 0.0f) {
-            lig[0].p = v0.p;
-            lig[0].n = v0.n;
-            lig[0].v = -w;
-            lig[0].transport = mk(1.0f, 1.0f, 1.0f);
-            lig[0].mat = v0.mat;
-            light_len = extend_walk<true>(lig, 1, max_len, g);
+            const Pathlet l0{v0.p, v0.n, -w, mk(1.0f, 1.0f, 1.0f), v0.mat};
+            lig.store(0, l0);
+            light_len = extend_walk<true>(lig, l0, 1, max_len, g);
         }
         float cos_e = dot(em.n, w);
         f3 emitted = cos_e > 0.0f ? light_rad(S, em.light) * cos_e : mk(0.0f, 0.0f, 0.0f);
@@ -425,10 +449,10 @@ template <int WG, int LDSD> struct Tracer {
             for (; cp >= 0 && lp <= light_len; lp++, cp--) {
                 if (lp == 0 && cp == 0) {
                     if (first.light >= 0)
-                        psum = psum + light_radiance(S, first.light, cam[0].v, cam[0].n);
+                        psum = psum + light_radiance(S, first.light, c0.v, c0.n);
                     wsum = wsum + 1.0f;
                 } else if (lp == 0) {
-                    Pathlet const cj = cam[cp - 1];
+                    Pathlet const cj = cam.load(cp - 1);
                     f3 tr = mk(0.0f, 0.0f, 0.0f), wi;
                     float r2, cl;
                     if (!direct(em, cj.p, cj.n, cj.mat, cj.v, tr, wi, r2, cl))
@@ -436,8 +460,8 @@ template <int WG, int LDSD> struct Tracer {
                     psum = psum + mul(tr, cj.transport);
                     wsum = wsum + 1.0f;
                 } else if (cp > 0) {
-                    Pathlet const lj = lig[lp - 1];
-                    Pathlet const cj = cam[cp - 1];
+                    Pathlet const lj = lig.load(lp - 1);
+                    Pathlet const cj = cam.load(cp - 1);
                     f3 join = cj.p - lj.p;
                     float dist = norm(join);
                     join = join / dist;

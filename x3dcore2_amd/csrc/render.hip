@@ -214,7 +214,10 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ uint2 lds[kLdsDepth * kWG];
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
     Tracer<kWG, kLdsDepth> tr{P.S, st};
-    Pathlet cam[kMaxPathLen], lig[kMaxPathLen];
+    // this wave's block: both subpaths of its 64 lanes
+    const size_t wave = (blockIdx.x * kWG + threadIdx.x) >> 6;
+    const PathletRows cam{P.subpaths + wave * (2 * kPathletRows * kMaxPathLen * 64u) + __lane_id()};
+    const PathletRows lig{cam.base + kPathletRows * kMaxPathLen * 64u};
     uint32_t npix = P.width * P.height;
     uint32_t total = npix * gen_samples;
     uint32_t grid_threads = gridDim.x * kWG;
@@ -467,7 +470,7 @@ struct x3d_scene {
 
 struct x3d_renderer {
     x3d_render_options opt;
-    x3d::DevBuf first, acc, out, spill, counters, samples;
+    x3d::DevBuf first, acc, out, spill, counters, samples, subpaths;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;
     x3d::WavefrontEngine *wf = nullptr;
@@ -684,13 +687,17 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         break;
     }
     const size_t spill_block = static_cast<size_t>(kWG) * P.spill_per_thread * sizeof(uint2);
+    // the plain bidirectional kernel's subpaths: 2 x kMaxPathLen vertices of 4 float4 per lane
+    const bool bd_plain = tracer == X3D_PT_BIDIRECT_MIS && !wf;
+    const size_t subpath_block =
+        bd_plain ? static_cast<size_t>(kWG) * 2 * kMaxPathLen * kPathletRows * sizeof(float4) : 0;
     if (arena) {
-        // Keep room for one sample per generation; the traversal-stack spill of the resident
-        // lanes takes at most half of the rest, fewer workgroups where it would not fit (every
-        // kernel fetches its work dynamically, so any grid size gives the same film).
-        const size_t one_sample = ScratchArena::up(npix * sizeof(float4)) + 256;
+        // Keep room for one sample per generation; the traversal-stack spill (and subpaths) of the
+        // resident lanes take at most half of the rest, fewer workgroups where they would not fit
+        // (every kernel fetches its work dynamically, so any grid size gives the same film).
+        const size_t one_sample = ScratchArena::up(npix * sizeof(float4)) + 512;
         const size_t left = arena->left();
-        const size_t cap = left > one_sample ? (left - one_sample) / 2 / spill_block : 0;
+        const size_t cap = left > one_sample ? (left - one_sample) / 2 / (spill_block + subpath_block) : 0;
         if (cap == 0)
             return fail(X3D_ERR_INVALID_ARGUMENT, "the film needs more than the " + std::to_string(arena->bytes >> 20) +
                                                       " MiB of fixed scratch memory");
@@ -701,6 +708,12 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
     if (int rc = buffer(r->spill, static_cast<size_t>(grid) * spill_block, &p_spill))
         return rc;
     P.spill = static_cast<uint2 *>(p_spill);
+    if (bd_plain) {
+        void *p_sub;
+        if (int rc = buffer(r->subpaths, static_cast<size_t>(grid_render) * subpath_block, &p_sub))
+            return rc;
+        P.subpaths = static_cast<float4 *>(p_sub);
+    }
     // Counters are cleared by a kernel, not a memset, so the sequence stays capturable in a graph.
     uint32_t *next_pixel = reinterpret_cast<uint32_t *>(P.cnt + 2);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream, reinterpret_cast<unsigned long long *>(P.cnt),

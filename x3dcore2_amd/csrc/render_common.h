@@ -106,6 +106,8 @@ struct RenderParams {
     int32_t firefly; // firefly filter on the sample images (sampling_task::run, renderer.cpp:46-83)
     const float *rays; // caller rays (origin, direction: 6 floats per "pixel") instead of the
                        // camera's, or null (x3d_trace_rays)
+    float4 *subpaths;  // plain bidirectional kernel: both subpaths of every resident lane
+                       // (2 x kMaxPathLen x 4 float4 rows per lane)
 };
 
 // The primary ray of pixel `pix`: the caller's ray in ray-list mode, else the camera's.

@@ -111,10 +111,8 @@ constexpr int kAdjWaves = kWG / 64;
 
 // Built for 4 waves per SIMD (128 VGPRs, a few spilled): the compiler's own choice, 3 waves
 // without spills, measured 10% slower.
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(RenderParams P,
-                                                                                        const float4 *dl,
-                                                                                        uint32_t num_mats,
-                                                 unsigned long long *partial, float4 *records) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void
+k_adjoint(RenderParams P, const float4 *dl, uint32_t num_mats, unsigned long long *partial, float4 *records) {
     __shared__ uint2 lds[kLdsDepth * kWG];
     __shared__ unsigned long long slots[kAdjWaves][4 * kAdjMaxMats];
     for (uint32_t i = threadIdx.x; i < kAdjWaves * 4 * kAdjMaxMats; i += kWG)

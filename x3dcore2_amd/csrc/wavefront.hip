@@ -1394,12 +1394,22 @@ struct WavefrontEngine {
     WavefrontTuning key_tune{};
     bool have_graph = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // a bounce's shadow and extension traversals run concurrently: the shadow kernel on a second
+    // stream, forked from and joined back into the render's stream
+    hipStream_t s2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     double shadow_ms = 0, extend_ms = 0;
     uint32_t trace_launches = 0;
     ~WavefrontEngine() {
         for (auto &x : ev)
             if (x)
                 (void)hipEventDestroy(x);
+        if (fork)
+            (void)hipEventDestroy(fork);
+        if (join)
+            (void)hipEventDestroy(join);
+        if (s2)
+            (void)hipStreamDestroy(s2);
         if (exec)
             (void)hipGraphExecDestroy(exec);
         if (graph)
@@ -1671,7 +1681,14 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     uint32_t g_shadow = grid_for(tk.shadow, wg), g_extend = grid_for(tk.extend, wg);
     uint32_t trace_threads = (g_shadow > g_extend ? g_shadow : g_extend) * wg;
     uint32_t spill_per_thread = spill_entries_for(P.S, tk.ldsd);
-    WF_HIP(e->spill.ensure(sizeof(uint2) * static_cast<size_t>(trace_threads) * spill_per_thread));
+    // one stack-spill slice per thread of each of the two concurrent traversal kernels
+    const size_t spill_half = static_cast<size_t>(trace_threads) * spill_per_thread;
+    WF_HIP(e->spill.ensure(2 * sizeof(uint2) * spill_half));
+    if (!e->s2) {
+        WF_HIP(hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking));
+        WF_HIP(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+        WF_HIP(hipEventCreateWithFlags(&e->join, hipEventDisableTiming));
+    }
 
     WfArgs A{};
     A.S = P.S;
@@ -1777,13 +1794,26 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
 
     bool profile = tune.profile != 0;
     auto trace_bounce = [&](hipStream_t s, WfArgs const &B) {
-        if (profile)
-            (void)hipEventRecord(e->ev[0], s);
-        hipLaunchKernelGGL(tk.shadow, dim3(g_shadow), dim3(wg), 0, s, B);
-        if (profile)
-            (void)hipEventRecord(e->ev[1], s);
+        WfArgs Bs = B; // the shadow kernel's own stack-spill slices
+        Bs.spill = B.spill + spill_half;
+        if (!profile) {
+            // independent traversals: each fills the other's tail (lanes whose queue ran dry)
+            (void)hipEventRecord(e->fork, s);
+            (void)hipStreamWaitEvent(e->s2, e->fork, 0);
+            hipLaunchKernelGGL(tk.extend, dim3(g_extend), dim3(wg), 0, s, B);
+            hipLaunchKernelGGL(tk.shadow, dim3(g_shadow), dim3(wg), 0, e->s2, Bs);
+            (void)hipEventRecord(e->join, e->s2);
+            (void)hipStreamWaitEvent(s, e->join, 0);
+            hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, s, B.ctl, static_cast<const WfCounter *>(B.eq_cnt_in),
+                               B.cnt);
+            return;
+        }
+        // profiled: one after the other, each timed with events
+        (void)hipEventRecord(e->ev[0], s);
+        hipLaunchKernelGGL(tk.shadow, dim3(g_shadow), dim3(wg), 0, s, Bs);
+        (void)hipEventRecord(e->ev[1], s);
         hipLaunchKernelGGL(tk.extend, dim3(g_extend), dim3(wg), 0, s, B);
-        if (profile) {
+        {
             (void)hipEventRecord(e->ev[2], s);
             float a = 0, b = 0;
             (void)hipEventSynchronize(e->ev[2]);

@@ -171,12 +171,13 @@ def traffic_from_profile(name, kernel, config):
         return None
 
 
-def leg_bdpt(x3d, steps, warmup, spp=0, cpu=True, budget_s=8.0):
+def leg_bdpt(x3d, steps, warmup, spp=0, cpu=True, budget_s=8.0, plain=True):
     """The bidirectional tracer with the reference's uniform strategy weights on the indirectly lit
     room with four 20k-triangle GGX spheres, 512 x 512, 32 spp, subpaths of up to 6 vertices (film in
     HBM), in the wavefront form (the default) and the plain form, whose films must be identical;
     beside it the reference's own bidirect_mis_path_tracer (oracle/_ref, its fixed 8-vertex
-    subpaths) on the host cores at a reduced resolution sized to ~budget_s."""
+    subpaths) on the host cores at a reduced resolution sized to ~budget_s. plain=False skips the
+    plain form."""
     from x3dcore2_amd import scenes
     from x3dcore2_amd.device import DeviceArray, synchronize
 
@@ -196,17 +197,19 @@ def leg_bdpt(x3d, steps, warmup, spp=0, cpu=True, budget_s=8.0):
         return time.perf_counter() - t0, stats, film.to_host(np.float32, (w * h, 4))
 
     dt, stats, img = timed(x3d.FORM_WAVEFRONT, steps)
-    dp, _, img_p = timed(x3d.FORM_PLAIN, max(1, steps // 2))
     rays = sum(st.closest_queries + st.anyhit_queries for st in stats)
     res = {"workload": f"indirect room + 4 GGX spheres ({desc.num_tris} tris)", "width": w, "height": h, "spp": spp,
            "max_path_len": 6, "form": "wavefront", "msamples_per_s": w * h * spp * steps / dt / 1e6,
            "ms_per_step": dt * 1e3 / steps, "mrays_per_s": rays / dt / 1e6,
-           "rays_per_sample": rays / (w * h * spp * steps), "plain_ms_per_step": dp * 1e3 / max(1, steps // 2),
-           "films_bit_identical": bool(np.array_equal(img.view(np.uint32), img_p.view(np.uint32)))}
-    res["speedup_vs_plain"] = res["plain_ms_per_step"] / res["ms_per_step"]
-    if not res["films_bit_identical"]:
-        raise SystemExit("plain and wavefront bidirectional films differ")
-    log(f"bdpt leg: {res['ms_per_step']:.1f} ms per render ({res['plain_ms_per_step']:.1f} plain), "
+           "rays_per_sample": rays / (w * h * spp * steps)}
+    if plain:
+        dp, _, img_p = timed(x3d.FORM_PLAIN, max(1, steps // 2))
+        res["plain_ms_per_step"] = dp * 1e3 / max(1, steps // 2)
+        res["films_bit_identical"] = bool(np.array_equal(img.view(np.uint32), img_p.view(np.uint32)))
+        res["speedup_vs_plain"] = res["plain_ms_per_step"] / res["ms_per_step"]
+        if not res["films_bit_identical"]:
+            raise SystemExit("plain and wavefront bidirectional films differ")
+    log(f"bdpt leg: {res['ms_per_step']:.1f} ms per render ({res.get('plain_ms_per_step', 0):.1f} plain), "
         f"{res['msamples_per_s']:.1f} Msamples/s")
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -281,11 +284,12 @@ def bench_secondary(args):
     out = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded procedural scenes)"}
     if args.workload == "indirect":
-        res = leg_bdpt(x3d, args.steps, args.warmup, args.spp, cpu=not args.no_cpu_baseline)
+        res = leg_bdpt(x3d, args.steps, args.warmup, args.spp, cpu=not args.no_cpu_baseline,
+                       plain=args.form != "wavefront")
         out.update(metric="bidirectional path tracing Msamples/s", unit="Msamples/s", value=res["msamples_per_s"],
                    ms_per_step=res["ms_per_step"], mrays_per_s=res["mrays_per_s"],
-                   rays_per_sample=res["rays_per_sample"], plain_ms_per_step=res["plain_ms_per_step"],
-                   speedup_vs_plain=res["speedup_vs_plain"], films_bit_identical=res["films_bit_identical"],
+                   rays_per_sample=res["rays_per_sample"], plain_ms_per_step=res.get("plain_ms_per_step"),
+                   speedup_vs_plain=res.get("speedup_vs_plain"), films_bit_identical=res.get("films_bit_identical"),
                    cpu_reference=res.get("cpu_reference"),
                    config={"workload": res["workload"], "width": res["width"], "height": res["height"],
                            "spp": res["spp"], "max_path_len": 6, "form": "wavefront",

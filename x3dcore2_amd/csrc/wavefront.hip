@@ -88,7 +88,17 @@ struct WfArgs {
     int fused;
     // lt1: the first hit's emission per path, added when the path ends (w = 1 when there is one).
     float4 *le0;
+    // Fused form: path state follows the queues. xp ... rng above are the state of the paths in
+    // the extension queue this bounce consumes (entry k's state at index k; L.w = path id), the
+    // *_o arrays that of the queue it fills; lfin_in / lfin_out hold (radiance, path id) of the
+    // finishing queues. A shadow ray names the entry whose radiance it adds to: an extension
+    // entry, or a finishing entry with kWfFinBit set.
+    int compact;
+    float4 *xp_o, *xn_o, *wo_o, *beta_o, *L_o, *wi_o;
+    uint64_t *rng_o;
+    float4 *lfin_in, *lfin_out;
 };
+constexpr uint32_t kWfFinBit = 0x80000000u;
 
 // Ray keys: direction octant (3 bits) above the Morton code of the origin's cell in an 8^3 grid
 // over the scene box (9 bits). Rays with equal keys start close together and head the same
@@ -268,15 +278,15 @@ X3D_DI uint32_t gen_count(WfArgs const &A, uint32_t gen) {
 struct NeeSample {
     bool has_shadow, ok;
     f3 sc, sdir, edir;
-    float sdist;
+    float sdist, dens;
+    uint64_t rng; // the path's PCG state after both samples
 };
 
 // TR = kWfMis: the light sample's contribution carries the power-heuristic weight and the BSDF is
 // sampled with the evaluated mixture densities (Tracer::mis_bounce); kWfLt1: unweighted
 // (Tracer::direct as li_lt1 adds it) and the reference's mixture sampling; kWfUnidirect: BSDF
 // sample only (no light sample).
-template <int TR>
-X3D_DI NeeSample nee_sample(WfArgs const &A, uint32_t path, HitRec const &x, f3 wo, f3 beta, Pcg &g) {
+template <int TR> X3D_DI NeeSample nee_sample(WfArgs const &A, HitRec const &x, f3 wo, f3 beta, Pcg &g) {
     NeeSample r;
     r.has_shadow = false;
     r.sc = r.sdir = mk(0.0f, 0.0f, 0.0f);
@@ -308,11 +318,10 @@ X3D_DI NeeSample nee_sample(WfArgs const &A, uint32_t path, HitRec const &x, f3 
     }
     float dens;
     f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, TR != kWfMis);
-    A.rng[path] = g.state;
+    r.rng = g.state;
     // li_mis stops on !(dens > 0); li_lt1 / li_unidirect on dens == 0
     r.ok = TR == kWfMis ? dens > 0.0f : dens != 0.0f;
-    if (r.ok)
-        A.wi[path] = make_float4(wi.x, wi.y, wi.z, dens);
+    r.dens = dens;
     r.edir = wi;
     return r;
 }
@@ -337,20 +346,25 @@ template <int TR> X3D_DI void finish_path(WfArgs const &A, uint32_t path, f3 L) 
 
 // Queues a sampled vertex's shadow ray and extension ray (or finishes the path when the BSDF
 // sample failed). Block-wide: every thread of the block calls it at one uniform call site
-// (block_append synchronizes); `scratch` holds 6 zeroed words.
-X3D_DI void push_rays(WfArgs const &A, bool in, uint32_t path, f3 xp3, NeeSample const &r, uint32_t *scratch) {
+// (block_append synchronizes); `scratch` holds 6 zeroed words. idx[1] / idx[2] receive the
+// path's extension / finishing entry. In the compact form the caller stores the path's state at
+// that entry; otherwise the queues hold path ids.
+X3D_DI void push_rays(WfArgs const &A, bool in, uint32_t path, f3 xp3, NeeSample const &r, uint32_t *scratch,
+                      uint32_t idx[3]) {
     uint32_t *const queues[3] = {&A.ctl->sq_count.v, &A.eq_cnt_out->v, &A.fin_cnt_out->v};
     const bool want[3] = {in && r.has_shadow, in && r.ok, in && !r.ok};
-    uint32_t idx[3];
     block_append<3>(queues, want, idx, scratch);
     if (want[0]) {
         uint32_t q = idx[0];
+        const uint32_t tgt = !A.compact ? path : (r.ok ? idx[1] : (idx[2] | kWfFinBit));
         A.sq[3 * q] = make_float4(xp3.x, xp3.y, xp3.z, r.sdist - 1e-3f);
-        A.sq[3 * q + 1] = make_float4(r.sdir.x, r.sdir.y, r.sdir.z, __uint_as_float(path));
+        A.sq[3 * q + 1] = make_float4(r.sdir.x, r.sdir.y, r.sdir.z, __uint_as_float(tgt));
         A.sq[3 * q + 2] = make_float4(r.sc.x, r.sc.y, r.sc.z, 0.0f);
         if (A.sort_rays)
             A.ray_key[0][q] = ray_key(A.S, xp3, r.sdir);
     }
+    if (A.compact)
+        return;
     if (want[1]) {
         A.eq_out[idx[1]] = path;
         if (A.sort_rays)
@@ -358,6 +372,23 @@ X3D_DI void push_rays(WfArgs const &A, bool in, uint32_t path, f3 xp3, NeeSample
     }
     if (want[2])
         A.fin_out[idx[2]] = path;
+}
+
+// Compact form: the state of a live path after its vertex's samples, at its new queue entry.
+X3D_DI void store_state(WfArgs const &A, uint32_t path, HitRec const &x, f3 wo, uint32_t pix, f3 beta, f3 L,
+                        NeeSample const &r, uint32_t const idx[3]) {
+    if (r.ok) {
+        const uint32_t e = idx[1];
+        A.xp_o[e] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
+        A.xn_o[e] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
+        A.wo_o[e] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
+        A.beta_o[e] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+        A.L_o[e] = make_float4(L.x, L.y, L.z, __uint_as_float(path));
+        A.wi_o[e] = make_float4(r.edir.x, r.edir.y, r.edir.z, r.dens);
+        A.rng_o[e] = r.rng;
+    } else {
+        A.lfin_out[idx[2]] = make_float4(L.x, L.y, L.z, __uint_as_float(path));
+    }
 }
 
 template <bool FUSED, int TR> __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
@@ -373,19 +404,22 @@ template <bool FUSED, int TR> __global__ __launch_bounds__(256) void k_wf_genera
         NeeSample r;
         r.has_shadow = r.ok = false;
         f3 xp3 = mk(0.0f, 0.0f, 0.0f);
+        HitRec x{};
+        f3 L = mk(0.0f, 0.0f, 0.0f), wo = L;
+        uint32_t pix = 0;
         if (id < n) {
-            uint32_t s_local = id / A.npix, pix = id - s_local * A.npix;
+            uint32_t s_local = id / A.npix;
+            pix = id - s_local * A.npix;
             uint32_t s = A.s_begin + gen * A.gen_samples + s_local;
             f3 o, v;
             camera_ray(A.cam, pix % A.width, pix / A.width, A.width, A.height, o, v);
             RenderParams RP;
             RP.first = A.first;
             bool valid;
-            HitRec x = load_first(RP, pix, valid);
+            x = load_first(RP, pix, valid);
             // Tracer::mis_begin / the first vertex of li_lt1, li_unidirect
             Pcg g = pcg_for(A.seed, pix, s);
-            f3 L = mk(0.0f, 0.0f, 0.0f);
-            f3 wo = -v;
+            wo = -v;
             if (TR == kWfMis) {
                 if (valid && x.light >= 0)
                     L = L + light_radiance(A.S, x.light, wo, x.n);
@@ -403,25 +437,28 @@ template <bool FUSED, int TR> __global__ __launch_bounds__(256) void k_wf_genera
             }
             if (!live) {
                 A.lbuf[id] = make_float4(L.x, L.y, L.z, 0.0f);
+            } else if (FUSED) {
+                // the state is stored at the path's queue entry, after the append below
+                xp3 = x.p;
+                r = nee_sample<TR>(A, x, wo, mk(1.0f, 1.0f, 1.0f), g);
             } else {
                 A.xp[id] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
                 A.xn[id] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
                 A.wo[id] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
                 A.beta[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
                 A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
-                if (FUSED) {
-                    xp3 = x.p;
-                    r = nee_sample<TR>(A, id, x, wo, mk(1.0f, 1.0f, 1.0f), g);
-                } else {
-                    A.rng[id] = g.state;
-                    kind = kind_of(A.S, x.mat);
-                }
+                A.rng[id] = g.state;
+                kind = kind_of(A.S, x.mat);
             }
         }
-        if (FUSED)
-            push_rays(A, live, id, xp3, r, scratch);
-        else
+        if (FUSED) {
+            uint32_t idx[3];
+            push_rays(A, live, id, xp3, r, scratch, idx);
+            if (live)
+                store_state(A, id, x, wo, pix, mk(1.0f, 1.0f, 1.0f), L, r, idx);
+        } else {
             push_active(A, 0, kind, id, live, scratch);
+        }
     }
 }
 
@@ -451,10 +488,14 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
             x.light = static_cast<int32_t>(__float_as_uint(xn.w));
             xp3 = x.p;
             Pcg g = path_pcg(A.rng[path], __float_as_uint(wo4.w));
-            r = nee_sample<kWfMis>(A, path, x, xyz(wo4), xyz(b4), g);
+            r = nee_sample<kWfMis>(A, x, xyz(wo4), xyz(b4), g);
+            A.rng[path] = r.rng;
+            if (r.ok)
+                A.wi[path] = make_float4(r.edir.x, r.edir.y, r.edir.z, r.dens);
         }
         // one call site with all lanes active: block_append synchronizes the block
-        push_rays(A, in, path, xp3, r, scratch);
+        uint32_t idx[3];
+        push_rays(A, in, path, xp3, r, scratch, idx);
     }
 }
 
@@ -662,11 +703,13 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
             if (BD) {
                 A.sq[2 * static_cast<size_t>(i)].w = blocked ? 0.0f : 1.0f;
             } else if (!blocked) {
-                uint32_t path = __float_as_uint(A.sq_in[3 * i + 1].w);
+                const uint32_t tgt = __float_as_uint(A.sq_in[3 * i + 1].w);
+                float4 *const Lp =
+                    (A.compact && (tgt & kWfFinBit)) ? A.lfin_in + (tgt & ~kWfFinBit) : A.L + tgt;
                 float4 c = A.sq_in[3 * i + 2];
-                float4 L = A.L[path];
+                float4 L = *Lp;
                 f3 nl = xyz(L) + xyz(c);
-                A.L[path] = make_float4(nl.x, nl.y, nl.z, L.w);
+                *Lp = make_float4(nl.x, nl.y, nl.z, L.w);
             }
             i = wave_fetch_batched(&A.ctl->sq_head.v, slot);
             if (i < n)
@@ -706,7 +749,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     uint32_t best_leaf = 0, ref = kDone, leaf = kDone, cur = 0; // cur: next triangle of the parked leaf
     bool wide = false;
     auto init = [&](uint32_t k) {
-        uint32_t path = A.eq_in[k];
+        const uint32_t path = A.compact ? k : A.eq_in[k];
         o = xyz(A.xp[path]);
         v = xyz(A.wi[path]);
         vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
@@ -798,17 +841,26 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
         uint32_t t = b + threadIdx.x;
         bool live = false;
         int kind = 0;
-        uint32_t path = 0;
+        uint32_t path = 0, pix = 0;
         NeeSample r;
         r.has_shadow = r.ok = false;
         f3 xp3 = mk(0.0f, 0.0f, 0.0f);
+        HitRec nx{};
+        f3 L = xp3, beta = xp3, wi = xp3;
         if (t >= total) {
             // no item: only takes part in the block-wide append below
         } else if (t >= ne) {
-            path = A.fin_in[t - ne];
-            finish_path<TR>(A, path, xyz(A.L[path]));
+            if (FUSED) {
+                // compact form: the finishing entry holds the path's radiance and id
+                const float4 lf = A.lfin_in[t - ne];
+                finish_path<TR>(A, __float_as_uint(lf.w), xyz(lf));
+            } else {
+                path = A.fin_in[t - ne];
+                finish_path<TR>(A, path, xyz(A.L[path]));
+            }
         } else {
-            path = A.eq_in[t];
+            // compact form: entry t's state at index t; otherwise at its path id
+            const uint32_t si = FUSED ? t : A.eq_in[t];
             ClosestHit h = A.eh[t];
             // Every load of the item issues at once: the path's state and the hit triangle (a
             // miss reads triangle 0, which every scene with a possible hit has), so one memory
@@ -818,11 +870,12 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
             const float4 *attr = static_cast<const float4 *>(A.S.tri_attr);
             float4 ta = tris[3 * pi], tb = tris[3 * pi + 1], tc = tris[3 * pi + 2];
             float4 na = attr[3 * pi], nb = attr[3 * pi + 1], nc = attr[3 * pi + 2];
-            float4 L4 = A.L[path], wi4 = A.wi[path], xn = A.xn[path], wo4 = A.wo[path], b4 = A.beta[path];
-            float4 xp4 = A.xp[path];
-            f3 L = xyz(L4);
-            f3 wi = xyz(wi4);
-            HitRec nx;
+            float4 L4 = A.L[si], wi4 = A.wi[si], xn = A.xn[si], wo4 = A.wo[si], b4 = A.beta[si];
+            float4 xp4 = A.xp[si];
+            path = FUSED ? __float_as_uint(L4.w) : si;
+            pix = __float_as_uint(wo4.w);
+            L = xyz(L4);
+            wi = xyz(wi4);
             bool hit = h.prim >= 0;
             if (hit) {
                 nx = finish_hit(h, ta, tb, tc, na, nb, nc);
@@ -833,7 +886,8 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
             } else {
                 float dens = wi4.w;
                 int32_t xmat = static_cast<int32_t>(__float_as_uint(xp4.w));
-                f3 xnrm = xyz(xn), wo = xyz(wo4), beta = xyz(b4);
+                f3 xnrm = xyz(xn), wo = xyz(wo4);
+                beta = xyz(b4);
                 float cw = dot(xnrm, wi);
                 f3 f = bsdf_eval(A.S.mats, xmat, xnrm, wo, wi);
                 beta = mul(beta, f) * cw / dens;
@@ -849,14 +903,14 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
                     }
                     go = bounce + 1 < A.max_depth;
                     if (go && FUSED)
-                        g.state = A.rng[path];
+                        g.state = A.rng[si];
                 } else {
                     // li_lt1 / li_unidirect at depth bounce + 1: Russian roulette p = 0.5 from depth 2,
                     // then (unidirect) the vertex's emission; the loop stops after kRefModeMaxDepth vertices
                     const uint32_t depth = bounce + 1;
                     go = depth < kRefModeMaxDepth;
                     if (go) {
-                        g.state = A.rng[path];
+                        g.state = A.rng[si];
                         if (depth >= 2) {
                             if (g.draw() >= 0.5f)
                                 go = false;
@@ -870,27 +924,32 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
                 if (!go) {
                     finish_path<TR>(A, path, L);
                 } else {
-                    A.xp[path] =
-                        make_float4(nx.p.x, nx.p.y, nx.p.z, __uint_as_float(static_cast<uint32_t>(nx.mat)));
-                    A.xn[path] =
-                        make_float4(nx.n.x, nx.n.y, nx.n.z, __uint_as_float(static_cast<uint32_t>(nx.light)));
-                    A.wo[path] = make_float4(-wi.x, -wi.y, -wi.z, wo4.w);
-                    A.beta[path] = make_float4(beta.x, beta.y, beta.z, b4.w);
-                    A.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
                     live = true;
                     if (FUSED) {
+                        // the state is stored at the path's new queue entry, after the append
                         xp3 = nx.p;
-                        r = nee_sample<TR>(A, path, nx, -wi, beta, g);
+                        r = nee_sample<TR>(A, nx, -wi, beta, g);
                     } else {
+                        A.xp[path] =
+                            make_float4(nx.p.x, nx.p.y, nx.p.z, __uint_as_float(static_cast<uint32_t>(nx.mat)));
+                        A.xn[path] =
+                            make_float4(nx.n.x, nx.n.y, nx.n.z, __uint_as_float(static_cast<uint32_t>(nx.light)));
+                        A.wo[path] = make_float4(-wi.x, -wi.y, -wi.z, wo4.w);
+                        A.beta[path] = make_float4(beta.x, beta.y, beta.z, b4.w);
+                        A.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
                         kind = kind_of(A.S, nx.mat);
                     }
                 }
             }
         }
-        if (FUSED)
-            push_rays(A, live, path, xp3, r, scratch);
-        else
+        if (FUSED) {
+            uint32_t idx[3];
+            push_rays(A, live, path, xp3, r, scratch, idx);
+            if (live)
+                store_state(A, path, nx, -wi, pix, beta, L, r, idx);
+        } else {
             push_active(A, parity ^ 1, kind, path, live, scratch);
+        }
     }
 }
 
@@ -1383,6 +1442,8 @@ __global__ void k_bd_count(WfCounter const *c, Counters *cnt, int anyhit) {
 struct WavefrontEngine {
     DevBuf xp, xn, wo, beta, L, wi, rng, lbuf, act0, act1, sq, eq, fin, eh, ctl, spill;
     DevBuf eq_b, fin_b; // the fused form's second queue parity
+    // the fused form's second parity of the path state, and the finishing queues' (radiance, id)
+    DevBuf xp_b, xn_b, wo_b, beta_b, L_b, wi_b, rng_b, lfin_a, lfin_b;
     DevBuf le0;         // lt1's first-hit emission per path
     DevBuf sq_sorted, eq_sorted, ray_key, ray_rank, bin_mat, bin_tot; // ray reordering
     // bidirectional form: per item state, PCG state, walk ray, emission sample, vertices,
@@ -1656,6 +1717,16 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     if (tune.tracer == kWfLt1)
         WF_HIP(e->le0.ensure(f4));
     if (fused) {
+        // path state follows the queues: a second parity of every state array
+        WF_HIP(e->xp_b.ensure(f4));
+        WF_HIP(e->xn_b.ensure(f4));
+        WF_HIP(e->wo_b.ensure(f4));
+        WF_HIP(e->beta_b.ensure(f4));
+        WF_HIP(e->L_b.ensure(f4));
+        WF_HIP(e->wi_b.ensure(f4));
+        WF_HIP(e->rng_b.ensure(sizeof(uint64_t) * cap));
+        WF_HIP(e->lfin_a.ensure(f4));
+        WF_HIP(e->lfin_b.ensure(f4));
         WF_HIP(e->eq_b.ensure(sizeof(uint32_t) * cap));
         WF_HIP(e->fin_b.ensure(sizeof(uint32_t) * cap));
     } else {
@@ -1738,6 +1809,7 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     A.eq_cnt_in = A.eq_cnt_out = &A.ctl->eq_count;
     A.fin_cnt_in = A.fin_cnt_out = &A.ctl->fin_count;
     A.fused = fused ? 1 : 0;
+    A.compact = fused ? 1 : 0;
     A.le0 = tune.tracer == kWfLt1 ? static_cast<float4 *>(e->le0.p) : nullptr;
     if (tune.sort_rays) {
         A.sq_sorted = static_cast<float4 *>(e->sq_sorted.p);
@@ -1780,6 +1852,28 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
         WfCounter *eqc[2] = {&A.ctl->eq_count, &A.ctl->eq_count_b};
         WfCounter *finc[2] = {&A.ctl->fin_count, &A.ctl->fin_count_b};
         const uint32_t p = k & 1u, q = p ^ 1u;
+        auto f4p = [](DevBuf &d) { return static_cast<float4 *>(d.p); };
+        float4 *xps[2] = {f4p(e->xp), f4p(e->xp_b)}, *xns[2] = {f4p(e->xn), f4p(e->xn_b)},
+               *wos[2] = {f4p(e->wo), f4p(e->wo_b)}, *betas[2] = {f4p(e->beta), f4p(e->beta_b)},
+               *Ls[2] = {f4p(e->L), f4p(e->L_b)}, *wis[2] = {f4p(e->wi), f4p(e->wi_b)},
+               *lfins[2] = {f4p(e->lfin_a), f4p(e->lfin_b)};
+        uint64_t *rngs[2] = {static_cast<uint64_t *>(e->rng.p), static_cast<uint64_t *>(e->rng_b.p)};
+        B.xp = xps[p];
+        B.xn = xns[p];
+        B.wo = wos[p];
+        B.beta = betas[p];
+        B.L = Ls[p];
+        B.wi = wis[p];
+        B.rng = rngs[p];
+        B.xp_o = xps[q];
+        B.xn_o = xns[q];
+        B.wo_o = wos[q];
+        B.beta_o = betas[q];
+        B.L_o = Ls[q];
+        B.wi_o = wis[q];
+        B.rng_o = rngs[q];
+        B.lfin_in = lfins[p];
+        B.lfin_out = lfins[q];
         B.eq = B.eq_out = eqs[q];
         B.eq_in = eqs[p];
         B.fin = B.fin_out = fins[q];

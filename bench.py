@@ -274,7 +274,8 @@ def bench_secondary(args):
     indirect: the bidirectional tracer on the indirectly lit room with four 20k-triangle GGX
               spheres, 512 x 512, 32 spp, subpaths of up to 6 vertices;
     batch:    64 seeded scenes (1k-2M triangles each, 128^2-1024^2, 4-64 spp, depth 6) in one
-              persistent launch, against rendering the same jobs one after another;
+              x3d_render_batch call (wavefront form: every job's paths in shared generations),
+              against rendering the same jobs one after another and the plain form's batch;
     adjoint:  gradient evaluations (forward render, L2 loss on the host, path-replay backward)
               of the 16-surface inverse-rendering room, 256 x 256, 32 spp, depth 4."""
     import x3dcore2_amd as x3d
@@ -315,7 +316,7 @@ def bench_secondary(args):
     items = [(built[k], d.camera, d.width, d.height, d.spp) for k, d in enumerate(descs)]
     ptrs = [f.ptr for f in films]
     samples = sum(d.width * d.height * d.spp for d in descs)
-    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=1)
+    rp = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=1)  # plain form
     rw = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=6, seed=1)
 
     def sequential(rr):
@@ -328,38 +329,44 @@ def bench_secondary(args):
             synchronize()
         return time.perf_counter() - t0
 
-    seq = sequential(r)
+    seq = sequential(rp)
     seq_wf = sequential(rw)
-    del rw
     log(f"sequential single-job renders: plain {seq * 1e3:.1f} ms, wavefront {seq_wf * 1e3:.1f} ms")
+
+    def batches(rr, steps, warmup):
+        for _ in range(warmup):
+            rr.render_batch_device(items, ptrs)
+        synchronize()
+        t0 = time.perf_counter()
+        res = [rr.render_batch_device(items, ptrs) for _ in range(steps)]
+        synchronize()
+        return time.perf_counter() - t0, res[-1]
+
+    # the plain form's batch (shared launch of the small jobs, large jobs alone), for comparison
+    dt_plain, (st_plain, _) = batches(rp, 1, 1)
+    log(f"plain-form batch: {dt_plain * 1e3:.1f} ms ({st_plain.alone_jobs} jobs alone)")
     tune = {}
     if not args.no_autotune:
-        # probe with an eighth of each job's samples: same scenes, tiles and ordering problem
-        rep = r.autotune_batch([(sc, cam, w, h, max(1, spp // 8)) for (sc, cam, w, h, spp) in items])
+        # probe with an eighth of each job's samples: same scenes and packing problem
+        rep = rw.autotune_batch([(sc, cam, w, h, max(1, spp // 8)) for (sc, cam, w, h, spp) in items])
         tune = {"workgroup_size": rep.workgroup_size, "stack_depth": rep.stack_depth,
-                "sort_key": ["default", "cost", "job order"][rep.sort_key], "candidates": rep.candidates,
-                "best_ms": rep.best_ms, "default_ms": rep.default_ms}
+                "candidates": rep.candidates, "best_ms": rep.best_ms, "default_ms": rep.default_ms}
         log(f"batch autotune: {tune}")
-    for _ in range(args.warmup):
-        r.render_batch_device(items, ptrs)
-    synchronize()
-    t0 = time.perf_counter()
-    res = [r.render_batch_device(items, ptrs) for _ in range(args.steps)]
-    synchronize()
-    dt = time.perf_counter() - t0
-    st, lat = res[-1]
+    # the wavefront form's batch (every job's paths in shared generations): the measured line
+    dt, (st, lat) = batches(rw, args.steps, args.warmup)
     lat = np.array(lat)
     out.update(metric="batched rendering Msamples/s (64 scenes)", unit="Msamples/s",
                value=samples * args.steps / dt / 1e6, ms_per_step=dt * 1e3 / args.steps,
                kernel_ms=st.kernel_ms, sequential_ms=seq * 1e3, sequential_wavefront_ms=seq_wf * 1e3,
+               plain_form_batch_ms=dt_plain * 1e3,
                speedup_vs_sequential=min(seq, seq_wf) / (dt / args.steps),
                mrays_per_s=(st.closest_queries + st.anyhit_queries) / (st.kernel_ms * 1e-3) / 1e6,
                latency_ms={"p50": float(np.percentile(lat, 50)), "p90": float(np.percentile(lat, 90)),
                            "max": float(lat.max())},
                autotune=tune,
                config={"workload": f"64 seeded scenes, {sum(d.num_tris for d in descs)} tris", "samples": samples,
-                       "max_depth": 6, "tiles": st.tiles, "alone_jobs": st.alone_jobs,
-                       "batch_split": 1 << 22, "tracer": "unidirectional MIS",
+                       "max_depth": 6, "form": "wavefront (every job's paths in shared generations)",
+                       "generations": st.generations, "tracer": "unidirectional MIS",
                        "scene_bytes": scene_bytes})
     print(json.dumps(out), flush=True)
 

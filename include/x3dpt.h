@@ -408,14 +408,19 @@ int x3d_render_device_noalloc(x3d_renderer *r, const x3d_scene *scene, const x3d
 int x3d_pcg_draws(uint64_t seed, uint64_t pixel, uint32_t sample, uint32_t n, uint32_t *out);
 
 /* ---- batched rendering (no reference counterpart: pt_image_renderer::render takes one scene,
- * renderer.h:57-59). Jobs (scene, camera, film size, spp) with fewer than batch_split samples
- * share one persistent launch, captured as a hipGraph and replayed while the batch keeps its
- * shape; larger jobs then run one after another in the wavefront form (fewest samples first).
- * The renderer must be a UNIDIRECT_MIS renderer (of either form); each job's film is bit-identical
- * to x3d_render of that job. Options used: max_depth, seed, batch_split; for the shared launch
- * workgroup_size (64/128/256), stack_depth (8/16), sort_key (NONE: hand tiles out in job order;
- * otherwise most expensive job first); for the large jobs queue_capacity. Jobs may share
- * scenes. ---- */
+ * renderer.h:57-59). The renderer must be a UNIDIRECT_MIS renderer; each job's film is
+ * bit-identical to x3d_render of that job, in either form. Jobs may share scenes.
+ * WAVEFRONT form: after each job's first-hit pass, the paths of all jobs share the generations of
+ * one wavefront render (queues, traversal and shading kernels of every bounce, replayed as one
+ * hipGraph per generation): each generation holds whole samples of several jobs' pixels, up to
+ * queue_capacity paths (fewest-sample jobs first; a job's samples may span generations, in
+ * order). Options used: max_depth, seed, queue_capacity, workgroup_size, stack_depth (sort_key,
+ * ray_order and batch_split do not apply).
+ * PLAIN form: jobs with fewer than batch_split samples share one persistent launch, captured as a
+ * hipGraph and replayed while the batch keeps its shape; larger jobs then run one after another
+ * in the wavefront form (fewest samples first). Options used: max_depth, seed, batch_split; for
+ * the shared launch workgroup_size (64/128/256), stack_depth (8/16), sort_key (NONE: hand tiles
+ * out in job order; otherwise most expensive job first); for the large jobs queue_capacity. ---- */
 typedef struct x3d_batch_job {
     const x3d_scene *scene;
     x3d_camera camera;
@@ -433,8 +438,10 @@ typedef struct x3d_batch_stats {
     uint64_t closest_nodes, closest_tests, anyhit_nodes, anyhit_tests;
     double kernel_ms;         /* device time of the whole batch (HIP events) */
     double total_ms;          /* wall time of the call */
-    uint32_t tiles;           /* work tiles of 64 pixels in the shared launch */
-    uint32_t alone_jobs;      /* jobs at or above batch_split, rendered on their own */
+    uint32_t tiles;           /* PLAIN: work tiles of 64 pixels in the shared launch */
+    uint32_t alone_jobs;      /* PLAIN: jobs at or above batch_split, rendered on their own */
+    uint32_t generations;     /* WAVEFRONT: shared generations */
+    uint32_t reserved;
 } x3d_batch_stats;
 
 int x3d_render_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs, x3d_batch_stats *stats);
@@ -468,8 +475,9 @@ int x3d_render_backward_device(x3d_renderer *r, const x3d_scene *scene, const x3
                                uint32_t width, uint32_t height, uint32_t spp, const float *d_dl_dfilm,
                                float *d_grad, x3d_render_stats *stats);
 
-/* Autotuner for a batch: times every workgroup size x LDS stack depth x sort key on the given
- * jobs (their films are written), keeps the fastest in the renderer's options. */
+/* Autotuner for a batch: times every workgroup size x LDS stack depth (x sort key, PLAIN form:
+ * 12 candidates; WAVEFRONT form: 6) on the given jobs (their films are written), keeps the
+ * fastest in the renderer's options. */
 int x3d_renderer_autotune_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_job *jobs,
                                 x3d_autotune_report *report);
 

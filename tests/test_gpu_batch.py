@@ -1,7 +1,9 @@
 """Batched rendering through the C ABI: every job's film must be bit-identical to rendering that
-job alone with the same renderer (and so to the oracle), whatever the workgroup size, stack depth,
-tile order and split between the shared launch and jobs rendered on their own in the wavefront
-form, on first capture and on graph replay."""
+job alone with the same renderer (and so to the oracle), on first capture and on graph replay.
+Plain form: whatever the workgroup size, stack depth, tile order and split between the shared
+launch and jobs rendered on their own in the wavefront form. Wavefront form (all jobs' paths in
+shared generations): whatever the generation size (one generation for the whole batch, jobs split
+over several generations, generations of a single film's pixels), workgroup size and stack depth."""
 
 import numpy as np
 import pytest
@@ -80,4 +82,55 @@ def test_batch_through_a_wavefront_form_renderer(x3d, batch):
     plain, _, _ = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=9).render_batch(items)
     wf, _, _ = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=6, seed=9).render_batch(items)
     for a, b in zip(plain, wf):
+        assert np.array_equal(_bits(a), _bits(b))
+
+
+def _generations(items, capacity):
+    """Generation count of the wavefront batch: whole samples of the jobs' pixels packed, fewest
+    samples first, into generations of max(capacity, largest film) paths."""
+    cap = max([capacity] + [w * h for (_, _, w, h, _) in items])
+    order = sorted(range(len(items)), key=lambda k: items[k][2] * items[k][3] * items[k][4])
+    gens, used = 0, 0
+    for k in order:
+        npix, left = items[k][2] * items[k][3], items[k][4]
+        while left:
+            fit = (cap - used) // npix
+            if fit == 0:
+                gens, used = gens + 1, 0
+                continue
+            c = min(fit, left)
+            used, left = used + c * npix, left - c
+    return gens + (used > 0)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(queue_capacity=5000), dict(queue_capacity=7000, workgroup_size=256),
+                                dict(queue_capacity=1, workgroup_size=128, stack_depth=8)])
+def test_wavefront_batch_films_identical_to_single_renders(x3d, batch, kw):
+    descs, built, items = batch
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=6, seed=9, **kw)
+    singles = [r.render(s, cam, w, h, spp)[0] for (s, cam, w, h, spp) in items]
+    gens = _generations(items, kw.get("queue_capacity", 1 << 26))
+    for rep in range(2):  # capture, then replay
+        print(f"[gpu] wavefront batch {kw} pass {rep}: {gens} generations", flush=True)
+        films, st, lat = r.render_batch(items)
+        for k, (f, ref) in enumerate(zip(films, singles)):
+            assert np.array_equal(_bits(f), _bits(ref)), (k, np.abs(f - ref).max())
+        assert st.generations == gens and st.alone_jobs == 0 and st.tiles == 0
+        assert st.num_samples == sum(w * h * spp for (_, _, w, h, spp) in items)
+        assert st.closest_queries >= sum(w * h for (_, _, w, h, _) in items)
+        assert all(0 < x <= st.kernel_ms * 1.01 + 0.05 for x in lat), (lat, st.kernel_ms)
+
+
+def test_wavefront_batch_matches_oracle_and_autotunes(x3d, oracle_mod, batch):
+    descs, built, items = batch
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=6, seed=3, queue_capacity=4000)
+    films, st, _ = r.render_batch(items[:3] + items[-2:])
+    assert st.generations > 1
+    for k, (f, d) in enumerate(zip(films, descs[:3] + descs[-1:])):
+        ref = oracle_mod.Oracle(d).render(oracle_mod.TR_UNIDIRECT_MIS, d.width, d.height, d.spp, seed=3, max_depth=6)
+        assert np.array_equal(_bits(f), _bits(ref)), k
+    rep = r.autotune_batch(items)
+    assert rep.candidates == 6 and rep.best_ms > 0 and rep.workgroup_size in (64, 128, 256)
+    again, _, _ = r.render_batch(items[:3] + items[-2:])
+    for a, b in zip(films, again):
         assert np.array_equal(_bits(a), _bits(b))

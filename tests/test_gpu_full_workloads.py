@@ -6,8 +6,9 @@
   does the indirect room's; a 1024 x 1024 render (depth 8, MIS, reduced to 2 spp) equals the
   oracle's render of a seeded 4096-pixel subset bit for bit, in the plain and the wavefront form;
 - the 64-scene batch (1k-2M triangles per scene, 14.2M in all, 128^2 to 1024^2 films, 4-64 spp,
-  depth 6) in one x3d_render_batch call: every job's film is bit-identical to rendering that job
-  alone.
+  depth 6) in one x3d_render_batch call, in the plain form (shared launch plus large jobs alone)
+  and in the wavefront form (every job's paths in shared generations): every job's film is
+  bit-identical to rendering that job alone.
 """
 
 import hashlib
@@ -68,17 +69,26 @@ def test_full_spheres_render_pixel_subset_bit_exact(x3d, oracle_mod, spheres):
     assert plain[:, :3].max() > 0
 
 
-def test_full_64_scene_batch_identical_to_single_renders(x3d):
+@pytest.fixture(scope="module")
+def batch64(x3d):
     descs = scenes.batch_scenes()
     assert len(descs) == 64
     built = [x3d.Scene(d) for d in descs]
     print(f"[gpu] batch: {sum(d.num_tris for d in descs)} tris", flush=True)
     items = [(built[k], d.camera, d.width, d.height, d.spp) for k, d in enumerate(descs)]
-    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=1)
+    solo = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=6, seed=1)
+    refs = [solo.render(*it)[0] for it in items]
+    return descs, built, items, refs
+
+
+@pytest.mark.parametrize("form", ["plain", "wavefront"])
+def test_full_64_scene_batch_identical_to_single_renders(x3d, batch64, form):
+    descs, built, items, refs = batch64
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT if form == "wavefront" else x3d.FORM_PLAIN,
+                     max_depth=6, seed=1)
     films, st, lat = r.render_batch(items)
     assert st.num_samples == sum(d.width * d.height * d.spp for d in descs)
-    print(f"[gpu] batch: {st.kernel_ms:.1f} ms, {st.alone_jobs} jobs alone", flush=True)
-    solo = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, max_depth=6, seed=1)
-    for k, (f, it) in enumerate(zip(films, items)):
-        ref, _ = solo.render(*it)
+    print(f"[gpu] batch ({form}): {st.kernel_ms:.1f} ms, {st.alone_jobs} jobs alone, {st.generations} shared "
+          f"generations", flush=True)
+    for k, (f, ref) in enumerate(zip(films, refs)):
         assert np.array_equal(_bits(f), _bits(ref)), k

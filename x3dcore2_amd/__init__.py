@@ -103,7 +103,8 @@ class BatchStats(C.Structure):
     _fields_ = [("num_samples", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
                 ("closest_nodes", C.c_uint64), ("closest_tests", C.c_uint64), ("anyhit_nodes", C.c_uint64),
                 ("anyhit_tests", C.c_uint64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
-                ("tiles", C.c_uint32), ("alone_jobs", C.c_uint32)]
+                ("tiles", C.c_uint32), ("alone_jobs", C.c_uint32), ("generations", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class ComposeStats(C.Structure):

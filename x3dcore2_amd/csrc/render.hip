@@ -897,9 +897,97 @@ int check_batch(x3d_renderer *r, uint32_t n, x3d_batch_job const *jobs) {
 
 constexpr uint64_t kBatchSplitDefault = 1ull << 22;
 
-// Runs the batch with films already in device memory (films[i] for job i): the jobs below the
-// split share one persistent launch (small films cannot fill the GPU on their own); each job at
-// or above it then runs alone in the wavefront form, fewest samples first. Either way a job's
+// Wavefront form of a batch: each job's first-hit pass, then one multi-job wavefront render in
+// which the paths of all jobs share every generation (x3d::wf_render_jobs).
+int batch_wavefront(x3d_renderer *r, uint32_t n, x3d_batch_job *jobs, float *const *films, x3d_batch_stats *stats,
+                    std::chrono::steady_clock::time_point t0) {
+    std::vector<uint64_t> off(n);
+    uint64_t px = 0, samples = 0;
+    uint32_t spill = 0, grid_first = 1;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint64_t npix = static_cast<uint64_t>(jobs[i].width) * jobs[i].height;
+        off[i] = px;
+        px += npix;
+        samples += npix * jobs[i].spp;
+        const uint32_t sp = spill_entries(jobs[i].scene->s.dev);
+        spill = sp > spill ? sp : spill;
+        const uint32_t g = persistent_grid(k_first_hit, npix);
+        grid_first = g > grid_first ? g : grid_first;
+    }
+    X3D_HIP(r->first.ensure(px * 3 * sizeof(float4)));
+    X3D_HIP(r->acc.ensure(px * sizeof(float4)));
+    X3D_HIP(r->spill.ensure(static_cast<size_t>(grid_first) * kWG * spill * sizeof(uint2)));
+    Counters *cnt = static_cast<Counters *>(r->counters.p);
+    float4 *first = static_cast<float4 *>(r->first.p), *acc = static_cast<float4 *>(r->acc.p);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, r->stream, reinterpret_cast<unsigned long long *>(cnt),
+                       static_cast<uint32_t>((2 * sizeof(Counters) + 64) / 8));
+    X3D_HIP(hipEventRecord(r->bev0, r->stream));
+    std::vector<x3d::WfJob> wj(n);
+    for (uint32_t i = 0; i < n; i++) {
+        RenderParams P{};
+        P.S = jobs[i].scene->s.dev;
+        P.cam = cam_params(jobs[i].camera);
+        P.width = jobs[i].width;
+        P.height = jobs[i].height;
+        P.first = first + 3 * off[i];
+        P.cnt = cnt;
+        P.spill = static_cast<uint2 *>(r->spill.p);
+        P.spill_per_thread = spill;
+        const uint64_t npix = static_cast<uint64_t>(P.width) * P.height;
+        hipLaunchKernelGGL(k_first_hit, dim3(persistent_grid(k_first_hit, npix)), dim3(kWG), 0, r->stream, P);
+        x3d::WfJob &J = wj[i];
+        J.S = P.S;
+        J.cam = P.cam;
+        J.first = P.first;
+        J.acc = acc + off[i];
+        J.out = reinterpret_cast<float4 *>(films[i]);
+        J.width = P.width;
+        J.height = P.height;
+        J.npix = static_cast<uint32_t>(npix);
+        J.spp = jobs[i].spp;
+    }
+    X3D_HIP(hipGetLastError());
+    // the multi-job form runs fused: no material or ray sort
+    x3d::WavefrontTuning wt = wf_tuning(r->opt);
+    wt.tracer = x3d::kWfMis;
+    wt.sort_by_material = 0;
+    wt.sort_rays = 0;
+    wt.profile = 0;
+    if (!r->batch_wf)
+        r->batch_wf = x3d::wf_create();
+    x3d::WfJobsResult res;
+    std::string e =
+        x3d::wf_render_jobs(r->batch_wf, wj, r->opt.max_depth, r->opt.seed, cnt + 1, wt, r->stream, r->bev0, &res);
+    if (!e.empty())
+        return fail(X3D_ERR_DEVICE, e);
+    X3D_HIP(hipEventRecord(r->bev1, r->stream));
+    X3D_HIP(hipEventSynchronize(r->bev1));
+    for (uint32_t i = 0; i < n; i++)
+        jobs[i].latency_ms = res.job_ms[i];
+    if (stats) {
+        Counters cc[2];
+        X3D_HIP(hipMemcpy(cc, cnt, sizeof cc, hipMemcpyDeviceToHost));
+        float ms = 0;
+        X3D_HIP(hipEventElapsedTime(&ms, r->bev0, r->bev1));
+        *stats = x3d_batch_stats{};
+        stats->num_samples = samples;
+        stats->closest_queries = cc[0].closest + cc[1].closest;
+        stats->anyhit_queries = cc[1].anyhit;
+        stats->closest_nodes = cc[0].pops + cc[1].pops;
+        stats->closest_tests = cc[0].tests + cc[1].tests;
+        stats->anyhit_nodes = cc[1].spops;
+        stats->anyhit_tests = cc[1].stests;
+        stats->kernel_ms = ms;
+        stats->generations = res.generations;
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return X3D_OK;
+}
+
+// Runs the batch with films already in device memory (films[i] for job i). Wavefront form: every
+// job's paths share the generations of one multi-job wavefront render. Plain form: the jobs below
+// the split share one persistent launch (small films cannot fill the GPU on their own); each job
+// at or above it then runs alone in the wavefront form, fewest samples first. Either way a job's
 // film is bit-identical to rendering it alone.
 int batch_impl(x3d_renderer *r, uint32_t n, x3d_batch_job *jobs, float *const *films, x3d_batch_stats *stats) {
     auto t0 = std::chrono::steady_clock::now();
@@ -915,6 +1003,8 @@ int batch_impl(x3d_renderer *r, uint32_t n, x3d_batch_job *jobs, float *const *f
         X3D_HIP(hipEventCreate(&r->bev1));
     }
     X3D_HIP(r->counters.ensure(2 * sizeof(Counters) + 64));
+    if (r->opt.form == X3D_FORM_WAVEFRONT)
+        return batch_wavefront(r, n, jobs, films, stats, t0);
     const uint64_t split = r->opt.batch_split ? r->opt.batch_split : kBatchSplitDefault;
     std::vector<x3d::BatchJob> bj;
     std::vector<uint32_t> shared, alone;
@@ -1956,7 +2046,10 @@ int x3d_renderer_autotune_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_jo
     rep.best_ms = 1e300;
     const uint32_t wgs[] = {64, 128, 256};
     const uint32_t depths[] = {8, 16};
-    const int32_t sorts[] = {X3D_SORT_MATERIAL, X3D_SORT_NONE};
+    // the plain form's shared launch: tile order by cost or by job; the wavefront form has no sort
+    std::vector<int32_t> sorts = {X3D_SORT_MATERIAL, X3D_SORT_NONE};
+    if (r->opt.form == X3D_FORM_WAVEFRONT)
+        sorts = {X3D_SORT_DEFAULT};
     auto time_config = [&](uint32_t wg, uint32_t d, int32_t so, double *ms) -> int {
         r->opt.workgroup_size = wg;
         r->opt.stack_depth = d;

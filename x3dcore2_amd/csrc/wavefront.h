@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "render_common.h"
 
@@ -44,6 +45,33 @@ void wf_last_timing(WavefrontEngine const *e, double *shadow_ms, double *extend_
 // samples [P.s_begin, P.s_end) of every pixel, in generations of `paths_in_flight / pixels`
 // samples; writes P.out (P.finalize) or P.acc. Returns an empty string or an error message.
 std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, hipStream_t stream);
+
+// One job of a multi-job render (a batch whose jobs share every generation): its scene, camera,
+// first-hit cache (3 float4 per pixel, filled beforehand by the first-hit pass), running sums
+// kept between the job's generations, and film.
+struct WfJob {
+    DevScene S;
+    CamParams cam;
+    float4 *first, *acc, *out;
+    uint32_t width, height, npix, spp;
+};
+
+struct WfJobsResult {
+    uint32_t generations = 0;
+    std::vector<double> job_ms; // per job: device time from `start` to the end of its last generation
+};
+
+// Renders every job with the fixed-depth MIS tracer (max_depth bounces, PCG streams of `seed`)
+// in shared generations of up to tune.paths_in_flight paths: each generation holds whole samples
+// of several jobs' pixels, a job's samples run in order and may span generations, and the paths
+// of all its jobs share the queues and kernels of every bounce. Each film is bit-identical to
+// wf_render of that job alone. The fused form only (no material or ray sort). Returns after the
+// last generation has finished (res->job_ms is measured from the event `start`, recorded
+// earlier on `stream`), with an empty string or an error message. `cnt` (device) receives the
+// ray-query counters.
+std::string wf_render_jobs(WavefrontEngine *e, std::vector<WfJob> const &jobs, uint32_t max_depth, uint64_t seed,
+                           Counters *cnt, WavefrontTuning const &tune, hipStream_t stream, hipEvent_t start,
+                           WfJobsResult *res);
 
 // Wavefront form of the bidirectional tracer. Working memory per (pixel, sample) item of a
 // generation for subpaths of up to max_len vertices (bd_item_bytes), and the largest generation

@@ -20,6 +20,7 @@
 
 #include "wavefront.h"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -39,6 +40,20 @@ struct WfCtl {
     WfCounter eq_count_b, fin_count_b; // second parity of the fused form's double-buffered queues
     uint32_t gen;
     uint32_t pad[2];
+};
+
+// Multi-job renders: a generation holds slices, each samples [s_first, s_first + count) of every
+// pixel of one job; the slice's paths are [path_begin, path_begin + count * npix) of the
+// generation (sample-major, as in a single-job generation) and its pixels [pix_begin, pix_begin
+// + npix) of the generation's accumulation pass. Slices of a generation are stored in order,
+// each beginning where the previous one ends.
+struct WfSlice {
+    uint32_t job, s_first, count, path_begin, pix_begin;
+    uint32_t pad[3];
+};
+struct WfGen {
+    uint32_t slice_begin, slice_end; // its slices
+    uint32_t paths, pixels;          // totals
 };
 
 struct WfArgs {
@@ -97,8 +112,28 @@ struct WfArgs {
     float4 *xp_o, *xn_o, *wo_o, *beta_o, *L_o, *wi_o;
     uint64_t *rng_o;
     float4 *lfin_in, *lfin_out;
+    // Multi-job renders (the MULTI kernels): per job scene, camera, first hits and films, and per
+    // generation its slices; the fields of one render above (S, cam, width ... first, acc, out)
+    // are then unused. A path's job travels with it: in beta.w of its state and in the third
+    // float4's w of its shadow ray (0 in a single-job render).
+    const WfJob *jobs;
+    const WfSlice *slices;
+    const WfGen *gens;
 };
 constexpr uint32_t kWfFinBit = 0x80000000u;
+
+// Slice of generation G that holds index q: a path id, or (by_pix) a pixel of the accumulation pass.
+X3D_DI uint32_t find_slice(const WfSlice *sl, WfGen const &G, uint32_t q, bool by_pix) {
+    uint32_t lo = G.slice_begin, hi = G.slice_end;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((by_pix ? sl[mid].pix_begin : sl[mid].path_begin) <= q)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
 
 // Ray keys: direction octant (3 bits) above the Morton code of the origin's cell in an 8^3 grid
 // over the scene box (9 bits). Rays with equal keys start close together and head the same
@@ -286,24 +321,24 @@ struct NeeSample {
 // sampled with the evaluated mixture densities (Tracer::mis_bounce); kWfLt1: unweighted
 // (Tracer::direct as li_lt1 adds it) and the reference's mixture sampling; kWfUnidirect: BSDF
 // sample only (no light sample).
-template <int TR> X3D_DI NeeSample nee_sample(WfArgs const &A, HitRec const &x, f3 wo, f3 beta, Pcg &g) {
+template <int TR> X3D_DI NeeSample nee_sample(DevScene const &S, HitRec const &x, f3 wo, f3 beta, Pcg &g) {
     NeeSample r;
     r.has_shadow = false;
     r.sc = r.sdir = mk(0.0f, 0.0f, 0.0f);
     r.sdist = 0.0f;
-    if (TR != kWfUnidirect && A.S.num_lights > 0) {
-        LightSample ls = sample_light(A.S, g);
+    if (TR != kWfUnidirect && S.num_lights > 0) {
+        LightSample ls = sample_light(S, g);
         f3 l, illum;
         float r2;
-        if (light_arrives(A.S, ls, x.p, x.n, illum, l, r2)) {
+        if (light_arrives(S, ls, x.p, x.n, illum, l, r2)) {
             float dist = norm(l);
             f3 wl = -l / dist;
-            f3 f = bsdf_eval(A.S.mats, x.mat, x.n, wo, wl);
+            f3 f = bsdf_eval(S.mats, x.mat, x.n, wo, wl);
             f3 d = mul(illum, f) / ls.area_dens;
             if (TR == kWfMis && !ls.sky) {
                 float cl = dot(ls.n, -wl);
                 float pl = ls.area_dens * r2 / cl;
-                float pb = bsdf_pdf(A.S.mats, x.mat, x.n, wo, wl);
+                float pb = bsdf_pdf(S.mats, x.mat, x.n, wo, wl);
                 float w = pl * pl / (pl * pl + pb * pb);
                 r.sc = mul(beta, d * w);
             } else {
@@ -317,7 +352,7 @@ template <int TR> X3D_DI NeeSample nee_sample(WfArgs const &A, HitRec const &x, 
         }
     }
     float dens;
-    f3 wi = bsdf_sample(A.S.mats, x.mat, x.n, wo, g, &dens, TR != kWfMis);
+    f3 wi = bsdf_sample(S.mats, x.mat, x.n, wo, g, &dens, TR != kWfMis);
     r.rng = g.state;
     // li_mis stops on !(dens > 0); li_lt1 / li_unidirect on dens == 0
     r.ok = TR == kWfMis ? dens > 0.0f : dens != 0.0f;
@@ -348,9 +383,9 @@ template <int TR> X3D_DI void finish_path(WfArgs const &A, uint32_t path, f3 L) 
 // sample failed). Block-wide: every thread of the block calls it at one uniform call site
 // (block_append synchronizes); `scratch` holds 6 zeroed words. idx[1] / idx[2] receive the
 // path's extension / finishing entry. In the compact form the caller stores the path's state at
-// that entry; otherwise the queues hold path ids.
+// that entry; otherwise the queues hold path ids. The shadow ray carries the path's job.
 X3D_DI void push_rays(WfArgs const &A, bool in, uint32_t path, f3 xp3, NeeSample const &r, uint32_t *scratch,
-                      uint32_t idx[3]) {
+                      uint32_t idx[3], uint32_t job) {
     uint32_t *const queues[3] = {&A.ctl->sq_count.v, &A.eq_cnt_out->v, &A.fin_cnt_out->v};
     const bool want[3] = {in && r.has_shadow, in && r.ok, in && !r.ok};
     block_append<3>(queues, want, idx, scratch);
@@ -359,7 +394,7 @@ X3D_DI void push_rays(WfArgs const &A, bool in, uint32_t path, f3 xp3, NeeSample
         const uint32_t tgt = !A.compact ? path : (r.ok ? idx[1] : (idx[2] | kWfFinBit));
         A.sq[3 * q] = make_float4(xp3.x, xp3.y, xp3.z, r.sdist - 1e-3f);
         A.sq[3 * q + 1] = make_float4(r.sdir.x, r.sdir.y, r.sdir.z, __uint_as_float(tgt));
-        A.sq[3 * q + 2] = make_float4(r.sc.x, r.sc.y, r.sc.z, 0.0f);
+        A.sq[3 * q + 2] = make_float4(r.sc.x, r.sc.y, r.sc.z, __uint_as_float(job));
         if (A.sort_rays)
             A.ray_key[0][q] = ray_key(A.S, xp3, r.sdir);
     }
@@ -376,13 +411,13 @@ X3D_DI void push_rays(WfArgs const &A, bool in, uint32_t path, f3 xp3, NeeSample
 
 // Compact form: the state of a live path after its vertex's samples, at its new queue entry.
 X3D_DI void store_state(WfArgs const &A, uint32_t path, HitRec const &x, f3 wo, uint32_t pix, f3 beta, f3 L,
-                        NeeSample const &r, uint32_t const idx[3]) {
+                        NeeSample const &r, uint32_t const idx[3], uint32_t job) {
     if (r.ok) {
         const uint32_t e = idx[1];
         A.xp_o[e] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
         A.xn_o[e] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
         A.wo_o[e] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
-        A.beta_o[e] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+        A.beta_o[e] = make_float4(beta.x, beta.y, beta.z, __uint_as_float(job));
         A.L_o[e] = make_float4(L.x, L.y, L.z, __uint_as_float(path));
         A.wi_o[e] = make_float4(r.edir.x, r.edir.y, r.edir.z, r.dens);
         A.rng_o[e] = r.rng;
@@ -391,11 +426,63 @@ X3D_DI void store_state(WfArgs const &A, uint32_t path, HitRec const &x, f3 wo, 
     }
 }
 
-template <bool FUSED, int TR> __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
+// Tracer::mis_begin / the first vertex of li_lt1, li_unidirect for path `id` (pixel pix, sample s
+// of a film width x height of scene S seen through cam, first hits in `first`): the first hit's
+// emission and, in the fused form, the vertex's light and BSDF samples. Returns whether the
+// path goes on.
+template <bool FUSED, int TR>
+X3D_DI bool gen_path(WfArgs const &A, DevScene const &S, CamParams const &cam, uint32_t width, uint32_t height,
+                     float4 *first, uint32_t id, uint32_t pix, uint32_t s, HitRec &x, f3 &wo, f3 &L, f3 &xp3,
+                     NeeSample &r, int &kind) {
+    f3 o, v;
+    camera_ray(cam, pix % width, pix / width, width, height, o, v);
+    RenderParams RP;
+    RP.first = first;
+    bool valid;
+    x = load_first(RP, pix, valid);
+    Pcg g = pcg_for(A.seed, pix, s);
+    wo = -v;
+    bool live;
+    if (TR == kWfMis) {
+        if (valid && x.light >= 0)
+            L = L + light_radiance(S, x.light, wo, x.n);
+        live = valid && A.max_depth > 0;
+    } else if (TR == kWfLt1) {
+        live = valid;
+        if (valid) {
+            f3 e = x.light >= 0 ? light_radiance(S, x.light, wo, x.n) : mk(0.0f, 0.0f, 0.0f);
+            A.le0[id] = make_float4(e.x, e.y, e.z, x.light >= 0 ? 1.0f : 0.0f);
+        }
+    } else {
+        live = valid;
+        if (valid && x.light >= 0)
+            L = L + mul(mk(1.0f, 1.0f, 1.0f), light_radiance(S, x.light, wo, x.n));
+    }
+    if (!live) {
+        A.lbuf[id] = make_float4(L.x, L.y, L.z, 0.0f);
+    } else if (FUSED) {
+        // the state is stored at the path's queue entry, after the append
+        xp3 = x.p;
+        r = nee_sample<TR>(S, x, wo, mk(1.0f, 1.0f, 1.0f), g);
+    } else {
+        A.xp[id] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
+        A.xn[id] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
+        A.wo[id] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
+        A.beta[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
+        A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
+        A.rng[id] = g.state;
+        kind = kind_of(S, x.mat);
+    }
+    return live;
+}
+
+// MULTI: the generation's paths belong to the jobs of its slices (the fused MIS form only).
+template <bool FUSED, int TR, bool MULTI = false> __global__ __launch_bounds__(256) void k_wf_generate(WfArgs A) {
+    static_assert(!MULTI || (FUSED && TR == kWfMis), "multi-job generations: fused MIS form");
     __shared__ uint32_t scratch[2 * kKinds];
     clear_scratch(scratch, 2 * kKinds);
     uint32_t gen = A.ctl->gen;
-    uint32_t n = gen_count(A, gen) * A.npix;
+    uint32_t n = MULTI ? A.gens[gen].paths : gen_count(A, gen) * A.npix;
     // block-uniform trip count (block_append synchronizes the block)
     for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
         uint32_t id = b + threadIdx.x;
@@ -406,56 +493,30 @@ template <bool FUSED, int TR> __global__ __launch_bounds__(256) void k_wf_genera
         f3 xp3 = mk(0.0f, 0.0f, 0.0f);
         HitRec x{};
         f3 L = mk(0.0f, 0.0f, 0.0f), wo = L;
-        uint32_t pix = 0;
+        uint32_t pix = 0, job = 0;
         if (id < n) {
-            uint32_t s_local = id / A.npix;
-            pix = id - s_local * A.npix;
-            uint32_t s = A.s_begin + gen * A.gen_samples + s_local;
-            f3 o, v;
-            camera_ray(A.cam, pix % A.width, pix / A.width, A.width, A.height, o, v);
-            RenderParams RP;
-            RP.first = A.first;
-            bool valid;
-            x = load_first(RP, pix, valid);
-            // Tracer::mis_begin / the first vertex of li_lt1, li_unidirect
-            Pcg g = pcg_for(A.seed, pix, s);
-            wo = -v;
-            if (TR == kWfMis) {
-                if (valid && x.light >= 0)
-                    L = L + light_radiance(A.S, x.light, wo, x.n);
-                live = valid && A.max_depth > 0;
-            } else if (TR == kWfLt1) {
-                live = valid;
-                if (valid) {
-                    f3 e = x.light >= 0 ? light_radiance(A.S, x.light, wo, x.n) : mk(0.0f, 0.0f, 0.0f);
-                    A.le0[id] = make_float4(e.x, e.y, e.z, x.light >= 0 ? 1.0f : 0.0f);
-                }
+            if constexpr (MULTI) {
+                const WfSlice sl = A.slices[find_slice(A.slices, A.gens[gen], id, false)];
+                WfJob const &J = A.jobs[sl.job];
+                const uint32_t local = id - sl.path_begin, npix = J.npix;
+                const uint32_t s_local = local / npix;
+                pix = local - s_local * npix;
+                job = sl.job;
+                live = gen_path<FUSED, TR>(A, J.S, J.cam, J.width, J.height, J.first, id, pix, sl.s_first + s_local,
+                                           x, wo, L, xp3, r, kind);
             } else {
-                live = valid;
-                if (valid && x.light >= 0)
-                    L = L + mul(mk(1.0f, 1.0f, 1.0f), light_radiance(A.S, x.light, wo, x.n));
-            }
-            if (!live) {
-                A.lbuf[id] = make_float4(L.x, L.y, L.z, 0.0f);
-            } else if (FUSED) {
-                // the state is stored at the path's queue entry, after the append below
-                xp3 = x.p;
-                r = nee_sample<TR>(A, x, wo, mk(1.0f, 1.0f, 1.0f), g);
-            } else {
-                A.xp[id] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
-                A.xn[id] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
-                A.wo[id] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
-                A.beta[id] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(0u));
-                A.L[id] = make_float4(L.x, L.y, L.z, 0.0f);
-                A.rng[id] = g.state;
-                kind = kind_of(A.S, x.mat);
+                uint32_t s_local = id / A.npix;
+                pix = id - s_local * A.npix;
+                uint32_t s = A.s_begin + gen * A.gen_samples + s_local;
+                live = gen_path<FUSED, TR>(A, A.S, A.cam, A.width, A.height, A.first, id, pix, s, x, wo, L, xp3, r,
+                                           kind);
             }
         }
         if (FUSED) {
             uint32_t idx[3];
-            push_rays(A, live, id, xp3, r, scratch, idx);
+            push_rays(A, live, id, xp3, r, scratch, idx, job);
             if (live)
-                store_state(A, id, x, wo, pix, mk(1.0f, 1.0f, 1.0f), L, r, idx);
+                store_state(A, id, x, wo, pix, mk(1.0f, 1.0f, 1.0f), L, r, idx, job);
         } else {
             push_active(A, 0, kind, id, live, scratch);
         }
@@ -488,14 +549,14 @@ __global__ __launch_bounds__(256) void k_wf_nee_bsdf(WfArgs A, int parity) {
             x.light = static_cast<int32_t>(__float_as_uint(xn.w));
             xp3 = x.p;
             Pcg g = path_pcg(A.rng[path], __float_as_uint(wo4.w));
-            r = nee_sample<kWfMis>(A, x, xyz(wo4), xyz(b4), g);
+            r = nee_sample<kWfMis>(A.S, x, xyz(wo4), xyz(b4), g);
             A.rng[path] = r.rng;
             if (r.ok)
                 A.wi[path] = make_float4(r.edir.x, r.edir.y, r.edir.z, r.dens);
         }
         // one call site with all lanes active: block_append synchronizes the block
         uint32_t idx[3];
-        push_rays(A, in, path, xp3, r, scratch, idx);
+        push_rays(A, in, path, xp3, r, scratch, idx, 0u);
     }
 }
 
@@ -605,15 +666,29 @@ __global__ __launch_bounds__(kSortWG) void k_wf_sort_scatter(WfArgs A) {
 // radiance. BD (bidirectional connections): 2 x float4 (o, tmax), (d, slot | kBdLongTmin); tmin
 // is 1e-3 where the flag is set (subpath connections), else 1e-4 (light samples), and the ray's
 // visibility (1 = unoccluded, 0 = occluded) replaces its tmax.
+// A multi-job ray's scene: the fields the traversal reads (both trees, triangles, roots).
+X3D_DI void use_job_scene(WfArgs const &A, uint32_t job, DevScene &S, const float4 *&tris) {
+    DevScene const &J = A.jobs[job].S;
+    S.nodes = J.nodes;
+    S.nodes4 = J.nodes4;
+    S.tris = J.tris;
+    S.root_ref = J.root_ref;
+    S.root4 = J.root4;
+    S.far_coords = J.far_coords;
+    tris = static_cast<const float4 *>(J.tris);
+}
+
 constexpr uint32_t kBdLongTmin = 0x80000000u;
-template <int WG, int LDSD, bool BD = false>
+// MULTI: each ray traverses the scene of its path's job (the rays of one wave may differ).
+template <int WG, int LDSD, bool BD = false, bool MULTI = false>
 __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
+    static_assert(!(BD && MULTI), "multi-job renders run the MIS form");
     __shared__ uint2 lds[LDSD * WG];
     __shared__ uint32_t cursor[2 * (WG / 64)];
     __shared__ float4 top[kTopNodes > 0 ? kTopNodes * kWideRows : 1];
     TravStack<WG, LDSD> st = make_stack<WG, LDSD>(lds, A.spill, A.spill_per_thread);
-    DevScene const &S = A.S;
-    if (kTopNodes > 0) {
+    DevScene S = A.S;
+    if (kTopNodes > 0 && !MULTI) {
         const uint32_t nt = min(kTopNodes, S.num_nodes4) * kWideRows;
         for (uint32_t q = threadIdx.x; q < nt; q += WG)
             top[q] = static_cast<const float4 *>(S.nodes4)[q];
@@ -635,6 +710,8 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     constexpr uint32_t kStride = BD ? 2u : 3u;
     auto init = [&](uint32_t k) {
         float4 a = A.sq_in[kStride * k], b = A.sq_in[kStride * k + 1];
+        if constexpr (MULTI)
+            use_job_scene(A, __float_as_uint(A.sq_in[kStride * k + 2].w), S, tris);
         o = xyz(a);
         tmax = a.w;
         v = xyz(b);
@@ -722,14 +799,15 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     }
 }
 
-template <int WG, int LDSD>
+// MULTI: each ray traverses the scene of its path's job (compact form).
+template <int WG, int LDSD, bool MULTI = false>
 __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     __shared__ uint2 lds[LDSD * WG];
     __shared__ uint32_t cursor[2 * (WG / 64)];
     __shared__ float4 top[kTopNodes > 0 ? kTopNodes * kWideRows : 1];
     TravStack<WG, LDSD> st = make_stack<WG, LDSD>(lds, A.spill, A.spill_per_thread);
-    DevScene const &S = A.S;
-    if (kTopNodes > 0) {
+    DevScene S = A.S;
+    if (kTopNodes > 0 && !MULTI) {
         const uint32_t nt = min(kTopNodes, S.num_nodes4) * kWideRows;
         for (uint32_t q = threadIdx.x; q < nt; q += WG)
             top[q] = static_cast<const float4 *>(S.nodes4)[q];
@@ -750,6 +828,8 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     bool wide = false;
     auto init = [&](uint32_t k) {
         const uint32_t path = A.compact ? k : A.eq_in[k];
+        if constexpr (MULTI)
+            use_job_scene(A, __float_as_uint(A.beta[k].w), S, tris);
         o = xyz(A.xp[path]);
         v = xyz(A.wi[path]);
         vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
@@ -830,8 +910,98 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     }
 }
 
+// One extension entry of a bounce (state at si, hit t) in scene S: the hit vertex, its emission
+// with the MIS weight (or lt1 / unidirect's Russian roulette and emission) and, in the fused
+// form, its light and BSDF samples. Returns whether the path goes on; a path that ends is
+// finished here.
 template <bool FUSED, int TR>
+X3D_DI bool shade_path(WfArgs const &A, DevScene const &S, uint32_t si, uint32_t t, uint32_t bounce, uint32_t &path,
+                       uint32_t &pix, HitRec &nx, f3 &L, f3 &beta, f3 &wi, f3 &xp3, NeeSample &r, int &kind) {
+    ClosestHit h = A.eh[t];
+    // Every load of the item issues at once: the path's state and the hit triangle (a miss reads
+    // triangle 0, which every scene with a possible hit has), so one memory round trip instead of
+    // three.
+    const uint32_t pi = h.prim >= 0 ? static_cast<uint32_t>(h.prim) : 0u;
+    const float4 *tris = static_cast<const float4 *>(S.tris);
+    const float4 *attr = static_cast<const float4 *>(S.tri_attr);
+    float4 ta = tris[3 * pi], tb = tris[3 * pi + 1], tc = tris[3 * pi + 2];
+    float4 na = attr[3 * pi], nb = attr[3 * pi + 1], nc = attr[3 * pi + 2];
+    float4 L4 = A.L[si], wi4 = A.wi[si], xn = A.xn[si], wo4 = A.wo[si], b4 = A.beta[si];
+    float4 xp4 = A.xp[si];
+    path = FUSED ? __float_as_uint(L4.w) : si;
+    pix = __float_as_uint(wo4.w);
+    L = xyz(L4);
+    wi = xyz(wi4);
+    bool hit = h.prim >= 0, live = false;
+    if (hit) {
+        nx = finish_hit(h, ta, tb, tc, na, nb, nc);
+        hit = !(dot(nx.n, -wi) <= 0.0f);
+    }
+    if (!hit) {
+        finish_path<TR>(A, path, L);
+    } else {
+        float dens = wi4.w;
+        int32_t xmat = static_cast<int32_t>(__float_as_uint(xp4.w));
+        f3 xnrm = xyz(xn), wo = xyz(wo4);
+        beta = xyz(b4);
+        float cw = dot(xnrm, wi);
+        f3 f = bsdf_eval(S.mats, xmat, xnrm, wo, wi);
+        beta = mul(beta, f) * cw / dens;
+        bool go;
+        Pcg g = path_pcg(0, __float_as_uint(wo4.w));
+        if (TR == kWfMis) {
+            if (nx.light >= 0) {
+                float cl = dot(nx.n, -wi);
+                DevLight const &Ln = S.lights[nx.light];
+                float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
+                float w = dens * dens / (dens * dens + pl * pl);
+                L = L + mul(beta, light_radiance(S, nx.light, -wi, nx.n) * w);
+            }
+            go = bounce + 1 < A.max_depth;
+            if (go && FUSED)
+                g.state = A.rng[si];
+        } else {
+            // li_lt1 / li_unidirect at depth bounce + 1: Russian roulette p = 0.5 from depth 2,
+            // then (unidirect) the vertex's emission; the loop stops after kRefModeMaxDepth vertices
+            const uint32_t depth = bounce + 1;
+            go = depth < kRefModeMaxDepth;
+            if (go) {
+                g.state = A.rng[si];
+                if (depth >= 2) {
+                    if (g.draw() >= 0.5f)
+                        go = false;
+                    else
+                        beta = beta / 0.5f;
+                }
+            }
+            if (go && TR == kWfUnidirect && nx.light >= 0)
+                L = L + mul(beta, light_radiance(S, nx.light, -wi, nx.n));
+        }
+        if (!go) {
+            finish_path<TR>(A, path, L);
+        } else {
+            live = true;
+            if (FUSED) {
+                // the state is stored at the path's new queue entry, after the append
+                xp3 = nx.p;
+                r = nee_sample<TR>(S, nx, -wi, beta, g);
+            } else {
+                A.xp[path] = make_float4(nx.p.x, nx.p.y, nx.p.z, __uint_as_float(static_cast<uint32_t>(nx.mat)));
+                A.xn[path] = make_float4(nx.n.x, nx.n.y, nx.n.z, __uint_as_float(static_cast<uint32_t>(nx.light)));
+                A.wo[path] = make_float4(-wi.x, -wi.y, -wi.z, wo4.w);
+                A.beta[path] = make_float4(beta.x, beta.y, beta.z, b4.w);
+                A.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
+                kind = kind_of(S, nx.mat);
+            }
+        }
+    }
+    return live;
+}
+
+// MULTI: each entry is shaded in the scene of its path's job (the fused MIS form only).
+template <bool FUSED, int TR, bool MULTI = false>
 __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t bounce) {
+    static_assert(!MULTI || (FUSED && TR == kWfMis), "multi-job generations: fused MIS form");
     __shared__ uint32_t scratch[2 * kKinds];
     clear_scratch(scratch, 2 * kKinds);
     uint32_t ne = A.eq_cnt_in->v, nf = A.fin_cnt_in->v;
@@ -841,7 +1011,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
         uint32_t t = b + threadIdx.x;
         bool live = false;
         int kind = 0;
-        uint32_t path = 0, pix = 0;
+        uint32_t path = 0, pix = 0, job = 0;
         NeeSample r;
         r.has_shadow = r.ok = false;
         f3 xp3 = mk(0.0f, 0.0f, 0.0f);
@@ -861,94 +1031,46 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
         } else {
             // compact form: entry t's state at index t; otherwise at its path id
             const uint32_t si = FUSED ? t : A.eq_in[t];
-            ClosestHit h = A.eh[t];
-            // Every load of the item issues at once: the path's state and the hit triangle (a
-            // miss reads triangle 0, which every scene with a possible hit has), so one memory
-            // round trip instead of three.
-            const uint32_t pi = h.prim >= 0 ? static_cast<uint32_t>(h.prim) : 0u;
-            const float4 *tris = static_cast<const float4 *>(A.S.tris);
-            const float4 *attr = static_cast<const float4 *>(A.S.tri_attr);
-            float4 ta = tris[3 * pi], tb = tris[3 * pi + 1], tc = tris[3 * pi + 2];
-            float4 na = attr[3 * pi], nb = attr[3 * pi + 1], nc = attr[3 * pi + 2];
-            float4 L4 = A.L[si], wi4 = A.wi[si], xn = A.xn[si], wo4 = A.wo[si], b4 = A.beta[si];
-            float4 xp4 = A.xp[si];
-            path = FUSED ? __float_as_uint(L4.w) : si;
-            pix = __float_as_uint(wo4.w);
-            L = xyz(L4);
-            wi = xyz(wi4);
-            bool hit = h.prim >= 0;
-            if (hit) {
-                nx = finish_hit(h, ta, tb, tc, na, nb, nc);
-                hit = !(dot(nx.n, -wi) <= 0.0f);
-            }
-            if (!hit) {
-                finish_path<TR>(A, path, L);
+            if constexpr (MULTI) {
+                job = __float_as_uint(A.beta[si].w);
+                live = shade_path<FUSED, TR>(A, A.jobs[job].S, si, t, bounce, path, pix, nx, L, beta, wi, xp3, r,
+                                             kind);
             } else {
-                float dens = wi4.w;
-                int32_t xmat = static_cast<int32_t>(__float_as_uint(xp4.w));
-                f3 xnrm = xyz(xn), wo = xyz(wo4);
-                beta = xyz(b4);
-                float cw = dot(xnrm, wi);
-                f3 f = bsdf_eval(A.S.mats, xmat, xnrm, wo, wi);
-                beta = mul(beta, f) * cw / dens;
-                bool go;
-                Pcg g = path_pcg(0, __float_as_uint(wo4.w));
-                if (TR == kWfMis) {
-                    if (nx.light >= 0) {
-                        float cl = dot(nx.n, -wi);
-                        DevLight const &Ln = A.S.lights[nx.light];
-                        float pl = (Ln.prob / Ln.area) * (nx.t * nx.t) / cl;
-                        float w = dens * dens / (dens * dens + pl * pl);
-                        L = L + mul(beta, light_radiance(A.S, nx.light, -wi, nx.n) * w);
-                    }
-                    go = bounce + 1 < A.max_depth;
-                    if (go && FUSED)
-                        g.state = A.rng[si];
-                } else {
-                    // li_lt1 / li_unidirect at depth bounce + 1: Russian roulette p = 0.5 from depth 2,
-                    // then (unidirect) the vertex's emission; the loop stops after kRefModeMaxDepth vertices
-                    const uint32_t depth = bounce + 1;
-                    go = depth < kRefModeMaxDepth;
-                    if (go) {
-                        g.state = A.rng[si];
-                        if (depth >= 2) {
-                            if (g.draw() >= 0.5f)
-                                go = false;
-                            else
-                                beta = beta / 0.5f;
-                        }
-                    }
-                    if (go && TR == kWfUnidirect && nx.light >= 0)
-                        L = L + mul(beta, light_radiance(A.S, nx.light, -wi, nx.n));
-                }
-                if (!go) {
-                    finish_path<TR>(A, path, L);
-                } else {
-                    live = true;
-                    if (FUSED) {
-                        // the state is stored at the path's new queue entry, after the append
-                        xp3 = nx.p;
-                        r = nee_sample<TR>(A, nx, -wi, beta, g);
-                    } else {
-                        A.xp[path] =
-                            make_float4(nx.p.x, nx.p.y, nx.p.z, __uint_as_float(static_cast<uint32_t>(nx.mat)));
-                        A.xn[path] =
-                            make_float4(nx.n.x, nx.n.y, nx.n.z, __uint_as_float(static_cast<uint32_t>(nx.light)));
-                        A.wo[path] = make_float4(-wi.x, -wi.y, -wi.z, wo4.w);
-                        A.beta[path] = make_float4(beta.x, beta.y, beta.z, b4.w);
-                        A.L[path] = make_float4(L.x, L.y, L.z, 0.0f);
-                        kind = kind_of(A.S, nx.mat);
-                    }
-                }
+                live = shade_path<FUSED, TR>(A, A.S, si, t, bounce, path, pix, nx, L, beta, wi, xp3, r, kind);
             }
         }
         if (FUSED) {
             uint32_t idx[3];
-            push_rays(A, live, path, xp3, r, scratch, idx);
+            push_rays(A, live, path, xp3, r, scratch, idx, job);
             if (live)
-                store_state(A, path, nx, -wi, pix, beta, L, r, idx);
+                store_state(A, path, nx, -wi, pix, beta, L, r, idx, job);
         } else {
             push_active(A, parity ^ 1, kind, path, live, scratch);
+        }
+    }
+}
+
+// Multi-job form: every pixel of each slice of the generation adds the slice's samples of that
+// pixel, in sample order, to its job's running sum (the job's first slice starts from zero, its
+// last writes the film), as a single-job render of the job does over its generations.
+__global__ __launch_bounds__(256) void k_wf_accumulate_jobs(WfArgs A) {
+    const WfGen G = A.gens[A.ctl->gen];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < G.pixels; q += gridDim.x * blockDim.x) {
+        const WfSlice sl = A.slices[find_slice(A.slices, G, q, true)];
+        WfJob const &J = A.jobs[sl.job];
+        const uint32_t pix = q - sl.pix_begin, npix = J.npix;
+        f3 acc = mk(0.0f, 0.0f, 0.0f);
+        if (sl.s_first > 0) {
+            float4 a = J.acc[pix];
+            acc = mk(a.x, a.y, a.z);
+        }
+        for (uint32_t s = 0; s < sl.count; s++)
+            acc = acc + xyz(A.lbuf[sl.path_begin + static_cast<size_t>(s) * npix + pix]);
+        if (sl.s_first + sl.count >= J.spp) {
+            const float scale = 1.0f / static_cast<float>(J.spp);
+            J.out[pix] = make_float4(scale * acc.x, scale * acc.y, scale * acc.z, scale);
+        } else {
+            J.acc[pix] = make_float4(acc.x, acc.y, acc.z, 0.0f);
         }
     }
 }
@@ -1449,6 +1571,9 @@ struct WavefrontEngine {
     // bidirectional form: per item state, PCG state, walk ray, emission sample, vertices,
     // visibilities
     DevBuf bd_state, bd_rng, bd_ro, bd_rd, bd_em, bd_cv, bd_lv, bd_rix;
+    // multi-job renders: job, slice and generation tables; an event at the end of each generation
+    DevBuf mj_jobs, mj_slices, mj_gens;
+    std::vector<hipEvent_t> gen_ev;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     WfArgs key{};
@@ -1465,6 +1590,8 @@ struct WavefrontEngine {
         for (auto &x : ev)
             if (x)
                 (void)hipEventDestroy(x);
+        for (auto &x : gen_ev)
+            (void)hipEventDestroy(x);
         if (fork)
             (void)hipEventDestroy(fork);
         if (join)
@@ -1505,19 +1632,32 @@ struct TraceKernels {
     int wg, ldsd;
 };
 
-template <int WG, int D> TraceKernels trace_kernels() {
-    return {k_wf_shadow<WG, D>, k_wf_extend<WG, D>, k_wf_shadow<WG, D, true>, WG, D};
+template <int WG, int D, bool M> TraceKernels trace_kernels() {
+    return {k_wf_shadow<WG, D, false, M>, k_wf_extend<WG, D, M>, k_wf_shadow<WG, D, true>, WG, D};
 }
 
-// Workgroup sizes 64 / 128 / 256 and LDS stack depths 8 / 16 (deeper entries spill to memory).
-TraceKernels pick_trace(int wg, int ldsd) {
-    bool shallow = ldsd == 8;
+template <bool M> TraceKernels pick_trace_of(int wg, bool shallow) {
     switch (wg) {
-    case 64: return shallow ? trace_kernels<64, 8>() : trace_kernels<64, 16>();
-    case 128: return shallow ? trace_kernels<128, 8>() : trace_kernels<128, 16>();
-    default: return shallow ? trace_kernels<256, 8>() : trace_kernels<256, 16>();
+    case 64: return shallow ? trace_kernels<64, 8, M>() : trace_kernels<64, 16, M>();
+    case 128: return shallow ? trace_kernels<128, 8, M>() : trace_kernels<128, 16, M>();
+    default: return shallow ? trace_kernels<256, 8, M>() : trace_kernels<256, 16, M>();
     }
 }
+
+// Workgroup sizes 64 / 128 / 256 and LDS stack depths 8 / 16 (deeper entries spill to memory);
+// `multi`: the kernels of multi-job renders.
+TraceKernels pick_trace(int wg, int ldsd, bool multi = false) {
+    return multi ? pick_trace_of<true>(wg, ldsd == 8) : pick_trace_of<false>(wg, ldsd == 8);
+}
+
+// A multi-job render's generations (device tables) and the host copy of its jobs.
+struct JobsPlan {
+    std::vector<WfJob> const *jobs;
+    const WfJob *d_jobs;
+    const WfSlice *d_slices;
+    const WfGen *d_gens;
+    uint32_t generations, cap;
+};
 
 bool same_config(WfArgs const &a, WfArgs const &b, WavefrontTuning const &ta, WavefrontTuning const &tb) {
     return std::memcmp(&a, &b, sizeof(WfArgs)) == 0 && ta.paths_in_flight == tb.paths_in_flight &&
@@ -1686,22 +1826,34 @@ std::string bd_generation(WavefrontEngine *e, RenderParams const &P, WavefrontTu
 #undef BD_HIP
 }
 
-std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, hipStream_t stream) {
+namespace {
+
 #define WF_HIP(call)                                                                                 \
     do {                                                                                             \
         hipError_t err_ = (call);                                                                    \
         if (err_ != hipSuccess)                                                                      \
             return std::string(#call) + ": " + hipGetErrorString(err_);                              \
     } while (0)
+
+// A single-job render (M null: P's film, samples [P.s_begin, P.s_end)) or a multi-job render
+// (M's generations; P then gives only the seed, depth and counters).
+std::string wf_run(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, hipStream_t stream,
+                   JobsPlan const *M) {
+    const bool multi = M != nullptr;
     uint32_t npix = P.width * P.height;
-    if (P.s_end <= P.s_begin)
+    if (!multi && P.s_end <= P.s_begin)
         return "";
-    uint32_t gen_samples = tune.paths_in_flight / npix;
-    if (gen_samples < 1)
-        gen_samples = 1;
-    if (gen_samples > P.s_end - P.s_begin)
-        gen_samples = P.s_end - P.s_begin;
-    uint32_t cap = gen_samples * npix;
+    uint32_t gen_samples = 0, cap = 0;
+    if (multi) {
+        cap = M->cap;
+    } else {
+        gen_samples = tune.paths_in_flight / npix;
+        if (gen_samples < 1)
+            gen_samples = 1;
+        if (gen_samples > P.s_end - P.s_begin)
+            gen_samples = P.s_end - P.s_begin;
+        cap = gen_samples * npix;
+    }
     size_t f4 = sizeof(float4) * static_cast<size_t>(cap);
     WF_HIP(e->xp.ensure(f4));
     WF_HIP(e->xn.ensure(f4));
@@ -1714,6 +1866,8 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     const bool fused = !tune.sort_by_material && !tune.sort_rays;
     if (tune.tracer != kWfMis && !fused)
         return "the lt1 and unidirect tracers run the wavefront form without material or ray sorting only";
+    if (multi && (!fused || tune.tracer != kWfMis))
+        return "multi-job renders run the MIS tracer's wavefront form without material or ray sorting only";
     if (tune.tracer == kWfLt1)
         WF_HIP(e->le0.ensure(f4));
     if (fused) {
@@ -1747,11 +1901,19 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
         WF_HIP(e->bin_tot.ensure(2 * sizeof(uint32_t) * kSortBins));
     }
 
-    TraceKernels tk = pick_trace(tune.trace_wg, tune.stack_depth);
+    TraceKernels tk = pick_trace(tune.trace_wg, tune.stack_depth, multi);
     int wg = tk.wg;
     uint32_t g_shadow = grid_for(tk.shadow, wg), g_extend = grid_for(tk.extend, wg);
     uint32_t trace_threads = (g_shadow > g_extend ? g_shadow : g_extend) * wg;
-    uint32_t spill_per_thread = spill_entries_for(P.S, tk.ldsd);
+    uint32_t spill_per_thread = 0;
+    if (multi) {
+        for (WfJob const &J : *M->jobs) {
+            const uint32_t sp = spill_entries_for(J.S, tk.ldsd);
+            spill_per_thread = sp > spill_per_thread ? sp : spill_per_thread;
+        }
+    } else {
+        spill_per_thread = spill_entries_for(P.S, tk.ldsd);
+    }
     // one stack-spill slice per thread of each of the two concurrent traversal kernels
     const size_t spill_half = static_cast<size_t>(trace_threads) * spill_per_thread;
     WF_HIP(e->spill.ensure(2 * sizeof(uint2) * spill_half));
@@ -1811,6 +1973,11 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     A.fused = fused ? 1 : 0;
     A.compact = fused ? 1 : 0;
     A.le0 = tune.tracer == kWfLt1 ? static_cast<float4 *>(e->le0.p) : nullptr;
+    if (multi) {
+        A.jobs = M->d_jobs;
+        A.slices = M->d_slices;
+        A.gens = M->d_gens;
+    }
     if (tune.sort_rays) {
         A.sq_sorted = static_cast<float4 *>(e->sq_sorted.p);
         A.eq_sorted = static_cast<uint32_t *>(e->eq_sorted.p);
@@ -1830,7 +1997,11 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     using ShadeKernel = void (*)(WfArgs, int, uint32_t);
     GenKernel gen_k = k_wf_generate<false, kWfMis>;
     ShadeKernel shade_k = k_wf_shade<false, kWfMis>;
-    if (fused) {
+    GenKernel acc_k = multi ? k_wf_accumulate_jobs : k_wf_accumulate;
+    if (multi) {
+        gen_k = k_wf_generate<true, kWfMis, true>;
+        shade_k = k_wf_shade<true, kWfMis, true>;
+    } else if (fused) {
         gen_k = tune.tracer == kWfLt1         ? k_wf_generate<true, kWfLt1>
                 : tune.tracer == kWfUnidirect ? k_wf_generate<true, kWfUnidirect>
                                               : k_wf_generate<true, kWfMis>;
@@ -1843,7 +2014,7 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
     // microseconds of launch each).
     const uint32_t bounces = tune.tracer == kWfMis ? P.max_depth : kRefModeMaxDepth;
     uint32_t g_gen = grid_for(gen_k, 256), g_nee = grid_for(k_wf_nee_bsdf, 256), g_shade = grid_for(shade_k, 256),
-             g_acc = grid_for(k_wf_accumulate, 256);
+             g_acc = grid_for(acc_k, 256);
     // The fused form's queue parities: bounce k consumes parity k & 1 and its shade fills the other.
     auto parity_args = [&](uint32_t k) {
         WfArgs B = A;
@@ -1884,7 +2055,7 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
         B.fin_cnt_out = finc[q];
         return B;
     };
-    uint32_t generations = (P.s_end - P.s_begin + gen_samples - 1) / gen_samples;
+    uint32_t generations = multi ? M->generations : (P.s_end - P.s_begin + gen_samples - 1) / gen_samples;
 
     bool profile = tune.profile != 0;
     auto trace_bounce = [&](hipStream_t s, WfArgs const &B) {
@@ -1948,8 +2119,12 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
                 hipLaunchKernelGGL(shade_k, dim3(g_shade), dim3(256), 0, s, A, par, k);
             }
         }
-        hipLaunchKernelGGL(k_wf_accumulate, dim3(g_acc), dim3(256), 0, s, A);
+        hipLaunchKernelGGL(acc_k, dim3(g_acc), dim3(256), 0, s, A);
         hipLaunchKernelGGL(k_wf_next_gen, dim3(1), dim3(64), 0, s, A.ctl);
+    };
+    // multi-job renders: the end of each generation (job latencies)
+    auto generation_done = [&](uint32_t g) -> hipError_t {
+        return multi ? hipEventRecord(e->gen_ev[g], stream) : hipSuccess;
     };
 
     e->shadow_ms = e->extend_ms = 0.0;
@@ -1976,15 +2151,129 @@ std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning
             e->key_tune = tune;
             e->have_graph = true;
         }
-        for (uint32_t g = 0; g < generations; g++)
+        for (uint32_t g = 0; g < generations; g++) {
             WF_HIP(hipGraphLaunch(e->exec, stream));
+            WF_HIP(generation_done(g));
+        }
     } else {
-        for (uint32_t g = 0; g < generations; g++)
+        for (uint32_t g = 0; g < generations; g++) {
             record_generation(stream);
+            WF_HIP(generation_done(g));
+        }
     }
     WF_HIP(hipGetLastError());
     return "";
-#undef WF_HIP
 }
+
+} // namespace
+
+std::string wf_render(WavefrontEngine *e, RenderParams const &P, WavefrontTuning const &tune, hipStream_t stream) {
+    return wf_run(e, P, tune, stream, nullptr);
+}
+
+std::string wf_render_jobs(WavefrontEngine *e, std::vector<WfJob> const &jobs, uint32_t max_depth, uint64_t seed,
+                           Counters *cnt, WavefrontTuning const &tune, hipStream_t stream, hipEvent_t start,
+                           WfJobsResult *res) {
+    const uint32_t nj = static_cast<uint32_t>(jobs.size());
+    if (res) {
+        res->generations = 0;
+        res->job_ms.assign(nj, 0.0);
+    }
+    if (nj == 0)
+        return "";
+    // Pack whole samples of the jobs' pixels into generations of at most `cap` paths, fewest
+    // samples first (small jobs finish in the first generation); a job that does not fit the rest
+    // of a generation continues in the next one, its samples in order.
+    uint64_t cap_paths = tune.paths_in_flight ? tune.paths_in_flight : kDefaultQueueCapacity;
+    for (WfJob const &J : jobs) {
+        if (J.npix == 0 || J.spp == 0 || J.npix != J.width * J.height)
+            return "multi-job render: a job with an empty film or no samples";
+        cap_paths = J.npix > cap_paths ? J.npix : cap_paths; // one sample of every film fits
+    }
+    if (cap_paths >= kWfFinBit)
+        return "multi-job render: a film of 2^31 pixels or more";
+    std::vector<uint32_t> order(nj);
+    for (uint32_t j = 0; j < nj; j++)
+        order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return static_cast<uint64_t>(jobs[a].npix) * jobs[a].spp < static_cast<uint64_t>(jobs[b].npix) * jobs[b].spp;
+    });
+    std::vector<WfSlice> slices;
+    std::vector<WfGen> gens;
+    std::vector<uint32_t> last_gen(nj, 0);
+    WfGen cur{0, 0, 0, 0};
+    auto close = [&]() {
+        cur.slice_end = static_cast<uint32_t>(slices.size());
+        gens.push_back(cur);
+        cur = WfGen{cur.slice_end, cur.slice_end, 0, 0};
+    };
+    for (uint32_t j : order) {
+        WfJob const &J = jobs[j];
+        uint32_t s = 0;
+        while (s < J.spp) {
+            const uint64_t fit = (cap_paths - cur.paths) / J.npix;
+            if (fit == 0) {
+                close();
+                continue;
+            }
+            const uint32_t c = static_cast<uint32_t>(fit < J.spp - s ? fit : J.spp - s);
+            WfSlice sl{};
+            sl.job = j;
+            sl.s_first = s;
+            sl.count = c;
+            sl.path_begin = cur.paths;
+            sl.pix_begin = cur.pixels;
+            slices.push_back(sl);
+            cur.paths += c * J.npix;
+            cur.pixels += J.npix;
+            s += c;
+            last_gen[j] = static_cast<uint32_t>(gens.size());
+        }
+    }
+    if (cur.paths > 0)
+        close();
+    uint32_t cap = 0;
+    for (WfGen const &G : gens)
+        cap = G.paths > cap ? G.paths : cap;
+    const uint32_t ng = static_cast<uint32_t>(gens.size());
+    WF_HIP(e->mj_jobs.ensure(sizeof(WfJob) * nj));
+    WF_HIP(e->mj_slices.ensure(sizeof(WfSlice) * slices.size()));
+    WF_HIP(e->mj_gens.ensure(sizeof(WfGen) * ng));
+    // synchronous copies of small host tables: the previous render has finished (this function
+    // waits for its last generation) and the kernels below are launched after them
+    WF_HIP(hipMemcpy(e->mj_jobs.p, jobs.data(), sizeof(WfJob) * nj, hipMemcpyHostToDevice));
+    WF_HIP(hipMemcpy(e->mj_slices.p, slices.data(), sizeof(WfSlice) * slices.size(), hipMemcpyHostToDevice));
+    WF_HIP(hipMemcpy(e->mj_gens.p, gens.data(), sizeof(WfGen) * ng, hipMemcpyHostToDevice));
+    while (e->gen_ev.size() < ng) {
+        hipEvent_t ev;
+        WF_HIP(hipEventCreate(&ev));
+        e->gen_ev.push_back(ev);
+    }
+    JobsPlan M{&jobs, static_cast<const WfJob *>(e->mj_jobs.p), static_cast<const WfSlice *>(e->mj_slices.p),
+               static_cast<const WfGen *>(e->mj_gens.p), ng, cap};
+    RenderParams P{};
+    P.seed = seed;
+    P.max_depth = max_depth;
+    P.cnt = cnt;
+    WavefrontTuning t = tune;
+    t.profile = 0;
+    std::string err = wf_run(e, P, t, stream, &M);
+    if (!err.empty())
+        return err;
+    WF_HIP(hipEventSynchronize(e->gen_ev[ng - 1]));
+    if (res) {
+        res->generations = ng;
+        std::vector<double> gen_ms(ng, 0.0);
+        for (uint32_t g = 0; g < ng; g++) {
+            float ms = 0;
+            WF_HIP(hipEventElapsedTime(&ms, start, e->gen_ev[g]));
+            gen_ms[g] = ms;
+        }
+        for (uint32_t j = 0; j < nj; j++)
+            res->job_ms[j] = gen_ms[last_gen[j]];
+    }
+    return "";
+}
+#undef WF_HIP
 
 } // namespace x3d

@@ -120,20 +120,20 @@ k_adjoint(RenderParams P, const float4 *dl, uint32_t num_mats, unsigned long lon
     __syncthreads();
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
     Tracer<kWG, kLdsDepth> tr{P.S, st};
-    unsigned long long *my = slots[threadIdx.x >> 6];
-    uint32_t npix = P.width * P.height;
-    uint64_t total = static_cast<uint64_t>(npix) * P.spp_total;
-    float inv = 1.0f / static_cast<float>(P.spp_total);
+    unsigned long long *my = slots[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
+    const uint32_t npix = P.width * P.height;
+    const float inv = 1.0f / static_cast<float>(P.spp_total);
     const AdjRecord rec = AdjRecord::of_lane(records, (blockIdx.x * kWG + threadIdx.x) >> 6);
-    for (uint64_t q = blockIdx.x * static_cast<uint64_t>(kWG) + threadIdx.x; q < total;
-         q += static_cast<uint64_t>(gridDim.x) * kWG) {
-        uint32_t pix = static_cast<uint32_t>(q % npix), s = static_cast<uint32_t>(q / npix);
+    // item q = s * npix + pix, grid-strided; (pix, s) advance by the stride's quotient and
+    // remainder instead of dividing a 64-bit q every item
+    const uint32_t q0 = blockIdx.x * kWG + threadIdx.x, stride = gridDim.x * kWG;
+    const uint32_t ds = stride / npix, dp = stride - ds * npix;
+    uint32_t pix = q0 % npix, s = q0 / npix;
+    for (; s < P.spp_total; s += ds, pix += dp, s += pix >= npix ? 1u : 0u, pix -= pix >= npix ? npix : 0u) {
         f3 o, v;
         camera_ray(P.cam, pix % P.width, pix / P.width, P.width, P.height, o, v);
         bool valid;
         HitRec first = load_first(P, pix, valid);
-        float4 d = dl[pix];
-        f3 gw = mk(d.x * inv, d.y * inv, d.z * inv);
         PathState ps;
         tr.mis_begin(ps, first, valid, v, pcg_for(P.seed, pix, s), P.max_depth);
         uint32_t nrec = 0;
@@ -141,6 +141,8 @@ k_adjoint(RenderParams P, const float4 *dl, uint32_t num_mats, unsigned long lon
             nrec = ps.k + 1;
             tr.mis_bounce<true>(ps, P.max_depth, &rec);
         }
+        const float4 d = dl[pix];
+        const f3 gw = mk(d.x * inv, d.y * inv, d.z * inv);
         f3 tail = mk(0.0f, 0.0f, 0.0f);
         for (int k = static_cast<int>(nrec) - 1; k >= 0; k--) {
             const float4 r0 = rec.row(k, 0), r1 = rec.row(k, 1);

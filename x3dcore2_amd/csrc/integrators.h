@@ -134,8 +134,8 @@ struct Pathlet {
 // so the subpaths take no private (scratch) memory.
 constexpr uint32_t kPathletRows = 4;
 struct PathletRows {
-    float4 *base; // this lane's row of vertex 0, field 0 (rows of one lane 64 float4 apart)
-    X3D_DI float4 &row(uint32_t k, uint32_t f) const { return base[(kPathletRows * k + f) * 64u]; }
+    float4 *base; // the wave's block (wave-uniform); lane l's rows at l, l + 64, ...
+    X3D_DI float4 &row(uint32_t k, uint32_t f) const { return base[(kPathletRows * k + f) * 64u + __lane_id()]; }
     X3D_DI Pathlet load(uint32_t k) const {
         float4 a = row(k, 0), b = row(k, 1), c = row(k, 2), d = row(k, 3);
         return Pathlet{xyz(a), xyz(b), xyz(c), xyz(d), static_cast<int32_t>(__float_as_uint(a.w))};
@@ -159,11 +159,11 @@ constexpr uint32_t kAdjNee = 1u, kAdjCont = 2u;
 constexpr uint32_t kAdjMaxDepth = 8; // deepest path the adjoint records
 constexpr uint32_t kAdjRows = 6;     // float4 rows per recorded vertex
 struct AdjRecord {
-    float4 *base; // this lane's row of vertex 0, field 0 (rows of one lane 64 float4 apart)
-    X3D_DI float4 &row(uint32_t k, uint32_t f) const { return base[(kAdjRows * k + f) * 64u]; }
-    // the record of the calling lane in `buf`, given its wave's global index
-    X3D_DI static AdjRecord of_lane(float4 *buf, size_t wave) {
-        return AdjRecord{buf + wave * (kAdjRows * kAdjMaxDepth * 64u) + __lane_id()};
+    float4 *base; // the wave's block (wave-uniform); lane l's rows at l, l + 64, ...
+    X3D_DI float4 &row(uint32_t k, uint32_t f) const { return base[(kAdjRows * k + f) * 64u + __lane_id()]; }
+    // the record block of the calling lane's wave in `buf`, given the wave's global index
+    X3D_DI static AdjRecord of_lane(float4 *buf, uint32_t wave) {
+        return AdjRecord{buf + static_cast<size_t>(__builtin_amdgcn_readfirstlane(wave)) * (kAdjRows * kAdjMaxDepth * 64u)};
     }
 };
 

@@ -214,9 +214,9 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k
     __shared__ uint2 lds[kLdsDepth * kWG];
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
     Tracer<kWG, kLdsDepth> tr{P.S, st};
-    // this wave's block: both subpaths of its 64 lanes
-    const size_t wave = (blockIdx.x * kWG + threadIdx.x) >> 6;
-    const PathletRows cam{P.subpaths + wave * (2 * kPathletRows * kMaxPathLen * 64u) + __lane_id()};
+    // this wave's block: both subpaths of its 64 lanes (a wave-uniform base)
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kWG + threadIdx.x) >> 6);
+    const PathletRows cam{P.subpaths + static_cast<size_t>(wave) * (2 * kPathletRows * kMaxPathLen * 64u)};
     const PathletRows lig{cam.base + kPathletRows * kMaxPathLen * 64u};
     uint32_t npix = P.width * P.height;
     uint32_t total = npix * gen_samples;

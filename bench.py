@@ -11,8 +11,9 @@ HBM. Both execution forms are timed on the same workload — the plain one-threa
 reference's own lt1 tracer is also timed in both forms at 16 spp (`lt1_tracer`), and the line
 carries two more configurations' rates, a few seconds each: the bidirectional tracer on the
 indirectly lit room at 512 x 512 x 32 spp with the reference's own BDPT timed on the host beside
-it (`bdpt_512x512x32`), and gradient evaluations of the 256 x 256 x 32 spp inverse-rendering room
-(`adjoint_256x256x32`). Prints one JSON line.
+it (`bdpt_512x512x32`), gradient evaluations of the 256 x 256 x 32 spp inverse-rendering room
+(`adjoint_256x256x32`), and the 64-scene batch in one call against its jobs one after another
+(`batch_64scenes`, about 20 s of it building the scenes). Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--workload spheres|box|indirect|batch|adjoint] [--spp N]
                     [--no-cpu-baseline] [--no-autotune | --wf-config WG,DEPTH,SORT,CAP]
@@ -231,6 +232,44 @@ def leg_bdpt(x3d, steps, warmup, spp=0, cpu=True, budget_s=8.0, plain=True):
                                     "sample": f"reference bidirect_mis_path_tracer (8-vertex subpaths) at {side}x{side}, "
                                               f"{taken} samples per pixel, {d1:.1f} s"}
     return res
+
+
+def leg_batch(x3d, steps, warmup):
+    """The 64-scene batch (1k-2M triangles per scene, 14.2M in all, 128^2-1024^2, 4-64 spp,
+    depth 6) in one x3d_render_batch call in the wavefront form, against the same jobs rendered
+    one after another in the wavefront form (one untimed pass first)."""
+    from x3dcore2_amd import scenes
+    from x3dcore2_amd.device import DeviceArray, synchronize
+
+    descs = scenes.batch_scenes()
+    built = [x3d.Scene(d) for d in descs]
+    films = [DeviceArray(d.width * d.height * 16) for d in descs]
+    items = [(built[k], d.camera, d.width, d.height, d.spp) for k, d in enumerate(descs)]
+    ptrs = [f.ptr for f in films]
+    samples = sum(d.width * d.height * d.spp for d in descs)
+    r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=6, seed=1)
+    for timed in (False, True):
+        synchronize()
+        t0 = time.perf_counter()
+        for k, d in enumerate(descs):
+            r.render_device(built[k], d.camera, d.width, d.height, d.spp, ptrs[k])
+        synchronize()
+    seq = time.perf_counter() - t0
+    for _ in range(warmup):
+        r.render_batch_device(items, ptrs)
+    synchronize()
+    t0 = time.perf_counter()
+    res = [r.render_batch_device(items, ptrs) for _ in range(steps)]
+    synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st, lat = res[-1]
+    out = {"workload": f"64 seeded scenes, {sum(d.num_tris for d in descs)} tris", "samples": samples,
+           "max_depth": 6, "form": "wavefront (every job's paths in shared generations)",
+           "msamples_per_s": samples / dt / 1e6, "ms_per_step": dt * 1e3, "kernel_ms": st.kernel_ms,
+           "generations": st.generations, "sequential_wavefront_ms": seq * 1e3, "speedup_vs_sequential": seq / dt,
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p90": float(np.percentile(lat, 90))}}
+    log(f"batch leg: {dt * 1e3:.1f} ms per batch ({seq * 1e3:.1f} ms one after another)")
+    return out
 
 
 def leg_adjoint(x3d, steps, warmup, spp=0):
@@ -567,6 +606,7 @@ def main():
         # the other configurations' headline rates, a few seconds each
         out["bdpt_512x512x32"] = leg_bdpt(x3d, 3, 1, cpu=not args.no_cpu_baseline)
         out["adjoint_256x256x32"] = leg_adjoint(x3d, 20, 3)
+        out["batch_64scenes"] = leg_batch(x3d, 2, 1)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(desc)
     print(json.dumps(out), flush=True)

@@ -134,3 +134,23 @@ def test_wavefront_batch_matches_oracle_and_autotunes(x3d, oracle_mod, batch):
     again, _, _ = r.render_batch(items[:3] + items[-2:])
     for a, b in zip(films, again):
         assert np.array_equal(_bits(a), _bits(b))
+
+
+def test_wavefront_batch_of_mixed_scenes(x3d):
+    """One generation holding paths of scenes with one emitter, several emitters of unequal power
+    and area, a sky light, a furnace and two triangle-heavy rooms: each job reads its own lights
+    and materials per path, so every film equals the job rendered alone."""
+    descs = [scenes.box_scene(), scenes.multi_light_scene(), scenes.sky_scene(), scenes.furnace_sphere(),
+             scenes.indirect_scene(level=3), scenes.spheres_scene(count=12, target_tris=30000, seed=77)]
+    built = [x3d.Scene(d) for d in descs]
+    sizes = [(40, 24, 3), (33, 20, 4), (24, 24, 5), (16, 16, 2), (48, 32, 2), (31, 17, 3)]
+    items = [(b, d.camera, w, h, spp) for b, d, (w, h, spp) in zip(built, descs, sizes)]
+    for cap in (1 << 26, 2500):
+        r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=7, seed=5, queue_capacity=cap)
+        singles = [r.render(*it)[0] for it in items]
+        films, st, _ = r.render_batch(items)
+        print(f"[gpu] mixed-scene wavefront batch, capacity {cap}: {st.generations} generations", flush=True)
+        assert st.generations == _generations(items, cap)
+        for k, (f, ref) in enumerate(zip(films, singles)):
+            assert np.array_equal(_bits(f), _bits(ref)), (cap, k)
+            assert f[:, :3].max() > 0, k

@@ -103,13 +103,14 @@ struct WfArgs {
     int fused;
     // lt1: the first hit's emission per path, added when the path ends (w = 1 when there is one).
     float4 *le0;
-    // Fused form: path state follows the queues. xp ... rng above are the state of the paths in
-    // the extension queue this bounce consumes (entry k's state at index k; L.w = path id), the
-    // *_o arrays that of the queue it fills; lfin_in / lfin_out hold (radiance, path id) of the
-    // finishing queues. A shadow ray names the entry whose radiance it adds to: an extension
-    // entry, or a finishing entry with kWfFinBit set.
+    // Fused form: path state follows the queues. xp, wi, beta, L, rng above are the state of the
+    // paths in the extension queue this bounce consumes (entry k's state at index k: (position,
+    // pixel), (direction, density), (throughput after the sample, job), (radiance, path id)), the
+    // *_o arrays that of the queue it fills (xn and wo are not used); lfin_in / lfin_out hold
+    // (radiance, path id) of the finishing queues. A shadow ray names the entry whose radiance it
+    // adds to: an extension entry, or a finishing entry with kWfFinBit set.
     int compact;
-    float4 *xp_o, *xn_o, *wo_o, *beta_o, *L_o, *wi_o;
+    float4 *xp_o, *beta_o, *L_o, *wi_o;
     uint64_t *rng_o;
     float4 *lfin_in, *lfin_out;
     // Multi-job renders (the MULTI kernels): per job scene, camera, first hits and films, and per
@@ -314,7 +315,8 @@ struct NeeSample {
     bool has_shadow, ok;
     f3 sc, sdir, edir;
     float sdist, dens;
-    uint64_t rng; // the path's PCG state after both samples
+    uint64_t rng;  // the path's PCG state after both samples
+    f3 beta_next;  // throughput after the BSDF sample: beta * f * cos / density (ok only)
 };
 
 // TR = kWfMis: the light sample's contribution carries the power-heuristic weight and the BSDF is
@@ -358,6 +360,16 @@ template <int TR> X3D_DI NeeSample nee_sample(DevScene const &S, HitRec const &x
     r.ok = TR == kWfMis ? dens > 0.0f : dens != 0.0f;
     r.dens = dens;
     r.edir = wi;
+    // The throughput update the tracers make once the sampled ray has hit (mis_bounce, li_lt1,
+    // li_unidirect: beta * f * cos / density), computed here where the vertex is at hand, with the
+    // same expression: the path state then needs neither the vertex's normal and material nor
+    // its outgoing direction.
+    r.beta_next = mk(0.0f, 0.0f, 0.0f);
+    if (r.ok) {
+        const float cw = dot(x.n, wi);
+        const f3 f = bsdf_eval(S.mats, x.mat, x.n, wo, wi);
+        r.beta_next = mul(beta, f) * cw / dens;
+    }
     return r;
 }
 
@@ -409,17 +421,17 @@ X3D_DI void push_rays(WfArgs const &A, bool in, uint32_t path, f3 xp3, NeeSample
         A.fin_out[idx[2]] = path;
 }
 
-// Compact form: the state of a live path after its vertex's samples, at its new queue entry.
-X3D_DI void store_state(WfArgs const &A, uint32_t path, HitRec const &x, f3 wo, uint32_t pix, f3 beta, f3 L,
-                        NeeSample const &r, uint32_t const idx[3], uint32_t job) {
+// Compact form: the state of a live path after its vertex's samples, at its new queue entry:
+// (vertex position, pixel), (sampled direction, its density), (throughput after the sample, job),
+// (radiance, path id), PCG state.
+X3D_DI void store_state(WfArgs const &A, uint32_t path, HitRec const &x, uint32_t pix, f3 L, NeeSample const &r,
+                        uint32_t const idx[3], uint32_t job) {
     if (r.ok) {
         const uint32_t e = idx[1];
-        A.xp_o[e] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(static_cast<uint32_t>(x.mat)));
-        A.xn_o[e] = make_float4(x.n.x, x.n.y, x.n.z, __uint_as_float(static_cast<uint32_t>(x.light)));
-        A.wo_o[e] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(pix));
-        A.beta_o[e] = make_float4(beta.x, beta.y, beta.z, __uint_as_float(job));
-        A.L_o[e] = make_float4(L.x, L.y, L.z, __uint_as_float(path));
+        A.xp_o[e] = make_float4(x.p.x, x.p.y, x.p.z, __uint_as_float(pix));
         A.wi_o[e] = make_float4(r.edir.x, r.edir.y, r.edir.z, r.dens);
+        A.beta_o[e] = make_float4(r.beta_next.x, r.beta_next.y, r.beta_next.z, __uint_as_float(job));
+        A.L_o[e] = make_float4(L.x, L.y, L.z, __uint_as_float(path));
         A.rng_o[e] = r.rng;
     } else {
         A.lfin_out[idx[2]] = make_float4(L.x, L.y, L.z, __uint_as_float(path));
@@ -516,7 +528,7 @@ template <bool FUSED, int TR, bool MULTI = false> __global__ __launch_bounds__(2
             uint32_t idx[3];
             push_rays(A, live, id, xp3, r, scratch, idx, job);
             if (live)
-                store_state(A, id, x, wo, pix, mk(1.0f, 1.0f, 1.0f), L, r, idx, job);
+                store_state(A, id, x, pix, L, r, idx, job);
         } else {
             push_active(A, 0, kind, id, live, scratch);
         }
@@ -926,10 +938,13 @@ X3D_DI bool shade_path(WfArgs const &A, DevScene const &S, uint32_t si, uint32_t
     const float4 *attr = static_cast<const float4 *>(S.tri_attr);
     float4 ta = tris[3 * pi], tb = tris[3 * pi + 1], tc = tris[3 * pi + 2];
     float4 na = attr[3 * pi], nb = attr[3 * pi + 1], nc = attr[3 * pi + 2];
-    float4 L4 = A.L[si], wi4 = A.wi[si], xn = A.xn[si], wo4 = A.wo[si], b4 = A.beta[si];
-    float4 xp4 = A.xp[si];
+    // compact form: (position, pixel), (direction, density), (throughput after the sample, job),
+    // (radiance, path id); otherwise the per-path arrays with the vertex's normal and material
+    // and its outgoing direction, the throughput before the sample
+    float4 L4 = A.L[si], wi4 = A.wi[si], b4 = A.beta[si], xp4 = A.xp[si];
+    float4 xn = FUSED ? float4{} : A.xn[si], wo4 = FUSED ? float4{} : A.wo[si];
     path = FUSED ? __float_as_uint(L4.w) : si;
-    pix = __float_as_uint(wo4.w);
+    pix = __float_as_uint(FUSED ? xp4.w : wo4.w);
     L = xyz(L4);
     wi = xyz(wi4);
     bool hit = h.prim >= 0, live = false;
@@ -941,14 +956,16 @@ X3D_DI bool shade_path(WfArgs const &A, DevScene const &S, uint32_t si, uint32_t
         finish_path<TR>(A, path, L);
     } else {
         float dens = wi4.w;
-        int32_t xmat = static_cast<int32_t>(__float_as_uint(xp4.w));
-        f3 xnrm = xyz(xn), wo = xyz(wo4);
         beta = xyz(b4);
-        float cw = dot(xnrm, wi);
-        f3 f = bsdf_eval(S.mats, xmat, xnrm, wo, wi);
-        beta = mul(beta, f) * cw / dens;
+        if (!FUSED) {
+            int32_t xmat = static_cast<int32_t>(__float_as_uint(xp4.w));
+            f3 xnrm = xyz(xn), wo = xyz(wo4);
+            float cw = dot(xnrm, wi);
+            f3 f = bsdf_eval(S.mats, xmat, xnrm, wo, wi);
+            beta = mul(beta, f) * cw / dens;
+        }
         bool go;
-        Pcg g = path_pcg(0, __float_as_uint(wo4.w));
+        Pcg g = path_pcg(0, pix);
         if (TR == kWfMis) {
             if (nx.light >= 0) {
                 float cl = dot(nx.n, -wi);
@@ -1043,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfArgs A, int parity, uint32_t
             uint32_t idx[3];
             push_rays(A, live, path, xp3, r, scratch, idx, job);
             if (live)
-                store_state(A, path, nx, -wi, pix, beta, L, r, idx, job);
+                store_state(A, path, nx, pix, L, r, idx, job);
         } else {
             push_active(A, parity ^ 1, kind, path, live, scratch);
         }
@@ -1565,7 +1582,7 @@ struct WavefrontEngine {
     DevBuf xp, xn, wo, beta, L, wi, rng, lbuf, act0, act1, sq, eq, fin, eh, ctl, spill;
     DevBuf eq_b, fin_b; // the fused form's second queue parity
     // the fused form's second parity of the path state, and the finishing queues' (radiance, id)
-    DevBuf xp_b, xn_b, wo_b, beta_b, L_b, wi_b, rng_b, lfin_a, lfin_b;
+    DevBuf xp_b, beta_b, L_b, wi_b, rng_b, lfin_a, lfin_b;
     DevBuf le0;         // lt1's first-hit emission per path
     DevBuf sq_sorted, eq_sorted, ray_key, ray_rank, bin_mat, bin_tot; // ray reordering
     // bidirectional form: per item state, PCG state, walk ray, emission sample, vertices,
@@ -1855,15 +1872,19 @@ std::string wf_run(WavefrontEngine *e, RenderParams const &P, WavefrontTuning co
         cap = gen_samples * npix;
     }
     size_t f4 = sizeof(float4) * static_cast<size_t>(cap);
+    const bool fused = !tune.sort_by_material && !tune.sort_rays;
     WF_HIP(e->xp.ensure(f4));
-    WF_HIP(e->xn.ensure(f4));
-    WF_HIP(e->wo.ensure(f4));
+    if (!fused) {
+        // the vertex's normal and outgoing direction (the fused form folds them into the
+        // throughput when it samples)
+        WF_HIP(e->xn.ensure(f4));
+        WF_HIP(e->wo.ensure(f4));
+    }
     WF_HIP(e->beta.ensure(f4));
     WF_HIP(e->L.ensure(f4));
     WF_HIP(e->wi.ensure(f4));
     WF_HIP(e->lbuf.ensure(f4));
     WF_HIP(e->rng.ensure(sizeof(uint64_t) * cap));
-    const bool fused = !tune.sort_by_material && !tune.sort_rays;
     if (tune.tracer != kWfMis && !fused)
         return "the lt1 and unidirect tracers run the wavefront form without material or ray sorting only";
     if (multi && (!fused || tune.tracer != kWfMis))
@@ -1873,8 +1894,6 @@ std::string wf_run(WavefrontEngine *e, RenderParams const &P, WavefrontTuning co
     if (fused) {
         // path state follows the queues: a second parity of every state array
         WF_HIP(e->xp_b.ensure(f4));
-        WF_HIP(e->xn_b.ensure(f4));
-        WF_HIP(e->wo_b.ensure(f4));
         WF_HIP(e->beta_b.ensure(f4));
         WF_HIP(e->L_b.ensure(f4));
         WF_HIP(e->wi_b.ensure(f4));
@@ -2024,21 +2043,17 @@ std::string wf_run(WavefrontEngine *e, RenderParams const &P, WavefrontTuning co
         WfCounter *finc[2] = {&A.ctl->fin_count, &A.ctl->fin_count_b};
         const uint32_t p = k & 1u, q = p ^ 1u;
         auto f4p = [](DevBuf &d) { return static_cast<float4 *>(d.p); };
-        float4 *xps[2] = {f4p(e->xp), f4p(e->xp_b)}, *xns[2] = {f4p(e->xn), f4p(e->xn_b)},
-               *wos[2] = {f4p(e->wo), f4p(e->wo_b)}, *betas[2] = {f4p(e->beta), f4p(e->beta_b)},
+        float4 *xps[2] = {f4p(e->xp), f4p(e->xp_b)}, *betas[2] = {f4p(e->beta), f4p(e->beta_b)},
                *Ls[2] = {f4p(e->L), f4p(e->L_b)}, *wis[2] = {f4p(e->wi), f4p(e->wi_b)},
                *lfins[2] = {f4p(e->lfin_a), f4p(e->lfin_b)};
         uint64_t *rngs[2] = {static_cast<uint64_t *>(e->rng.p), static_cast<uint64_t *>(e->rng_b.p)};
         B.xp = xps[p];
-        B.xn = xns[p];
-        B.wo = wos[p];
+        B.xn = B.wo = nullptr; // not part of the compact state
         B.beta = betas[p];
         B.L = Ls[p];
         B.wi = wis[p];
         B.rng = rngs[p];
         B.xp_o = xps[q];
-        B.xn_o = xns[q];
-        B.wo_o = wos[q];
         B.beta_o = betas[q];
         B.L_o = Ls[q];
         B.wi_o = wis[q];

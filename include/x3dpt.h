@@ -280,7 +280,7 @@ typedef struct x3d_render_options {
     uint64_t seed;         /* PCG seed; sample s of pixel p uses stream (seed, p, s) */
     uint32_t workgroup_size; /* WAVEFRONT: traversal workgroup size (64/128/256), 0 = default (64) */
     uint32_t queue_capacity; /* WAVEFRONT: paths in flight per generation, 0 = default = max = 64M
-                              (about 340 B per path; a render allocates for min(capacity,
+                              (about 270 B per path; a render allocates for min(capacity,
                               w*h*spp));
                               PLAIN (every tracer) and WAVEFRONT BIDIRECT_MIS: (pixel, sample)
                               items per generation, whole samples of every pixel (default 16M;

@@ -12,7 +12,7 @@ namespace x3d {
 
 struct WavefrontEngine;
 
-constexpr uint32_t kDefaultQueueCapacity = 1u << 26; // 64M paths: ~13.5 GiB of path state and queues
+constexpr uint32_t kDefaultQueueCapacity = 1u << 26; // 64M paths: ~17 GiB of path state and queues
 constexpr uint32_t kMaxQueueCapacity = 1u << 26;
 
 // Estimator a wavefront render reproduces (bit for bit, against its plain kernel): the MIS tracer,

@@ -1740,7 +1740,7 @@ int x3d_render_options_default(int32_t pt_type, x3d_render_options *opts) {
     std::memset(opts, 0, sizeof(*opts));
     opts->pt_type = pt_type;
     // The wavefront form wherever it exists: the same film, 3.5x (MIS) / 2.0x (lt1) faster on the
-    // 1.3M-triangle spheres workload, 1.7x (BDPT, 6-vertex subpaths) on the 82k-triangle room.
+    // 1.3M-triangle spheres workload, 1.9x (BDPT, 6-vertex subpaths) on the 82k-triangle room.
     const bool wf = pt_type == X3D_PT_UNIDIRECT_MIS || pt_type == X3D_PT_UNIDIRECT_LT1 ||
                     pt_type == X3D_PT_UNIDIRECT || pt_type == X3D_PT_BIDIRECT_MIS;
     opts->form = wf ? X3D_FORM_WAVEFRONT : X3D_FORM_PLAIN;

@@ -1594,8 +1594,13 @@ struct WavefrontEngine {
     // multi-job renders: job, slice and generation tables; an event at the end of each generation
     DevBuf mj_jobs, mj_slices, mj_gens;
     std::vector<hipEvent_t> gen_ev;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> execs;
+    // segmented generations: per segment the sizes of the queues after it (pinned host memory)
+    // and an event after their copy
+    uint32_t *pinned = nullptr;
+    uint32_t pinned_n = 0;
+    std::vector<hipEvent_t> seg_ev;
     WfArgs key{};
     WavefrontTuning key_tune{};
     bool have_graph = false;
@@ -1618,10 +1623,14 @@ struct WavefrontEngine {
             (void)hipEventDestroy(join);
         if (s2)
             (void)hipStreamDestroy(s2);
-        if (exec)
-            (void)hipGraphExecDestroy(exec);
-        if (graph)
-            (void)hipGraphDestroy(graph);
+        for (hipGraphExec_t x : execs)
+            (void)hipGraphExecDestroy(x);
+        for (hipGraph_t x : graphs)
+            (void)hipGraphDestroy(x);
+        for (auto &x : seg_ev)
+            (void)hipEventDestroy(x);
+        if (pinned)
+            (void)hipHostFree(pinned);
     }
 };
 
@@ -2107,21 +2116,27 @@ std::string wf_run(WavefrontEngine *e, RenderParams const &P, WavefrontTuning co
         hipLaunchKernelGGL(k_wf_count, dim3(1), dim3(64), 0, s, B.ctl, static_cast<const WfCounter *>(B.eq_cnt_in),
                            B.cnt);
     };
-    auto record_generation = [&](hipStream_t s) {
+    // A generation: head (reset, generate), bounces [k0, k1), tail (accumulate, next generation).
+    auto record_head = [&](hipStream_t s) {
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, -1);
         if (fused) {
             // generate fills bounce 0's queues (parity 0); bounce k's shade fills bounce k + 1's
             WfArgs G = parity_args(1);
             hipLaunchKernelGGL(gen_k, dim3(g_gen), dim3(256), 0, s, G);
-            for (uint32_t k = 0; k < bounces; k++) {
+        } else {
+            hipLaunchKernelGGL(gen_k, dim3(g_gen), dim3(256), 0, s, A);
+        }
+    };
+    auto record_bounces = [&](hipStream_t s, uint32_t k0, uint32_t k1) {
+        if (fused) {
+            for (uint32_t k = k0; k < k1; k++) {
                 WfArgs B = parity_args(k);
                 trace_bounce(s, B);
                 hipLaunchKernelGGL(k_wf_reset_fused, dim3(1), dim3(64), 0, s, B.ctl, B.eq_cnt_out, B.fin_cnt_out);
                 hipLaunchKernelGGL(shade_k, dim3(g_shade), dim3(256), 0, s, B, 0, k);
             }
         } else {
-            hipLaunchKernelGGL(gen_k, dim3(g_gen), dim3(256), 0, s, A);
-            for (uint32_t k = 0; k < P.max_depth; k++) {
+            for (uint32_t k = k0; k < k1; k++) {
                 int par = static_cast<int>(k & 1u);
                 hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, s, A.ctl, par);
                 hipLaunchKernelGGL(k_wf_nee_bsdf, dim3(g_nee), dim3(256), 0, s, A, par);
@@ -2135,9 +2150,24 @@ std::string wf_run(WavefrontEngine *e, RenderParams const &P, WavefrontTuning co
                 hipLaunchKernelGGL(shade_k, dim3(g_shade), dim3(256), 0, s, A, par, k);
             }
         }
+    };
+    auto record_tail = [&](hipStream_t s) {
         hipLaunchKernelGGL(acc_k, dim3(g_acc), dim3(256), 0, s, A);
         hipLaunchKernelGGL(k_wf_next_gen, dim3(1), dim3(64), 0, s, A.ctl);
     };
+    auto record_generation = [&](hipStream_t s) {
+        record_head(s);
+        record_bounces(s, 0, bounces);
+        record_tail(s);
+    };
+    // The reference-mode tracers' up to kRefModeMaxDepth bounces (Russian roulette ends nearly
+    // every path within about 30): their graph is cut into segments of kSegBounces bounces, and
+    // after each segment the host reads the queue sizes the next bounce consumes; once both are
+    // empty the remaining segments are not launched (one segment runs ahead of the check, so the
+    // GPU never waits for the host).
+    constexpr uint32_t kSegBounces = 4;
+    const bool segmented = fused && bounces > 2 * kSegBounces;
+    const uint32_t nseg = segmented ? (bounces + kSegBounces - 1) / kSegBounces : 0;
     // multi-job renders: the end of each generation (job latencies)
     auto generation_done = [&](uint32_t g) -> hipError_t {
         return multi ? hipEventRecord(e->gen_ev[g], stream) : hipSuccess;
@@ -2151,24 +2181,75 @@ std::string wf_run(WavefrontEngine *e, RenderParams const &P, WavefrontTuning co
     hipLaunchKernelGGL(k_wf_begin_render, dim3(1), dim3(64), 0, stream, A.ctl);
     WF_HIP(hipGetLastError());
     if (tune.use_graph && !profile) {
-        if (!e->have_graph || !same_config(A, e->key, tune, e->key_tune)) {
-            if (e->exec)
-                WF_HIP(hipGraphExecDestroy(e->exec));
-            if (e->graph)
-                WF_HIP(hipGraphDestroy(e->graph));
-            e->exec = nullptr;
-            e->graph = nullptr;
+        // one graph per generation, or (segmented) head, segments and tail
+        const uint32_t ngraphs = segmented ? nseg + 2 : 1;
+        if (!e->have_graph || !same_config(A, e->key, tune, e->key_tune) || e->execs.size() != ngraphs) {
+            for (hipGraphExec_t x : e->execs)
+                WF_HIP(hipGraphExecDestroy(x));
+            for (hipGraph_t x : e->graphs)
+                WF_HIP(hipGraphDestroy(x));
+            e->execs.clear();
+            e->graphs.clear();
             e->have_graph = false;
-            WF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            record_generation(stream);
-            WF_HIP(hipStreamEndCapture(stream, &e->graph));
-            WF_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+            for (uint32_t gi = 0; gi < ngraphs; gi++) {
+                hipGraph_t gr = nullptr;
+                hipGraphExec_t ex = nullptr;
+                WF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                if (!segmented)
+                    record_generation(stream);
+                else if (gi == 0)
+                    record_head(stream);
+                else if (gi + 1 == ngraphs)
+                    record_tail(stream);
+                else
+                    record_bounces(stream, (gi - 1) * kSegBounces, std::min(gi * kSegBounces, bounces));
+                WF_HIP(hipStreamEndCapture(stream, &gr));
+                e->graphs.push_back(gr);
+                WF_HIP(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+                e->execs.push_back(ex);
+            }
             std::memcpy(&e->key, &A, sizeof A);
             e->key_tune = tune;
             e->have_graph = true;
         }
+        if (segmented) {
+            if (!e->pinned)
+                WF_HIP(hipHostMalloc(reinterpret_cast<void **>(&e->pinned), 2 * sizeof(uint32_t) * nseg));
+            else if (e->pinned_n < nseg) {
+                WF_HIP(hipHostFree(e->pinned));
+                e->pinned = nullptr;
+                WF_HIP(hipHostMalloc(reinterpret_cast<void **>(&e->pinned), 2 * sizeof(uint32_t) * nseg));
+            }
+            e->pinned_n = std::max(e->pinned_n, nseg);
+            while (e->seg_ev.size() < nseg) {
+                hipEvent_t ev;
+                WF_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                e->seg_ev.push_back(ev);
+            }
+        }
         for (uint32_t g = 0; g < generations; g++) {
-            WF_HIP(hipGraphLaunch(e->exec, stream));
+            if (!segmented) {
+                WF_HIP(hipGraphLaunch(e->execs[0], stream));
+            } else {
+                WF_HIP(hipGraphLaunch(e->execs[0], stream));
+                for (uint32_t i = 0; i < nseg; i++) {
+                    WF_HIP(hipGraphLaunch(e->execs[1 + i], stream));
+                    // the queues the bounce after this segment consumes (parity kend & 1)
+                    const uint32_t kend = std::min((i + 1) * kSegBounces, bounces);
+                    WfCounter *eqc = (kend & 1u) ? &A.ctl->eq_count_b : &A.ctl->eq_count;
+                    WfCounter *finc = (kend & 1u) ? &A.ctl->fin_count_b : &A.ctl->fin_count;
+                    WF_HIP(hipMemcpyAsync(e->pinned + 2 * i, &eqc->v, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+                    WF_HIP(hipMemcpyAsync(e->pinned + 2 * i + 1, &finc->v, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                          stream));
+                    WF_HIP(hipEventRecord(e->seg_ev[i], stream));
+                    if (i > 0) {
+                        WF_HIP(hipEventSynchronize(e->seg_ev[i - 1]));
+                        if (e->pinned[2 * (i - 1)] == 0 && e->pinned[2 * (i - 1) + 1] == 0)
+                            break; // every path of the generation has ended
+                    }
+                }
+                WF_HIP(hipGraphLaunch(e->execs[ngraphs - 1], stream));
+            }
             WF_HIP(generation_done(g));
         }
     } else {

@@ -259,8 +259,8 @@ typedef struct x3d_renderer x3d_renderer;
 #define X3D_PT_UNIDIRECT_MIS 16
 
 /* Execution forms of the same estimator; every form gives the bit-identical film.
- * x3d_render_options_default picks the wavefront form for UNIDIRECT_MIS, UNIDIRECT_LT1,
- * UNIDIRECT and BIDIRECT_MIS, the plain form for the other tracers. */
+ * x3d_render_options_default picks X3D_FORM_AUTO for UNIDIRECT_MIS, UNIDIRECT_LT1, UNIDIRECT
+ * and BIDIRECT_MIS, the plain form for the other tracers. */
 #define X3D_FORM_PLAIN 0     /* persistent kernels over the (pixel, sample) items of a generation,
                                 each lane taking the next item when its path ends, then an in-order
                                 per-pixel sum (the drop-in) */
@@ -271,6 +271,12 @@ typedef struct x3d_renderer x3d_renderer;
                                 (unsorted, no graph; generations of whole samples sized by
                                 queue_capacity and a 24 GiB working-memory budget, about 2.4 KB
                                 per (pixel, sample) item at 6-vertex subpaths) */
+#define X3D_FORM_AUTO 2      /* per render: the plain form below a per-tracer size, the wavefront
+                                form from it on (w*h*spp samples: UNIDIRECT_MIS 2^20,
+                                UNIDIRECT_LT1 2^21, UNIDIRECT 2^22, BIDIRECT_MIS 2^18 — where the
+                                two forms measured equal on a 1.3M- and an 82k-triangle scene);
+                                the plain form for the other tracers; batches run the wavefront
+                                form */
 
 typedef struct x3d_render_options {
     int32_t pt_type;       /* X3D_PT_* */
@@ -332,7 +338,7 @@ typedef struct x3d_render_stats {
     double extend_ms;           /* WAVEFRONT + profile: closest-hit traversal kernels, summed */
     double shadow_ms;           /* WAVEFRONT + profile: any-hit traversal kernels, summed */
     uint32_t trace_launches;    /* WAVEFRONT + profile: launches of each traversal kernel */
-    uint32_t reserved;
+    int32_t form;               /* the form that rendered: X3D_FORM_PLAIN or X3D_FORM_WAVEFRONT */
 } x3d_render_stats;
 
 /* Autotuner (no reference counterpart): renders `probe_spp` samples per pixel of the given

@@ -67,6 +67,8 @@ def test_render_option_errors_without_gpu(x3d):
         dict(pt_type=x3d.PT_DIRECT, form=x3d.FORM_WAVEFRONT),
         dict(pt_type=x3d.PT_UNIDIRECT_LT1, form=x3d.FORM_WAVEFRONT, sort_key=1),   # sorting: MIS only
         dict(pt_type=x3d.PT_UNIDIRECT, form=x3d.FORM_WAVEFRONT, ray_order=1),
+        dict(pt_type=x3d.PT_UNIDIRECT_LT1, form=x3d.FORM_AUTO, sort_key=1),        # AUTO may pick the wavefront
+        dict(pt_type=x3d.PT_UNIDIRECT_MIS, form=3),
         dict(pt_type=x3d.PT_UNIDIRECT_MIS, workgroup_size=96),
         dict(pt_type=x3d.PT_UNIDIRECT_MIS, stack_depth=12),
         dict(pt_type=x3d.PT_UNIDIRECT_MIS, queue_capacity=(1 << 26) + 1),
@@ -89,12 +91,13 @@ def test_every_export_has_a_ctypes_signature(x3d):
 
 
 def test_option_defaults_pick_the_wavefront_form_where_it_exists(x3d):
-    """x3d_render_options_default: the wavefront form (same film, faster) for the tracers that
-    have one, the plain form for the others; the reference's m_max_path_len of 8."""
+    """x3d_render_options_default: the form chosen per render by size (AUTO: the wavefront form,
+    same film, from the size where it is faster) for the tracers that have a wavefront form, the
+    plain form for the others; the reference's m_max_path_len of 8."""
     import ctypes as C
 
-    for pt, form in ((x3d.PT_UNIDIRECT_MIS, x3d.FORM_WAVEFRONT), (x3d.PT_UNIDIRECT_LT1, x3d.FORM_WAVEFRONT),
-                     (x3d.PT_UNIDIRECT, x3d.FORM_WAVEFRONT), (x3d.PT_BIDIRECT_MIS, x3d.FORM_WAVEFRONT),
+    for pt, form in ((x3d.PT_UNIDIRECT_MIS, x3d.FORM_AUTO), (x3d.PT_UNIDIRECT_LT1, x3d.FORM_AUTO),
+                     (x3d.PT_UNIDIRECT, x3d.FORM_AUTO), (x3d.PT_BIDIRECT_MIS, x3d.FORM_AUTO),
                      (x3d.PT_DIRECT, x3d.FORM_PLAIN), (x3d.PT_NORMAL, x3d.FORM_PLAIN)):
         o = x3d.RenderOptions()
         assert x3d.lib().x3d_render_options_default(pt, C.byref(o)) == 0

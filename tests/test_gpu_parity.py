@@ -312,6 +312,29 @@ def test_wavefront_bdpt_identical_to_plain(x3d, scene_name, kw):
         assert st.trace_launches == 2 * kw["max_path_len"] and st.extend_ms > 0 and st.shadow_ms > 0
 
 
+@pytest.mark.parametrize("tracer,small,large", [
+    ("UNIDIRECT_MIS", (48, 40, 5), (512, 512, 4)),   # 2^20 samples and up: wavefront
+    ("UNIDIRECT_LT1", (48, 40, 5), (512, 512, 8)),   # 2^21
+    ("UNIDIRECT", (48, 40, 5), (1024, 512, 8)),      # 2^22
+    ("BIDIRECT_MIS", (48, 40, 5), (256, 256, 4)),    # 2^18
+])
+def test_auto_form_picks_by_size_with_the_same_film(x3d, tracer, small, large):
+    """X3D_FORM_AUTO renders small films in the plain form and large ones in the wavefront form,
+    and either way the film equals the plain form's bit for bit."""
+    desc = QUERY_SCENES["box"]()
+    gs = x3d.Scene(desc)
+    pt = getattr(x3d, "PT_" + tracer)
+    kw = dict(max_path_len=6) if tracer == "BIDIRECT_MIS" else {}
+    auto = x3d.Renderer(pt, form=x3d.FORM_AUTO, seed=31, **kw)
+    plain = x3d.Renderer(pt, form=x3d.FORM_PLAIN, seed=31, **kw)
+    for (w, h, spp), want in ((small, x3d.FORM_PLAIN), (large, x3d.FORM_WAVEFRONT)):
+        print(f"[gpu] auto form {tracer} {w}x{h}x{spp}", flush=True)
+        a, sa = auto.render(gs, desc.camera, w, h, spp)
+        p, sp = plain.render(gs, desc.camera, w, h, spp)
+        assert sa.form == want and sp.form == x3d.FORM_PLAIN
+        assert np.array_equal(_bits(a), _bits(p))
+
+
 @pytest.mark.parametrize("tracer", ["UNIDIRECT_LT1", "UNIDIRECT", "UNIDIRECT_MIS", "BIDIRECT_MIS"])
 def test_wavefront_without_lights_and_ragged(x3d, tracer):
     """Edge cases of the wavefront form: a scene without lights (no light sample is drawn, the film

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libx3dpt.so")
 # Tracer types (include/x3dpt.h): pathtracer_factory::pt_type order plus the MIS tracer.
 PT_NORMAL, PT_POSITION, PT_DIRECT, PT_UNIDIRECT, PT_UNIDIRECT_LT1, PT_BIDIRECT_LT2, PT_BIDIRECT_MIS = range(7)
 PT_UNIDIRECT_MIS = 16
-FORM_PLAIN, FORM_WAVEFRONT = 0, 1
+FORM_PLAIN, FORM_WAVEFRONT, FORM_AUTO = 0, 1, 2
 SORT_DEFAULT, SORT_MATERIAL, SORT_NONE = 0, 1, 2
 RAY_ORDER_DEFAULT, RAY_ORDER_SORTED, RAY_ORDER_APPEND = 0, 1, 2
 COMPOSITOR_CLAMP, COMPOSITOR_ACES = 0, 1
@@ -83,7 +83,7 @@ class RenderStats(C.Structure):
                 ("anyhit_tests", C.c_uint64), ("first_hit_nodes", C.c_uint64), ("first_hit_tests", C.c_uint64),
                 ("kernel_ms", C.c_double), ("first_hit_ms", C.c_double),
                 ("trace_ms", C.c_double), ("total_ms", C.c_double), ("extend_ms", C.c_double),
-                ("shadow_ms", C.c_double), ("trace_launches", C.c_uint32), ("reserved", C.c_uint32)]
+                ("shadow_ms", C.c_double), ("trace_launches", C.c_uint32), ("form", C.c_int32)]
 
 
 class AutotuneReport(C.Structure):

@@ -577,6 +577,39 @@ x3d::WavefrontTuning wf_tuning(x3d_render_options const &o) {
     return tune;
 }
 
+bool has_wavefront(int32_t pt) {
+    return pt == X3D_PT_UNIDIRECT_MIS || pt == X3D_PT_UNIDIRECT_LT1 || pt == X3D_PT_UNIDIRECT ||
+           pt == X3D_PT_BIDIRECT_MIS;
+}
+
+// The smallest render (w * h * spp samples) X3D_FORM_AUTO runs in the wavefront form: where the
+// two forms measured equal on the spheres workload (1.3M triangles) and the indirect room (82k);
+// below, the wavefront form's per-bounce launches and queue tails cost more than its lanes save.
+uint64_t auto_wavefront_min(int32_t pt) {
+    switch (pt) {
+    case X3D_PT_UNIDIRECT_MIS: return 1ull << 20;
+    case X3D_PT_UNIDIRECT_LT1: return 1ull << 21;
+    case X3D_PT_UNIDIRECT: return 1ull << 22;
+    case X3D_PT_BIDIRECT_MIS: return 1ull << 18;
+    default: return ~0ull;
+    }
+}
+
+// The wavefront engine a render of w x h x spp runs with, or null for the plain form.
+x3d::WavefrontEngine *form_engine(x3d_renderer *r, uint32_t w, uint32_t h, uint32_t spp) {
+    if (!r->wf || r->opt.form == X3D_FORM_PLAIN)
+        return nullptr;
+    if (r->opt.form == X3D_FORM_WAVEFRONT)
+        return r->wf;
+    const uint64_t npix = static_cast<uint64_t>(w) * h;
+    if (npix * spp < auto_wavefront_min(r->opt.pt_type))
+        return nullptr;
+    // the bidirectional wavefront needs one sample of every pixel in one generation
+    if (r->opt.pt_type == X3D_PT_BIDIRECT_MIS && npix > x3d::bd_max_items(r->opt.max_path_len))
+        return nullptr;
+    return r->wf;
+}
+
 // One render. `wf` (with `wf_tune`) selects the wavefront form of the MIS tracer, null the plain
 // kernels; d_dl / d_grad select the adjoint pass.
 // Ray-list mode of render_impl (x3d_trace_rays): the caller's rays replace the camera's (width =
@@ -858,6 +891,7 @@ int render_impl(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camer
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         stats->extend_ms = stats->shadow_ms = 0.0;
         stats->trace_launches = 0;
+        stats->form = wf ? X3D_FORM_WAVEFRONT : X3D_FORM_PLAIN;
         if (wf && wf_tune->profile)
             x3d::wf_last_timing(wf, &stats->shadow_ms, &stats->extend_ms, &stats->trace_launches);
     }
@@ -1003,7 +1037,7 @@ int batch_impl(x3d_renderer *r, uint32_t n, x3d_batch_job *jobs, float *const *f
         X3D_HIP(hipEventCreate(&r->bev1));
     }
     X3D_HIP(r->counters.ensure(2 * sizeof(Counters) + 64));
-    if (r->opt.form == X3D_FORM_WAVEFRONT)
+    if (r->opt.form != X3D_FORM_PLAIN)
         return batch_wavefront(r, n, jobs, films, stats, t0);
     const uint64_t split = r->opt.batch_split ? r->opt.batch_split : kBatchSplitDefault;
     std::vector<x3d::BatchJob> bj;
@@ -1743,7 +1777,7 @@ int x3d_render_options_default(int32_t pt_type, x3d_render_options *opts) {
     // 1.3M-triangle spheres workload, 1.9x (BDPT, 6-vertex subpaths) on the 82k-triangle room.
     const bool wf = pt_type == X3D_PT_UNIDIRECT_MIS || pt_type == X3D_PT_UNIDIRECT_LT1 ||
                     pt_type == X3D_PT_UNIDIRECT || pt_type == X3D_PT_BIDIRECT_MIS;
-    opts->form = wf ? X3D_FORM_WAVEFRONT : X3D_FORM_PLAIN;
+    opts->form = wf ? X3D_FORM_AUTO : X3D_FORM_PLAIN;
     opts->max_depth = 5;
     opts->max_path_len = 8; // bidirect_mis_path_tracer::m_max_path_len (pathtracer.h:201)
     opts->seed = 1;
@@ -1769,14 +1803,14 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
     if (opts->pt_type == X3D_PT_BIDIRECT_MIS && (opts->max_path_len < 1 || opts->max_path_len > x3d::kMaxPathLen))
         return fail(X3D_ERR_INVALID_ARGUMENT,
                     "max_path_len must be in [1, " + std::to_string(x3d::kMaxPathLen) + "] for BIDIRECT_MIS");
-    if (opts->form != X3D_FORM_PLAIN && opts->form != X3D_FORM_WAVEFRONT)
+    if (opts->form != X3D_FORM_PLAIN && opts->form != X3D_FORM_WAVEFRONT && opts->form != X3D_FORM_AUTO)
         return fail(X3D_ERR_INVALID_ARGUMENT, "unknown render form " + std::to_string(opts->form));
     if (opts->form == X3D_FORM_WAVEFRONT && opts->pt_type != X3D_PT_UNIDIRECT_MIS &&
         opts->pt_type != X3D_PT_UNIDIRECT_LT1 && opts->pt_type != X3D_PT_UNIDIRECT &&
         opts->pt_type != X3D_PT_BIDIRECT_MIS)
         return fail(X3D_ERR_UNSUPPORTED, "the wavefront form runs the UNIDIRECT_MIS, UNIDIRECT_LT1, UNIDIRECT and "
                                          "BIDIRECT_MIS tracers only");
-    if (opts->form == X3D_FORM_WAVEFRONT && opts->pt_type != X3D_PT_UNIDIRECT_MIS &&
+    if (opts->form != X3D_FORM_PLAIN && opts->pt_type != X3D_PT_UNIDIRECT_MIS &&
         (opts->sort_key == X3D_SORT_MATERIAL || opts->ray_order == X3D_RAY_ORDER_SORTED))
         return fail(X3D_ERR_UNSUPPORTED, "material and ray sorting apply to the wavefront UNIDIRECT_MIS tracer only");
     if (opts->workgroup_size != 0 && opts->workgroup_size != 64 && opts->workgroup_size != 128 &&
@@ -1801,7 +1835,7 @@ int x3d_renderer_create(const x3d_render_options *opts, x3d_renderer **out) {
     X3D_HIP(hipEventCreate(&r->ev0));
     X3D_HIP(hipEventCreate(&r->ev1));
     X3D_HIP(hipEventCreate(&r->evm));
-    if (opts->form == X3D_FORM_WAVEFRONT)
+    if (opts->form == X3D_FORM_WAVEFRONT || (opts->form == X3D_FORM_AUTO && has_wavefront(opts->pt_type)))
         r->wf = x3d::wf_create();
     *out = r.release();
     return X3D_OK;
@@ -1821,7 +1855,7 @@ int x3d_render_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera 
     if (!d_rgba)
         return fail(X3D_ERR_INVALID_ARGUMENT, "d_rgba is null");
     x3d::WavefrontTuning tune = wf_tuning(r->opt);
-    return render_impl(r, scene, camera, width, height, spp, d_rgba, stats, r->wf, &tune);
+    return render_impl(r, scene, camera, width, height, spp, d_rgba, stats, form_engine(r, width, height, spp), &tune);
 }
 
 namespace {
@@ -1901,8 +1935,8 @@ int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera
     X3D_HIP(r->out.ensure(npix * sizeof(float4)));
     auto t0 = std::chrono::steady_clock::now();
     x3d::WavefrontTuning tune = wf_tuning(r->opt);
-    rc = render_impl(r, scene, camera, width, height, spp, static_cast<float *>(r->out.p), stats, r->wf, &tune,
-                     nullptr, nullptr, firefly_filter != 0);
+    rc = render_impl(r, scene, camera, width, height, spp, static_cast<float *>(r->out.p), stats,
+                     form_engine(r, width, height, spp), &tune, nullptr, nullptr, firefly_filter != 0);
     if (rc)
         return rc;
     X3D_HIP(hipMemcpy(rgba, r->out.p, npix * sizeof(float4), hipMemcpyDeviceToHost));
@@ -2048,7 +2082,7 @@ int x3d_renderer_autotune_batch(x3d_renderer *r, uint32_t num_jobs, x3d_batch_jo
     const uint32_t depths[] = {8, 16};
     // the plain form's shared launch: tile order by cost or by job; the wavefront form has no sort
     std::vector<int32_t> sorts = {X3D_SORT_MATERIAL, X3D_SORT_NONE};
-    if (r->opt.form == X3D_FORM_WAVEFRONT)
+    if (r->opt.form != X3D_FORM_PLAIN)
         sorts = {X3D_SORT_DEFAULT};
     auto time_config = [&](uint32_t wg, uint32_t d, int32_t so, double *ms) -> int {
         r->opt.workgroup_size = wg;

@@ -154,3 +154,27 @@ def test_wavefront_batch_of_mixed_scenes(x3d):
         for k, (f, ref) in enumerate(zip(films, singles)):
             assert np.array_equal(_bits(f), _bits(ref)), (cap, k)
             assert f[:, :3].max() > 0, k
+
+
+def test_wavefront_batch_random_packings(x3d, batch):
+    """Seeded random batches (film sizes 2-70 pixels a side, 1-9 spp, generation sizes from below
+    the largest film up): slices that start a job mid-way through its samples, generations of one
+    slice, jobs spread over many generations; every film equals its job rendered alone."""
+    descs, built, _ = batch
+    rng = np.random.default_rng(1234)
+    for trial in range(4):
+        n = int(rng.integers(3, 9))
+        items = []
+        for _ in range(n):
+            k = int(rng.integers(0, len(built)))
+            w, h, spp = int(rng.integers(2, 71)), int(rng.integers(2, 71)), int(rng.integers(1, 10))
+            items.append((built[k], descs[k].camera, w, h, spp))
+        cap = int(rng.choice([1, 97, 1000, 4096, 20000]))
+        r = x3d.Renderer(x3d.PT_UNIDIRECT_MIS, form=x3d.FORM_WAVEFRONT, max_depth=4, seed=trial, queue_capacity=cap)
+        singles = [r.render(*it)[0] for it in items]
+        films, st, _ = r.render_batch(items)
+        print(f"[gpu] random wavefront batch {trial}: {n} jobs, capacity {cap}, {st.generations} generations",
+              flush=True)
+        assert st.generations == _generations(items, cap)
+        for k, (f, ref) in enumerate(zip(films, singles)):
+            assert np.array_equal(_bits(f), _bits(ref)), (trial, k)

@@ -402,7 +402,9 @@ class Scene:
 
 
 class Renderer:
-    """pt_image_renderer equivalent (x3d_renderer)."""
+    """pt_image_renderer equivalent (x3d_renderer). `form` defaults to FORM_PLAIN here so that a
+    test names the form it exercises; x3d_render_options_default (the C default) gives FORM_AUTO
+    to the tracers that have a wavefront form. Every form gives the same film."""
 
     def __init__(self, pt_type=PT_UNIDIRECT_MIS, form=FORM_PLAIN, max_depth=5, max_path_len=8, seed=1,
                  workgroup_size=0, queue_capacity=0, sort_key=0, stack_depth=0, profile=0, batch_split=0,

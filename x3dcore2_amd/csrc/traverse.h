@@ -139,48 +139,62 @@ X3D_DI void cswap(float *t, uint32_t *r, int a, int b) {
     r[b] = rb;
 }
 
+// Rows of a wide node that hold a ray's near and far plane of each axis. For a ray whose slab
+// test cannot meet NaN (finite, nonzero inverse direction; every box has min <= max, and an
+// empty slot's planes are equal), min((lo - o) * vinv, (hi - o) * vinv) is the lo plane's
+// value when vinv > 0 and the hi plane's when vinv < 0, and max is the other one (rounding is
+// monotone), so loading the planes in that order per ray gives the reference's slab values
+// without the per-axis min and max.
+struct WideSel {
+    uint32_t nx, fx, ny, fy, nz, fz; // row offsets in float4 units within a wide node
+};
+
+X3D_DI WideSel wide_sel(f3 vinv) {
+    constexpr uint32_t g = kWideGroups;
+    WideSel s;
+    s.nx = vinv.x < 0.0f ? g : 0u;
+    s.fx = g - s.nx;
+    s.ny = vinv.y < 0.0f ? 3u * g : 2u * g;
+    s.fy = 5u * g - s.ny;
+    s.nz = vinv.z < 0.0f ? 5u * g : 4u * g;
+    s.fz = 9u * g - s.nz;
+    return s;
+}
+
 // Visits wide node `ref` (a ray that cannot meet NaN in the slab test): tests the child boxes,
-// drops children entering beyond `cut`, pushes the hit children but the nearest, farthest
-// first, and returns the nearest (or the next stacked node, or kDone). Children with equal entry
-// distances may come in either order; the order never changes a result. Unordered (any-hit
-// queries, which stop at the first hit anywhere on the segment): the first hit child is
-// returned and the others pushed as they come, without the sort.
+// drops children entering beyond `cut` (at most tmax), pushes the hit children but the
+// nearest, farthest first, and returns the nearest (or the next stacked node, or kDone).
+// Children with equal entry distances may come in either order; the order never changes a
+// result. Unordered (any-hit queries, which stop at the first hit anywhere on the segment):
+// the first hit child is returned and the others pushed as they come, without the sort.
+// Hit and cull in one comparison: without NaN, !(lo > min(hi, tmax)) && !(lo > cut) is
+// !(lo > min(hi, min(tmax, cut))).
 // `top` (LDS) holds the first kTopNodes nodes when a kernel staged them, else null.
 template <bool ORDERED = true, int WG, int LDS_DEPTH>
-X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, float tmin, float tmax, float cut,
+X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideSel const &sel, float tmin, float cut,
                            TravStack<WG, LDS_DEPTH> &st, const float4 *top = nullptr) {
-    float4 row[7 * kWideGroups];
-    if (kTopNodes > 0 && top && ref < kTopNodes) {
-#pragma unroll
-        for (uint32_t i = 0; i < 7 * kWideGroups; i++)
-            row[i] = top[kWideRows * ref + i];
-    } else {
-        const float4 *n = static_cast<const float4 *>(S.nodes4) + kWideRows * ref;
-#pragma unroll
-        for (uint32_t i = 0; i < 7 * kWideGroups; i++)
-            row[i] = n[i];
-    }
+    const float4 *n = (kTopNodes > 0 && top && ref < kTopNodes) ? top + kWideRows * ref
+                                                                : static_cast<const float4 *>(S.nodes4) + kWideRows * ref;
     float t[kTreeWidth];
     uint32_t r[kTreeWidth];
 #pragma unroll
     for (uint32_t g = 0; g < kWideGroups; g++) {
-        const float4 *p = row + g;
-        const float lx[4] = {p[0].x, p[0].y, p[0].z, p[0].w};
-        const float hx[4] = {p[kWideGroups].x, p[kWideGroups].y, p[kWideGroups].z, p[kWideGroups].w};
-        const float ly[4] = {p[2 * kWideGroups].x, p[2 * kWideGroups].y, p[2 * kWideGroups].z, p[2 * kWideGroups].w};
-        const float hy[4] = {p[3 * kWideGroups].x, p[3 * kWideGroups].y, p[3 * kWideGroups].z, p[3 * kWideGroups].w};
-        const float lz[4] = {p[4 * kWideGroups].x, p[4 * kWideGroups].y, p[4 * kWideGroups].z, p[4 * kWideGroups].w};
-        const float hz[4] = {p[5 * kWideGroups].x, p[5 * kWideGroups].y, p[5 * kWideGroups].z, p[5 * kWideGroups].w};
-        const float rf[4] = {p[6 * kWideGroups].x, p[6 * kWideGroups].y, p[6 * kWideGroups].z, p[6 * kWideGroups].w};
+        const float4 qnx = n[sel.nx + g], qfx = n[sel.fx + g], qny = n[sel.ny + g], qfy = n[sel.fy + g];
+        const float4 qnz = n[sel.nz + g], qfz = n[sel.fz + g], qrf = n[6 * kWideGroups + g];
+        const float nx[4] = {qnx.x, qnx.y, qnx.z, qnx.w}, fx[4] = {qfx.x, qfx.y, qfx.z, qfx.w};
+        const float ny[4] = {qny.x, qny.y, qny.z, qny.w}, fy[4] = {qfy.x, qfy.y, qfy.z, qfy.w};
+        const float nz[4] = {qnz.x, qnz.y, qnz.z, qnz.w}, fz[4] = {qfz.x, qfz.y, qfz.z, qfz.w};
+        const float rf[4] = {qrf.x, qrf.y, qrf.z, qrf.w};
 #pragma unroll
         for (uint32_t j = 0; j < 4; j++) {
-            float tt;
-            bool h = slab_fast(lx[j], hx[j], ly[j], hy[j], lz[j], hz[j], o, vinv, tmin, tmax, &tt);
+            const float lo = fmaxf(fmaxf((nx[j] - o.x) * vinv.x, (ny[j] - o.y) * vinv.y),
+                                   fmaxf((nz[j] - o.z) * vinv.z, tmin));
+            const float hi = fminf(fminf((fx[j] - o.x) * vinv.x, (fy[j] - o.y) * vinv.y),
+                                   fminf((fz[j] - o.z) * vinv.z, cut));
             // an empty slot's box [+inf, +inf] always misses (the ray's vinv is finite, nonzero)
-            uint32_t rr = __float_as_uint(rf[j]);
-            h = h && !(tt > cut);
-            t[4 * g + j] = h ? tt : __builtin_inff(); // misses sort last
-            r[4 * g + j] = h ? rr : kDone;
+            const bool h = !(lo > hi);
+            t[4 * g + j] = h ? lo : __builtin_inff(); // misses sort last
+            r[4 * g + j] = h ? __float_as_uint(rf[j]) : kDone;
         }
     }
     if (!ORDERED) {
@@ -241,6 +255,7 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
     // the wide tree, unless the ray may meet NaN or the reference order is asked for
     const bool wide = !REF_ORDER && !slab_may_nan(o, vinv, S.far_coords != 0);
+    const WideSel sel = wide_sel(vinv);
     const float4 *tris = static_cast<const float4 *>(S.tris);
     uint32_t best_leaf = 0;
     uint32_t ref = wide ? S.root4 : S.root_ref, leaf = kDone;
@@ -255,7 +270,7 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
             float cut = REF_ORDER ? __builtin_inff() : h.t + h.t * kCullSlack;
             if (wide) {
                 pops += kTreeWidth / 2; // a wide node counts as W/2 binary nodes (same bytes and tests)
-                ref = visit_wide(S, ref, o, vinv, kTMin, kTMax, cut, st);
+                ref = visit_wide(S, ref, o, vinv, sel, kTMin, fminf(kTMax, cut), st);
             } else {
                 pops++;
                 ref = visit_interior<REF_ORDER>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);
@@ -305,6 +320,7 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
         return false;
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
     const bool wide = !slab_may_nan(o, vinv, S.far_coords != 0);
+    const WideSel sel = wide_sel(vinv);
     const float4 *tris = static_cast<const float4 *>(S.tris);
     uint32_t ref = wide ? S.root4 : S.root_ref, leaf = kDone;
     if (!is_interior(ref)) {
@@ -316,7 +332,7 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
         while (is_interior(ref)) {
             if (wide) {
                 pops += kTreeWidth / 2;
-                ref = visit_wide<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st);
+                ref = visit_wide<false>(S, ref, o, vinv, sel, tmin, tmax, st);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, true);

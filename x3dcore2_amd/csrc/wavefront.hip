@@ -716,6 +716,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     }
     uint32_t i = wave_fetch_batched(&A.ctl->sq_head.v, slot);
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
+    WideSel sel{};
     float tmin = 1e-4f, tmax = 0.0f;
     uint32_t ref = kDone, leaf = kDone, cur = 0; // cur: next triangle of the parked leaf
     bool blocked = false, wide = false;
@@ -731,6 +732,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
             tmin = (__float_as_uint(b.w) & kBdLongTmin) ? 1e-3f : 1e-4f;
         vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
         wide = !slab_may_nan(o, vinv, S.far_coords != 0);
+        sel = wide_sel(vinv);
         blocked = false;
         st.sp = 0;
         ref = wide ? S.root4 : S.root_ref;
@@ -747,7 +749,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
         while (is_interior(ref)) {
             if (wide) {
                 pops += kTreeWidth / 2; // a wide node counts as W/2 binary nodes (same bytes and tests)
-                ref = visit_wide<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, top);
+                ref = visit_wide<false>(S, ref, o, vinv, sel, tmin, tmax, st, top);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, true);
@@ -835,6 +837,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     }
     uint32_t i = wave_fetch_batched(&A.ctl->eq_head.v, slot);
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
+    WideSel sel{};
     ClosestHit h{-1, __builtin_inff(), 0.0f, 0.0f};
     uint32_t best_leaf = 0, ref = kDone, leaf = kDone, cur = 0; // cur: next triangle of the parked leaf
     bool wide = false;
@@ -846,6 +849,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
         v = xyz(A.wi[path]);
         vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
         wide = !slab_may_nan(o, vinv, S.far_coords != 0);
+        sel = wide_sel(vinv);
         h = ClosestHit{-1, __builtin_inff(), 0.0f, 0.0f};
         best_leaf = 0;
         st.sp = 0;
@@ -864,7 +868,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
             float cut = h.t + h.t * kCullSlack;
             if (wide) {
                 pops += kTreeWidth / 2;
-                ref = visit_wide(S, ref, o, vinv, kTMin, kTMax, cut, st, top);
+                ref = visit_wide(S, ref, o, vinv, sel, kTMin, fminf(kTMax, cut), st, top);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);

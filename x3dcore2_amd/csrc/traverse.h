@@ -162,21 +162,26 @@ X3D_DI WideSel wide_sel(f3 vinv) {
 }
 
 // Visits wide node `ref` (a ray that cannot meet NaN in the slab test): tests the child boxes,
-// drops children entering beyond `cut` (at most tmax), pushes the hit children but the
-// nearest, farthest first, and returns the nearest (or the next stacked node, or kDone).
+// drops children entering beyond `cut` (the caller's min(tmax, cull distance): finite), pushes
+// the hit children but the nearest, farthest first, and returns the nearest (or the next
+// stacked node, or kDone).
 // Children with equal entry distances may come in either order; the order never changes a
 // result. Unordered (any-hit queries, which stop at the first hit anywhere on the segment):
 // the first hit child is returned and the others pushed as they come, without the sort.
-// Hit and cull in one comparison: without NaN, !(lo > min(hi, tmax)) && !(lo > cut) is
-// !(lo > min(hi, min(tmax, cut))).
+// Hit and cull in one comparison: without NaN, !(lo > min(hi, tmax)) && !(lo > cull) is
+// !(lo > min(hi, min(tmax, cull))).
 // `top` (LDS) holds the first kTopNodes nodes when a kernel staged them, else null.
 template <bool ORDERED = true, int WG, int LDS_DEPTH>
 X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideSel const &sel, float tmin, float cut,
                            TravStack<WG, LDS_DEPTH> &st, const float4 *top = nullptr) {
     const float4 *n = (kTopNodes > 0 && top && ref < kTopNodes) ? top + kWideRows * ref
                                                                 : static_cast<const float4 *>(S.nodes4) + kWideRows * ref;
+    // t: entry distance of a hit child, +inf for a miss (a hit child enters at most at
+    // min(tmax, cut), which is finite), so a child is valid exactly when t is finite; r: the
+    // child refs as stored.
     float t[kTreeWidth];
     uint32_t r[kTreeWidth];
+    bool hit[kTreeWidth];
 #pragma unroll
     for (uint32_t g = 0; g < kWideGroups; g++) {
         const float4 qnx = n[sel.nx + g], qfx = n[sel.fx + g], qny = n[sel.ny + g], qfy = n[sel.fy + g];
@@ -192,16 +197,16 @@ X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideS
             const float hi = fminf(fminf((fx[j] - o.x) * vinv.x, (fy[j] - o.y) * vinv.y),
                                    fminf((fz[j] - o.z) * vinv.z, cut));
             // an empty slot's box [+inf, +inf] always misses (the ray's vinv is finite, nonzero)
-            const bool h = !(lo > hi);
-            t[4 * g + j] = h ? lo : __builtin_inff(); // misses sort last
-            r[4 * g + j] = h ? __float_as_uint(rf[j]) : kDone;
+            hit[4 * g + j] = !(lo > hi);
+            t[4 * g + j] = ORDERED && !hit[4 * g + j] ? __builtin_inff() : lo; // misses sort last
+            r[4 * g + j] = __float_as_uint(rf[j]);
         }
     }
     if (!ORDERED) {
         uint32_t next = kDone;
 #pragma unroll
         for (uint32_t i = 0; i < kTreeWidth; i++)
-            if (r[i] != kDone) {
+            if (hit[i]) {
                 if (next == kDone)
                     next = r[i];
                 else
@@ -220,13 +225,13 @@ X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideS
         cswap2(t0, r0, t2, r2);
         cswap2(t1, r1, t3, r3);
         cswap2(t1, r1, t2, r2);
-        if (r3 != kDone)
+        if (t3 != __builtin_inff())
             st.push(r3, t3);
-        if (r2 != kDone)
+        if (t2 != __builtin_inff())
             st.push(r2, t2);
-        if (r1 != kDone)
+        if (t1 != __builtin_inff())
             st.push(r1, t1);
-        if (r0 != kDone)
+        if (t0 != __builtin_inff())
             return r0;
         return st.pop_next(cut);
     } else {
@@ -238,9 +243,9 @@ X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideS
     }
 #pragma unroll
     for (int i = kTreeWidth - 1; i > 0; i--)
-        if (r[i] != kDone)
+        if (t[i] != __builtin_inff())
             st.push(r[i], t[i]);
-    if (r[0] != kDone)
+    if (t[0] != __builtin_inff())
         return r[0];
     return st.pop_next(cut);
 }
@@ -263,14 +268,16 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
         leaf = ref;
         ref = kDone;
     }
+    // cull distance min(tmax, best t (1 + slack)), kept in step with h.t: children that pass the
+    // slab test enter at most at tmax, so the tmax bound changes no cull decision
+    float cut = REF_ORDER ? __builtin_inff() : kTMax;
     st.sp = 0;
     for (;;) {
         // Descend; park the first leaf met and keep going until all active lanes hold one.
         while (is_interior(ref)) {
-            float cut = REF_ORDER ? __builtin_inff() : h.t + h.t * kCullSlack;
             if (wide) {
                 pops += kTreeWidth / 2; // a wide node counts as W/2 binary nodes (same bytes and tests)
-                ref = visit_wide(S, ref, o, vinv, sel, kTMin, fminf(kTMax, cut), st);
+                ref = visit_wide(S, ref, o, vinv, sel, kTMin, cut, st);
             } else {
                 pops++;
                 ref = visit_interior<REF_ORDER>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);
@@ -296,11 +303,13 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
                     h.b2 = c2;
                     h.prim = static_cast<int32_t>(i);
                     best_leaf = s0;
+                    if (!REF_ORDER)
+                        cut = fminf(kTMax, t0 + t0 * kCullSlack);
                 }
             }
             if (is_leaf_ref(ref)) {
                 leaf = ref;
-                ref = st.pop_next(REF_ORDER ? __builtin_inff() : h.t + h.t * kCullSlack);
+                ref = st.pop_next(cut);
             } else {
                 leaf = kDone;
             }

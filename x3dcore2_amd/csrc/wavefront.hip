@@ -839,6 +839,9 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
     f3 o = mk(0, 0, 0), v = mk(0, 0, 0), vinv = mk(0, 0, 0);
     WideSel sel{};
     ClosestHit h{-1, __builtin_inff(), 0.0f, 0.0f};
+    // cull distance min(tmax, best t (1 + slack)), kept in step with h.t: children that pass the
+    // slab test enter at most at tmax, so the tmax bound changes no cull decision
+    float cutc = kTMax;
     uint32_t best_leaf = 0, ref = kDone, leaf = kDone, cur = 0; // cur: next triangle of the parked leaf
     bool wide = false;
     auto init = [&](uint32_t k) {
@@ -851,6 +854,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
         wide = !slab_may_nan(o, vinv, S.far_coords != 0);
         sel = wide_sel(vinv);
         h = ClosestHit{-1, __builtin_inff(), 0.0f, 0.0f};
+        cutc = kTMax;
         best_leaf = 0;
         st.sp = 0;
         ref = wide ? S.root4 : S.root_ref;
@@ -865,18 +869,17 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
         init(i);
     while (i < n) {
         while (is_interior(ref)) {
-            float cut = h.t + h.t * kCullSlack;
             if (wide) {
                 pops += kTreeWidth / 2;
-                ref = visit_wide(S, ref, o, vinv, sel, kTMin, fminf(kTMax, cut), st, top);
+                ref = visit_wide(S, ref, o, vinv, sel, kTMin, cutc, st, top);
             } else {
                 pops++;
-                ref = visit_interior<false>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);
+                ref = visit_interior<false>(S, ref, o, vinv, kTMin, kTMax, cutc, st, true);
             }
             if (is_leaf_ref(ref) && leaf == kDone) {
                 leaf = ref;
                 cur = leaf_start(leaf);
-                ref = st.pop_next(cut);
+                ref = st.pop_next(cutc);
             }
             // descend until fewer than refill_below lanes still look for their first leaf
             if (__popcll(__ballot(leaf == kDone)) < A.refill_below)
@@ -897,6 +900,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
                     h.b2 = c2;
                     h.prim = static_cast<int32_t>(p);
                     best_leaf = s0;
+                    cutc = fminf(kTMax, t0 + t0 * kCullSlack);
                 }
             }
             cur++;
@@ -904,7 +908,7 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
                 if (is_leaf_ref(ref)) {
                     leaf = ref;
                     cur = leaf_start(leaf);
-                    ref = st.pop_next(h.t + h.t * kCullSlack);
+                    ref = st.pop_next(cutc);
                 } else {
                     leaf = kDone;
                 }

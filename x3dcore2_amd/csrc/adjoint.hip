@@ -119,7 +119,7 @@ k_adjoint(RenderParams P, const float4 *dl, uint32_t num_mats, unsigned long lon
         (&slots[0][0])[i] = 0ull;
     __syncthreads();
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
-    Tracer<kWG, kLdsDepth> tr{P.S, st};
+    Tracer<kWG, kLdsDepth, false> tr{P.S, st};
     unsigned long long *my = slots[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
     const uint32_t npix = P.width * P.height;
     const float inv = 1.0f / static_cast<float>(P.spp_total);

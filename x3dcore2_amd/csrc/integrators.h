@@ -176,14 +176,16 @@ struct PathState {
     bool alive;
 };
 
-template <int WG, int LDSD> struct Tracer {
+// NEAR_ROWS: whether traversal keeps the ray's near-plane row offsets in registers (fewer VALU
+// instructions per wide visit) or takes per-axis min / max (three registers fewer).
+template <int WG, int LDSD, bool NEAR_ROWS = true> struct Tracer {
     DevScene const &S;
     TravStack<WG, LDSD> &st;
     uint32_t pops = 0, tests = 0, spops = 0, stests = 0, nclosest = 0, nany = 0;
 
     X3D_DI bool trace(f3 o, f3 v, HitRec &out) {
         nclosest++;
-        ClosestHit h = closest_hit<WG, LDSD, false>(S, o, v, st, pops, tests);
+        ClosestHit h = closest_hit<WG, LDSD, false, NEAR_ROWS>(S, o, v, st, pops, tests);
         if (h.prim < 0)
             return false;
         out = finish_hit(S, h);
@@ -191,7 +193,7 @@ template <int WG, int LDSD> struct Tracer {
     }
     X3D_DI bool occluded(f3 o, f3 v, float tmin, float tmax) {
         nany++;
-        return any_hit<WG, LDSD>(S, o, v, tmin, tmax, st, spops, stests);
+        return any_hit<WG, LDSD, NEAR_ROWS>(S, o, v, tmin, tmax, st, spops, stests);
     }
 
     // transport_illum_source (pathtracer.cpp:152-176) divided by the light sample's area

@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                      float4 *samples, uint32_t *next_item) {
     __shared__ uint2 lds[kLdsDepth * kWG];
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
-    Tracer<kWG, kLdsDepth> tr{P.S, st};
+    Tracer<kWG, kLdsDepth, false> tr{P.S, st};
     // this wave's block: both subpaths of its 64 lanes (a wave-uniform base)
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * kWG + threadIdx.x) >> 6);
     const PathletRows cam{P.subpaths + static_cast<size_t>(wave) * (2 * kPathletRows * kMaxPathLen * 64u)};
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                            float4 *samples, uint32_t *next_item) {
     __shared__ uint2 lds[kLdsDepth * kWG];
     TravStack<kWG, kLdsDepth> st = make_stack<kWG>(lds, P.spill, P.spill_per_thread);
-    Tracer<kWG, kLdsDepth> tr{P.S, st};
+    Tracer<kWG, kLdsDepth, false> tr{P.S, st};
     const uint32_t npix = P.width * P.height, total = npix * gen_samples;
     for (uint32_t q = blockIdx.x * kWG + threadIdx.x; q < total; q = gridDim.x * kWG + atomicAdd(next_item, 1u)) {
         uint32_t pix = q % npix, s = gen_begin + q / npix;

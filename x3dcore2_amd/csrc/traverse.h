@@ -146,18 +146,16 @@ X3D_DI void cswap(float *t, uint32_t *r, int a, int b) {
 // monotone), so loading the planes in that order per ray gives the reference's slab values
 // without the per-axis min and max.
 struct WideSel {
-    uint32_t nx, fx, ny, fy, nz, fz; // row offsets in float4 units within a wide node
+    uint32_t nx, ny, nz; // byte offsets of the near-plane rows within a wide node; the far-plane
+                         // row of an axis is the other row of its pair (offset ^ row bytes)
 };
 
 X3D_DI WideSel wide_sel(f3 vinv) {
-    constexpr uint32_t g = kWideGroups;
+    constexpr uint32_t g = kWideGroups * 16u; // bytes per row
     WideSel s;
     s.nx = vinv.x < 0.0f ? g : 0u;
-    s.fx = g - s.nx;
     s.ny = vinv.y < 0.0f ? 3u * g : 2u * g;
-    s.fy = 5u * g - s.ny;
     s.nz = vinv.z < 0.0f ? 5u * g : 4u * g;
-    s.fz = 9u * g - s.nz;
     return s;
 }
 
@@ -171,11 +169,18 @@ X3D_DI WideSel wide_sel(f3 vinv) {
 // Hit and cull in one comparison: without NaN, !(lo > min(hi, tmax)) && !(lo > cull) is
 // !(lo > min(hi, min(tmax, cull))).
 // `top` (LDS) holds the first kTopNodes nodes when a kernel staged them, else null.
-template <bool ORDERED = true, int WG, int LDS_DEPTH>
+// NEAR_ROWS false: the rows are loaded in stored order and each axis takes the min and max of
+// its two plane distances (the same values; for kernels that cannot keep `sel` in registers).
+template <bool ORDERED = true, bool NEAR_ROWS = true, int WG, int LDS_DEPTH>
 X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideSel const &sel, float tmin, float cut,
                            TravStack<WG, LDS_DEPTH> &st, const float4 *top = nullptr) {
-    const float4 *n = (kTopNodes > 0 && top && ref < kTopNodes) ? top + kWideRows * ref
-                                                                : static_cast<const float4 *>(S.nodes4) + kWideRows * ref;
+    // the node's rows addressed by 32-bit byte offsets from the array base (a wave-uniform base
+    // then needs no 64-bit address arithmetic per row)
+    const bool in_top = kTopNodes > 0 && top && ref < kTopNodes;
+    const char *base = in_top ? reinterpret_cast<const char *>(top) : static_cast<const char *>(S.nodes4);
+    const uint32_t nb = ref * (kWideRows * 16u);
+    constexpr uint32_t kRowPair = kWideGroups * 16u; // near ^ kRowPair = far row of the same axis
+    auto row = [&](uint32_t off) { return *reinterpret_cast<const float4 *>(base + (nb + off)); };
     // t: entry distance of a hit child, +inf for a miss (a hit child enters at most at
     // min(tmax, cut), which is finite), so a child is valid exactly when t is finite; r: the
     // child refs as stored.
@@ -184,18 +189,33 @@ X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideS
     bool hit[kTreeWidth];
 #pragma unroll
     for (uint32_t g = 0; g < kWideGroups; g++) {
-        const float4 qnx = n[sel.nx + g], qfx = n[sel.fx + g], qny = n[sel.ny + g], qfy = n[sel.fy + g];
-        const float4 qnz = n[sel.nz + g], qfz = n[sel.fz + g], qrf = n[6 * kWideGroups + g];
+        const uint32_t gb = 16u * g;
+        const uint32_t ox = NEAR_ROWS ? sel.nx : 0u, oy = NEAR_ROWS ? sel.ny : 2u * kRowPair;
+        const uint32_t oz = NEAR_ROWS ? sel.nz : 4u * kRowPair;
+        const float4 qnx = row(ox + gb), qfx = row((ox ^ kRowPair) + gb);
+        const float4 qny = row(oy + gb), qfy = row((oy ^ kRowPair) + gb);
+        const float4 qnz = row(oz + gb), qfz = row((oz ^ kRowPair) + gb);
+        const float4 qrf = row(6u * kRowPair + gb);
         const float nx[4] = {qnx.x, qnx.y, qnx.z, qnx.w}, fx[4] = {qfx.x, qfx.y, qfx.z, qfx.w};
         const float ny[4] = {qny.x, qny.y, qny.z, qny.w}, fy[4] = {qfy.x, qfy.y, qfy.z, qfy.w};
         const float nz[4] = {qnz.x, qnz.y, qnz.z, qnz.w}, fz[4] = {qfz.x, qfz.y, qfz.z, qfz.w};
         const float rf[4] = {qrf.x, qrf.y, qrf.z, qrf.w};
 #pragma unroll
         for (uint32_t j = 0; j < 4; j++) {
-            const float lo = fmaxf(fmaxf((nx[j] - o.x) * vinv.x, (ny[j] - o.y) * vinv.y),
-                                   fmaxf((nz[j] - o.z) * vinv.z, tmin));
-            const float hi = fminf(fminf((fx[j] - o.x) * vinv.x, (fy[j] - o.y) * vinv.y),
-                                   fminf((fz[j] - o.z) * vinv.z, cut));
+            float ax = (nx[j] - o.x) * vinv.x, bx = (fx[j] - o.x) * vinv.x;
+            float ay = (ny[j] - o.y) * vinv.y, by = (fy[j] - o.y) * vinv.y;
+            float az = (nz[j] - o.z) * vinv.z, bz = (fz[j] - o.z) * vinv.z;
+            if (!NEAR_ROWS) {
+                const float mx = fminf(ax, bx), my = fminf(ay, by), mz = fminf(az, bz);
+                bx = fmaxf(ax, bx);
+                by = fmaxf(ay, by);
+                bz = fmaxf(az, bz);
+                ax = mx;
+                ay = my;
+                az = mz;
+            }
+            const float lo = fmaxf(fmaxf(ax, ay), fmaxf(az, tmin));
+            const float hi = fminf(fminf(bx, by), fminf(bz, cut));
             // an empty slot's box [+inf, +inf] always misses (the ray's vinv is finite, nonzero)
             hit[4 * g + j] = !(lo > hi);
             t[4 * g + j] = ORDERED && !hit[4 * g + j] ? __builtin_inff() : lo; // misses sort last
@@ -251,7 +271,7 @@ X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideS
 }
 
 // REF_ORDER: the reference's traversal order without culling.
-template <int WG, int LDS_DEPTH, bool REF_ORDER>
+template <int WG, int LDS_DEPTH, bool REF_ORDER, bool NEAR_ROWS = true>
 X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_DEPTH> &st,
                               uint32_t &pops, uint32_t &tests) {
     ClosestHit h{-1, __builtin_inff(), 0.0f, 0.0f};
@@ -260,7 +280,7 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
     // the wide tree, unless the ray may meet NaN or the reference order is asked for
     const bool wide = !REF_ORDER && !slab_may_nan(o, vinv, S.far_coords != 0);
-    const WideSel sel = wide_sel(vinv);
+    const WideSel sel = NEAR_ROWS ? wide_sel(vinv) : WideSel{};
     const float4 *tris = static_cast<const float4 *>(S.tris);
     uint32_t best_leaf = 0;
     uint32_t ref = wide ? S.root4 : S.root_ref, leaf = kDone;
@@ -277,7 +297,7 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
         while (is_interior(ref)) {
             if (wide) {
                 pops += kTreeWidth / 2; // a wide node counts as W/2 binary nodes (same bytes and tests)
-                ref = visit_wide(S, ref, o, vinv, sel, kTMin, cut, st);
+                ref = visit_wide<true, NEAR_ROWS>(S, ref, o, vinv, sel, kTMin, cut, st);
             } else {
                 pops++;
                 ref = visit_interior<REF_ORDER>(S, ref, o, vinv, kTMin, kTMax, cut, st, true);
@@ -322,14 +342,14 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
 
 // has_intersect (pathspace.cpp:415-456): any triangle hit in [tmin, tmax] under boxes clipped to
 // the same interval; the result does not depend on the visiting order.
-template <int WG, int LDS_DEPTH>
+template <int WG, int LDS_DEPTH, bool NEAR_ROWS = true>
 X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravStack<WG, LDS_DEPTH> &st,
                     uint32_t &pops, uint32_t &tests) {
     if (S.root_ref == kEmptyRef)
         return false;
     f3 vinv = mk(1.0f / v.x, 1.0f / v.y, 1.0f / v.z);
     const bool wide = !slab_may_nan(o, vinv, S.far_coords != 0);
-    const WideSel sel = wide_sel(vinv);
+    const WideSel sel = NEAR_ROWS ? wide_sel(vinv) : WideSel{};
     const float4 *tris = static_cast<const float4 *>(S.tris);
     uint32_t ref = wide ? S.root4 : S.root_ref, leaf = kDone;
     if (!is_interior(ref)) {
@@ -341,7 +361,7 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
         while (is_interior(ref)) {
             if (wide) {
                 pops += kTreeWidth / 2;
-                ref = visit_wide<false>(S, ref, o, vinv, sel, tmin, tmax, st);
+                ref = visit_wide<false, NEAR_ROWS>(S, ref, o, vinv, sel, tmin, tmax, st);
             } else {
                 pops++;
                 ref = visit_interior<false>(S, ref, o, vinv, tmin, tmax, __builtin_inff(), st, true);

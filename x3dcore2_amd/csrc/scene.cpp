@@ -469,30 +469,188 @@ std::string Scene::upload() {
         q[2] = {l.mn[2], l.mx[2], r.mn[2], r.mx[2]};
         q[3] = {bits_f(ref_of(static_cast<uint32_t>(n + 1))), bits_f(ref_of(flat[n].next_child)), 0.0f, 0.0f};
     }
-    // Wide traversal tree over the same leaves: each wide node takes a binary interior node and
-    // repeatedly replaces its interior child of largest surface area by that child's two
-    // children, up to kTreeWidth children. The child boxes are boxes of the binary tree, so a ray
-    // reaches exactly the leaves it reaches there (an inner box contains its descendants' boxes,
-    // and the slab test is monotone in the box for rays that meet no NaN).
+    // Wide traversal tree over the reference's leaves. For a ray whose slab test cannot meet NaN
+    // the reference reaches exactly the leaves whose own box passes the test: an inner box
+    // contains its descendants' boxes, and the entry distance only shrinks and the exit distance
+    // only grows with the box (subtraction and multiplication round monotonically), so every
+    // ancestor of a passing leaf passes too. Which leaves a ray reaches therefore does not depend
+    // on the tree above the leaves, and neither does the closest hit (its tie rule names leaves,
+    // not tree positions) or an any-hit answer. The tree above the leaves is rebuilt here for
+    // traversal cost: a binned SAH over the leaf boxes on all three axes (the reference bins the
+    // widest axis only and stops at four triangles per leaf), inner boxes the exact unions of
+    // their leaves' boxes. Then each wide node takes a binary node and repeatedly replaces its
+    // interior child of largest surface area by that child's two children, up to kTreeWidth
+    // children.
+    struct UNode {
+        float mn[3], mx[3];
+        uint32_t left, right; // children in `up`; leaf nodes: kDone4
+        uint32_t ref;         // leaf nodes: the reference leaf's ref
+    };
+    std::vector<UNode> up;
+    {
+        struct ULeaf {
+            float mn[3], mx[3], c[3];
+            uint32_t ref;
+            float w; // triangles: the SAH weight
+        };
+        std::vector<ULeaf> lv;
+        for (auto const &f : flat)
+            if (f.num_prims > 0) {
+                ULeaf l;
+                for (int i = 0; i < 3; i++) {
+                    l.mn[i] = f.mn[i];
+                    l.mx[i] = f.mx[i];
+                    l.c[i] = 0.5f * (f.mn[i] + f.mx[i]);
+                }
+                l.ref = leaf_ref(f.prim_start, f.num_prims);
+                l.w = static_cast<float>(f.num_prims);
+                lv.push_back(l);
+            }
+        up.reserve(2 * lv.size());
+        constexpr int kBins = 32;
+        auto area = [](const float *mn, const float *mx) {
+            double dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+            return dx * dy + dy * dz + dz * dx;
+        };
+        const uint32_t median_depth = 2 * static_cast<uint32_t>(std::log2(std::max<size_t>(lv.size(), 2))) + 8;
+        std::function<uint32_t(uint32_t, uint32_t, uint32_t)> build = [&](uint32_t b, uint32_t e,
+                                                                         uint32_t depth) -> uint32_t {
+            uint32_t idx = static_cast<uint32_t>(up.size());
+            up.push_back(UNode{});
+            float mn[3], mx[3], cmn[3], cmx[3];
+            for (int i = 0; i < 3; i++) {
+                mn[i] = cmn[i] = std::numeric_limits<float>::infinity();
+                mx[i] = cmx[i] = -std::numeric_limits<float>::infinity();
+            }
+            for (uint32_t k = b; k < e; k++)
+                for (int i = 0; i < 3; i++) {
+                    mn[i] = std::min(mn[i], lv[k].mn[i]);
+                    mx[i] = std::max(mx[i], lv[k].mx[i]);
+                    cmn[i] = std::min(cmn[i], lv[k].c[i]);
+                    cmx[i] = std::max(cmx[i], lv[k].c[i]);
+                }
+            UNode n;
+            std::copy(mn, mn + 3, n.mn);
+            std::copy(mx, mx + 3, n.mx);
+            n.left = n.right = kDone4;
+            n.ref = kDone4;
+            if (e - b == 1) {
+                n.ref = lv[b].ref;
+                std::copy(lv[b].mn, lv[b].mn + 3, n.mn); // the reference leaf's own box
+                std::copy(lv[b].mx, lv[b].mx + 3, n.mx);
+                up[idx] = n;
+                return idx;
+            }
+            int best_axis = -1, best_bin = 0;
+            double best_cost = std::numeric_limits<double>::infinity();
+            auto bin_of = [&](ULeaf const &l, int a) {
+                float f = (l.c[a] - cmn[a]) / (cmx[a] - cmn[a]) * kBins;
+                return !(f >= 0.0f) ? 0 : f >= kBins ? kBins - 1 : static_cast<int>(f); // NaN: bin 0
+            };
+            if (depth < median_depth)
+                for (int a = 0; a < 3; a++) {
+                    if (!(cmx[a] > cmn[a]))
+                        continue;
+                    float bmn[kBins][3], bmx[kBins][3];
+                    double bw[kBins] = {};
+                    for (int q = 0; q < kBins; q++)
+                        for (int i = 0; i < 3; i++) {
+                            bmn[q][i] = std::numeric_limits<float>::infinity();
+                            bmx[q][i] = -std::numeric_limits<float>::infinity();
+                        }
+                    for (uint32_t k = b; k < e; k++) {
+                        int q = bin_of(lv[k], a);
+                        bw[q] += lv[k].w;
+                        for (int i = 0; i < 3; i++) {
+                            bmn[q][i] = std::min(bmn[q][i], lv[k].mn[i]);
+                            bmx[q][i] = std::max(bmx[q][i], lv[k].mx[i]);
+                        }
+                    }
+                    // suffix sweep: right-side area and weight of a split after bin q
+                    double ra[kBins], rw[kBins];
+                    float smn[3], smx[3];
+                    for (int i = 0; i < 3; i++) {
+                        smn[i] = std::numeric_limits<float>::infinity();
+                        smx[i] = -std::numeric_limits<float>::infinity();
+                    }
+                    double sw = 0;
+                    for (int q = kBins - 1; q > 0; q--) {
+                        for (int i = 0; i < 3; i++) {
+                            smn[i] = std::min(smn[i], bmn[q][i]);
+                            smx[i] = std::max(smx[i], bmx[q][i]);
+                        }
+                        sw += bw[q];
+                        ra[q - 1] = sw > 0 ? area(smn, smx) : 0.0;
+                        rw[q - 1] = sw;
+                    }
+                    for (int i = 0; i < 3; i++) {
+                        smn[i] = std::numeric_limits<float>::infinity();
+                        smx[i] = -std::numeric_limits<float>::infinity();
+                    }
+                    sw = 0;
+                    for (int q = 0; q < kBins - 1; q++) {
+                        for (int i = 0; i < 3; i++) {
+                            smn[i] = std::min(smn[i], bmn[q][i]);
+                            smx[i] = std::max(smx[i], bmx[q][i]);
+                        }
+                        sw += bw[q];
+                        if (sw <= 0 || rw[q] <= 0)
+                            continue;
+                        double c = area(smn, smx) * sw + ra[q] * rw[q];
+                        if (c < best_cost) {
+                            best_cost = c;
+                            best_axis = a;
+                            best_bin = q;
+                        }
+                    }
+                }
+            uint32_t mid;
+            if (best_axis >= 0) {
+                mid = static_cast<uint32_t>(
+                    std::partition(lv.begin() + b, lv.begin() + e,
+                                   [&](ULeaf const &l) { return bin_of(l, best_axis) <= best_bin; }) -
+                    lv.begin());
+            } else {
+                // all centroids equal, or too deep: median of the widest centroid axis
+                int a = 0;
+                for (int i = 1; i < 3; i++)
+                    if (cmx[i] - cmn[i] > cmx[a] - cmn[a])
+                        a = i;
+                mid = (b + e) / 2;
+                std::nth_element(lv.begin() + b, lv.begin() + mid, lv.begin() + e,
+                                 [a](ULeaf const &x, ULeaf const &y) { return x.c[a] < y.c[a]; });
+            }
+            if (mid == b || mid == e)
+                mid = (b + e) / 2;
+            uint32_t l = build(b, mid, depth + 1);
+            uint32_t r = build(mid, e, depth + 1);
+            n.left = l;
+            n.right = r;
+            up[idx] = n;
+            return idx;
+        };
+        if (!lv.empty())
+            build(0, static_cast<uint32_t>(lv.size()), 0);
+    }
     // Record (kWideRows x float4): for each plane (min x, max x, min y, max y, min z, max z) the
     // children in groups of four, then the child refs in groups of four (interior: wide node
     // index; kDone4: no child), padding.
     std::vector<F4> nodes4;
     uint32_t max_push4 = 0;
     std::function<uint32_t(uint32_t, uint32_t)> collapse = [&](uint32_t n, uint32_t pushes) -> uint32_t {
-        FlatNode const &f = flat[n];
-        if (f.num_prims > 0) {
+        UNode const &f = up[n];
+        if (f.ref != kDone4) {
             max_push4 = std::max(max_push4, pushes);
-            return leaf_ref(f.prim_start, f.num_prims);
+            return f.ref;
         }
-        uint32_t c[kTreeWidth] = {n + 1, f.next_child};
+        uint32_t c[kTreeWidth] = {f.left, f.right};
         uint32_t k = 2;
         while (k < kTreeWidth) {
             int best = -1;
             float best_area = -1.0f;
             for (uint32_t i = 0; i < k; i++) {
-                FlatNode const &q = flat[c[i]];
-                if (q.num_prims > 0)
+                UNode const &q = up[c[i]];
+                if (q.ref != kDone4)
                     continue;
                 float dx = q.mx[0] - q.mn[0], dy = q.mx[1] - q.mn[1], dz = q.mx[2] - q.mn[2];
                 float a = dx * dy + dy * dz + dz * dx;
@@ -504,8 +662,8 @@ std::string Scene::upload() {
             if (best < 0)
                 break;
             uint32_t m = c[best];
-            c[best] = m + 1;
-            c[k++] = flat[m].next_child;
+            c[best] = up[m].left;
+            c[k++] = up[m].right;
         }
         uint32_t idx = static_cast<uint32_t>(nodes4.size() / kWideRows);
         nodes4.resize(nodes4.size() + kWideRows);
@@ -521,8 +679,8 @@ std::string Scene::upload() {
                 for (uint32_t j = 0; j < 4; j++) {
                     uint32_t i = 4 * g + j;
                     // an empty slot gets the box [+inf, +inf]: every wide-tree ray misses it
-                    lo[j] = i < k ? flat[c[i]].mn[a] : std::numeric_limits<float>::infinity();
-                    hi[j] = i < k ? flat[c[i]].mx[a] : std::numeric_limits<float>::infinity();
+                    lo[j] = i < k ? up[c[i]].mn[a] : std::numeric_limits<float>::infinity();
+                    hi[j] = i < k ? up[c[i]].mx[a] : std::numeric_limits<float>::infinity();
                 }
                 q[(2 * a) * kWideGroups + g] = {lo[0], lo[1], lo[2], lo[3]};
                 q[(2 * a + 1) * kWideGroups + g] = {hi[0], hi[1], hi[2], hi[3]};
@@ -532,7 +690,7 @@ std::string Scene::upload() {
         }
         return idx;
     };
-    uint32_t root4 = flat.empty() ? kEmptyRef : collapse(0, 0);
+    uint32_t root4 = up.empty() ? kEmptyRef : collapse(0, 0);
     num_nodes4 = static_cast<uint32_t>(nodes4.size() / kWideRows);
     max_stack4 = max_push4 + 1;
     if (kTopNodes > 0 && root4 != kEmptyRef && !(root4 & kLeafBit)) {

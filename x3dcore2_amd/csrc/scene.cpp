@@ -507,7 +507,14 @@ std::string Scene::upload() {
                 lv.push_back(l);
             }
         up.reserve(2 * lv.size());
-        constexpr int kBins = 32;
+        constexpr int kBins = 128;
+        // bin tables, filled and read before a node's recursion (shared by all levels)
+        struct Bins {
+            float bmn[kBins][3], bmx[kBins][3];
+            double bw[kBins], ra[kBins], rw[kBins];
+        };
+        std::vector<Bins> binbuf(1);
+        Bins &B = binbuf[0];
         auto area = [](const float *mn, const float *mx) {
             double dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
             return dx * dy + dy * dz + dz * dx;
@@ -551,13 +558,18 @@ std::string Scene::upload() {
                 for (int a = 0; a < 3; a++) {
                     if (!(cmx[a] > cmn[a]))
                         continue;
-                    float bmn[kBins][3], bmx[kBins][3];
-                    double bw[kBins] = {};
-                    for (int q = 0; q < kBins; q++)
+                    auto &bmn = B.bmn;
+                    auto &bmx = B.bmx;
+                    auto &bw = B.bw;
+                    auto &ra = B.ra;
+                    auto &rw = B.rw;
+                    for (int q = 0; q < kBins; q++) {
+                        bw[q] = 0.0;
                         for (int i = 0; i < 3; i++) {
                             bmn[q][i] = std::numeric_limits<float>::infinity();
                             bmx[q][i] = -std::numeric_limits<float>::infinity();
                         }
+                    }
                     for (uint32_t k = b; k < e; k++) {
                         int q = bin_of(lv[k], a);
                         bw[q] += lv[k].w;
@@ -567,7 +579,6 @@ std::string Scene::upload() {
                         }
                     }
                     // suffix sweep: right-side area and weight of a split after bin q
-                    double ra[kBins], rw[kBins];
                     float smn[3], smx[3];
                     for (int i = 0; i < 3; i++) {
                         smn[i] = std::numeric_limits<float>::infinity();

@@ -223,14 +223,17 @@ X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideS
         }
     }
     if (!ORDERED) {
+        // children in stored order: the first hit one is descended into, the others are pushed
+        // last first, so they come off the stack in stored order as well
         uint32_t next = kDone;
+        float tn = 0.0f;
 #pragma unroll
-        for (uint32_t i = 0; i < kTreeWidth; i++)
+        for (int i = kTreeWidth - 1; i >= 0; i--)
             if (hit[i]) {
-                if (next == kDone)
-                    next = r[i];
-                else
-                    st.push(r[i], t[i]);
+                if (next != kDone)
+                    st.push(next, tn);
+                next = r[i];
+                tn = t[i];
             }
         return next != kDone ? next : st.pop_next(cut);
     }

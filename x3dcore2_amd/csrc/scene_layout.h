@@ -7,10 +7,11 @@
 //             f2 = (c0.min.z, c0.max.z, c1.min.z, c1.max.z)
 //             f3 = (ref c0, ref c1, -, -) as uint32 bits; c0 = left child (flattened n + 1),
 //             c1 = right child (flattened next_child) of the reference layout.
-//   nodes4    kWideRows x float4 per wide node (128 B for 4 children): the same tree with
-//             levels folded in: (min x)[W], (max x)[W], (min y)[W], (max y)[W], (min z)[W],
-//             (max z)[W] of the children, their refs (kDone4 = empty slot, whose box is
-//             [+inf, +inf] on every axis), padding.
+//   nodes4    kWideRows x float4 per wide node (128 B for 4 children): a tree over the
+//             reference layout's leaves (inner nodes built for traversal cost, children
+//             smallest surface area first): (min x)[W], (max x)[W], (min y)[W], (max y)[W],
+//             (min z)[W], (max z)[W] of the children, their refs (kDone4 = empty slot, whose
+//             box is [+inf, +inf] on every axis), padding.
 //   tris      3 x float4 per primitive, in BVH primitive order: v0 (w = mesh id bits),
 //             v1 (w = triangle index bits), v2 (w = 0).
 //   tri_attr  3 x float4 per primitive: n0 (w = material id bits), n1 (w = light id bits or

@@ -1420,6 +1420,31 @@ template <bool WRITE> X3D_DI uint32_t bd_connections(BdArgs const &B, uint32_t j
         }
         cnt++;
     };
+    // The first kBdRegLight light vertices' positions and normals are read once per item and
+    // kept in registers (every camera vertex is joined to each of them); deeper ones are read
+    // per camera vertex.
+    constexpr uint32_t kBdRegLight = 8;
+    f3 rpos[kBdRegLight], rnrm[kBdRegLight];
+#pragma unroll
+    for (uint32_t k = 0; k < kBdRegLight; k++) {
+        rpos[k] = rnrm[k] = mk(0.0f, 0.0f, 0.0f);
+        if (k < light_len) {
+            const size_t ql = static_cast<size_t>(k) * B.n + j;
+            rpos[k] = xyz(B.lv[4 * ql]);
+            rnrm[k] = xyz(B.lv[4 * ql + 1]);
+        }
+    }
+    auto connect = [&](f3 cpos, f3 cnrm, f3 lpos, f3 lnrm, uint32_t slot) {
+        f3 join = cpos - lpos;
+        float dist = norm(join);
+        join = join / dist;
+        float cos_wo = dot(lnrm, join);
+        float cos_wi = dot(cnrm, -join);
+        if (cos_wo > 0.0f && cos_wi > 0.0f)
+            emit(lpos, join, dist - 1e-3f, slot, kBdLongTmin);
+        else if (WRITE)
+            B.rix[slot] = kBdNoRay;
+    };
     for (uint32_t cp = 1; cp <= cam_len; cp++) {
         const size_t qc = static_cast<size_t>(cp - 1) * B.n + j;
         const f3 cpos = xyz(B.cv[4 * qc]), cnrm = xyz(B.cv[4 * qc + 1]);
@@ -1432,18 +1457,13 @@ template <bool WRITE> X3D_DI uint32_t bd_connections(BdArgs const &B, uint32_t j
         } else if (WRITE) {
             B.rix[row] = kBdNoRay;
         }
-        for (uint32_t lp = 1; lp <= light_len; lp++) {
+#pragma unroll
+        for (uint32_t k = 0; k < kBdRegLight; k++)
+            if (k < light_len)
+                connect(cpos, cnrm, rpos[k], rnrm[k], row + (k + 1) * B.n);
+        for (uint32_t lp = kBdRegLight + 1; lp <= light_len; lp++) {
             const size_t ql = static_cast<size_t>(lp - 1) * B.n + j;
-            const f3 lpos = xyz(B.lv[4 * ql]), lnrm = xyz(B.lv[4 * ql + 1]);
-            f3 join = cpos - lpos;
-            float dist = norm(join);
-            join = join / dist;
-            float cos_wo = dot(lnrm, join);
-            float cos_wi = dot(cnrm, -join);
-            if (cos_wo > 0.0f && cos_wi > 0.0f)
-                emit(lpos, join, dist - 1e-3f, row + lp * B.n, kBdLongTmin);
-            else if (WRITE)
-                B.rix[row + lp * B.n] = kBdNoRay;
+            connect(cpos, cnrm, xyz(B.lv[4 * ql]), xyz(B.lv[4 * ql + 1]), row + lp * B.n);
         }
     }
     return cnt;

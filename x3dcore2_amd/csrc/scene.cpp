@@ -927,7 +927,8 @@ std::string Scene::upload() {
     dev.max_stack = std::max(max_leaf_depth + 1, max_stack4);
     dev.num_prims = static_cast<uint32_t>(prims.size());
     dev.num_interior = num_interior;
-    dev.far_coords = 0;
+    // the wide tree's rows are addressed by 32-bit byte offsets: a larger tree is not used
+    dev.far_coords = static_cast<uint64_t>(num_nodes4) * kWideRows * sizeof(F4) < (1ull << 32) ? 0u : 1u;
     for (int a = 0; a < 3; a++) {
         dev.space_lo[a] = space_lo[a];
         dev.space_range[a] = space_hi[a] - space_lo[a];

@@ -107,7 +107,9 @@ struct DevScene {
     uint32_t num_interior;
     float box_lo[3];  // scene bounds (root box): lower corner
     float box_inv[3]; // 1 / extent per axis (0 for a flat axis)
-    uint32_t far_coords; // some box coordinate exceeds 1e30 in magnitude (slab_may_nan)
+    uint32_t far_coords; // every ray traverses the binary tree: some box coordinate exceeds 1e30
+                         // in magnitude (slab_may_nan), or the wide tree spans 4 GiB or more (its
+                         // rows are addressed by 32-bit byte offsets from `nodes4`)
     // if_path_space::aabb (pathspace.cpp:13-18): union of the vertex bounds of every mesh
     // (trimesh::update_aabb, geometry.cpp:97-103); lower corner and extent max - min
     float space_lo[3];

@@ -175,7 +175,8 @@ template <bool ORDERED = true, bool NEAR_ROWS = true, int WG, int LDS_DEPTH>
 X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideSel const &sel, float tmin, float cut,
                            TravStack<WG, LDS_DEPTH> &st, const float4 *top = nullptr) {
     // the node's rows addressed by 32-bit byte offsets from the array base (a wave-uniform base
-    // then needs no 64-bit address arithmetic per row)
+    // then needs no 64-bit address arithmetic per row; a scene whose wide tree spans 4 GiB or
+    // more traverses the binary tree only, DevScene::far_coords)
     const bool in_top = kTopNodes > 0 && top && ref < kTopNodes;
     const char *base = in_top ? reinterpret_cast<const char *>(top) : static_cast<const char *>(S.nodes4);
     const uint32_t nb = ref * (kWideRows * 16u);
@@ -316,7 +317,8 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
         while (leaf != kDone) {
             uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
             for (uint32_t i = s0; i < s0 + cnt; i++) {
-                float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
+                const float4 *tp = tris + 3 * static_cast<size_t>(i);
+                float4 a = tp[0], b = tp[1], c = tp[2];
                 float c1, c2, t0;
                 tests++;
                 if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), kTMin, kTMax, &c1, &c2, &t0) &&
@@ -379,7 +381,8 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
         while (leaf != kDone) {
             uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
             for (uint32_t i = s0; i < s0 + cnt; i++) {
-                float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
+                const float4 *tp = tris + 3 * static_cast<size_t>(i);
+                float4 a = tp[0], b = tp[1], c = tp[2];
                 float c1, c2, t0;
                 tests++;
                 if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), tmin, tmax, &c1, &c2, &t0))

@@ -768,7 +768,8 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
         while (leaf != kDone) {
             {
                 const uint32_t p = cur;
-                float4 a = tris[3 * p], b = tris[3 * p + 1], c = tris[3 * p + 2];
+                const float4 *tp = tris + 3 * static_cast<size_t>(p);
+                float4 a = tp[0], b = tp[1], c = tp[2];
                 float c1, c2, t0;
                 tests++;
                 blocked = tri_hit(o, v, xyz(a), xyz(b), xyz(c), tmin, tmax, &c1, &c2, &t0);
@@ -890,7 +891,8 @@ __global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
         while (leaf != kDone) {
             const uint32_t s0 = leaf_start(leaf), p = cur;
             {
-                float4 a = tris[3 * p], b = tris[3 * p + 1], c = tris[3 * p + 2];
+                const float4 *tp = tris + 3 * static_cast<size_t>(p);
+                float4 a = tp[0], b = tp[1], c = tp[2];
                 float c1, c2, t0;
                 tests++;
                 if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), kTMin, kTMax, &c1, &c2, &t0) &&

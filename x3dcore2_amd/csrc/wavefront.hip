@@ -814,9 +814,15 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
     }
 }
 
+// LDS stack depth of the closest-hit kernel paired with an any-hit depth D, and the resident
+// waves per SIMD its registers are sized for (12 entries x 8 B x 64 lanes = 6 KiB per wave:
+// six waves fit 160 KiB of LDS per CU, at <= 80 VGPRs).
+template <int D> constexpr int kExtendDepth = D == 16 ? 12 : D;
+template <int LDSD> constexpr int kExtendWaves = LDSD == 12 ? 6 : 4;
+
 // MULTI: each ray traverses the scene of its path's job (compact form).
 template <int WG, int LDSD, bool MULTI = false>
-__global__ __launch_bounds__(WG) void k_wf_extend(WfArgs A) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(kExtendWaves<LDSD>))) void k_wf_extend(WfArgs A) {
     __shared__ uint2 lds[LDSD * WG];
     __shared__ uint32_t cursor[2 * (WG / 64)];
     __shared__ float4 top[kTopNodes > 0 ? kTopNodes * kWideRows : 1];
@@ -1688,11 +1694,17 @@ using TraceKernel = void (*)(WfArgs);
 
 struct TraceKernels {
     TraceKernel shadow, extend, bd_shadow;
-    int wg, ldsd;
+    int wg;
+    int ldsd; // the shallowest LDS stack of the three kernels: the spill slices are sized for it
 };
 
+// Stack depth D pairs the any-hit kernels' D LDS entries with kExtendDepth<D> for the
+// closest-hit kernel (12 beside 16: 6 instead of 5 resident waves per SIMD for the extension
+// rays, whose kernel is VALU-bound with divergent lanes, while more resident shadow rays thrash
+// the caches).
 template <int WG, int D, bool M> TraceKernels trace_kernels() {
-    return {k_wf_shadow<WG, D, false, M>, k_wf_extend<WG, D, M>, k_wf_shadow<WG, D, true>, WG, D};
+    return {k_wf_shadow<WG, D, false, M>, k_wf_extend<WG, kExtendDepth<D>, M>, k_wf_shadow<WG, D, true>, WG,
+            kExtendDepth<D> < D ? kExtendDepth<D> : D};
 }
 
 template <bool M> TraceKernels pick_trace_of(int wg, bool shallow) {

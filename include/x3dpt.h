@@ -273,7 +273,7 @@ typedef struct x3d_renderer x3d_renderer;
                                 per (pixel, sample) item at 6-vertex subpaths) */
 #define X3D_FORM_AUTO 2      /* per render: the plain form below a per-tracer size, the wavefront
                                 form from it on (w*h*spp samples: UNIDIRECT_MIS 2^20,
-                                UNIDIRECT_LT1 2^21, UNIDIRECT 2^22, BIDIRECT_MIS 2^18 — where the
+                                UNIDIRECT_LT1 2^22, UNIDIRECT 2^23, BIDIRECT_MIS 2^21 — where the
                                 two forms measured equal on a 1.3M- and an 82k-triangle scene);
                                 the plain form for the other tracers; batches run the wavefront
                                 form */

@@ -588,9 +588,9 @@ bool has_wavefront(int32_t pt) {
 uint64_t auto_wavefront_min(int32_t pt) {
     switch (pt) {
     case X3D_PT_UNIDIRECT_MIS: return 1ull << 20;
-    case X3D_PT_UNIDIRECT_LT1: return 1ull << 21;
-    case X3D_PT_UNIDIRECT: return 1ull << 22;
-    case X3D_PT_BIDIRECT_MIS: return 1ull << 18;
+    case X3D_PT_UNIDIRECT_LT1: return 1ull << 22;
+    case X3D_PT_UNIDIRECT: return 1ull << 23;
+    case X3D_PT_BIDIRECT_MIS: return 1ull << 21;
     default: return ~0ull;
     }
 }

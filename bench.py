@@ -537,7 +537,9 @@ def main():
         nodes = (sp.closest_nodes - sp.first_hit_nodes) / launches
         tests = (sp.closest_tests - sp.first_hit_tests) / launches
         launch_ms = sp.extend_ms / launches
-        kernel = f"k_wf_extend<{wg or 64},{sd or 16}>"
+        # the closest-hit kernel's own LDS stack depth: 12 beside the any-hit kernels' 16
+        ed = 12 if (sd or 16) == 16 else sd
+        kernel = f"k_wf_extend<{wg or 64},{ed}>"
         launch_cfg = (f"{wg or 64},{sd or 16},{'material' if sort == 1 else 'none'},"
                       f"{tuned.get('queue_capacity', 0) or 1 << 26}")
         traffic = traffic_from_profile(f"traffic_{desc.name}_wavefront.md", kernel, launch_cfg)

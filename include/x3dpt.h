@@ -293,7 +293,9 @@ typedef struct x3d_render_options {
                               the sample buffer takes 16 B per item, 256 MiB at the default,
                               min(capacity, w*h*spp) items) */
     int32_t sort_key;        /* WAVEFRONT: X3D_SORT_* */
-    uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16 */
+    uint32_t stack_depth;    /* WAVEFRONT: LDS traversal-stack entries per lane (8/16), 0 = 16;
+                              16 gives the any-hit kernels 16 entries and the closest-hit kernel
+                              12 (deeper entries spill to device memory either way) */
     int32_t profile;         /* WAVEFRONT: 1 = time traversal kernels with events (no graph) */
     uint32_t batch_split;    /* batches: jobs of at least this many samples (w*h*spp) run in the
                               wavefront form after the shared launch of the others; 0 = default

@@ -45,6 +45,18 @@ X3D_DI float smax(float a, float b) { return (a < b) ? b : a; }
 X3D_DI float smin(float a, float b) { return (b < a) ? b : a; }
 X3D_DI f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 
+// A load through a pointer known to address device (global) memory. Scene arrays reached through
+// pointers read from memory (a batch job's scene) are otherwise generic to the compiler, and a
+// generic (flat) load counts against the LDS wait counter too, so every LDS stack access would
+// wait for the node loads in flight.
+typedef float x3d_v4f __attribute__((ext_vector_type(4)));
+X3D_DI float4 ldg(const float4 *p) {
+    // (through the builtin vector type: float4's copy constructor would load through a generic
+    // reference again)
+    const x3d_v4f v = *(const __attribute__((address_space(1))) x3d_v4f *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kTMin = 1e-4f;   // pathspace.cpp:335
 constexpr float kTMax = 1000.0f; // pathspace.cpp:336

@@ -73,10 +73,10 @@ X3D_DI uint32_t leaf_count(uint32_t ref) { return ((ref >> 28) & 3u) + 1u; }
 
 X3D_DI void load_node(DevScene const &S, uint32_t idx, float4 &f0, float4 &f1, float4 &f2, float4 &f3) {
     const float4 *n = static_cast<const float4 *>(S.nodes) + 4u * idx;
-    f0 = n[0];
-    f1 = n[1];
-    f2 = n[2];
-    f3 = n[3];
+    f0 = ldg(n);
+    f1 = ldg(n + 1);
+    f2 = ldg(n + 2);
+    f3 = ldg(n + 3);
 }
 
 // Visits interior node `ref`: tests both child boxes against [tmin, tmax], drops children
@@ -181,7 +181,10 @@ X3D_DI uint32_t visit_wide(DevScene const &S, uint32_t ref, f3 o, f3 vinv, WideS
     const char *base = in_top ? reinterpret_cast<const char *>(top) : static_cast<const char *>(S.nodes4);
     const uint32_t nb = ref * (kWideRows * 16u);
     constexpr uint32_t kRowPair = kWideGroups * 16u; // near ^ kRowPair = far row of the same axis
-    auto row = [&](uint32_t off) { return *reinterpret_cast<const float4 *>(base + (nb + off)); };
+    auto row = [&](uint32_t off) {
+        const float4 *q = reinterpret_cast<const float4 *>(base + (nb + off));
+        return in_top ? *q : ldg(q);
+    };
     // t: entry distance of a hit child, +inf for a miss (a hit child enters at most at
     // min(tmax, cut), which is finite), so a child is valid exactly when t is finite; r: the
     // child refs as stored.
@@ -318,7 +321,7 @@ X3D_DI ClosestHit closest_hit(DevScene const &S, f3 o, f3 v, TravStack<WG, LDS_D
             uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
             for (uint32_t i = s0; i < s0 + cnt; i++) {
                 const float4 *tp = tris + 3 * static_cast<size_t>(i);
-                float4 a = tp[0], b = tp[1], c = tp[2];
+                float4 a = ldg(tp), b = ldg(tp + 1), c = ldg(tp + 2);
                 float c1, c2, t0;
                 tests++;
                 if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), kTMin, kTMax, &c1, &c2, &t0) &&
@@ -382,7 +385,7 @@ X3D_DI bool any_hit(DevScene const &S, f3 o, f3 v, float tmin, float tmax, TravS
             uint32_t s0 = leaf_start(leaf), cnt = leaf_count(leaf);
             for (uint32_t i = s0; i < s0 + cnt; i++) {
                 const float4 *tp = tris + 3 * static_cast<size_t>(i);
-                float4 a = tp[0], b = tp[1], c = tp[2];
+                float4 a = ldg(tp), b = ldg(tp + 1), c = ldg(tp + 2);
                 float c1, c2, t0;
                 tests++;
                 if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), tmin, tmax, &c1, &c2, &t0))

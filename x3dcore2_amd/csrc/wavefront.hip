@@ -769,7 +769,7 @@ __global__ __launch_bounds__(WG) void k_wf_shadow(WfArgs A) {
             {
                 const uint32_t p = cur;
                 const float4 *tp = tris + 3 * static_cast<size_t>(p);
-                float4 a = tp[0], b = tp[1], c = tp[2];
+                float4 a = ldg(tp), b = ldg(tp + 1), c = ldg(tp + 2);
                 float c1, c2, t0;
                 tests++;
                 blocked = tri_hit(o, v, xyz(a), xyz(b), xyz(c), tmin, tmax, &c1, &c2, &t0);
@@ -898,7 +898,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(kExtendWaves
             const uint32_t s0 = leaf_start(leaf), p = cur;
             {
                 const float4 *tp = tris + 3 * static_cast<size_t>(p);
-                float4 a = tp[0], b = tp[1], c = tp[2];
+                float4 a = ldg(tp), b = ldg(tp + 1), c = ldg(tp + 2);
                 float c1, c2, t0;
                 tests++;
                 if (tri_hit(o, v, xyz(a), xyz(b), xyz(c), kTMin, kTMax, &c1, &c2, &t0) &&

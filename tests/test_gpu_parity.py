@@ -26,6 +26,7 @@ QUERY_SCENES = {
     "spheres_small": lambda: scenes.spheres_scene(count=12, target_tris=30000, seed=77),
     "multi_light": lambda: scenes.multi_light_scene(),
     "sky": lambda: scenes.sky_scene(),
+    "ties": lambda: scenes.tie_scene(),
 }
 
 

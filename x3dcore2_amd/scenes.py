@@ -522,6 +522,34 @@ def random_soup(num_tris: int, seed: int, extent=1.0) -> SceneDesc:
     return SceneDesc("soup", mats, [Mesh(v, n, t, 0)], [], translate_camera(0.5, 0.5, 2.0))
 
 
+def tie_scene(seed: int = 5) -> SceneDesc:
+    """Geometry where hits tie: a 10 x 10 grid of quads on the plane z = 0.5 repeated in 8 meshes
+    (identical triangles in different leaves, hit at equal distances), 400 copies of one
+    triangle in one mesh (leaves whose boxes and centroids coincide) and 1000 random triangles."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    g = np.linspace(0.1, 0.9, 11, dtype=np.float32)
+    quads = []
+    for i in range(10):
+        for j in range(10):
+            a, b = (g[i], g[j]), (g[i + 1], g[j + 1])
+            quads += [(a[0], a[1]), (b[0], a[1]), (b[0], b[1]), (a[0], a[1]), (b[0], b[1]), (a[0], b[1])]
+    gv = np.array([(x, y, 0.5) for x, y in quads], np.float32)
+    gn = np.tile(np.array([0, 0, 1], np.float32), (gv.shape[0], 1))
+    gt = np.arange(gv.shape[0], dtype=np.uint32).reshape(-1, 3)
+    for _ in range(8):
+        meshes.append(Mesh(gv.copy(), gn.copy(), gt.copy(), 0))
+    one = np.array([[0.3, 0.2, 0.7], [0.8, 0.3, 0.7], [0.5, 0.8, 0.75]], np.float32)
+    dv = np.tile(one, (400, 1))
+    fn = np.cross(one[1] - one[0], one[2] - one[0])
+    fn = (fn / np.linalg.norm(fn)).astype(np.float32)
+    meshes.append(Mesh(dv, np.tile(fn, (dv.shape[0], 1)), np.arange(dv.shape[0], dtype=np.uint32).reshape(-1, 3), 0))
+    soup = random_soup(1000, seed)
+    meshes.append(soup.meshes[0])
+    mats = [Material(OREN_NAYAR, (0.5, 0.5, 0.5))]
+    return SceneDesc("ties", mats, meshes, [], translate_camera(0.5, 0.5, 2.0))
+
+
 def seeded_rays(desc: SceneDesc, n: int, seed: int, inside=True) -> np.ndarray:
     """Rays as float32 [n, 6] (origin, unit direction): origins uniform in the room (or
     around the scene bounds), directions uniform on the sphere, plus a share of camera-like rays

@@ -271,12 +271,13 @@ typedef struct x3d_renderer x3d_renderer;
                                 (unsorted, no graph; generations of whole samples sized by
                                 queue_capacity and a 24 GiB working-memory budget, about 2.4 KB
                                 per (pixel, sample) item at 6-vertex subpaths) */
-#define X3D_FORM_AUTO 2      /* per render: the plain form below a per-tracer size, the wavefront
-                                form from it on (w*h*spp samples: UNIDIRECT_MIS 2^20,
-                                UNIDIRECT_LT1 2^22, UNIDIRECT 2^23, BIDIRECT_MIS 2^21 — where the
-                                two forms measured equal on a 1.3M- and an 82k-triangle scene);
-                                the plain form for the other tracers; batches run the wavefront
-                                form */
+#define X3D_FORM_AUTO 2      /* per render: the plain form below a per-tracer size that grows as
+                                the scene's triangle count falls, the wavefront form from it on
+                                (w*h*spp samples; from 2^18 triangles on: UNIDIRECT_MIS 2^20,
+                                UNIDIRECT_LT1 2^22, UNIDIRECT 2^23, BIDIRECT_MIS 2^21; for a few
+                                dozen triangles: 2^24, 2^24, 2^24, 2^22 — where the two forms
+                                measured equal); the plain form for the other tracers; batches
+                                run the wavefront form */
 
 typedef struct x3d_render_options {
     int32_t pt_type;       /* X3D_PT_* */

@@ -314,10 +314,11 @@ def test_wavefront_bdpt_identical_to_plain(x3d, scene_name, kw):
 
 
 @pytest.mark.parametrize("tracer,small,large", [
-    ("UNIDIRECT_MIS", (48, 40, 5), (512, 512, 4)),   # 2^20 samples and up: wavefront
-    ("UNIDIRECT_LT1", (48, 40, 5), (1024, 512, 8)),  # 2^22
-    ("UNIDIRECT", (48, 40, 5), (1024, 1024, 8)),     # 2^23
-    ("BIDIRECT_MIS", (48, 40, 5), (512, 512, 8)),    # 2^21
+    # the box room (34 triangles): from 2^24 samples on (MIS, lt1, unidirect), 2^22 (BDPT)
+    ("UNIDIRECT_MIS", (48, 40, 5), (1024, 1024, 16)),
+    ("UNIDIRECT_LT1", (48, 40, 5), (1024, 1024, 16)),
+    ("UNIDIRECT", (48, 40, 5), (1024, 1024, 16)),
+    ("BIDIRECT_MIS", (48, 40, 5), (1024, 1024, 4)),
 ])
 def test_auto_form_picks_by_size_with_the_same_film(x3d, tracer, small, large):
     """X3D_FORM_AUTO renders small films in the plain form and large ones in the wavefront form,

@@ -582,27 +582,33 @@ bool has_wavefront(int32_t pt) {
            pt == X3D_PT_BIDIRECT_MIS;
 }
 
-// The smallest render (w * h * spp samples) X3D_FORM_AUTO runs in the wavefront form: where the
-// two forms measured equal on the spheres workload (1.3M triangles) and the indirect room (82k);
-// below, the wavefront form's per-bounce launches and queue tails cost more than its lanes save.
-uint64_t auto_wavefront_min(int32_t pt) {
+// The smallest render (w * h * spp samples) X3D_FORM_AUTO runs in the wavefront form, per tracer
+// and scene size: where the two forms measured equal. The wavefront form wins back the lanes that
+// divergent traversal costs the plain kernels, so the fewer triangles a scene has, the more
+// samples it takes to pay for its per-bounce launches and queue tails. Measured crossovers (MIS,
+// depth 8): 34 triangles 2^24, 30k 2^23, 82k 2^22, 300k and 1.3M 2^20; lt1: 34 to 82k 2^24, 300k
+// and 1.3M 2^22; unidirect: 1.3M 2^23; BDPT (6-vertex subpaths): 82k 2^21. Unidirect below 2^18
+// triangles and BDPT below 2^16 take one power of two more, as lt1 and MIS do.
+uint64_t auto_wavefront_min(int32_t pt, uint32_t tris) {
+    int lg;
     switch (pt) {
-    case X3D_PT_UNIDIRECT_MIS: return 1ull << 20;
-    case X3D_PT_UNIDIRECT_LT1: return 1ull << 22;
-    case X3D_PT_UNIDIRECT: return 1ull << 23;
-    case X3D_PT_BIDIRECT_MIS: return 1ull << 21;
+    case X3D_PT_UNIDIRECT_MIS: lg = tris < (1u << 14) ? 24 : tris < (1u << 16) ? 23 : tris < (1u << 18) ? 22 : 20; break;
+    case X3D_PT_UNIDIRECT_LT1: lg = tris < (1u << 18) ? 24 : 22; break;
+    case X3D_PT_UNIDIRECT: lg = tris < (1u << 18) ? 24 : 23; break;
+    case X3D_PT_BIDIRECT_MIS: lg = tris < (1u << 16) ? 22 : 21; break;
     default: return ~0ull;
     }
+    return 1ull << lg;
 }
 
 // The wavefront engine a render of w x h x spp runs with, or null for the plain form.
-x3d::WavefrontEngine *form_engine(x3d_renderer *r, uint32_t w, uint32_t h, uint32_t spp) {
+x3d::WavefrontEngine *form_engine(x3d_renderer *r, const x3d_scene *scene, uint32_t w, uint32_t h, uint32_t spp) {
     if (!r->wf || r->opt.form == X3D_FORM_PLAIN)
         return nullptr;
     if (r->opt.form == X3D_FORM_WAVEFRONT)
         return r->wf;
     const uint64_t npix = static_cast<uint64_t>(w) * h;
-    if (npix * spp < auto_wavefront_min(r->opt.pt_type))
+    if (npix * spp < auto_wavefront_min(r->opt.pt_type, scene->s.dev.num_prims))
         return nullptr;
     // the bidirectional wavefront needs one sample of every pixel in one generation
     if (r->opt.pt_type == X3D_PT_BIDIRECT_MIS && npix > x3d::bd_max_items(r->opt.max_path_len))
@@ -1855,7 +1861,7 @@ int x3d_render_device(x3d_renderer *r, const x3d_scene *scene, const x3d_camera 
     if (!d_rgba)
         return fail(X3D_ERR_INVALID_ARGUMENT, "d_rgba is null");
     x3d::WavefrontTuning tune = wf_tuning(r->opt);
-    return render_impl(r, scene, camera, width, height, spp, d_rgba, stats, form_engine(r, width, height, spp), &tune);
+    return render_impl(r, scene, camera, width, height, spp, d_rgba, stats, form_engine(r, scene, width, height, spp), &tune);
 }
 
 namespace {
@@ -1936,7 +1942,7 @@ int x3d_render(x3d_renderer *r, const x3d_scene *scene, const x3d_camera *camera
     auto t0 = std::chrono::steady_clock::now();
     x3d::WavefrontTuning tune = wf_tuning(r->opt);
     rc = render_impl(r, scene, camera, width, height, spp, static_cast<float *>(r->out.p), stats,
-                     form_engine(r, width, height, spp), &tune, nullptr, nullptr, firefly_filter != 0);
+                     form_engine(r, scene, width, height, spp), &tune, nullptr, nullptr, firefly_filter != 0);
     if (rc)
         return rc;
     X3D_HIP(hipMemcpy(rgba, r->out.p, npix * sizeof(float4), hipMemcpyDeviceToHost));

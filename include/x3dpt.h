@@ -157,7 +157,10 @@ int x3d_transform_mesh(const float transform[16], uint32_t num_verts, const floa
 
 /* Builds the SAH BVH on the host with the reference's layout (bvh_path_space_layout::commit,
  * pathspace.cpp:292-328), the light power CDF (lightsources.cpp:45-52), and uploads the
- * scene to device memory. Scenes built with x3d_scene_add_* are immutable afterwards. */
+ * scene to device memory. Beside the reference layout it builds the 4-wide traversal tree most
+ * rays use: the reference's leaves under inner nodes of its own (a 3-axis SAH over the leaf
+ * boxes); which leaves a ray reaches, and so every hit, does not depend on the inner nodes.
+ * Scenes built with x3d_scene_add_* are immutable afterwards. */
 int x3d_scene_commit(x3d_scene *scene);
 
 /* Material parameters of a committed scene (no reference counterpart; for inverse rendering):

@@ -165,7 +165,8 @@ X3D_DI WideSel wide_sel(f3 vinv) {
 // stacked node, or kDone).
 // Children with equal entry distances may come in either order; the order never changes a
 // result. Unordered (any-hit queries, which stop at the first hit anywhere on the segment):
-// the first hit child is returned and the others pushed as they come, without the sort.
+// no sort; the first hit child in stored order (smallest surface area first) is returned and
+// the others are pushed so that they pop in that order.
 // Hit and cull in one comparison: without NaN, !(lo > min(hi, tmax)) && !(lo > cull) is
 // !(lo > min(hi, min(tmax, cull))).
 // `top` (LDS) holds the first kTopNodes nodes when a kernel staged them, else null.
